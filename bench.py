@@ -1,0 +1,176 @@
+"""Benchmark of the fused PBD fluid step on one MI355X.
+
+Workload: the 1,048,576-particle jittered lattice (128 x 64 x 128, spacing
+0.005, h = 0.0125, 5 PBD iterations, dt = 1e-3, scene [0,1]^3), inputs resident
+in HBM. A step is one PositionBasedFluidModel::step (neighbor search,
+integration, 5 density/lambda/displacement iterations, velocity update).
+
+Prints one JSON line: particle-steps/s over the timed steps, the roofline
+position of the dominant kernel (per-pass HIP-event timing, on the stream the
+kernels run on) and the CPU baseline (the oracle on this host's cores, a
+bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s at 1M PBD fluid; batched vs serial; bwd/fwd time ratio"
+HBM_PEAK_GBS = 8000.0
+FP32_PEAK_TFLOPS = 157.3
+
+# Algorithmic traffic per particle and flops per pair (fp32, the pass
+# structure of the reference; neighbor data is on-chip):
+#   sort/bin 60 B, integrate 48 B, per iteration density+lambda 32 B and
+#   displacement 40 B, velocity update 48 B; grid arrays 12 B per cell.
+#   Flops per neighbor pair: 9 in the neighbor test, 33 in density+lambda,
+#   24 in displacement; 30 per particle.
+BYTES_PASS_A = 32
+BYTES_PASS_B = 40
+FLOPS_PAIR_A = 33
+FLOPS_PAIR_B = 24
+
+
+def algorithmic_bytes(n, ncell, iters):
+    return (60 + 48 + 72 * iters + 48) * n + 12 * ncell
+
+
+def algorithmic_flops(n, pairs, iters):
+    return (9 + 57 * iters) * pairs + 30 * n
+
+
+def cpu_baseline(x, steps, threads):
+    """The oracle (C++ restatement of the reference path, OpenMP over
+    particles) on this host: one warm-up step, then `steps` timed steps."""
+    import __graft_entry__ as G
+    O = G.load_oracle()
+    used = O.set_threads(threads)
+    p = O.make_params()
+    v = np.zeros_like(x)
+    xw, vw, _, _ = O.step(p, x, v, 1)
+    t0 = time.perf_counter()
+    O.step(p, xw, vw, steps)
+    dt = time.perf_counter() - t0
+    return {"value": len(x) * steps / dt, "unit": "particle-steps/s", "cores": used, "kind": "port",
+            "sample": f"{len(x)}-particle lattice (the benchmark workload), {steps} steps after 1 warm-up step, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workload", default="lattice_1m", choices=["lattice_1m", "blobs_1m"])
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not part of this "
+              "build", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import __graft_entry__ as G
+    pkg = G.load_package()
+    from importlib import import_module
+    fluid = import_module(G.PKG_NAME + ".fluid")
+
+    x_np = getattr(pkg.scene, args.workload)()
+    n = len(x_np)
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(x_np).to(dev)
+    v = torch.zeros_like(x)
+    rho = torch.empty(n, dtype=torch.float32, device=dev)
+    ctx = fluid.Context(dev)
+    params = ctx.params()
+    ctx.reserve(n)
+
+    # warm-up (also grows the neighbor-mask reservation to what the scene needs)
+    ctx.step(x, v, rho, max(args.warmup, 1))
+    torch.cuda.synchronize()
+    ctx.step(x, v, rho, 1)
+    torch.cuda.synchronize()
+
+    stream = torch.cuda.current_stream(dev)
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    t0 = time.perf_counter()
+    ctx.step(x, v, rho, args.steps)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_total = e0.elapsed_time(e1)
+    ms_step = ms_total / args.steps
+    value = n * args.steps / (ms_total / 1e3)
+
+    stats = ctx.stats()
+    pairs = stats["neighbor_pairs"]
+    ncell = int(np.prod(stats["grid_dims"]))
+    iters = params["iterations"]
+
+    # per-pass timing: the same number of steps again with HIP events around
+    # every pass (recorded on the kernels' stream)
+    ctx.set_profiling(True)
+    ctx.step(x, v, rho, args.steps)
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    pass_ms = {k: (ms / max(cnt, 1), cnt) for k, (ms, cnt) in prof.items()}
+    dom = max(("density_lambda", "displacement", "neighbors", "sort"), key=lambda k: pass_ms[k][0] * pass_ms[k][1])
+    t_dom = pass_ms[dom][0] / 1e3
+    per_launch_bytes = {"density_lambda": BYTES_PASS_A * n, "displacement": BYTES_PASS_B * n,
+                        "neighbors": 28 * n, "sort": 60 * n + 12 * ncell}[dom]
+    per_launch_flops = {"density_lambda": FLOPS_PAIR_A * pairs, "displacement": FLOPS_PAIR_B * pairs,
+                        "neighbors": 9 * pairs, "sort": 0}[dom]
+    achieved_tflops = per_launch_flops / t_dom / 1e12
+    achieved_gbs = per_launch_bytes / t_dom / 1e9
+    roofline = {
+        "bound": "valu",
+        "kernel": {"density_lambda": "k_density_lambda", "displacement": "k_displacement",
+                   "neighbors": "k_neighbors", "sort": "sort passes"}[dom],
+        "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
+        "traffic": None,
+        "hbm": {"achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved_gbs / HBM_PEAK_GBS, 4)},
+        "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2),
+    }
+    b_alg = algorithmic_bytes(n, ncell, iters)
+    f_alg = algorithmic_flops(n, pairs, iters)
+    step_s = ms_step / 1e3
+    whole_step = {"bytes_alg": b_alg, "flops_alg": f_alg,
+                  "hbm_fraction": round(b_alg / step_s / (HBM_PEAK_GBS * 1e9), 4),
+                  "valu_fraction": round(f_alg / step_s / (FP32_PEAK_TFLOPS * 1e12), 4)}
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_step, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": args.workload, "particles": n, "smoothing_length": params["smoothing_length"],
+                   "spacing": params["sampling_distance"], "iterations": iters, "dt": params["dt"],
+                   "grid_cells": ncell, "neighbor_pairs": pairs},
+        "roofline": roofline,
+        "passes_ms": {k: round(v[0], 4) for k, v in pass_ms.items()},
+        "step_roofline": whole_step,
+        "wall_s": round(wall, 3),
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        out["cpu_baseline"] = cpu_baseline(x_np, args.cpu_steps, threads)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

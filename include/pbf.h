@@ -1,0 +1,185 @@
+/* pbf.h - C ABI of the MI355X position-based-fluid step (libpbf.so).
+ *
+ * Replaces, for PhysIKA's PBD particle fluid, the module methods that
+ * PositionBasedFluidModel::step calls
+ * (Dynamics/ParticleSystem/PositionBasedFluidModel.cpp:83-101):
+ *   NeighborQuery::compute            (Framework/Topology/NeighborQuery.cu:152-199)
+ *   ParticleIntegrator::begin/integrate (Dynamics/ParticleSystem/ParticleIntegrator.cu:20-34,129-138)
+ *   DensityPBD::constrain             (Dynamics/ParticleSystem/DensityPBD.cu:271-302)
+ *   SummationDensity::compute         (Dynamics/ParticleSystem/SummationDensity.cu:59-110)
+ * with field semantics kept: positions and velocities are packed xyz float
+ * triples (the reference's Vector<float,3>, Core/Vector/vector_3d.h:31), the
+ * neighborhood is a CSR list (Framework/Topology/NeighborList.h:27-66), the
+ * density is one float per particle.
+ *
+ * Conventions
+ *  - Every function returns an int status: PBF_OK (0) or a negative
+ *    pbf_status; pbf_last_error(ctx) gives the text. No C++ exception crosses
+ *    the ABI.
+ *  - Pointers named d_* are device pointers (hipMalloc'd or any device memory
+ *    of the current HIP device); `stream` is a hipStream_t (NULL = the null
+ *    stream). Calls are stream-ordered and asynchronous unless documented as
+ *    blocking.
+ *  - The caller owns x, v, rho, forces and CSR buffers. The context owns all
+ *    scratch, sized by pbf_reserve() or on first use; a steady-state call with
+ *    a particle count within the reservation allocates nothing.
+ *  - A context is used by one host thread at a time; contexts are independent.
+ */
+#ifndef PBF_H_
+#define PBF_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PBF_ABI_VERSION 1
+
+typedef enum pbf_status {
+  PBF_OK = 0,
+  PBF_ERR_INVALID = -1,     /* bad argument: null pointer, negative size, bad params */
+  PBF_ERR_HIP = -2,         /* a HIP runtime call failed */
+  PBF_ERR_CAPACITY = -3,    /* caller buffer too small (the needed size is reported) */
+  PBF_ERR_ALLOC = -4,       /* device allocation failed */
+  PBF_ERR_UNSUPPORTED = -5  /* size or configuration outside what the library handles */
+} pbf_status;
+
+/* Parameters of the step. pbf_params_default() gives the reference's values
+ * with the fluid workload's smoothing length. */
+typedef struct pbf_params {
+  float smoothing_length;  /* h: NeighborQuery.inRadius (search radius and grid
+                              cell, NeighborQuery.cu:142) and DensityPBD /
+                              SummationDensity varSmoothingLength */
+  float sampling_distance; /* d: DensityPBD.varSamplingDistance; particle mass
+                              rho0*d^3 and the lattice normalisation
+                              (SummationDensity.cu:113-138) */
+  float rest_density;      /* DensityPBD.varRestDensity (mass only) */
+  float dt;                /* Node::m_dt (Framework/Framework/Node.cpp:15) */
+  float gravity[3];        /* SceneGraph::getGravity (SceneGraph.h:80) */
+  int iterations;          /* DensityPBD.varIterationNumber */
+  float lambda_rest;       /* 1000.0f literal of K_ComputeLambdas (DensityPBD.cu:87) */
+  float lambda_eps;        /* 0.1f literal of K_ComputeLambdas (DensityPBD.cu:87) */
+  float corr_scale;        /* 10.0f literal of K_ComputeDisplacement (DensityPBD.cu:158) */
+  float pair_eps;          /* EPSILON 1e-6 of the lambda/dx pair skip (cuda_utilities.h:30) */
+  float scene_lo[3];       /* SceneGraph lower bound (SceneGraph.h:77) */
+  float scene_hi[3];       /* SceneGraph upper bound (SceneGraph.h:78) */
+  int mask_words;          /* neighbor-mask reservation, 32-bit words per particle
+                              (0 = 32); grows automatically, see pbf_step */
+  unsigned flags;          /* reserved, must be 0 */
+} pbf_params;
+
+/* Diagnostics of the last neighbor search of a context. */
+typedef struct pbf_stats {
+  long long neighbor_pairs;      /* sum of neighbor-list sizes (self included) */
+  long long out_of_grid;         /* particles outside the grid: not hashed, still querying */
+  long long nonfinite;           /* particles with a NaN/Inf coordinate (counted in out_of_grid) */
+  long long mask_words_needed;   /* neighbor-mask words the last search wanted */
+  long long mask_words_reserved; /* neighbor-mask words reserved */
+  long long mask_fallback_waves; /* wavefronts that re-tested candidates (mask full) */
+  int grid_dims[3];              /* cells per axis (GridHash::setSpace) */
+  float grid_lo[3];              /* grid origin = scene_lo - 2 cells */
+  float cell_size;
+  float density_factor;          /* m_factor of SummationDensity */
+  float particle_mass;           /* rest_density * d^3 */
+  float neighbor_threshold;      /* squared-distance bound T: sqrtf(s) < h  <=>  s < T */
+} pbf_stats;
+
+typedef struct pbf_ctx pbf_ctx;
+
+int pbf_abi_version(void);
+const char* pbf_status_string(int status);
+
+/* Reference defaults: h 0.0125, d 0.005, rho0 1000, dt 0.001, g (0,-9.8,0),
+ * 5 iterations, lambda constants 1000 / 0.1, correction 10, pair eps 1e-6,
+ * scene [0,1]^3. */
+void pbf_params_default(pbf_params* p);
+
+/* Creates a context on the current HIP device. Replaces module construction
+ * + Module::initialize (NeighborQuery::initializeImpl builds the grid once,
+ * NeighborQuery.cu:107-149; SummationDensity computes m_factor,
+ * SummationDensity.cu:113-138). */
+int pbf_create(const pbf_params* p, pbf_ctx** out);
+int pbf_destroy(pbf_ctx* ctx);
+
+/* Replaces the VarField setters and their callbacks (FieldVar.h:110-145):
+ * recomputes m_factor and the mass; rebuilds the grid when h or the scene
+ * bounds change. Blocking (may reallocate the grid arrays). */
+int pbf_set_params(pbf_ctx* ctx, const pbf_params* p);
+int pbf_get_params(const pbf_ctx* ctx, pbf_params* out);
+
+/* Sizes the context's scratch for up to n particles. Blocking. */
+int pbf_reserve(pbf_ctx* ctx, int n);
+
+/* NeighborQuery::compute (NeighborQuery.cu:152-199): CSR neighbor lists of n
+ * particles at d_x (original order, original ids, self included, strict
+ * |x_i - x_j| < h). d_offsets receives n+1 exclusive offsets (the reference's
+ * m_index plus the total). Rows are written when the total fits in `cap`
+ * elements (d_elems may be NULL to size only); *total receives the total.
+ * Returns PBF_ERR_CAPACITY when cap < total. Blocking: the total is read back,
+ * as the reference does (NeighborQuery.cu:463). */
+int pbf_neighbor_query(pbf_ctx* ctx, const float* d_x, int n, int* d_offsets, int* d_elems,
+                       long long cap, long long* total, void* stream);
+
+/* ParticleIntegrator::begin + integrate (ParticleIntegrator.cu:20-34,129-138):
+ * v += dt*(f + g), x += dt*v. d_f may be NULL (the zeroed force of begin()). */
+int pbf_integrate(pbf_ctx* ctx, float* d_x, float* d_v, const float* d_f, int n, void* stream);
+
+/* SummationDensity::compute (SummationDensity.cu:59-110) over a CSR list. */
+int pbf_density(pbf_ctx* ctx, const float* d_x, const int* d_offsets, const int* d_elems,
+                float* d_rho, int n, void* stream);
+
+/* DensityPBD::constrain (DensityPBD.cu:271-302) over a caller CSR list (any
+ * list, symmetric or not): `iterations` Jacobi rounds of density, lambda and
+ * pairwise displacement (float atomics, as the reference), then
+ * v += (x - x_old)/dt. x and v are updated in place; d_rho (may be NULL)
+ * receives the last iteration's density. */
+int pbf_pbd_constrain(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_offsets,
+                      const int* d_elems, float* d_rho, int n, void* stream);
+
+/* nsteps of PositionBasedFluidModel::step without viscosity
+ * (PositionBasedFluidModel.cpp:83-101): begin -> NeighborQuery on x^n ->
+ * integrate -> DensityPBD::constrain, fused. x and v (original order) are
+ * updated in place, d_rho (may be NULL) receives the density of the last
+ * iteration of the last step. Internally the particles are sorted by cell, the
+ * neighbor sets are kept as bit masks over cell runs and the displacement is
+ * gathered without atomics (deterministic: repeated runs give identical
+ * bits). The neighbor-mask reservation grows between calls when a previous
+ * call needed more (read without blocking once that call has finished). */
+int pbf_step(pbf_ctx* ctx, float* d_x, float* d_v, float* d_rho, int n, int nsteps, void* stream);
+
+/* CSR export of the neighbor lists built by the last pbf_step (its last step,
+ * on the positions before that step's integration), original ids; same
+ * contract as pbf_neighbor_query. Blocking. */
+int pbf_step_neighbors(pbf_ctx* ctx, int* d_offsets, int* d_elems, long long cap, long long* total,
+                       void* stream);
+
+/* Diagnostics of the last neighbor search. Blocking (synchronises the stream
+ * of the last call). */
+int pbf_get_stats(pbf_ctx* ctx, pbf_stats* out);
+
+/* Per-pass GPU timing (HIP events recorded on the call's stream around each
+ * pass of pbf_step; the reference's GTimer, Core/Utility/GTimer.cpp:21-27, has
+ * no fluid hook). Not part of the reference interface. */
+typedef enum pbf_pass {
+  PBF_PASS_SORT = 0,            /* clear + bin + scan + scatter + reorder/integrate */
+  PBF_PASS_NEIGHBORS = 1,       /* neighbor masks (one kernel) */
+  PBF_PASS_DENSITY_LAMBDA = 2,  /* density + lambda (one kernel per iteration) */
+  PBF_PASS_DISPLACEMENT = 3,    /* displacement + position update (one kernel per iteration) */
+  PBF_PASS_COUNT = 4
+} pbf_pass;
+
+typedef struct pbf_profile {
+  double ms[PBF_PASS_COUNT];          /* summed event time per pass */
+  long long launches[PBF_PASS_COUNT]; /* timed segments per pass */
+} pbf_profile;
+
+/* enable != 0 starts (and clears) timing; 0 stops it. */
+int pbf_set_profiling(pbf_ctx* ctx, int enable);
+/* Accumulated timings since profiling was enabled. Blocking. */
+int pbf_get_profile(pbf_ctx* ctx, pbf_profile* out);
+
+const char* pbf_last_error(const pbf_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PBF_H_ */

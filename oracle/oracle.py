@@ -1,0 +1,212 @@
+"""ctypes front end of the CPU oracle (oracle/_build/liborc.so).
+
+Test infrastructure: the checker. Only tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg import this module; the product never does.
+Arrays are numpy, positions and velocities (n, 3) packed xyz.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+_DIR = Path(__file__).resolve().parent
+_LIB_PATH = _DIR / "_build" / "liborc.so"
+
+
+def build(force: bool = False) -> Path:
+    if force or not _LIB_PATH.exists():
+        subprocess.run(["make", "-C", str(_DIR)] + (["-B"] if force else []), check=True,
+                       stdout=subprocess.DEVNULL)
+    return _LIB_PATH
+
+
+def _params_struct(real):
+    class P(C.Structure):
+        _fields_ = [("h", real), ("d", real), ("rho0", real), ("dt", real), ("g", real * 3),
+                    ("iters", C.c_int), ("lam_rest", real), ("lam_eps", real), ("corr", real),
+                    ("pair_eps", real), ("lo", real * 3), ("hi", real * 3), ("dx_mode", C.c_int)]
+    return P
+
+
+ParamsF = _params_struct(C.c_float)
+ParamsD = _params_struct(C.c_double)
+
+DEFAULTS = dict(h=0.0125, d=0.005, rho0=1000.0, dt=0.001, g=(0.0, -9.8, 0.0), iters=5,
+                lam_rest=1000.0, lam_eps=0.1, corr=10.0, pair_eps=1e-6,
+                lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dx_mode=0)
+
+
+def make_params(double: bool = False, **kw):
+    v = dict(DEFAULTS)
+    v.update(kw)
+    P = ParamsD if double else ParamsF
+    p = P()
+    for k, val in v.items():
+        if k in ("g", "lo", "hi"):
+            arr = getattr(p, k)
+            for i in range(3):
+                arr[i] = val[i]
+        else:
+            setattr(p, k, val)
+    return p
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(str(_LIB_PATH))
+        fp, dp, ip, llp = (C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_int),
+                           C.POINTER(C.c_longlong))
+        PF, PD = C.POINTER(ParamsF), C.POINTER(ParamsD)
+        sig = {
+            "orc_abi_version": (C.c_int, []),
+            "orc_set_threads": (C.c_int, [C.c_int]),
+            "orc_grid_f": (None, [PF, fp, fp, ip]),
+            "orc_cell3_f": (None, [PF, fp, C.c_int, ip, llp]),
+            "orc_spiky_w_f": (C.c_float, [C.c_float, C.c_float]),
+            "orc_spiky_grad_f": (C.c_float, [C.c_float, C.c_float]),
+            "orc_spiky_w_d": (C.c_double, [C.c_double, C.c_double]),
+            "orc_spiky_grad_d": (C.c_double, [C.c_double, C.c_double]),
+            "orc_scaling_factor_f": (C.c_float, [C.c_float, C.c_float]),
+            "orc_scaling_factor_d": (C.c_double, [C.c_double, C.c_double]),
+            "orc_mass_f": (C.c_float, [PF]),
+            "orc_neighbor_query_f": (C.c_longlong, [PF, fp, C.c_int, ip, ip, C.c_longlong]),
+            "orc_neighbor_query_d": (C.c_longlong, [PD, dp, C.c_int, ip, ip, C.c_longlong]),
+            "orc_density_f": (None, [PF, fp, C.c_int, ip, ip, fp]),
+            "orc_lambdas_f": (None, [PF, fp, fp, C.c_int, ip, ip, fp]),
+            "orc_displacement_f": (None, [PF, fp, fp, C.c_int, ip, ip, fp, C.c_int]),
+            "orc_pbd_constrain_f": (None, [PF, fp, fp, C.c_int, ip, ip, fp]),
+            "orc_pbd_constrain_d": (None, [PD, dp, dp, C.c_int, ip, ip, dp]),
+            "orc_integrate_f": (None, [PF, fp, fp, fp, C.c_int]),
+            "orc_integrate_d": (None, [PD, dp, dp, dp, C.c_int]),
+            "orc_step_f": (C.c_longlong, [PF, fp, fp, fp, C.c_int, C.c_int]),
+            "orc_step_d": (C.c_longlong, [PD, dp, dp, dp, C.c_int, C.c_int]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a
+
+
+def _ptr(a, ct):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def set_threads(t: int) -> int:
+    return lib().orc_set_threads(int(t))
+
+
+def grid(params):
+    ds = C.c_float()
+    lo = (C.c_float * 3)()
+    dims = (C.c_int * 3)()
+    lib().orc_grid_f(C.byref(params), C.byref(ds), lo, dims)
+    return ds.value, np.array(lo[:], dtype=np.float32), tuple(dims[:])
+
+
+def cell3(params, x):
+    x = _f32(x)
+    n = len(x)
+    c3 = np.zeros((n, 3), dtype=np.int32)
+    cid = np.zeros(n, dtype=np.int64)
+    lib().orc_cell3_f(C.byref(params), _ptr(x, C.c_float), n, _ptr(c3, C.c_int), _ptr(cid, C.c_longlong))
+    return c3, cid
+
+
+def neighbor_query(params, x):
+    """CSR (offsets[n+1], elems) of NeighborQuery::compute on x."""
+    dbl = isinstance(params, ParamsD)
+    x = np.ascontiguousarray(x, dtype=np.float64 if dbl else np.float32)
+    n = len(x)
+    off = np.zeros(n + 1, dtype=np.int32)
+    fn = lib().orc_neighbor_query_d if dbl else lib().orc_neighbor_query_f
+    ct = C.c_double if dbl else C.c_float
+    total = fn(C.byref(params), _ptr(x, ct), n, _ptr(off, C.c_int), None, 0)
+    el = np.zeros(max(total, 1), dtype=np.int32)
+    fn(C.byref(params), _ptr(x, ct), n, _ptr(off, C.c_int), _ptr(el, C.c_int), total)
+    return off, el[:total]
+
+
+def mass(params) -> float:
+    return lib().orc_mass_f(C.byref(params))
+
+
+def density(params, x, off, el):
+    x = _f32(x)
+    rho = np.zeros(len(x), dtype=np.float32)
+    lib().orc_density_f(C.byref(params), _ptr(x, C.c_float), len(x), _ptr(off, C.c_int), _ptr(el, C.c_int),
+                        _ptr(rho, C.c_float))
+    return rho
+
+
+def lambdas(params, rho, x, off, el):
+    x = _f32(x)
+    rho = _f32(rho)
+    lam = np.zeros(len(x), dtype=np.float32)
+    lib().orc_lambdas_f(C.byref(params), _ptr(rho, C.c_float), _ptr(x, C.c_float), len(x), _ptr(off, C.c_int),
+                        _ptr(el, C.c_int), _ptr(lam, C.c_float))
+    return lam
+
+
+def displacement(params, lam, x, off, el, mode=1):
+    x = _f32(x)
+    lam = _f32(lam)
+    dpos = np.zeros_like(x)
+    lib().orc_displacement_f(C.byref(params), _ptr(lam, C.c_float), _ptr(x, C.c_float), len(x),
+                             _ptr(off, C.c_int), _ptr(el, C.c_int), _ptr(dpos, C.c_float), int(mode))
+    return dpos
+
+
+def pbd_constrain(params, x, v, off, el):
+    dbl = isinstance(params, ParamsD)
+    dt = np.float64 if dbl else np.float32
+    ct = C.c_double if dbl else C.c_float
+    x = np.array(x, dtype=dt, copy=True, order="C")
+    v = np.array(v, dtype=dt, copy=True, order="C")
+    rho = np.zeros(len(x), dtype=dt)
+    fn = lib().orc_pbd_constrain_d if dbl else lib().orc_pbd_constrain_f
+    fn(C.byref(params), _ptr(x, ct), _ptr(v, ct), len(x), _ptr(np.ascontiguousarray(off, np.int32), C.c_int),
+       _ptr(np.ascontiguousarray(el, np.int32), C.c_int), _ptr(rho, ct))
+    return x, v, rho
+
+
+def integrate(params, x, v, f=None):
+    dbl = isinstance(params, ParamsD)
+    dt = np.float64 if dbl else np.float32
+    ct = C.c_double if dbl else C.c_float
+    x = np.array(x, dtype=dt, copy=True, order="C")
+    v = np.array(v, dtype=dt, copy=True, order="C")
+    f = None if f is None else np.ascontiguousarray(f, dtype=dt)
+    fn = lib().orc_integrate_d if dbl else lib().orc_integrate_f
+    fn(C.byref(params), _ptr(x, ct), _ptr(v, ct), _ptr(f, ct), len(x))
+    return x, v
+
+
+def step(params, x, v, nsteps=1):
+    """nsteps of the fluid step; returns (x, v, rho_of_last_step, pairs_of_last_step)."""
+    dbl = isinstance(params, ParamsD)
+    dt = np.float64 if dbl else np.float32
+    ct = C.c_double if dbl else C.c_float
+    x = np.array(x, dtype=dt, copy=True, order="C")
+    v = np.array(v, dtype=dt, copy=True, order="C")
+    rho = np.zeros(len(x), dtype=dt)
+    fn = lib().orc_step_d if dbl else lib().orc_step_f
+    pairs = fn(C.byref(params), _ptr(x, ct), _ptr(v, ct), _ptr(rho, ct), len(x), int(nsteps))
+    return x, v, rho, pairs

@@ -1,0 +1,398 @@
+// CPU oracle for the PhysIKA position-based-fluid step (test infrastructure:
+// the checker, never the product). Built with -ffp-contract=off; see
+// pbf_oracle.h for what pins each function.
+#include "pbf_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace pbf_oracle {
+
+// The reference's Vector<Real,3> wraps glm::tvec3 (Core/Vector/vector_3d.h:31).
+// dot() is glm's compute_dot for tvec3: tmp = a*b; (tmp.x + tmp.y) + tmp.z
+// (Extern/glm-0.9.9.7/glm/detail/func_geometric.inl:64-71), and norm() is
+// glm::length = sqrt(dot(v, v)) (func_geometric.inl:95-100,
+// Core/Vector/vector_3d.inl:208-212).
+template <class R> static inline V3<R> sub(const V3<R>& a, const V3<R>& b) {
+  return {a.x - b.x, a.y - b.y, a.z - b.z};
+}
+template <class R> static inline R dot(const V3<R>& a, const V3<R>& b) {
+  R tx = a.x * b.x, ty = a.y * b.y, tz = a.z * b.z;
+  return (tx + ty) + tz;
+}
+template <class R> static inline R norm(const V3<R>& a) { return std::sqrt(dot(a, a)); }
+
+// 1.0/r evaluated in double and stored to Real, as `* (1.0 / r)` does in
+// K_ComputeDisplacement (DensityPBD.cu:158). For float this equals the
+// correctly rounded float reciprocal (double rounding is innocuous for a
+// quotient of floats).
+template <class R> static inline R inv_double(R r) { return (R)(1.0 / (double)r); }
+
+// M_PI literal of Core/Utility/cuda_utilities.h:31, cast to Real as in
+// Kernel.h:43 ((Real)M_PI).
+static constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// GridHash::setSpace, Framework/Topology/GridHash.cu:47-75.
+//   padding = 2; ds = h; lo = lo - padding*ds; nSeg = (hi - lo_orig)/ds;
+//   n = ceil(nSeg) + 1 + 2*padding.
+template <class R> Grid<R> grid_setup(R h, const R lo[3], const R hi[3]) {
+  Grid<R> g;
+  const int padding = 2;
+  g.ds = h;
+  R pad = (R)padding * g.ds;
+  int n[3];
+  for (int c = 0; c < 3; ++c) {
+    g.lo[c] = lo[c] - pad;
+    R nseg = (hi[c] - lo[c]) / g.ds;
+    n[c] = (int)(std::ceil(nseg) + (R)1 + (R)(2 * padding));
+  }
+  g.nx = n[0];
+  g.ny = n[1];
+  g.nz = n[2];
+  g.num = (long long)g.nx * g.ny * g.nz;
+  return g;
+}
+
+// GridHash::getIndex3 (GridHash.h:55-62): floor((p - lo)/ds) per axis with a
+// true division. Out-of-range values are clamped to [-2, n+1] (the set of
+// valid cells in the +-1 stencil is the same), NaN maps to -2.
+template <class R> static inline int cell_coord(R p, R lo, R ds, int n) {
+  R f = std::floor((p - lo) / ds);
+  if (!(f >= (R)-2)) return -2;  // also NaN
+  if (f > (R)(n + 1)) return n + 1;
+  return (int)f;
+}
+template <class R> void cell3(const Grid<R>& g, const V3<R>& p, int out[3]) {
+  out[0] = cell_coord(p.x, g.lo[0], g.ds, g.nx);
+  out[1] = cell_coord(p.y, g.lo[1], g.ds, g.ny);
+  out[2] = cell_coord(p.z, g.lo[2], g.ds, g.nz);
+}
+
+// GridHash::getIndex(i,j,k), GridHash.h:37-44.
+template <class R> long long cell_id(const Grid<R>& g, int i, int j, int k) {
+  if (i < 0 || i >= g.nx) return -1;
+  if (j < 0 || j >= g.ny) return -1;
+  if (k < 0 || k >= g.nz) return -1;
+  return (long long)i + (long long)j * g.nx + (long long)k * g.nx * g.ny;
+}
+
+// The 27 stencil offsets in the order of NeighborQuery.cu:14-41 (centre, the
+// 6 face neighbours, the 12 edge neighbours, the 8 corners).
+static const int kOffsets[27][3] = {
+    {0, 0, 0},   {0, 0, 1},   {0, 1, 0},   {1, 0, 0},   {0, 0, -1},  {0, -1, 0},  {-1, 0, 0},
+    {0, 1, 1},   {0, 1, -1},  {0, -1, 1},  {0, -1, -1}, {1, 0, 1},   {1, 0, -1},  {-1, 0, 1},
+    {-1, 0, -1}, {1, 1, 0},   {1, -1, 0},  {-1, 1, 0},  {-1, -1, 0}, {1, 1, 1},   {1, 1, -1},
+    {1, -1, 1},  {-1, 1, 1},  {1, -1, -1}, {-1, 1, -1}, {-1, -1, 1}, {-1, -1, -1}};
+
+// SpikyKernel::Weight, Dynamics/ParticleSystem/Kernel.h:36-45.
+template <class R> R spiky_w(R r, R h) {
+  const R q = r / h;
+  if (q > (R)1) return (R)0;
+  const R d = (R)(1.0 - (double)q);
+  const R hh = h * h;
+  return (R)15 / ((R)kPi * hh * h) * d * d * d * (R)1;
+}
+
+// SpikyKernel::Gradient, Dynamics/ParticleSystem/Kernel.h:47-57.
+template <class R> R spiky_grad(R r, R h) {
+  const R q = r / h;
+  if (q > (R)1) return (R)0;
+  const R d = (R)(1.0 - (double)q);
+  const R hh = h * h;
+  return (R)-45 / ((R)kPi * hh * h) * d * d * (R)1;
+}
+
+// SummationDensity::calculateScalingFactor, SummationDensity.cu:113-138.
+template <class R> R scaling_factor(R d, R H) {
+  R V = d * d * d;
+  R total = (R)0;
+  int half_res = (int)(H / d + (R)1);
+  for (int i = -half_res; i <= half_res; i++)
+    for (int j = -half_res; j <= half_res; j++)
+      for (int k = -half_res; k <= half_res; k++) {
+        R x = (R)i * d;
+        R y = (R)j * d;
+        R z = (R)k * d;
+        R r = std::sqrt(x * x + y * y + z * z);
+        total += V * spiky_w(r, H);
+      }
+  return (R)(1.0 / (double)total);
+}
+
+// GridHash::construct (GridHash.cu:298-323) followed by
+// NeighborQuery::queryNeighborDynamic (NeighborQuery.cu:450-478) with
+// K_CalNeighborSize (:245-276) and K_GetNeighborElements (:337-367).
+// Particles outside the grid are not inserted (GridHash.cu:85,194) but still
+// query. Ids inside a cell are kept in ascending order (the reference's order
+// comes from atomics and is not defined; the neighbor SET is the same).
+template <class R>
+void neighbor_query(const Grid<R>& g, const V3<R>* x, int n, R h, CSR& out) {
+  std::vector<long long> cid(n);
+  std::vector<int> c3(3 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    cell3(g, x[i], &c3[3 * (size_t)i]);
+    cid[i] = cell_id(g, c3[3 * i], c3[3 * i + 1], c3[3 * i + 2]);
+  }
+  // counting sort of in-grid ids: index[c] = exclusive scan of counts
+  std::vector<int> index((size_t)g.num + 1, 0);
+  for (int i = 0; i < n; ++i)
+    if (cid[i] >= 0) index[(size_t)cid[i] + 1]++;
+  for (long long c = 0; c < g.num; ++c) index[(size_t)c + 1] += index[(size_t)c];
+  std::vector<int> ids(index[(size_t)g.num]);
+  {
+    std::vector<int> fill(index.begin(), index.end() - 1);
+    for (int i = 0; i < n; ++i)
+      if (cid[i] >= 0) ids[fill[(size_t)cid[i]]++] = i;
+  }
+  std::vector<int> cnt(n);
+  auto visit = [&](int i, auto&& emit) {
+    const int* gi = &c3[3 * (size_t)i];
+    for (int c = 0; c < 27; ++c) {
+      long long cell = cell_id(g, gi[0] + kOffsets[c][0], gi[1] + kOffsets[c][1], gi[2] + kOffsets[c][2]);
+      if (cell < 0) continue;
+      for (int k = index[(size_t)cell]; k < index[(size_t)cell + 1]; ++k) {
+        int j = ids[k];
+        R dij = norm(sub(x[i], x[j]));
+        if (dij < h) emit(j);
+      }
+    }
+  };
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    int c = 0;
+    visit(i, [&](int) { ++c; });
+    cnt[i] = c;
+  }
+  out.offsets.assign((size_t)n + 1, 0);
+  for (int i = 0; i < n; ++i) out.offsets[(size_t)i + 1] = out.offsets[(size_t)i] + cnt[i];
+  out.elems.assign((size_t)out.offsets[(size_t)n], 0);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    int w = out.offsets[(size_t)i];
+    visit(i, [&](int j) { out.elems[(size_t)w++] = j; });
+  }
+}
+
+// K_ComputeDensity, SummationDensity.cu:13-36 (mass = m_factor * particle
+// mass, SummationDensity.cu:104-109).
+template <class R>
+void density(const V3<R>* x, int n, const CSR& nb, R h, R mass, R* rho) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    R rho_i = (R)0;
+    for (int k = nb.offsets[i]; k < nb.offsets[i + 1]; ++k) {
+      int j = nb.elems[k];
+      R r = norm(sub(x[i], x[j]));
+      rho_i += mass * spiky_w(r, h);
+    }
+    rho[i] = rho_i;
+  }
+}
+
+// K_ComputeLambdas, DensityPBD.cu:46-90.
+template <class R>
+void lambdas(const Params<R>& p, const R* rho, const V3<R>* x, int n, const CSR& nb, R* lam) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    R lamda_i = (R)0;
+    V3<R> grad_ci{0, 0, 0};
+    for (int k = nb.offsets[i]; k < nb.offsets[i + 1]; ++k) {
+      int j = nb.elems[k];
+      V3<R> dx = sub(x[i], x[j]);
+      R r = norm(dx);
+      if (r > p.pair_eps) {
+        R gr = spiky_grad(r, p.h);
+        R ir = (R)1 / r;
+        V3<R> gv{(dx.x * gr) * ir, (dx.y * gr) * ir, (dx.z * gr) * ir};
+        grad_ci.x += gv.x;
+        grad_ci.y += gv.y;
+        grad_ci.z += gv.z;
+        lamda_i += dot(gv, gv);
+      }
+    }
+    lamda_i += dot(grad_ci, grad_ci);
+    R rho_i = rho[i];
+    lamda_i = -(rho_i - p.lam_rest) / (lamda_i + p.lam_eps);
+    lam[i] = lamda_i > (R)0 ? (R)0 : lamda_i;
+  }
+}
+
+// dp_ij of K_ComputeDisplacement, DensityPBD.cu:158:
+//   10*(x_i - x_j)*(lambda_i + lambda_j)*Gradient(r)*(1.0/r), left to right.
+template <class R>
+static inline bool pair_dp(const Params<R>& p, const V3<R>& xi, const V3<R>& xj, R li, R lj, V3<R>& dp) {
+  V3<R> dx = sub(xi, xj);
+  R r = norm(dx);
+  if (!(r > p.pair_eps)) return false;
+  R s = li + lj;
+  R gr = spiky_grad(r, p.h);
+  R ir = inv_double(r);
+  dp.x = (((dx.x * p.corr) * s) * gr) * ir;
+  dp.y = (((dx.y * p.corr) * s) * gr) * ir;
+  dp.z = (((dx.z * p.corr) * s) * gr) * ir;
+  return true;
+}
+
+// Gather form of K_ComputeDisplacement for a symmetric neighbor list: the
+// reference adds +dp_ij to i and -dp_ij to j from both endpoints
+// (DensityPBD.cu:161-174) and dp_ji == -dp_ij bit for bit, so
+// dPos_i = 2 * sum_j dp_ij.
+template <class R>
+void displacement_gather(const Params<R>& p, const R* lam, const V3<R>* x, int n, const CSR& nb,
+                         V3<R>* dpos) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    V3<R> acc{0, 0, 0};
+    for (int k = nb.offsets[i]; k < nb.offsets[i + 1]; ++k) {
+      int j = nb.elems[k];
+      V3<R> dp;
+      if (pair_dp(p, x[i], x[j], lam[i], lam[j], dp)) {
+        acc.x += dp.x;
+        acc.y += dp.y;
+        acc.z += dp.z;
+      }
+    }
+    dpos[i] = {(R)2 * acc.x, (R)2 * acc.y, (R)2 * acc.z};
+  }
+}
+
+// Literal sequential emulation of the reference's pairwise atomic scatter
+// (DensityPBD.cu:152-176): for every i in order, for every listed j,
+// dPos[i] += dp_ij and dPos[j] -= dp_ij. Exact for any list, including the
+// asymmetric lists produced when particles leave the grid.
+template <class R>
+void displacement_atomic_order(const Params<R>& p, const R* lam, const V3<R>* x, int n, const CSR& nb,
+                               V3<R>* dpos) {
+  for (int i = 0; i < n; ++i) dpos[i] = {0, 0, 0};  // m_deltaPos.reset(), :315
+  for (int i = 0; i < n; ++i) {
+    for (int k = nb.offsets[i]; k < nb.offsets[i + 1]; ++k) {
+      int j = nb.elems[k];
+      V3<R> dp;
+      if (pair_dp(p, x[i], x[j], lam[i], lam[j], dp)) {
+        dpos[i].x += dp.x;
+        dpos[j].x += -dp.x;
+        dpos[i].y += dp.y;
+        dpos[j].y += -dp.y;
+        dpos[i].z += dp.z;
+        dpos[j].z += -dp.z;
+      }
+    }
+  }
+}
+
+template <class R> static bool list_symmetric(const CSR& nb, int n) {
+  // i in N_j for every j in N_i (rows need not be sorted)
+  std::vector<std::vector<int>> rows(n);
+  for (int i = 0; i < n; ++i) {
+    rows[i].assign(nb.elems.begin() + nb.offsets[i], nb.elems.begin() + nb.offsets[i + 1]);
+    std::sort(rows[i].begin(), rows[i].end());
+  }
+  for (int i = 0; i < n; ++i)
+    for (int j : rows[i])
+      if (!std::binary_search(rows[j].begin(), rows[j].end(), i)) return false;
+  return true;
+}
+
+// DensityPBD::constrain (DensityPBD.cu:271-302), takeOneIteration (:306-365)
+// with K_UpdatePosition (:227-237), and DP_UpdateVelocity (:368-392).
+// rho receives the density of the last iteration, computed before its Δx
+// (outDensity is SummationDensity's output, DensityPBD.cu:259).
+template <class R>
+void pbd_constrain(const Params<R>& p, V3<R>* x, V3<R>* v, int n, const CSR& nb, R* rho) {
+  std::vector<V3<R>> x_old(x, x + n);
+  std::vector<R> lam(n);
+  std::vector<V3<R>> dpos(n);
+  std::vector<R> rho_tmp(n);
+  R* rho_w = rho ? rho : rho_tmp.data();
+  const R mass = scaling_factor(p.d, p.h) * (p.rho0 * (p.d * p.d * p.d));
+  const bool gather = p.dx_mode == 1 || (p.dx_mode == 0 && list_symmetric<R>(nb, n));
+  for (int it = 0; it < p.iters; ++it) {
+    density(x, n, nb, p.h, mass, rho_w);
+    lambdas(p, rho_w, x, n, nb, lam.data());
+    if (gather)
+      displacement_gather(p, lam.data(), x, n, nb, dpos.data());
+    else
+      displacement_atomic_order(p, lam.data(), x, n, nb, dpos.data());
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      x[i].x += dpos[i].x;
+      x[i].y += dpos[i].y;
+      x[i].z += dpos[i].z;
+    }
+  }
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    v[i].x += (x[i].x - x_old[i].x) / p.dt;
+    v[i].y += (x[i].y - x_old[i].y) / p.dt;
+    v[i].z += (x[i].z - x_old[i].z) / p.dt;
+  }
+}
+
+// ParticleIntegrator::integrate (ParticleIntegrator.cu:129-138):
+// K_UpdateVelocity (:60-70) v += dt*(f + g), then K_UpdatePosition (:103-112)
+// x += dt*v. A null force means the zeroed ForceDensity of begin() (:32).
+template <class R>
+void integrate(const Params<R>& p, V3<R>* x, V3<R>* v, const V3<R>* f, int n) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    V3<R> fi = f ? f[i] : V3<R>{0, 0, 0};
+    v[i].x += (fi.x + p.g[0]) * p.dt;
+    v[i].y += (fi.y + p.g[1]) * p.dt;
+    v[i].z += (fi.z + p.g[2]) * p.dt;
+    x[i].x += v[i].x * p.dt;
+    x[i].y += v[i].y * p.dt;
+    x[i].z += v[i].z * p.dt;
+  }
+}
+
+// PositionBasedFluidModel::step (PositionBasedFluidModel.cpp:83-101) without
+// the viscosity module: begin (force := 0) -> NeighborQuery::compute on x^n ->
+// integrate -> DensityPBD::constrain.
+template <class R>
+void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out) {
+  Grid<R> g = grid_setup(p.h, p.lo, p.hi);
+  CSR local;
+  CSR& nb = nb_out ? *nb_out : local;
+  neighbor_query(g, x, n, p.h, nb);
+  // The list is symmetric exactly when every particle lies inside the grid
+  // (out-of-grid particles query but are not inserted, GridHash.cu:85,194).
+  Params<R> q = p;
+  if (q.dx_mode == 0) {
+    q.dx_mode = 1;
+    for (int i = 0; i < n; ++i) {
+      int c[3];
+      cell3(g, x[i], c);
+      if (cell_id(g, c[0], c[1], c[2]) < 0) {
+        q.dx_mode = 2;
+        break;
+      }
+    }
+  }
+  integrate<R>(q, x, v, nullptr, n);
+  pbd_constrain(q, x, v, n, nb, rho);
+}
+
+#define PBF_ORACLE_INSTANTIATE(R)                                                                  \
+  template Grid<R> grid_setup<R>(R, const R*, const R*);                                           \
+  template void cell3<R>(const Grid<R>&, const V3<R>&, int*);                                      \
+  template long long cell_id<R>(const Grid<R>&, int, int, int);                                    \
+  template R spiky_w<R>(R, R);                                                                     \
+  template R spiky_grad<R>(R, R);                                                                  \
+  template R scaling_factor<R>(R, R);                                                              \
+  template void neighbor_query<R>(const Grid<R>&, const V3<R>*, int, R, CSR&);                     \
+  template void density<R>(const V3<R>*, int, const CSR&, R, R, R*);                               \
+  template void lambdas<R>(const Params<R>&, const R*, const V3<R>*, int, const CSR&, R*);         \
+  template void displacement_gather<R>(const Params<R>&, const R*, const V3<R>*, int, const CSR&,  \
+                                       V3<R>*);                                                    \
+  template void displacement_atomic_order<R>(const Params<R>&, const R*, const V3<R>*, int,        \
+                                             const CSR&, V3<R>*);                                  \
+  template void pbd_constrain<R>(const Params<R>&, V3<R>*, V3<R>*, int, const CSR&, R*);           \
+  template void integrate<R>(const Params<R>&, V3<R>*, V3<R>*, const V3<R>*, int);                 \
+  template void step<R>(const Params<R>&, V3<R>*, V3<R>*, R*, int, CSR*);
+
+PBF_ORACLE_INSTANTIATE(float)
+PBF_ORACLE_INSTANTIATE(double)
+
+}  // namespace pbf_oracle

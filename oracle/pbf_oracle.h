@@ -1,0 +1,98 @@
+// CPU oracle for the PhysIKA position-based-fluid step.
+//
+// TEST INFRASTRUCTURE ONLY. This is the checker: a plain C++ restatement of the
+// reference algorithm that the GPU product is compared against. Only tests/,
+// __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it; the
+// product library never links, calls or falls back to it.
+//
+// Every function follows a reference file:line (paths relative to the
+// reference's Source/ tree unless stated) and keeps its fp evaluation order.
+// The file is compiled with -ffp-contract=off so that no multiply-add is fused.
+// Templated on the scalar type: float mirrors the reference (PRECISION_FLOAT,
+// Core/Platform.h:48); double serves finite-difference and adjoint checks.
+//
+// Pinning (what each function is checked against, tests/test_oracle_*.py):
+//   grid/cell index   closed form on hand-picked points (cell faces, padding,
+//                     out-of-grid)
+//   neighbor query    brute-force O(N^2) sets, symmetry, self-inclusion,
+//                     cell-size independence, h +- 1 ulp pairs
+//   spiky W / dW      FP64 closed form; dW = d/dr W by central differences
+//   scaling factor    independent 343-term sum; perfect lattice interior
+//                     density == rest density
+//   density           lattice closed form; isolated particle closed form
+//   lambda            rho <= 1000 -> 0; two-particle closed form
+//   displacement      momentum conservation; literal atomic-order emulation of
+//                     the reference's pairwise scatter (factor 2); 2-particle
+//                     closed form
+//   integrator        free fall closed form (bit-exact)
+//   step              committed golden fixtures (regression pins only)
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace pbf_oracle {
+
+template <class R> struct V3 { R x, y, z; };
+
+// Grid geometry of GridHash::setSpace (Framework/Topology/GridHash.cu:47-75).
+template <class R> struct Grid {
+  R ds;
+  R lo[3];
+  int nx, ny, nz;
+  long long num;  // nx*ny*nz
+};
+
+template <class R> struct Params {
+  R h;             // smoothing length = search radius = grid cell
+  R d;             // sampling distance (sets the mass and the lattice factor)
+  R rho0;          // rest density (mass only)
+  R dt;
+  R g[3];
+  int iters;       // DensityPBD iteration number
+  R lam_rest;      // 1000.0f literal, DensityPBD.cu:87
+  R lam_eps;       // 0.1f literal, DensityPBD.cu:87
+  R corr;          // 10.0f literal, DensityPBD.cu:158
+  R pair_eps;      // EPSILON, Core/Utility/cuda_utilities.h:30
+  R lo[3], hi[3];  // scene bounds, SceneGraph.h:77-78
+  // Δx accumulation: 0 = gather when the list is symmetric, else the
+  // atomic-order emulation; 1 = gather; 2 = atomic-order emulation.
+  int dx_mode;
+};
+
+struct CSR {
+  std::vector<int> offsets;  // n+1 exclusive offsets (NeighborList m_index + total)
+  std::vector<int> elems;    // neighbor ids
+};
+
+template <class R> Grid<R> grid_setup(R h, const R lo[3], const R hi[3]);
+// Cell coordinates of GridHash::getIndex3 (GridHash.h:55-62). Values far out of
+// range are clamped to [-2, n+1], which keeps the set of valid 27-stencil cells
+// unchanged; a NaN coordinate yields -2 (no valid stencil cell).
+template <class R> void cell3(const Grid<R>& g, const V3<R>& p, int out[3]);
+// GridHash::getIndex(i,j,k) (GridHash.h:37-44): linear id or -1.
+template <class R> long long cell_id(const Grid<R>& g, int i, int j, int k);
+
+template <class R> R spiky_w(R r, R h);     // Dynamics/ParticleSystem/Kernel.h:36-45
+template <class R> R spiky_grad(R r, R h);  // Dynamics/ParticleSystem/Kernel.h:47-57
+template <class R> R scaling_factor(R d, R H);  // SummationDensity.cu:113-138
+
+template <class R>
+void neighbor_query(const Grid<R>& g, const V3<R>* x, int n, R h, CSR& out);
+template <class R>
+void density(const V3<R>* x, int n, const CSR& nb, R h, R mass, R* rho);
+template <class R>
+void lambdas(const Params<R>& p, const R* rho, const V3<R>* x, int n, const CSR& nb, R* lam);
+template <class R>
+void displacement_gather(const Params<R>& p, const R* lam, const V3<R>* x, int n,
+                         const CSR& nb, V3<R>* dpos);
+template <class R>
+void displacement_atomic_order(const Params<R>& p, const R* lam, const V3<R>* x, int n,
+                               const CSR& nb, V3<R>* dpos);
+template <class R>
+void pbd_constrain(const Params<R>& p, V3<R>* x, V3<R>* v, int n, const CSR& nb, R* rho);
+template <class R>
+void integrate(const Params<R>& p, V3<R>* x, V3<R>* v, const V3<R>* f, int n);
+template <class R>
+void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out);
+
+}  // namespace pbf_oracle

@@ -1,0 +1,151 @@
+"""ctypes binding of the C ABI in include/pbf.h (libpbf.so).
+
+The library is required: loading fails loudly when libpbf.so has not been
+built (``python __graft_entry__.py build``); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_PKG = Path(__file__).resolve().parent
+_LIB = _PKG / "_build" / "libpbf.so"
+
+PBF_OK = 0
+PBF_ERR_INVALID = -1
+PBF_ERR_HIP = -2
+PBF_ERR_CAPACITY = -3
+PBF_ERR_ALLOC = -4
+PBF_ERR_UNSUPPORTED = -5
+
+
+class pbf_params(C.Structure):
+    _fields_ = [
+        ("smoothing_length", C.c_float),
+        ("sampling_distance", C.c_float),
+        ("rest_density", C.c_float),
+        ("dt", C.c_float),
+        ("gravity", C.c_float * 3),
+        ("iterations", C.c_int),
+        ("lambda_rest", C.c_float),
+        ("lambda_eps", C.c_float),
+        ("corr_scale", C.c_float),
+        ("pair_eps", C.c_float),
+        ("scene_lo", C.c_float * 3),
+        ("scene_hi", C.c_float * 3),
+        ("mask_words", C.c_int),
+        ("flags", C.c_uint),
+    ]
+
+
+class pbf_stats(C.Structure):
+    _fields_ = [
+        ("neighbor_pairs", C.c_longlong),
+        ("out_of_grid", C.c_longlong),
+        ("nonfinite", C.c_longlong),
+        ("mask_words_needed", C.c_longlong),
+        ("mask_words_reserved", C.c_longlong),
+        ("mask_fallback_waves", C.c_longlong),
+        ("grid_dims", C.c_int * 3),
+        ("grid_lo", C.c_float * 3),
+        ("cell_size", C.c_float),
+        ("density_factor", C.c_float),
+        ("particle_mass", C.c_float),
+        ("neighbor_threshold", C.c_float),
+    ]
+
+
+PASS_NAMES = ("sort", "neighbors", "density_lambda", "displacement")
+
+
+class pbf_profile(C.Structure):
+    _fields_ = [("ms", C.c_double * 4), ("launches", C.c_longlong * 4)]
+
+
+# every symbol include/pbf.h declares, with its ctypes signature
+_VP = C.c_void_p
+_CTX = C.c_void_p
+SIGNATURES = {
+    "pbf_abi_version": (C.c_int, []),
+    "pbf_status_string": (C.c_char_p, [C.c_int]),
+    "pbf_params_default": (None, [C.POINTER(pbf_params)]),
+    "pbf_create": (C.c_int, [C.POINTER(pbf_params), C.POINTER(_CTX)]),
+    "pbf_destroy": (C.c_int, [_CTX]),
+    "pbf_set_params": (C.c_int, [_CTX, C.POINTER(pbf_params)]),
+    "pbf_get_params": (C.c_int, [_CTX, C.POINTER(pbf_params)]),
+    "pbf_reserve": (C.c_int, [_CTX, C.c_int]),
+    "pbf_neighbor_query": (C.c_int, [_CTX, _VP, C.c_int, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
+    "pbf_integrate": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, _VP]),
+    "pbf_density": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+    "pbf_pbd_constrain": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+    "pbf_step": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, C.c_int, _VP]),
+    "pbf_step_neighbors": (C.c_int, [_CTX, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
+    "pbf_get_stats": (C.c_int, [_CTX, C.POINTER(pbf_stats)]),
+    "pbf_set_profiling": (C.c_int, [_CTX, C.c_int]),
+    "pbf_get_profile": (C.c_int, [_CTX, C.POINTER(pbf_profile)]),
+    "pbf_last_error": (C.c_char_p, [_CTX]),
+}
+
+
+class PBFError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"pbf status {status}: {message}")
+        self.status = status
+
+
+def library_path(must_exist: bool = True) -> Path:
+    if must_exist and not _LIB.exists():
+        raise RuntimeError(f"{_LIB} is not built; run `python __graft_entry__.py build` (no CPU fallback exists)")
+    return _LIB
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libpbf.so with typed entry points."""
+    global _lib
+    if _lib is None:
+        L = C.CDLL(str(library_path(True)))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(status: int, ctx=None) -> None:
+    if status != PBF_OK:
+        msg = lib().pbf_last_error(ctx).decode() if ctx else lib().pbf_status_string(status).decode()
+        raise PBFError(status, msg or lib().pbf_status_string(status).decode())
+
+
+def default_params() -> pbf_params:
+    p = pbf_params()
+    lib().pbf_params_default(C.byref(p))
+    return p
+
+
+def params_from(**kw) -> pbf_params:
+    """Default parameters with keyword overrides (vector fields take 3-sequences)."""
+    p = default_params()
+    for k, v in kw.items():
+        if k in ("gravity", "scene_lo", "scene_hi"):
+            arr = getattr(p, k)
+            for i in range(3):
+                arr[i] = float(v[i])
+        elif not hasattr(p, k):
+            raise KeyError(k)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def params_dict(p: pbf_params) -> dict:
+    out = {}
+    for name, _ in pbf_params._fields_:
+        v = getattr(p, name)
+        out[name] = list(v) if hasattr(v, "__len__") else v
+    return out

@@ -1,0 +1,686 @@
+// C ABI of libpbf.so (include/pbf.h): context, parameters and entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "pbf.h"
+#include "pbf_internal.h"
+
+using namespace pbf;
+
+struct pbf_ctx {
+  pbf_params p{};
+  GridDev g{};
+  PhysConst pc{};
+  float m_factor = 0.f, mass = 0.f;
+  int device = 0;
+  hipStream_t last_stream = nullptr;
+
+  // grid arrays
+  int* cell_cnt = nullptr;    // ncell + 1
+  int* cell_start = nullptr;  // ncell + 2
+  uint8_t* oog_flag = nullptr;
+  int grid_cells = -1;  // ncell the grid arrays were sized for
+
+  // particle arrays (capacity cap_n)
+  int cap_n = 0;
+  int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *inv = nullptr;
+  int *nbr_cnt = nullptr, *cnt_orig = nullptr;
+  float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr;
+  float* dpos = nullptr;
+  long long* wave_base = nullptr;
+  uint32_t* mask = nullptr;
+  long long mask_cap = 0;
+
+  int* scan_scratch = nullptr;
+  size_t scan_scratch_n = 0;
+
+  unsigned long long* counters = nullptr;     // device, C_COUNT
+  unsigned long long* h_counters = nullptr;   // pinned host copy
+  hipEvent_t counters_ev = nullptr;
+  bool counters_pending = false;
+
+  int last_n = -1;  // particle count of the last neighbor search (-1: none)
+
+  // per-pass timing
+  bool profiling = false;
+  struct Mark {
+    int pass;
+    hipEvent_t a, b;
+  };
+  std::vector<Mark> marks;        // recorded, not yet summed
+  std::vector<hipEvent_t> spare;  // reusable events
+  double prof_ms[PBF_PASS_COUNT] = {0, 0, 0, 0};
+  long long prof_n[PBF_PASS_COUNT] = {0, 0, 0, 0};
+
+  std::string err;
+};
+
+namespace {
+
+int fail(pbf_ctx* c, int status, const std::string& msg) {
+  if (c) c->err = msg;
+  return status;
+}
+
+#define PBF_HIP(ctx, call)                                                                           \
+  do {                                                                                               \
+    hipError_t e_ = (call);                                                                          \
+    if (e_ != hipSuccess)                                                                            \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? PBF_ERR_ALLOC : PBF_ERR_HIP,                      \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                               \
+  } while (0)
+
+#define PBF_LAUNCHED(ctx)                                                                            \
+  do {                                                                                               \
+    hipError_t e_ = hipGetLastError();                                                               \
+    if (e_ != hipSuccess) return fail(ctx, PBF_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T> void dfree(T*& p) {
+  if (p) (void)hipFree((void*)p);
+  p = nullptr;
+}
+
+constexpr double kPi = 3.14159265358979323846;  // M_PI, Core/Utility/cuda_utilities.h:31
+
+// SpikyKernel::Weight (Dynamics/ParticleSystem/Kernel.h:36-45), host side.
+float spiky_w_host(float r, float h) {
+#pragma clang fp contract(off)
+  const float q = r / h;
+  if (q > 1.0f) return 0.0f;
+  const float d = (float)(1.0 - (double)q);
+  const float hh = h * h;
+  return 15.0f / ((float)kPi * hh * h) * d * d * d * 1.0f;
+}
+
+// SummationDensity::calculateScalingFactor (SummationDensity.cu:113-138): the
+// lattice sum that makes a perfect lattice of spacing d have the rest density.
+float scaling_factor_host(float d, float H) {
+#pragma clang fp contract(off)
+  const float V = d * d * d;
+  float total = 0.0f;
+  const int half_res = (int)(H / d + 1.0f);
+  for (int i = -half_res; i <= half_res; i++)
+    for (int j = -half_res; j <= half_res; j++)
+      for (int k = -half_res; k <= half_res; k++) {
+        const float x = (float)i * d, y = (float)j * d, z = (float)k * d;
+        const float r = std::sqrt(x * x + y * y + z * z);
+        total += V * spiky_w_host(r, H);
+      }
+  return (float)(1.0 / (double)total);
+}
+
+// Least float s >= 0 with sqrtf(s) >= h (IEEE square root is monotone), so
+// that the strict test sqrtf(s) < h of NeighborQuery.cu:267 is s < T.
+float sqrt_threshold(float h) {
+  uint32_t lo = 0, hi = 0x7f800000u;  // [+0, +inf]
+  while (lo < hi) {
+    uint32_t mid = lo + (hi - lo) / 2;
+    float s;
+    std::memcpy(&s, &mid, 4);
+    if (std::sqrt(s) >= h)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  float t;
+  std::memcpy(&t, &lo, 4);
+  return t;
+}
+
+int validate(const pbf_params* p, std::string& why) {
+  auto pos = [](float v) { return std::isfinite(v) && v > 0.0f; };
+  if (!pos(p->smoothing_length)) return why = "smoothing_length must be finite and > 0", PBF_ERR_INVALID;
+  if (!pos(p->sampling_distance)) return why = "sampling_distance must be finite and > 0", PBF_ERR_INVALID;
+  if (!pos(p->dt)) return why = "dt must be finite and > 0", PBF_ERR_INVALID;
+  if (!std::isfinite(p->rest_density)) return why = "rest_density must be finite", PBF_ERR_INVALID;
+  if (p->iterations < 0 || p->iterations > 1000) return why = "iterations must be in [0, 1000]", PBF_ERR_INVALID;
+  if (p->mask_words < 0) return why = "mask_words must be >= 0", PBF_ERR_INVALID;
+  if (p->flags != 0) return why = "flags must be 0", PBF_ERR_INVALID;
+  for (int c = 0; c < 3; ++c) {
+    if (!std::isfinite(p->scene_lo[c]) || !std::isfinite(p->scene_hi[c]) || !(p->scene_hi[c] > p->scene_lo[c]))
+      return why = "scene bounds must be finite with hi > lo", PBF_ERR_INVALID;
+    if (!std::isfinite(p->gravity[c])) return why = "gravity must be finite", PBF_ERR_INVALID;
+  }
+  if (!std::isfinite(p->lambda_rest) || !std::isfinite(p->lambda_eps) || !std::isfinite(p->corr_scale) ||
+      !std::isfinite(p->pair_eps))
+    return why = "lambda/correction constants must be finite", PBF_ERR_INVALID;
+  return PBF_OK;
+}
+
+// GridHash::setSpace (Framework/Topology/GridHash.cu:47-75).
+int derive(pbf_ctx* c, const pbf_params* p) {
+#pragma clang fp contract(off)
+  GridDev g{};
+  const float ds = p->smoothing_length;
+  const float pad = 2.0f * ds;
+  long long n[3];
+  for (int a = 0; a < 3; ++a) {
+    g.lo[a] = p->scene_lo[a] - pad;
+    const float nseg = (p->scene_hi[a] - p->scene_lo[a]) / ds;
+    const float nf = std::ceil(nseg) + 1.0f + 4.0f;
+    if (!(nf < 2.0e6f)) return fail(c, PBF_ERR_UNSUPPORTED, "grid too large along one axis");
+    n[a] = (long long)nf;
+  }
+  const long long ncell = n[0] * n[1] * n[2];
+  if (ncell > (1ll << 30)) return fail(c, PBF_ERR_UNSUPPORTED, "grid has more than 2^30 cells");
+  g.ds = ds;
+  g.nx = (int)n[0];
+  g.ny = (int)n[1];
+  g.nz = (int)n[2];
+  g.ncell = (int)ncell;
+
+  PhysConst pc{};
+  const float h = p->smoothing_length;
+  pc.h = h;
+  pc.inv_h = 1.0f / h;
+  pc.thr2 = sqrt_threshold(h);
+  const float m_factor = scaling_factor_host(p->sampling_distance, h);
+  const float d = p->sampling_distance;
+  const float mass = p->rest_density * (d * d * d);
+  const float hh = h * h;
+  pc.cW = (m_factor * mass) * (15.0f / ((float)kPi * hh * h));
+  pc.cG = -45.0f / ((float)kPi * hh * h);
+  pc.lam_rest = p->lambda_rest;
+  pc.lam_eps = p->lambda_eps;
+  pc.corr = p->corr_scale;
+  pc.pair_eps = p->pair_eps;
+  pc.dt = p->dt;
+  for (int a = 0; a < 3; ++a) {
+    pc.g[a] = p->gravity[a];
+    pc.gdt[a] = (0.0f + p->gravity[a]) * p->dt;
+  }
+  c->g = g;
+  c->pc = pc;
+  c->m_factor = m_factor;
+  c->mass = mass;
+  c->p = *p;
+  return PBF_OK;
+}
+
+int ensure_grid(pbf_ctx* c) {
+  if (c->grid_cells == c->g.ncell) return PBF_OK;
+  dfree(c->cell_cnt);
+  dfree(c->cell_start);
+  dfree(c->oog_flag);
+  c->grid_cells = -1;
+  PBF_HIP(c, hipMalloc(&c->cell_cnt, sizeof(int) * ((size_t)c->g.ncell + 1)));
+  PBF_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * ((size_t)c->g.ncell + 2)));
+  PBF_HIP(c, hipMalloc(&c->oog_flag, (size_t)c->g.ncell + 1));
+  c->grid_cells = c->g.ncell;
+  c->last_n = -1;
+  return PBF_OK;
+}
+
+int ensure_scan_scratch(pbf_ctx* c, long long n) {
+  const size_t need = scan_scratch_ints(n);
+  if (need <= c->scan_scratch_n) return PBF_OK;
+  dfree(c->scan_scratch);
+  c->scan_scratch_n = 0;
+  PBF_HIP(c, hipMalloc(&c->scan_scratch, sizeof(int) * need));
+  c->scan_scratch_n = need;
+  return PBF_OK;
+}
+
+long long mask_words_for(const pbf_ctx* c, long long n) {
+  const long long per = c->p.mask_words > 0 ? c->p.mask_words : 32;
+  const long long waves = (n + 63) / 64;
+  return waves * 64 * per;
+}
+
+int ensure_mask(pbf_ctx* c, long long words) {
+  if (words <= c->mask_cap) return PBF_OK;
+  dfree(c->mask);
+  c->mask_cap = 0;
+  PBF_HIP(c, hipMalloc(&c->mask, sizeof(uint32_t) * (size_t)words));
+  c->mask_cap = words;
+  return PBF_OK;
+}
+
+int ensure_particles(pbf_ctx* c, int n) {
+  int st = ensure_grid(c);
+  if (st) return st;
+  st = ensure_scan_scratch(c, std::max<long long>((long long)c->g.ncell + 1, (long long)n + 1));
+  if (st) return st;
+  if (n <= c->cap_n) return PBF_OK;
+  const int cap = std::max(((n + 255) / 256) * 256, 256);
+  dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
+  dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
+  dfree(c->dpos); dfree(c->wave_base);
+  c->cap_n = 0;
+  const size_t ni = sizeof(int) * (size_t)cap, nf4 = sizeof(float4) * (size_t)cap;
+  PBF_HIP(c, hipMalloc(&c->key, ni));
+  PBF_HIP(c, hipMalloc(&c->slot, ni));
+  PBF_HIP(c, hipMalloc(&c->tmp, ni));
+  PBF_HIP(c, hipMalloc(&c->perm, ni));
+  PBF_HIP(c, hipMalloc(&c->inv, ni));
+  PBF_HIP(c, hipMalloc(&c->nbr_cnt, ni));
+  PBF_HIP(c, hipMalloc(&c->cnt_orig, ni));
+  PBF_HIP(c, hipMalloc(&c->xn, nf4));
+  PBF_HIP(c, hipMalloc(&c->x0, nf4));
+  PBF_HIP(c, hipMalloc(&c->xa, nf4));
+  PBF_HIP(c, hipMalloc(&c->xb, nf4));
+  PBF_HIP(c, hipMalloc(&c->vs, nf4));
+  PBF_HIP(c, hipMalloc(&c->dpos, 3 * sizeof(float) * (size_t)cap));
+  PBF_HIP(c, hipMemset(c->dpos, 0, 3 * sizeof(float) * (size_t)cap));
+  PBF_HIP(c, hipMalloc(&c->wave_base, sizeof(long long) * (size_t)(cap / 64 + 4)));
+  c->cap_n = cap;
+  st = ensure_mask(c, mask_words_for(c, cap));
+  if (st) return st;
+  c->last_n = -1;
+  return PBF_OK;
+}
+
+// Grows the mask reservation when the last finished call wanted more; never
+// blocks (the counters are copied asynchronously at the end of each call).
+int adapt_mask(pbf_ctx* c) {
+  if (!c->counters_pending) return PBF_OK;
+  if (hipEventQuery(c->counters_ev) != hipSuccess) return PBF_OK;
+  c->counters_pending = false;
+  const long long need = (long long)c->h_counters[C_MASK_WORDS];
+  if (need > c->mask_cap) return ensure_mask(c, need + need / 8);
+  return PBF_OK;
+}
+
+NbrView view_of(const pbf_ctx* c, int n) {
+  NbrView nv;
+  nv.g = c->g;
+  nv.cell_start = c->cell_start;
+  nv.xn = c->xn;
+  nv.mask = c->mask;
+  nv.wave_base = c->wave_base;
+  nv.oog_flag = c->oog_flag;
+  nv.thr2 = c->pc.thr2;
+  nv.n = n;
+  return nv;
+}
+
+// HIP-event pair around one pass on the call's stream (profiling only).
+hipEvent_t take_event(pbf_ctx* c) {
+  if (!c->spare.empty()) {
+    hipEvent_t e = c->spare.back();
+    c->spare.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+struct PassTimer {
+  pbf_ctx* c;
+  int pass;
+  hipStream_t st;
+  hipEvent_t a = nullptr;
+  PassTimer(pbf_ctx* c_, int pass_, hipStream_t st_) : c(c_), pass(pass_), st(st_) {
+    if (c->profiling && (a = take_event(c)) != nullptr) (void)hipEventRecord(a, st);
+  }
+  ~PassTimer() {
+    if (!a) return;
+    hipEvent_t b = take_event(c);
+    if (!b) {
+      c->spare.push_back(a);
+      return;
+    }
+    (void)hipEventRecord(b, st);
+    c->marks.push_back({pass, a, b});
+  }
+};
+
+int drain_marks(pbf_ctx* c) {
+  for (auto& m : c->marks) {
+    PBF_HIP(c, hipEventSynchronize(m.b));
+    float ms = 0.f;
+    PBF_HIP(c, hipEventElapsedTime(&ms, m.a, m.b));
+    c->prof_ms[m.pass] += ms;
+    c->prof_n[m.pass] += 1;
+    c->spare.push_back(m.a);
+    c->spare.push_back(m.b);
+  }
+  c->marks.clear();
+  return PBF_OK;
+}
+
+// NeighborQuery::compute on the positions x (GridHash::construct +
+// queryNeighborDynamic), leaving the cell-sorted state and neighbor masks in
+// the context; with `integrate` the integrator is fused into the reorder.
+int search(pbf_ctx* c, const float* x, const float* v, int n, bool integrate, hipStream_t st) {
+  {
+    PassTimer t(c, PBF_PASS_SORT, st);
+    launch_clear(c->cell_cnt, c->oog_flag, c->g.ncell, c->counters, st);
+    launch_bin(x, n, c->g, c->key, c->slot, c->cell_cnt, c->counters, st);
+    scan_exclusive(c->cell_cnt, c->cell_start, c->g.ncell + 1, c->scan_scratch, st);
+    launch_scatter(c->key, c->slot, c->cell_start, c->tmp, n, st);
+    launch_reorder(c->key, c->cell_start, c->tmp, x, v, n, c->perm, c->inv, c->xn, c->x0, c->vs,
+                   integrate ? &c->pc : nullptr, st);
+  }
+  NbrView nv = view_of(c, n);
+  {
+    PassTimer t(c, PBF_PASS_NEIGHBORS, st);
+    launch_neighbors(nv, c->mask, c->wave_base, c->nbr_cnt, c->mask_cap, c->counters, st);
+  }
+  launch_oog_mark(nv, c->oog_flag, st);
+  PBF_LAUNCHED(c);
+  c->last_n = n;
+  return PBF_OK;
+}
+
+int record_counters(pbf_ctx* c, hipStream_t st) {
+  PBF_HIP(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(unsigned long long) * C_COUNT,
+                            hipMemcpyDeviceToHost, st));
+  PBF_HIP(c, hipEventRecord(c->counters_ev, st));
+  c->counters_pending = true;
+  c->last_stream = st;
+  return PBF_OK;
+}
+
+int export_csr(pbf_ctx* c, int n, int* d_offsets, int* d_elems, long long cap, long long* total, hipStream_t st) {
+  if (n == 0) {
+    PBF_HIP(c, hipMemsetAsync(d_offsets, 0, sizeof(int), st));
+    PBF_HIP(c, hipStreamSynchronize(st));
+    *total = 0;
+    return PBF_OK;
+  }
+  launch_csr_count(c->nbr_cnt, c->inv, n, c->cnt_orig, st);
+  scan_exclusive(c->cnt_orig, d_offsets, n, c->scan_scratch, st);
+  PBF_LAUNCHED(c);
+  int tot = 0;
+  PBF_HIP(c, hipMemcpyAsync(&tot, d_offsets + n, sizeof(int), hipMemcpyDeviceToHost, st));
+  PBF_HIP(c, hipStreamSynchronize(st));
+  *total = tot;
+  if (d_elems == nullptr) return PBF_OK;
+  if (cap < (long long)tot)
+    return fail(c, PBF_ERR_CAPACITY, "neighbor element buffer holds " + std::to_string(cap) + ", need " +
+                                         std::to_string(tot));
+  launch_csr_fill(view_of(c, n), c->perm, d_offsets, d_elems, cap, st);
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+bool bad_n(int n) { return n < 0; }
+
+}  // namespace
+
+extern "C" {
+
+int pbf_abi_version(void) { return PBF_ABI_VERSION; }
+
+const char* pbf_status_string(int s) {
+  switch (s) {
+    case PBF_OK: return "ok";
+    case PBF_ERR_INVALID: return "invalid argument";
+    case PBF_ERR_HIP: return "HIP runtime error";
+    case PBF_ERR_CAPACITY: return "buffer capacity exceeded";
+    case PBF_ERR_ALLOC: return "device allocation failed";
+    case PBF_ERR_UNSUPPORTED: return "unsupported configuration";
+    default: return "unknown status";
+  }
+}
+
+void pbf_params_default(pbf_params* p) {
+  if (!p) return;
+  std::memset(p, 0, sizeof(*p));
+  p->smoothing_length = 0.0125f;
+  p->sampling_distance = 0.005f;
+  p->rest_density = 1000.0f;
+  p->dt = 0.001f;
+  p->gravity[0] = 0.0f;
+  p->gravity[1] = -9.8f;
+  p->gravity[2] = 0.0f;
+  p->iterations = 5;
+  p->lambda_rest = 1000.0f;
+  p->lambda_eps = 0.1f;
+  p->corr_scale = 10.0f;
+  p->pair_eps = 1e-6f;
+  for (int a = 0; a < 3; ++a) {
+    p->scene_lo[a] = 0.0f;
+    p->scene_hi[a] = 1.0f;
+  }
+  p->mask_words = 32;
+  p->flags = 0;
+}
+
+int pbf_create(const pbf_params* p, pbf_ctx** out) {
+  if (!out) return PBF_ERR_INVALID;
+  *out = nullptr;
+  pbf_params def;
+  if (!p) {
+    pbf_params_default(&def);
+    p = &def;
+  }
+  pbf_ctx* c = new (std::nothrow) pbf_ctx();
+  if (!c) return PBF_ERR_ALLOC;
+  std::string why;
+  int st = validate(p, why);
+  if (st) {
+    delete c;
+    return st;
+  }
+  st = derive(c, p);
+  if (st) {
+    delete c;
+    return st;
+  }
+  if (hipGetDevice(&c->device) != hipSuccess || hipMalloc(&c->counters, sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
+      hipHostMalloc(&c->h_counters, sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
+      hipEventCreateWithFlags(&c->counters_ev, hipEventDisableTiming) != hipSuccess) {
+    pbf_destroy(c);
+    return PBF_ERR_HIP;
+  }
+  std::memset(c->h_counters, 0, sizeof(unsigned long long) * C_COUNT);
+  (void)hipMemset(c->counters, 0, sizeof(unsigned long long) * C_COUNT);
+  st = ensure_grid(c);
+  if (st) {
+    pbf_destroy(c);
+    return st;
+  }
+  *out = c;
+  return PBF_OK;
+}
+
+int pbf_destroy(pbf_ctx* c) {
+  if (!c) return PBF_ERR_INVALID;
+  (void)hipDeviceSynchronize();
+  dfree(c->cell_cnt); dfree(c->cell_start); dfree(c->oog_flag);
+  dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
+  dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
+  dfree(c->dpos); dfree(c->wave_base); dfree(c->mask); dfree(c->scan_scratch); dfree(c->counters);
+  if (c->h_counters) (void)hipHostFree(c->h_counters);
+  if (c->counters_ev) (void)hipEventDestroy(c->counters_ev);
+  for (auto& m : c->marks) {
+    (void)hipEventDestroy(m.a);
+    (void)hipEventDestroy(m.b);
+  }
+  for (hipEvent_t e : c->spare) (void)hipEventDestroy(e);
+  delete c;
+  return PBF_OK;
+}
+
+int pbf_set_params(pbf_ctx* c, const pbf_params* p) {
+  if (!c || !p) return fail(c, PBF_ERR_INVALID, "null argument");
+  std::string why;
+  int st = validate(p, why);
+  if (st) return fail(c, st, why);
+  PBF_HIP(c, hipDeviceSynchronize());
+  st = derive(c, p);
+  if (st) return st;
+  st = ensure_grid(c);
+  if (st) return st;
+  if (c->cap_n > 0) return ensure_mask(c, mask_words_for(c, c->cap_n));
+  return PBF_OK;
+}
+
+int pbf_get_params(const pbf_ctx* c, pbf_params* out) {
+  if (!c || !out) return PBF_ERR_INVALID;
+  *out = c->p;
+  return PBF_OK;
+}
+
+int pbf_reserve(pbf_ctx* c, int n) {
+  if (!c || bad_n(n)) return fail(c, PBF_ERR_INVALID, "bad particle count");
+  return ensure_particles(c, n);
+}
+
+int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_offsets, int* d_elems, long long cap,
+                       long long* total, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || !total || !d_offsets || (n > 0 && !d_x) || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  int s = ensure_particles(c, n);
+  if (s) return s;
+  if (n == 0) {
+    c->last_n = 0;
+    return export_csr(c, 0, d_offsets, d_elems, cap, total, st);
+  }
+  s = adapt_mask(c);
+  if (s) return s;
+  s = search(c, d_x, nullptr, n, false, st);
+  if (s) return s;
+  s = record_counters(c, st);
+  if (s) return s;
+  return export_csr(c, n, d_offsets, d_elems, cap, total, st);
+}
+
+int pbf_step_neighbors(pbf_ctx* c, int* d_offsets, int* d_elems, long long cap, long long* total, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (!d_offsets || !total || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (c->last_n < 0) return fail(c, PBF_ERR_INVALID, "no neighbor search has run on this context");
+  return export_csr(c, c->last_n, d_offsets, d_elems, cap, total, (hipStream_t)stream);
+}
+
+int pbf_integrate(pbf_ctx* c, float* d_x, float* d_v, const float* d_f, int n, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n == 0) return PBF_OK;
+  launch_integrate(d_x, d_v, d_f, n, c->pc, (hipStream_t)stream);
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+int pbf_density(pbf_ctx* c, const float* d_x, const int* d_offsets, const int* d_elems, float* d_rho, int n,
+                void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_offsets || !d_elems || !d_rho))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n == 0) return PBF_OK;
+  launch_csr_density(d_x, d_offsets, d_elems, d_rho, n, c->pc, (hipStream_t)stream);
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+int pbf_pbd_constrain(pbf_ctx* c, float* d_x, float* d_v, const int* d_offsets, const int* d_elems, float* d_rho,
+                      int n, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n == 0) return PBF_OK;
+  int s = ensure_particles(c, n);
+  if (s) return s;
+  hipStream_t st = (hipStream_t)stream;
+  launch_aos_to_f4(d_x, c->xa, n, st);
+  for (int it = 0; it < c->p.iterations; ++it) {
+    const bool last = it + 1 == c->p.iterations;
+    launch_csr_lambda(c->xa, d_offsets, d_elems, last ? d_rho : nullptr, n, c->pc, st);
+    launch_csr_dx_atomic(c->xa, d_offsets, d_elems, c->dpos, n, c->pc, st);
+    launch_csr_apply(c->xa, c->dpos, n, st);
+  }
+  launch_csr_finish(c->xa, d_x, d_v, n, c->pc.dt, st);
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+int pbf_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, int nsteps, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || nsteps < 0 || (n > 0 && (!d_x || !d_v))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n == 0 || nsteps == 0) return PBF_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int s = ensure_particles(c, n);
+  if (s) return s;
+  s = adapt_mask(c);
+  if (s) return s;
+  const int K = c->p.iterations;
+  for (int step = 0; step < nsteps; ++step) {
+    s = search(c, d_x, d_v, n, true, st);
+    if (s) return s;
+    NbrView nv = view_of(c, n);
+    float4* bufs[2] = {c->xa, c->xb};
+    if (K == 0) launch_write_back(c->x0, c->vs, c->perm, n, d_x, d_v, st);
+    for (int it = 0; it < K; ++it) {
+      const bool last = it + 1 == K;
+      float4* src = it == 0 ? c->x0 : bufs[(it - 1) & 1];
+      float4* dst = bufs[it & 1];
+      {
+        PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st);
+        launch_pass_density_lambda(nv, src, c->pc, (last && step + 1 == nsteps) ? d_rho : nullptr, c->perm, st);
+      }
+      {
+        PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
+        launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, d_x, d_v, st);
+      }
+    }
+    PBF_LAUNCHED(c);
+  }
+  return record_counters(c, st);
+}
+
+int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
+  if (!c || !out) return fail(c, PBF_ERR_INVALID, "null argument");
+  std::memset(out, 0, sizeof(*out));
+  // counters_pending stays set: the next call still sees the completed copy
+  // and grows the mask reservation if needed.
+  if (c->counters_pending) PBF_HIP(c, hipEventSynchronize(c->counters_ev));
+  out->neighbor_pairs = (long long)c->h_counters[C_PAIRS];
+  out->out_of_grid = (long long)c->h_counters[C_OOG];
+  out->nonfinite = (long long)c->h_counters[C_NONFINITE];
+  out->mask_words_needed = (long long)c->h_counters[C_MASK_WORDS];
+  out->mask_words_reserved = c->mask_cap;
+  out->mask_fallback_waves = (long long)c->h_counters[C_FALLBACK];
+  out->grid_dims[0] = c->g.nx;
+  out->grid_dims[1] = c->g.ny;
+  out->grid_dims[2] = c->g.nz;
+  for (int a = 0; a < 3; ++a) out->grid_lo[a] = c->g.lo[a];
+  out->cell_size = c->g.ds;
+  out->density_factor = c->m_factor;
+  out->particle_mass = c->mass;
+  out->neighbor_threshold = c->pc.thr2;
+  return PBF_OK;
+}
+
+int pbf_set_profiling(pbf_ctx* c, int enable) {
+  if (!c) return PBF_ERR_INVALID;
+  int s = drain_marks(c);
+  if (s) return s;
+  if (enable) {
+    for (int i = 0; i < PBF_PASS_COUNT; ++i) {
+      c->prof_ms[i] = 0.0;
+      c->prof_n[i] = 0;
+    }
+  }
+  c->profiling = enable != 0;
+  return PBF_OK;
+}
+
+int pbf_get_profile(pbf_ctx* c, pbf_profile* out) {
+  if (!c || !out) return fail(c, PBF_ERR_INVALID, "null argument");
+  int s = drain_marks(c);
+  if (s) return s;
+  for (int i = 0; i < PBF_PASS_COUNT; ++i) {
+    out->ms[i] = c->prof_ms[i];
+    out->launches[i] = c->prof_n[i];
+  }
+  return PBF_OK;
+}
+
+const char* pbf_last_error(const pbf_ctx* c) {
+  if (!c) return "null context";
+  return c->err.c_str();
+}
+
+}  // extern "C"

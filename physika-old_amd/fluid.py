@@ -1,0 +1,365 @@
+"""Python mirror of the reference's fluid modules over the C ABI.
+
+Class and field names follow the reference so that code written against
+PhysIKA's modules reads the same:
+
+  NeighborQuery        Framework/Topology/NeighborQuery.h:13-95
+                       inRadius, inPosition, outNeighborhood; compute()
+  NeighborList         Framework/Topology/NeighborList.h:9-125
+  ParticleIntegrator   Dynamics/ParticleSystem/ParticleIntegrator.h:8-54
+                       inPosition, inVelocity, inForceDensity; begin/integrate/end
+  DensityPBD           Dynamics/ParticleSystem/DensityPBD.h:18-88
+                       varIterationNumber, varRestDensity, varSamplingDistance,
+                       varSmoothingLength, inPosition, inVelocity,
+                       inNeighborIndex, outDensity; constrain()
+  PositionBasedFluidModel
+                       Dynamics/ParticleSystem/PositionBasedFluidModel.h:25-60
+                       m_smoothingLength, m_position, m_velocity, m_forceDensity;
+                       step(dt)
+
+Array fields hold torch tensors on a HIP device ((n, 3) float32 for
+positions/velocities/forces, (n,) float32 for density); connected fields share
+one tensor, as the reference's fields share one device array
+(Framework/Framework/FieldArray.h:166-185). Every compute call goes through
+libpbf.so; there is no CPU path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Callable, Optional
+
+import numpy as np
+import torch
+
+from . import _capi
+
+
+def _ptr(t: Optional[torch.Tensor]):
+    return None if t is None else C.c_void_p(t.data_ptr())
+
+
+def _stream(device) -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _check_vec3(t: torch.Tensor, name: str) -> torch.Tensor:
+    if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 3 or not t.is_cuda:
+        raise ValueError(f"{name}: expected a (n, 3) float32 device tensor, got {tuple(t.shape)} {t.dtype} {t.device}")
+    return t.contiguous()
+
+
+class Context:
+    """A pbf_ctx (include/pbf.h) bound to one HIP device."""
+
+    def __init__(self, device="cuda:0", **params):
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise ValueError("the PBD fluid step runs on a HIP device only")
+        self._lib = _capi.lib()
+        self._ctx = C.c_void_p()
+        p = _capi.params_from(**params)
+        with torch.cuda.device(self.device):
+            _capi.check(self._lib.pbf_create(C.byref(p), C.byref(self._ctx)))
+
+    # -- parameters (the VarField setters of the reference) --------------------
+    def params(self) -> dict:
+        p = _capi.pbf_params()
+        _capi.check(self._lib.pbf_get_params(self._ctx, C.byref(p)), self._ctx)
+        return _capi.params_dict(p)
+
+    def set_params(self, **kw) -> None:
+        cur = self.params()
+        cur.update(kw)
+        p = _capi.params_from(**cur)
+        with torch.cuda.device(self.device):
+            _capi.check(self._lib.pbf_set_params(self._ctx, C.byref(p)), self._ctx)
+
+    def reserve(self, n: int) -> None:
+        with torch.cuda.device(self.device):
+            _capi.check(self._lib.pbf_reserve(self._ctx, int(n)), self._ctx)
+
+    def stats(self) -> dict:
+        s = _capi.pbf_stats()
+        _capi.check(self._lib.pbf_get_stats(self._ctx, C.byref(s)), self._ctx)
+        out = {}
+        for name, _ in _capi.pbf_stats._fields_:
+            v = getattr(s, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+    def set_profiling(self, enable: bool) -> None:
+        _capi.check(self._lib.pbf_set_profiling(self._ctx, 1 if enable else 0), self._ctx)
+
+    def profile(self) -> dict:
+        """Per pass: (summed ms, timed launches) since profiling was enabled."""
+        p = _capi.pbf_profile()
+        _capi.check(self._lib.pbf_get_profile(self._ctx, C.byref(p)), self._ctx)
+        return {name: (p.ms[i], p.launches[i]) for i, name in enumerate(_capi.PASS_NAMES)}
+
+    @property
+    def handle(self):
+        return self._ctx
+
+    def close(self) -> None:
+        if self._ctx:
+            self._lib.pbf_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- C ABI calls on tensors -------------------------------------------------
+    def neighbor_query(self, x: torch.Tensor) -> "NeighborList":
+        x = _check_vec3(x, "position")
+        n = x.shape[0]
+        off = torch.empty(n + 1, dtype=torch.int32, device=x.device)
+        total = C.c_longlong()
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_neighbor_query(self._ctx, _ptr(x), n, _ptr(off), None, 0, C.byref(total),
+                                                     _stream(x.device)), self._ctx)
+            el = torch.empty(max(total.value, 1), dtype=torch.int32, device=x.device)
+            _capi.check(self._lib.pbf_step_neighbors(self._ctx, _ptr(off), _ptr(el), total.value, C.byref(total),
+                                                     _stream(x.device)), self._ctx)
+        return NeighborList(off, el[: total.value])
+
+    def step_neighbors(self, n: int) -> "NeighborList":
+        off = torch.empty(n + 1, dtype=torch.int32, device=self.device)
+        total = C.c_longlong()
+        with torch.cuda.device(self.device):
+            _capi.check(self._lib.pbf_step_neighbors(self._ctx, _ptr(off), None, 0, C.byref(total),
+                                                     _stream(self.device)), self._ctx)
+            el = torch.empty(max(total.value, 1), dtype=torch.int32, device=self.device)
+            _capi.check(self._lib.pbf_step_neighbors(self._ctx, _ptr(off), _ptr(el), total.value, C.byref(total),
+                                                     _stream(self.device)), self._ctx)
+        return NeighborList(off, el[: total.value])
+
+    def integrate(self, x, v, f=None) -> None:
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        if f is not None:
+            f = _check_vec3(f, "force")
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_integrate(self._ctx, _ptr(x), _ptr(v), _ptr(f), x.shape[0], _stream(x.device)),
+                        self._ctx)
+
+    def density(self, x, nbr: "NeighborList", rho: torch.Tensor) -> None:
+        x = _check_vec3(x, "position")
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_density(self._ctx, _ptr(x), _ptr(nbr.index), _ptr(nbr.elements), _ptr(rho),
+                                              x.shape[0], _stream(x.device)), self._ctx)
+
+    def pbd_constrain(self, x, v, nbr: "NeighborList", rho: Optional[torch.Tensor]) -> None:
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_pbd_constrain(self._ctx, _ptr(x), _ptr(v), _ptr(nbr.index),
+                                                    _ptr(nbr.elements), _ptr(rho), x.shape[0], _stream(x.device)),
+                        self._ctx)
+
+    def step(self, x, v, rho: Optional[torch.Tensor] = None, nsteps: int = 1) -> None:
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_step(self._ctx, _ptr(x), _ptr(v), _ptr(rho), x.shape[0], int(nsteps),
+                                           _stream(x.device)), self._ctx)
+
+
+class NeighborList:
+    """CSR neighbor lists: ``index`` holds n+1 exclusive offsets (the
+    reference's m_index plus the total), ``elements`` the neighbor ids
+    (NeighborList.h:27-66)."""
+
+    def __init__(self, index: torch.Tensor, elements: torch.Tensor):
+        self.index = index
+        self.elements = elements
+
+    def size(self) -> int:
+        return self.index.numel() - 1
+
+    def getNeighborSize(self, i: int) -> int:
+        return int(self.index[i + 1] - self.index[i])
+
+    def getElement(self, i: int, j: int) -> int:
+        return int(self.elements[int(self.index[i]) + j])
+
+    def rows(self):
+        """Host copy as a list of sorted numpy rows."""
+        off = self.index.cpu().numpy()
+        el = self.elements.cpu().numpy()
+        return [np.sort(el[off[i]:off[i + 1]]) for i in range(len(off) - 1)]
+
+
+class VarField:
+    """A scalar parameter field (Framework/Framework/FieldVar.h): setValue
+    forwards to the context, as the reference's callbacks recompute m_factor
+    (SummationDensity.cu:47-51)."""
+
+    def __init__(self, ctx: Context, name: str, cast: Callable = float):
+        self._ctx, self._name, self._cast = ctx, name, cast
+
+    def setValue(self, v) -> None:
+        self._ctx.set_params(**{self._name: self._cast(v)})
+
+    def getValue(self):
+        return self._ctx.params()[self._name]
+
+
+class ArrayField:
+    """A device-array field; connect() makes the sink share the source's tensor
+    (FieldArray.h:166-185)."""
+
+    def __init__(self, value: Optional[torch.Tensor] = None):
+        self._src: Optional[ArrayField] = None
+        self._value = value
+
+    def _root(self) -> "ArrayField":
+        f = self
+        while f._src is not None:
+            f = f._src
+        return f
+
+    def setValue(self, t: Optional[torch.Tensor]) -> None:
+        self._root()._value = t
+
+    def getValue(self) -> Optional[torch.Tensor]:
+        return self._root()._value
+
+    def connect(self, sink: "ArrayField") -> None:
+        sink._src = self
+
+    def isEmpty(self) -> bool:
+        v = self.getValue()
+        return v is None or v.numel() == 0
+
+    def getElementCount(self) -> int:
+        v = self.getValue()
+        return 0 if v is None else v.shape[0]
+
+
+class NeighborQuery:
+    """Uniform-grid neighbor search (NeighborQuery.cu:152-199)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.inRadius = VarField(ctx, "smoothing_length")
+        self.inPosition = ArrayField()
+        self.outNeighborhood = ArrayField()
+
+    def compute(self) -> None:
+        self.outNeighborhood.setValue(self.ctx.neighbor_query(self.inPosition.getValue()))
+
+
+class ParticleIntegrator:
+    """Symplectic Euler (ParticleIntegrator.cu:20-138)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.inPosition = ArrayField()
+        self.inVelocity = ArrayField()
+        self.inForceDensity = ArrayField()
+
+    def begin(self) -> None:
+        f = self.inForceDensity.getValue()
+        if f is not None:
+            f.zero_()
+
+    def integrate(self) -> bool:
+        if not self.inPosition.isEmpty():
+            self.ctx.integrate(self.inPosition.getValue(), self.inVelocity.getValue(), self.inForceDensity.getValue())
+        return True
+
+    def end(self) -> None:
+        pass
+
+
+class DensityPBD:
+    """Position-based incompressibility (DensityPBD.cu:271-302)."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.varIterationNumber = VarField(ctx, "iterations", int)
+        self.varRestDensity = VarField(ctx, "rest_density")
+        self.varSamplingDistance = VarField(ctx, "sampling_distance")
+        self.varSmoothingLength = VarField(ctx, "smoothing_length")
+        self.inPosition = ArrayField()
+        self.inVelocity = ArrayField()
+        self.inNeighborIndex = ArrayField()
+        self.outDensity = ArrayField()
+
+    def constrain(self) -> bool:
+        x = self.inPosition.getValue()
+        rho = self.outDensity.getValue()
+        if rho is None or rho.shape[0] != x.shape[0]:
+            rho = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+            self.outDensity.setValue(rho)
+        self.ctx.pbd_constrain(x, self.inVelocity.getValue(), self.inNeighborIndex.getValue(), rho)
+        return True
+
+
+class PositionBasedFluidModel:
+    """NumericalModel of the PBD fluid (PositionBasedFluidModel.cpp:41-101)
+    without the viscosity module.
+
+    step(dt) runs the fused MI355X step (one library call);
+    step_modules(dt) runs the module sequence begin -> NeighborQuery::compute ->
+    integrate -> DensityPBD::constrain -> end through the module-granular
+    entry points, as the reference's step does."""
+
+    def __init__(self, smoothing_length: float = 0.0125, device="cuda:0", **params):
+        self.ctx = Context(device, smoothing_length=smoothing_length, **params)
+        self.m_smoothingLength = VarField(self.ctx, "smoothing_length")
+        self.m_position = ArrayField()
+        self.m_velocity = ArrayField()
+        self.m_forceDensity = ArrayField()
+        self.m_density = ArrayField()
+        self.nbrQuery = NeighborQuery(self.ctx)
+        self.pbdModule = DensityPBD(self.ctx)
+        self.integrator = ParticleIntegrator(self.ctx)
+        self.m_position.connect(self.nbrQuery.inPosition)
+        self.m_position.connect(self.pbdModule.inPosition)
+        self.m_velocity.connect(self.pbdModule.inVelocity)
+        self.m_position.connect(self.integrator.inPosition)
+        self.m_velocity.connect(self.integrator.inVelocity)
+        self.m_forceDensity.connect(self.integrator.inForceDensity)
+        self.nbrQuery.outNeighborhood.connect(self.pbdModule.inNeighborIndex)
+        self.pbdModule.outDensity.connect(self.m_density)
+
+    def _set_dt(self, dt: Optional[float]) -> None:
+        if dt is not None and abs(self.ctx.params()["dt"] - float(np.float32(dt))) > 0:
+            self.ctx.set_params(dt=float(dt))
+
+    def step(self, dt: Optional[float] = None, nsteps: int = 1) -> None:
+        self._set_dt(dt)
+        x = self.m_position.getValue()
+        rho = self.m_density.getValue()
+        if rho is None or rho.shape[0] != x.shape[0]:
+            rho = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+            self.m_density.setValue(rho)
+        self.ctx.step(x, self.m_velocity.getValue(), rho, nsteps)
+
+    def step_modules(self, dt: Optional[float] = None) -> None:
+        self._set_dt(dt)
+        self.integrator.begin()
+        self.nbrQuery.compute()
+        self.integrator.integrate()
+        self.pbdModule.constrain()
+        self.integrator.end()
+
+    def step_arrays(self, x: np.ndarray, v: np.ndarray, nsteps: int = 1, modules: bool = False):
+        """Convenience for numpy callers: copies in, steps, copies out
+        (positions, velocities, densities)."""
+        dev = self.ctx.device
+        self.m_position.setValue(torch.from_numpy(np.ascontiguousarray(x, np.float32)).to(dev))
+        self.m_velocity.setValue(torch.from_numpy(np.ascontiguousarray(v, np.float32)).to(dev))
+        self.m_forceDensity.setValue(torch.zeros_like(self.m_position.getValue()))
+        if modules:
+            for _ in range(nsteps):
+                self.step_modules()
+        else:
+            self.step(nsteps=nsteps)
+        torch.cuda.synchronize(dev)
+        return (self.m_position.getValue().cpu().numpy(), self.m_velocity.getValue().cpu().numpy(),
+                self.m_density.getValue().cpu().numpy())

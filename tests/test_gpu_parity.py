@@ -1,0 +1,379 @@
+"""Parity of libpbf.so (HIP, gfx950) against the CPU oracle.
+
+Bars: neighbor sets bit-exact (integer work); positions within 1e-6 m (1e-6 of
+the unit scene box) and velocities within 1e-3 m/s (the position bar divided
+by dt) of the FP32 restatement; over 100 steps mean density and kinetic
+energy within 0.5%; free fall bit-exact.
+
+Density after a step is held to 1e-5 relative against the FP64 instantiation
+of the oracle, or to twice the FP32 restatement's own deviation from FP64 on
+the same input when that is larger: fp32 positions near 0.5 m carry 6e-8 m
+of rounding per update, which moves the density of sparse (surface) particles
+by ~1e-5 relative after five iterations, so two fp32 evaluations that differ
+only in summation order (the reference's own order is that of float atomics)
+legitimately differ at that level. A single density pass on identical inputs
+(test_density_on_caller_list) meets 1e-5 directly.
+
+Every call goes through the C ABI."""
+import sys
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+RHO_REL = 1e-5
+X_ABS = 1e-6
+V_ABS = 1e-3
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float32)).to("cuda:0")
+
+
+def row_keys(off, el):
+    n = len(off) - 1
+    rows = np.repeat(np.arange(n, dtype=np.int64), np.diff(off).astype(np.int64))
+    return np.sort(rows * n + el.astype(np.int64))
+
+
+def row_checksums(off, el):
+    """Order-independent per-row (count, sum, xor) of SplitMix-mixed ids."""
+    z = el.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    with np.errstate(over="ignore"):
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    cnt = np.diff(off)
+    starts = off[:-1].astype(np.int64)
+    nz = cnt > 0
+    s = np.zeros(len(cnt), np.uint64)
+    x = np.zeros(len(cnt), np.uint64)
+    s[nz] = np.add.reduceat(z, starts[nz])
+    x[nz] = np.bitwise_xor.reduceat(z, starts[nz])
+    return cnt, s, x
+
+
+def assert_same_sets(gpu_nl, off, el):
+    go = gpu_nl.index.cpu().numpy()
+    ge = gpu_nl.elements.cpu().numpy()
+    np.testing.assert_array_equal(go, off)
+    np.testing.assert_array_equal(row_keys(go, ge), row_keys(off, el))
+
+
+def rel_err(a, b):
+    """|a - b| / |b| per element; 0 where equal (also 0 == 0), |a - b| where b == 0."""
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.abs(a - b)
+    den = np.where(np.abs(b) > 0, np.abs(b), 1.0)
+    return np.where(d == 0, 0.0, d / den)
+
+
+def assert_step_parity(gpu, f32, f64=None):
+    """gpu, f32, f64: (x, v, rho) of the GPU, the FP32 restatement and the FP64
+    instantiation on the same input."""
+    x, v, rho = gpu
+    xo, vo, ro = f32[:3]
+    ex = np.abs(x.astype(np.float64) - xo).max()
+    ev = np.abs(v.astype(np.float64) - vo).max()
+    if f64 is None:
+        er, bar = rel_err(rho, ro).max(), RHO_REL
+    else:
+        floor = rel_err(ro, f64[2]).max()
+        er, bar = rel_err(rho, f64[2]).max(), max(RHO_REL, 2 * floor)
+        say(f"rho: gpu-vs-fp64 {er:.3e}, fp32-oracle-vs-fp64 {floor:.3e}, bar {bar:.3e}; "
+            f"gpu-vs-fp32: rho {rel_err(rho, ro).max():.3e} x {ex:.3e} v {ev:.3e}")
+    assert er <= bar, f"rho rel err {er:.3e} > {bar:.3e}"
+    assert ex <= X_ABS, f"x err {ex:.3e}"
+    assert ev <= V_ABS, f"v err {ev:.3e}"
+
+
+def oracle_both(oracle, x, v, nsteps, **kw):
+    f32 = oracle.step(oracle.make_params(**kw), x, v, nsteps)
+    f64 = oracle.step(oracle.make_params(double=True, **kw), np.asarray(x, np.float64), np.asarray(v, np.float64),
+                      nsteps)
+    return f32, f64
+
+
+# ----------------------------------------------------------- neighbor search
+def test_neighbor_sets_bit_exact_dam_break(fluid, oracle, scene):
+    say("neighbor sets, 32768-particle dam break")
+    x = scene.dam_break_32k()
+    ctx = fluid.Context("cuda:0")
+    nl = ctx.neighbor_query(dev(x))
+    off, el = oracle.neighbor_query(oracle.make_params(), x)
+    assert_same_sets(nl, off, el)
+    st = ctx.stats()
+    assert st["neighbor_pairs"] == off[-1]
+    assert st["grid_dims"] == [85, 85, 85]
+    assert st["out_of_grid"] == 0
+
+
+def test_neighbor_sets_gaussian_blob(fluid, oracle, scene):
+    say("neighbor sets, uneven occupancy (Gaussian blobs)")
+    x = scene.gaussian_blobs(2, 12000, 0.03, seed=21)
+    ctx = fluid.Context("cuda:0")
+    nl = ctx.neighbor_query(dev(x))
+    off, el = oracle.neighbor_query(oracle.make_params(), x)
+    assert_same_sets(nl, off, el)
+    assert np.diff(off).max() > 150  # dense centre
+
+
+def test_neighbor_sets_crowded_cell(fluid, oracle, scene):
+    """More than 1024 particles in one cell (the in-cell rank falls back to
+    arrival order there); the sets must not change."""
+    say("neighbor sets, 1200 particles in one cell")
+    u = scene.uniform01(33, 3 * 1200).reshape(1200, 3)
+    x = (0.5 + 0.011 * u).astype(np.float32)
+    ctx = fluid.Context("cuda:0")
+    nl = ctx.neighbor_query(dev(x))
+    off, el = oracle.neighbor_query(oracle.make_params(), x)
+    assert_same_sets(nl, off, el)
+
+
+def test_neighbor_query_capacity_error(fluid, capi, scene):
+    import ctypes as C
+    say("neighbor query capacity error")
+    x = dev(scene.jittered_lattice((8, 8, 8), (0.3, 0.3, 0.3), 0.005, 0.1, seed=2))
+    ctx = fluid.Context("cuda:0")
+    off = torch.empty(513, dtype=torch.int32, device="cuda:0")
+    el = torch.empty(10, dtype=torch.int32, device="cuda:0")
+    total = C.c_longlong()
+    st = capi.lib().pbf_neighbor_query(ctx.handle, C.c_void_p(x.data_ptr()), 512, C.c_void_p(off.data_ptr()),
+                                       C.c_void_p(el.data_ptr()), 10, C.byref(total), None)
+    assert st == capi.PBF_ERR_CAPACITY
+    assert total.value > 10
+    assert b"need" in capi.lib().pbf_last_error(ctx.handle)
+
+
+# ------------------------------------------------------------- fused step
+def test_one_step_parity_dam_break(fluid, oracle, scene):
+    say("fused step, 32768-particle dam break, 1 step")
+    x = scene.dam_break_32k()
+    v = np.zeros_like(x)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    gpu = model.step_arrays(x, v, 1)
+    assert_step_parity(gpu, *oracle_both(oracle, x, v, 1))
+
+
+def test_module_path_parity_dam_break(fluid, oracle, scene):
+    say("module path NeighborQuery -> integrate -> DensityPBD, 1 step")
+    x = scene.dam_break_32k()
+    v = np.zeros_like(x)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    gpu = model.step_arrays(x, v, 1, modules=True)
+    assert_step_parity(gpu, *oracle_both(oracle, x, v, 1))
+
+
+def test_step_neighbor_export(fluid, oracle, scene):
+    say("CSR export after pbf_step")
+    x = scene.dam_break_32k()
+    ctx = fluid.Context("cuda:0")
+    xd, vd = dev(x), dev(np.zeros_like(x))
+    ctx.step(xd, vd, None, 1)
+    nl = ctx.step_neighbors(len(x))
+    off, el = oracle.neighbor_query(oracle.make_params(), x)
+    assert_same_sets(nl, off, el)
+
+
+def test_hundred_step_aggregates(fluid, oracle, scene):
+    say("fused step, 32768 particles, 100 steps, mean density and kinetic energy")
+    x = scene.dam_break_32k()
+    v = np.zeros_like(x)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    xs, vs, rho = model.step_arrays(x, v, 100)
+    xo, vo, ro, _ = oracle.step(oracle.make_params(), x, v, 100)
+    assert abs(rho.mean() / ro.mean() - 1) <= 5e-3
+    ke, keo = 0.5 * (vs.astype(np.float64) ** 2).sum(1).mean(), 0.5 * (vo.astype(np.float64) ** 2).sum(1).mean()
+    assert abs(ke / keo - 1) <= 5e-3
+
+
+def test_step_is_bitwise_reproducible(fluid, scene):
+    say("fused step reproducibility, 2 runs x 3 steps")
+    x = scene.dam_break_32k()
+    outs = []
+    for _ in range(2):
+        model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+        outs.append(model.step_arrays(x, np.zeros_like(x), 3))
+    for a, b in zip(*outs):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_free_fall_bit_exact(fluid):
+    say("free fall, 20 steps")
+    x0 = np.array([[0.2, 0.5, 0.2], [0.5, 0.6, 0.5], [0.8, 0.7, 0.3]], np.float32)
+    v0 = np.array([[0.1, 0.0, -0.2], [0.0, 0.3, 0.0], [-0.05, 0.0, 0.07]], np.float32)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    xs, vs, rho = model.step_arrays(x0, v0, 20)
+    ex, ev = x0.copy(), v0.copy()
+    g = np.array([0.0, -9.8, 0.0], np.float32)
+    for _ in range(20):
+        ev = ev + (np.float32(0) + g) * np.float32(0.001)
+        ex = ex + ev * np.float32(0.001)
+    np.testing.assert_array_equal(vs, ev)
+    np.testing.assert_array_equal(xs, ex)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 1000, 4097])
+def test_ragged_sizes(fluid, oracle, scene, n):
+    say(f"fused step, ragged n={n}")
+    x = scene.jittered_lattice((17, 17, 17), (0.3, 0.5, 0.3), 0.005, 0.1, seed=n, count=n)
+    v = np.zeros_like(x)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    gpu = model.step_arrays(x, v, 2)
+    assert_step_parity(gpu, *oracle_both(oracle, x, v, 2))
+
+
+def test_empty_input(fluid):
+    say("n = 0")
+    ctx = fluid.Context("cuda:0")
+    x = torch.empty((0, 3), dtype=torch.float32, device="cuda:0")
+    ctx.step(x, x.clone(), None, 3)
+    nl = ctx.neighbor_query(x)
+    assert nl.index.numel() == 1 and int(nl.index[0]) == 0
+
+
+def test_mask_fallback_is_identical(fluid, scene):
+    """A neighbor-mask reservation too small for the scene makes waves
+    re-test candidates; the results must be bit-identical."""
+    say("neighbor-mask fallback vs masks")
+    x = scene.dam_break_32k()
+    v = np.zeros_like(x)
+    a = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    ra = a.step_arrays(x, v, 1)
+    b = fluid.PositionBasedFluidModel(0.0125, "cuda:0", mask_words=1)
+    rb = b.step_arrays(x, v, 1)
+    st = b.ctx.stats()
+    assert st["mask_fallback_waves"] > 0
+    for u, w in zip(ra, rb):
+        np.testing.assert_array_equal(u, w)
+    rb2 = b.step_arrays(x, v, 1)  # reservation grown after the first call
+    assert b.ctx.stats()["mask_fallback_waves"] == 0
+    for u, w in zip(ra, rb2):
+        np.testing.assert_array_equal(u, w)
+
+
+def test_out_of_grid_particles(fluid, oracle, scene):
+    """Particles outside the padded grid query but are not hashed
+    (GridHash.cu:85,194); the one-sided pairs they create are reproduced."""
+    say("out-of-grid particles near the grid's lower x face")
+    x = scene.jittered_lattice((8, 8, 8), (-0.047, 0.5, 0.5), 0.005, 0.1, seed=12)
+    v = np.zeros_like(x)
+    p = oracle.make_params(g=(0.0, 0.0, 0.0))
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", gravity=(0.0, 0.0, 0.0))
+    gpu = model.step_arrays(x, v, 1)
+    c3, cid = oracle.cell3(p, x)
+    n_oog = int((cid < 0).sum())
+    assert 0 < n_oog < len(x)
+    assert model.ctx.stats()["out_of_grid"] == n_oog
+    assert_step_parity(gpu, *oracle_both(oracle, x, v, 1, g=(0.0, 0.0, 0.0)))
+
+
+def test_nonfinite_positions_reported(fluid):
+    say("NaN / Inf positions reported")
+    x = np.array([[0.5, 0.5, 0.5], [np.nan, 0.5, 0.5], [0.5, np.inf, 0.5]], np.float32)
+    ctx = fluid.Context("cuda:0")
+    ctx.neighbor_query(dev(x))
+    st = ctx.stats()
+    assert st["nonfinite"] == 2 and st["out_of_grid"] == 2
+
+
+# ----------------------------------------------------- module-granular calls
+def test_integrate_bit_exact(fluid, oracle, scene):
+    say("pbf_integrate with force")
+    x = scene.jittered_lattice((10, 10, 10), (0.3, 0.5, 0.3), 0.005, 0.1, seed=3)
+    v = (scene.uniform01(5, x.size).reshape(x.shape) - 0.5).astype(np.float32)
+    f = (scene.uniform01(6, x.size).reshape(x.shape) * 20 - 10).astype(np.float32)
+    ctx = fluid.Context("cuda:0")
+    xd, vd, fd = dev(x), dev(v), dev(f)
+    ctx.integrate(xd, vd, fd)
+    xo, vo = oracle.integrate(oracle.make_params(), x, v, f)
+    np.testing.assert_array_equal(vd.cpu().numpy(), vo)
+    np.testing.assert_array_equal(xd.cpu().numpy(), xo)
+
+
+def test_density_on_caller_list(fluid, oracle, scene):
+    say("pbf_density on the oracle's CSR")
+    x = scene.dam_break_32k()
+    p = oracle.make_params()
+    off, el = oracle.neighbor_query(p, x)
+    ctx = fluid.Context("cuda:0")
+    nl = fluid.NeighborList(torch.from_numpy(off).to("cuda:0"), torch.from_numpy(el).to("cuda:0"))
+    rho = torch.empty(len(x), dtype=torch.float32, device="cuda:0")
+    ctx.density(dev(x), nl, rho)
+    ro = oracle.density(p, x, off, el)
+    assert (np.abs(rho.cpu().numpy() - ro) / ro).max() <= RHO_REL
+
+
+def test_pbd_constrain_on_asymmetric_list(fluid, oracle, scene):
+    """DensityPBD over a caller list that is not symmetric (one-sided pairs):
+    the pairwise scatter of the reference, not the symmetric gather."""
+    say("pbf_pbd_constrain, asymmetric caller list")
+    x = scene.jittered_lattice((10, 10, 10), (0.3, 0.5, 0.3), 0.0045, 0.1, seed=9)
+    v = np.zeros_like(x)
+    p = oracle.make_params()
+    off, el = oracle.neighbor_query(p, x)
+    # drop every third row's last neighbor
+    keep = np.ones(len(el), bool)
+    rows = np.arange(len(x))
+    drop = off[1:][rows % 3 == 0] - 1
+    keep[drop] = False
+    cnt = np.diff(off) - (rows % 3 == 0)
+    off2 = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
+    el2 = el[keep].astype(np.int32)
+    ctx = fluid.Context("cuda:0")
+    nl = fluid.NeighborList(torch.from_numpy(off2).to("cuda:0"), torch.from_numpy(el2).to("cuda:0"))
+    xd, vd = dev(x), dev(v)
+    rho = torch.empty(len(x), dtype=torch.float32, device="cuda:0")
+    ctx.pbd_constrain(xd, vd, nl, rho)
+    f32 = oracle.pbd_constrain(p, x, v, off2, el2)
+    f64 = oracle.pbd_constrain(oracle.make_params(double=True), x.astype(np.float64), v.astype(np.float64), off2, el2)
+    assert_step_parity((xd.cpu().numpy(), vd.cpu().numpy(), rho.cpu().numpy()), f32, f64)
+
+
+def test_invalid_params_rejected(fluid, capi):
+    say("parameter validation")
+    with pytest.raises(capi.PBFError) as e:
+        fluid.Context("cuda:0", smoothing_length=-1.0)
+    assert e.value.status == capi.PBF_ERR_INVALID
+    ctx = fluid.Context("cuda:0")
+    with pytest.raises(capi.PBFError):
+        ctx.set_params(iterations=-2)
+    assert ctx.params()["iterations"] == 5
+
+
+# ------------------------------------------------------- full-size properties
+def test_million_particle_neighbor_checksums(fluid, oracle, scene):
+    say("1,048,576-particle lattice: per-row neighbor checksums vs oracle")
+    x = scene.lattice_1m()
+    ctx = fluid.Context("cuda:0")
+    nl = ctx.neighbor_query(dev(x))
+    off, el = oracle.neighbor_query(oracle.make_params(), x)
+    go = nl.index.cpu().numpy()
+    ge = nl.elements.cpu().numpy()
+    np.testing.assert_array_equal(go, off)
+    for a, b in zip(row_checksums(go, ge), row_checksums(off, el)):
+        np.testing.assert_array_equal(a, b)
+
+
+def test_million_particle_one_step(fluid, oracle, scene):
+    say("1,048,576-particle lattice: one fused step vs oracle")
+    x = scene.lattice_1m()
+    v = np.zeros_like(x)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    gpu = model.step_arrays(x, v, 1)
+    assert_step_parity(gpu, *oracle_both(oracle, x, v, 1))
