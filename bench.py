@@ -74,6 +74,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="lattice_1m", choices=["lattice_1m", "blobs_1m"])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 256 MiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not part of this "
@@ -89,6 +91,12 @@ def main():
     x_np = getattr(pkg.scene, args.workload)()
     n = len(x_np)
     dev = torch.device("cuda:0")
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     x = torch.from_numpy(x_np).to(dev)
     v = torch.zeros_like(x)
     rho = torch.empty(n, dtype=torch.float32, device=dev)

@@ -62,8 +62,11 @@ typedef struct pbf_params {
   float pair_eps;          /* EPSILON 1e-6 of the lambda/dx pair skip (cuda_utilities.h:30) */
   float scene_lo[3];       /* SceneGraph lower bound (SceneGraph.h:77) */
   float scene_hi[3];       /* SceneGraph upper bound (SceneGraph.h:78) */
-  int mask_words;          /* neighbor-mask reservation, 32-bit words per particle
-                              (0 = 32); grows automatically, see pbf_step */
+  int mask_words;          /* neighbor bit-mask scratch reservation, 32-bit words
+                              per particle (0 = 32) */
+  int list_slots;          /* neighbor-list reservation, 16-bit slots per particle
+                              (0 = 96). Both reservations grow automatically
+                              between calls, see pbf_step */
   unsigned flags;          /* reserved, must be 0 */
 } pbf_params;
 
@@ -72,9 +75,12 @@ typedef struct pbf_stats {
   long long neighbor_pairs;      /* sum of neighbor-list sizes (self included) */
   long long out_of_grid;         /* particles outside the grid: not hashed, still querying */
   long long nonfinite;           /* particles with a NaN/Inf coordinate (counted in out_of_grid) */
-  long long mask_words_needed;   /* neighbor-mask words the last search wanted */
-  long long mask_words_reserved; /* neighbor-mask words reserved */
-  long long mask_fallback_waves; /* wavefronts that re-tested candidates (mask full) */
+  long long mask_words_needed;   /* neighbor bit-mask words the last search wanted */
+  long long mask_words_reserved; /* neighbor bit-mask words reserved */
+  long long mask_fallback_waves; /* wavefronts that re-test candidates in every pass
+                                    (bit-mask reservation full) */
+  long long list_slots_needed;   /* neighbor-list slots the last search wanted */
+  long long list_slots_reserved; /* neighbor-list slots reserved */
   int grid_dims[3];              /* cells per axis (GridHash::setSpace) */
   float grid_lo[3];              /* grid origin = scene_lo - 2 cells */
   float cell_size;

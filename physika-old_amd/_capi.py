@@ -35,6 +35,7 @@ class pbf_params(C.Structure):
         ("scene_lo", C.c_float * 3),
         ("scene_hi", C.c_float * 3),
         ("mask_words", C.c_int),
+        ("list_slots", C.c_int),
         ("flags", C.c_uint),
     ]
 
@@ -47,6 +48,8 @@ class pbf_stats(C.Structure):
         ("mask_words_needed", C.c_longlong),
         ("mask_words_reserved", C.c_longlong),
         ("mask_fallback_waves", C.c_longlong),
+        ("list_slots_needed", C.c_longlong),
+        ("list_slots_reserved", C.c_longlong),
         ("grid_dims", C.c_int * 3),
         ("grid_lo", C.c_float * 3),
         ("cell_size", C.c_float),

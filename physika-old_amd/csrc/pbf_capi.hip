@@ -34,9 +34,15 @@ struct pbf_ctx {
   int *nbr_cnt = nullptr, *cnt_orig = nullptr;
   float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr;
   float* dpos = nullptr;
-  long long* wave_base = nullptr;
+  // neighbor pass output: per-wave metadata, bit masks (scratch), lists
+  long long* mask_base = nullptr;
+  long long* list_base = nullptr;
+  int* list_jbase = nullptr;
+  int* list_mode = nullptr;
   uint32_t* mask = nullptr;
-  long long mask_cap = 0;
+  long long mask_cap = 0;  // words
+  uint16_t* list = nullptr;
+  long long list_cap = 0;  // 16-bit slots
 
   int* scan_scratch = nullptr;
   size_t scan_scratch_n = 0;
@@ -143,6 +149,7 @@ int validate(const pbf_params* p, std::string& why) {
   if (!std::isfinite(p->rest_density)) return why = "rest_density must be finite", PBF_ERR_INVALID;
   if (p->iterations < 0 || p->iterations > 1000) return why = "iterations must be in [0, 1000]", PBF_ERR_INVALID;
   if (p->mask_words < 0) return why = "mask_words must be >= 0", PBF_ERR_INVALID;
+  if (p->list_slots < 0) return why = "list_slots must be >= 0", PBF_ERR_INVALID;
   if (p->flags != 0) return why = "flags must be 0", PBF_ERR_INVALID;
   for (int c = 0; c < 3; ++c) {
     if (!std::isfinite(p->scene_lo[c]) || !std::isfinite(p->scene_hi[c]) || !(p->scene_hi[c] > p->scene_lo[c]))
@@ -235,12 +242,27 @@ long long mask_words_for(const pbf_ctx* c, long long n) {
   return waves * 64 * per;
 }
 
+long long list_slots_for(const pbf_ctx* c, long long n) {
+  const long long per = c->p.list_slots > 0 ? c->p.list_slots : 96;
+  const long long waves = (n + 63) / 64;
+  return waves * 64 * per;
+}
+
 int ensure_mask(pbf_ctx* c, long long words) {
   if (words <= c->mask_cap) return PBF_OK;
   dfree(c->mask);
   c->mask_cap = 0;
   PBF_HIP(c, hipMalloc(&c->mask, sizeof(uint32_t) * (size_t)words));
   c->mask_cap = words;
+  return PBF_OK;
+}
+
+int ensure_list(pbf_ctx* c, long long slots) {
+  if (slots <= c->list_cap) return PBF_OK;
+  dfree(c->list);
+  c->list_cap = 0;
+  PBF_HIP(c, hipMalloc(&c->list, sizeof(uint16_t) * (size_t)slots));
+  c->list_cap = slots;
   return PBF_OK;
 }
 
@@ -254,7 +276,7 @@ int ensure_particles(pbf_ctx* c, int n) {
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
-  dfree(c->dpos); dfree(c->wave_base);
+  dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   c->cap_n = 0;
   const size_t ni = sizeof(int) * (size_t)cap, nf4 = sizeof(float4) * (size_t)cap;
   PBF_HIP(c, hipMalloc(&c->key, ni));
@@ -271,23 +293,33 @@ int ensure_particles(pbf_ctx* c, int n) {
   PBF_HIP(c, hipMalloc(&c->vs, nf4));
   PBF_HIP(c, hipMalloc(&c->dpos, 3 * sizeof(float) * (size_t)cap));
   PBF_HIP(c, hipMemset(c->dpos, 0, 3 * sizeof(float) * (size_t)cap));
-  PBF_HIP(c, hipMalloc(&c->wave_base, sizeof(long long) * (size_t)(cap / 64 + 4)));
+  const size_t nwv = (size_t)(cap / 64 + 4);
+  PBF_HIP(c, hipMalloc(&c->mask_base, sizeof(long long) * nwv));
+  PBF_HIP(c, hipMalloc(&c->list_base, sizeof(long long) * nwv));
+  PBF_HIP(c, hipMalloc(&c->list_jbase, sizeof(int) * nwv));
+  PBF_HIP(c, hipMalloc(&c->list_mode, sizeof(int) * nwv));
   c->cap_n = cap;
   st = ensure_mask(c, mask_words_for(c, cap));
+  if (st) return st;
+  st = ensure_list(c, list_slots_for(c, cap));
   if (st) return st;
   c->last_n = -1;
   return PBF_OK;
 }
 
-// Grows the mask reservation when the last finished call wanted more; never
-// blocks (the counters are copied asynchronously at the end of each call).
-int adapt_mask(pbf_ctx* c) {
+// Grows the mask and list reservations when the last finished call wanted
+// more; never blocks (the counters are copied asynchronously at the end of each
+// call).
+int adapt_reservations(pbf_ctx* c) {
   if (!c->counters_pending) return PBF_OK;
   if (hipEventQuery(c->counters_ev) != hipSuccess) return PBF_OK;
   c->counters_pending = false;
-  const long long need = (long long)c->h_counters[C_MASK_WORDS];
-  if (need > c->mask_cap) return ensure_mask(c, need + need / 8);
-  return PBF_OK;
+  const long long mw = (long long)c->h_counters[C_MASK_WORDS];
+  const long long ls = (long long)c->h_counters[C_LIST_SLOTS];
+  int st = PBF_OK;
+  if (mw > c->mask_cap) st = ensure_mask(c, mw + mw / 8);
+  if (!st && ls > c->list_cap) st = ensure_list(c, ls + ls / 8);
+  return st;
 }
 
 NbrView view_of(const pbf_ctx* c, int n) {
@@ -295,12 +327,31 @@ NbrView view_of(const pbf_ctx* c, int n) {
   nv.g = c->g;
   nv.cell_start = c->cell_start;
   nv.xn = c->xn;
+  nv.nbr_cnt = c->nbr_cnt;
+  nv.list_base = c->list_base;
+  nv.list_jbase = c->list_jbase;
+  nv.list_mode = c->list_mode;
+  nv.list = c->list;
+  nv.mask_base = c->mask_base;
   nv.mask = c->mask;
-  nv.wave_base = c->wave_base;
   nv.oog_flag = c->oog_flag;
   nv.thr2 = c->pc.thr2;
   nv.n = n;
   return nv;
+}
+
+NbrOut out_of(pbf_ctx* c) {
+  NbrOut o;
+  o.mask = c->mask;
+  o.mask_cap = c->mask_cap;
+  o.list = c->list;
+  o.list_cap = c->list_cap;
+  o.mask_base = c->mask_base;
+  o.list_base = c->list_base;
+  o.list_jbase = c->list_jbase;
+  o.list_mode = c->list_mode;
+  o.nbr_cnt = c->nbr_cnt;
+  return o;
 }
 
 // HIP-event pair around one pass on the call's stream (profiling only).
@@ -365,7 +416,7 @@ int search(pbf_ctx* c, const float* x, const float* v, int n, bool integrate, hi
   NbrView nv = view_of(c, n);
   {
     PassTimer t(c, PBF_PASS_NEIGHBORS, st);
-    launch_neighbors(nv, c->mask, c->wave_base, c->nbr_cnt, c->mask_cap, c->counters, st);
+    launch_neighbors(nv, out_of(c), c->counters, st);
   }
   launch_oog_mark(nv, c->oog_flag, st);
   PBF_LAUNCHED(c);
@@ -445,6 +496,7 @@ void pbf_params_default(pbf_params* p) {
     p->scene_hi[a] = 1.0f;
   }
   p->mask_words = 32;
+  p->list_slots = 96;
   p->flags = 0;
 }
 
@@ -493,7 +545,9 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
-  dfree(c->dpos); dfree(c->wave_base); dfree(c->mask); dfree(c->scan_scratch); dfree(c->counters);
+  dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
+  dfree(c->mask); dfree(c->list); dfree(c->scan_scratch);
+  dfree(c->counters);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->counters_ev) (void)hipEventDestroy(c->counters_ev);
   for (auto& m : c->marks) {
@@ -515,7 +569,11 @@ int pbf_set_params(pbf_ctx* c, const pbf_params* p) {
   if (st) return st;
   st = ensure_grid(c);
   if (st) return st;
-  if (c->cap_n > 0) return ensure_mask(c, mask_words_for(c, c->cap_n));
+  if (c->cap_n > 0) {
+    st = ensure_mask(c, mask_words_for(c, c->cap_n));
+    if (st) return st;
+    return ensure_list(c, list_slots_for(c, c->cap_n));
+  }
   return PBF_OK;
 }
 
@@ -541,7 +599,7 @@ int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_offsets, int*
     c->last_n = 0;
     return export_csr(c, 0, d_offsets, d_elems, cap, total, st);
   }
-  s = adapt_mask(c);
+  s = adapt_reservations(c);
   if (s) return s;
   s = search(c, d_x, nullptr, n, false, st);
   if (s) return s;
@@ -603,7 +661,7 @@ int pbf_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, int nsteps
   hipStream_t st = (hipStream_t)stream;
   int s = ensure_particles(c, n);
   if (s) return s;
-  s = adapt_mask(c);
+  s = adapt_reservations(c);
   if (s) return s;
   const int K = c->p.iterations;
   for (int step = 0; step < nsteps; ++step) {
@@ -642,6 +700,8 @@ int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
   out->mask_words_needed = (long long)c->h_counters[C_MASK_WORDS];
   out->mask_words_reserved = c->mask_cap;
   out->mask_fallback_waves = (long long)c->h_counters[C_FALLBACK];
+  out->list_slots_needed = (long long)c->h_counters[C_LIST_SLOTS];
+  out->list_slots_reserved = c->list_cap;
   out->grid_dims[0] = c->g.nx;
   out->grid_dims[1] = c->g.ny;
   out->grid_dims[2] = c->g.nz;

@@ -1,0 +1,101 @@
+// Device helpers shared by the kernel files of libpbf.so (gfx950, wave64).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "pbf_internal.h"
+
+namespace pbf {
+
+// GridHash::getIndex3 (Framework/Topology/GridHash.h:55-62): floor((p - lo)/ds)
+// with a correctly rounded division (the library is built with
+// -fhip-fp32-correctly-rounded-divide-sqrt), clamped to [-2, n+1] (the set of
+// valid +-1 stencil cells is unchanged; NaN maps to -2).
+__device__ __forceinline__ int cell_coord(float p, float lo, float ds, int n) {
+#pragma clang fp contract(off)
+  float f = floorf((p - lo) / ds);
+  if (!(f >= -2.0f)) return -2;
+  if (f > (float)(n + 1)) return n + 1;
+  return (int)f;
+}
+
+__device__ __forceinline__ void cell_of(const GridDev& g, float x, float y, float z, int& ci, int& cj, int& ck) {
+  ci = cell_coord(x, g.lo[0], g.ds, g.nx);
+  cj = cell_coord(y, g.lo[1], g.ds, g.ny);
+  ck = cell_coord(z, g.lo[2], g.ds, g.nz);
+}
+
+__device__ __forceinline__ bool in_grid(const GridDev& g, int ci, int cj, int ck) {
+  return ci >= 0 && ci < g.nx && cj >= 0 && cj < g.ny && ck >= 0 && ck < g.nz;
+}
+
+// |x_i - x_j| < h exactly as NeighborQuery.cu:266-267 evaluates it: the squared
+// norm in glm's order ((dx*dx + dy*dy) + dz*dz, no fused multiply-add) against
+// thr2, the least float whose correctly rounded square root reaches h (the
+// square root is monotone, so the two tests agree for every input).
+__device__ __forceinline__ bool within(float4 a, float4 b, float thr2) {
+#pragma clang fp contract(off)
+  float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  float s = (dx * dx + dy * dy) + dz * dz;
+  return s < thr2;
+}
+
+// Blocks are dispatched round-robin over the 8 XCDs of the chip; remap so that
+// each XCD (own L2) works on one contiguous range of cell-sorted particles.
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  int per = nb >> 3, rem = nb & 7, x = b & 7, idx = b >> 3;
+  return x * per + min(x, rem) + idx;
+}
+
+__device__ __forceinline__ int wave_max(int v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = max(v, __shfl_xor(v, d));
+  return v;
+}
+__device__ __forceinline__ int wave_min(int v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v = min(v, __shfl_xor(v, d));
+  return v;
+}
+__device__ __forceinline__ int wave_sum(int v) {
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
+  return v;
+}
+__device__ __forceinline__ int wave_inclusive_scan(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    int t = __shfl_up(v, d);
+    if (lane >= d) v += t;
+  }
+  return v;
+}
+
+__device__ __forceinline__ float4 load3(const float* x, int i) {
+  const size_t b = 3 * (size_t)i;
+  return make_float4(x[b], x[b + 1], x[b + 2], 0.0f);
+}
+
+// Pair geometry with t = max(1 - r/h, 0):
+//   W(r)  = 15/(pi h^3) t^3   (Dynamics/ParticleSystem/Kernel.h:36-45)
+//   Grad  = -45/(pi h^3) t^2  (Kernel.h:47-57)
+// r from v_rsq_f32 (within 2 ulp); r == 0 (self) gives t = 1.
+struct PairGeom {
+  float dx, dy, dz, r, rinv, t;
+};
+
+__device__ __forceinline__ PairGeom pair_geom(float4 xi, float4 xj, float inv_h) {
+  PairGeom p;
+  p.dx = xi.x - xj.x;
+  p.dy = xi.y - xj.y;
+  p.dz = xi.z - xj.z;
+  const float r2 = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
+  p.rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+  p.r = r2 * p.rinv;
+  p.t = fmaxf(1.0f - p.r * inv_h, 0.0f);
+  return p;
+}
+
+}  // namespace pbf

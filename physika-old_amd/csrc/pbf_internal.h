@@ -22,7 +22,7 @@ struct PhysConst {
   float h, inv_h;
   float thr2;      // squared-distance bound: sqrtf(s) < h  <=>  s < thr2
   float cW;        // m_factor * mass * 15/(pi h^3): rho_i = cW * sum (1-q)^3
-  float cG;        // -45/(pi h^3): dW/dr = cG (1-q)^2
+  float cG;        // -45/(pi h^3): Gradient = cG (1-q)^2
   float lam_rest;  // 1000
   float lam_eps;   // 0.1
   float corr;      // 10
@@ -32,18 +32,40 @@ struct PhysConst {
   float gdt[3];    // (0 + g) * dt: velocity increment of the integrator at zero force
 };
 
-enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_COUNT };
+enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_COUNT };
 
-// Cell-sorted view used by the neighbor-mask producer and every pass over it.
+// Cell-sorted view used by the neighbor pass and every pass over its lists.
+// Particles are processed in wavefronts of 64 consecutive sorted indices; a
+// wave's neighbor list holds entry k of lane l at list_base + 64*k + l (16-bit
+// j - list_jbase, or a 32-bit j in two slots when the wave's candidates span
+// more than 65536 indices), so the lanes of a wave read consecutive slots.
 struct NbrView {
   GridDev g;
-  const int* cell_start;      // ncell + 2 entries; cell_start[ncell] = in-grid count
-  const float4* xn;           // sorted positions the neighbor sets are defined on
-  const uint32_t* mask;       // wave-interleaved neighbor bit masks
-  const long long* wave_base; // per 64-particle wave: first mask word, -1 = re-test
-  const uint8_t* oog_flag;    // per cell: an out-of-grid particle lists this cell
+  const int* cell_start;       // ncell + 2 entries; cell_start[ncell] = in-grid count
+  const float4* xn;            // sorted positions the neighbor sets are defined on
+  const int* nbr_cnt;          // neighbors per sorted particle
+  const long long* list_base;  // per wave: first 16-bit slot of its list, -1 = no list
+  const int* list_jbase;       // per wave: index base of 16-bit entries
+  const int* list_mode;        // per wave: 0 = 16-bit deltas, 1 = 32-bit indices
+  const uint16_t* list;
+  const long long* mask_base;  // per wave: first word of its bit masks, -1 = none
+  const uint32_t* mask;        // wave-interleaved bit masks over the cell runs
+  const uint8_t* oog_flag;     // per cell: an out-of-grid particle lists this cell
   float thr2;
   int n;
+};
+
+// Output arrays of the neighbor pass.
+struct NbrOut {
+  uint32_t* mask;
+  long long mask_cap;   // words
+  uint16_t* list;
+  long long list_cap;   // 16-bit slots
+  long long* mask_base;
+  long long* list_base;
+  int* list_jbase;
+  int* list_mode;
+  int* nbr_cnt;
 };
 
 constexpr int kBlock = 256;
@@ -61,8 +83,7 @@ void launch_scatter(const int* key, const int* slot, const int* cell_start, int*
 void launch_reorder(const int* key, const int* cell_start, const int* tmp, const float* x,
                     const float* v, int n, int* perm, int* inv, float4* xn, float4* x0, float4* vs,
                     const PhysConst* integrate, hipStream_t st);
-void launch_neighbors(const NbrView& nv, uint32_t* mask, long long* wave_base, int* nbr_cnt,
-                      long long mask_cap, unsigned long long* counters, hipStream_t st);
+void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st);
 void launch_oog_mark(const NbrView& nv, uint8_t* oog_flag, hipStream_t st);
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
                                 float* rho_out, const int* perm, hipStream_t st);
@@ -75,7 +96,7 @@ void launch_csr_count(const int* nbr_cnt, const int* inv, int n, int* cnt_orig, 
 void launch_csr_fill(const NbrView& nv, const int* perm, const int* offsets, int* elems,
                      long long cap, hipStream_t st);
 
-// Module-granular kernels over a caller CSR list, original particle order.
+// Module-granular kernels, original particle order, caller CSR.
 void launch_integrate(float* x, float* v, const float* f, int n, const PhysConst& pc, hipStream_t st);
 void launch_csr_density(const float* x, const int* off, const int* el, float* rho, int n,
                         const PhysConst& pc, hipStream_t st);

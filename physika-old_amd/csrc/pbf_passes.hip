@@ -1,0 +1,428 @@
+// Neighbor lists and the PBD passes over them (NeighborQuery::compute,
+// SummationDensity, DensityPBD::takeOneIteration).
+//
+// A particle's 27 stencil cells form 9 runs of 3 x-consecutive cells; in cell
+// order each run is one contiguous index range [a, a+len). The neighbor pass
+// tests every candidate of the runs (the reference's set: every j in a valid
+// stencil cell with sqrtf(|x_i - x_j|^2) < h, NeighborQuery.cu:245-276), keeps
+// the result as bit masks (scratch) and decodes them into the wave's neighbor
+// list (16-bit offsets from a per-wave base, wave-interleaved: entry k of lane
+// l at base + 64*k + l). The ten PBD passes of a step then read one coalesced
+// list entry and gather one position per neighbor. Neighbor order is run
+// order, ascending index inside a run, on every path.
+#include <climits>
+
+#include "pbf_device.h"
+
+namespace pbf {
+
+// Run r (0..8) of cell (ci,cj,ck): cells (ci-1..ci+1, cj+dj, ck+dk) with
+// dj = r/3-1, dk = r%3-1, clipped to the grid.
+__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const GridDev& g, int ci, int cj,
+                                          int ck, int r, int& a, int& len) {
+  int j = cj + (r / 3) - 1, k = ck + (r % 3) - 1;
+  a = 0;
+  len = 0;
+  if (j < 0 || j >= g.ny || k < 0 || k >= g.nz) return;
+  int i0 = max(ci - 1, 0), i1 = min(ci + 1, g.nx - 1);
+  if (i0 > i1) return;
+  int row = (j + k * g.ny) * g.nx;
+  a = cell_start[row + i0];
+  len = cell_start[row + i1 + 1] - a;
+}
+
+// Bits of the candidates [j0, j0+cnt) (1 <= cnt <= 32) within h of xi. Loads
+// go out eight at a time; indices past the run repeat its last candidate and
+// their bits are masked off.
+__device__ __forceinline__ uint32_t test_word(const float4* __restrict__ xn, float4 xi, int j0, int cnt,
+                                              float thr2) {
+  uint32_t bits = 0;
+  const int last = j0 + cnt - 1;
+  for (int k0 = 0; k0 < cnt; k0 += 8) {
+    float4 c[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) c[u] = xn[min(j0 + k0 + u, last)];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) bits |= (uint32_t)within(xi, c[u], thr2) << (k0 + u);
+  }
+  return cnt >= 32 ? bits : bits & ((1u << cnt) - 1u);
+}
+
+// Neighbor position load: all four components (pass B reads lambda from w),
+// or xyz only (a float4 load whose w is dead lets the register allocator
+// reuse that register while the load is in flight, forcing an early wait).
+template <bool kW>
+__device__ __forceinline__ float4 load_pos(const float4* __restrict__ xs, int j) {
+  if (kW) return xs[j];
+  const float* q = reinterpret_cast<const float*>(xs + j);
+  return make_float4(q[0], q[1], q[2], 0.0f);
+}
+
+// List layout: a lane's entries come in groups of four that sit together
+// (8 bytes as 16-bit deltas, 16 bytes as 32-bit indices), and group b of the
+// wave's 64 lanes is contiguous: group b of lane l at base + (64*b + l) groups.
+__device__ __forceinline__ long long list_slot(long long lb, int k, int lane, int mode) {
+  return lb + ((((long long)(k >> 2) << 6) + lane) << (2 + mode)) + ((long long)(k & 3) << mode);
+}
+
+struct Group4 {
+  int j[4];
+};
+
+// Indices of group b (entries 4b..4b+3); entries past cnt become s.
+__device__ __forceinline__ Group4 decode16(uint2 e, int jb, int b, int cnt, int s) {
+  const int k = b << 2;
+  Group4 g;
+  g.j[0] = k + 0 < cnt ? jb + (int)(e.x & 0xffffu) : s;
+  g.j[1] = k + 1 < cnt ? jb + (int)(e.x >> 16) : s;
+  g.j[2] = k + 2 < cnt ? jb + (int)(e.y & 0xffffu) : s;
+  g.j[3] = k + 3 < cnt ? jb + (int)(e.y >> 16) : s;
+  return g;
+}
+__device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int s) {
+  const int k = b << 2;
+  Group4 g;
+  g.j[0] = k + 0 < cnt ? e.x : s;
+  g.j[1] = k + 1 < cnt ? e.y : s;
+  g.j[2] = k + 2 < cnt ? e.z : s;
+  g.j[3] = k + 3 < cnt ? e.w : s;
+  return g;
+}
+
+// Counted loop over one lane's list, software-pipelined two groups deep:
+// while the bodies of group b run, the positions of group b+1 and the indices
+// of group b+2 are in flight (one wait per group).
+template <bool kW, class GroupT, class LoadFn, class DecodeFn, class Body>
+__device__ __forceinline__ void list_loop(int cnt, const float4* __restrict__ xs, LoadFn&& load_group,
+                                          DecodeFn&& decode, Body&& body) {
+  const int ng = (cnt + 3) >> 2;
+  if (ng == 0) return;
+  GroupT e = load_group(0);
+  Group4 g = decode(e, 0);
+  float4 a[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) a[u] = load_pos<kW>(xs, g.j[u]);
+  if (ng > 1) e = load_group(1);
+  for (int b = 0; b < ng; ++b) {
+    const Group4 gn = decode(e, b + 1);
+    float4 an[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) an[u] = load_pos<kW>(xs, gn.j[u]);
+    if (b + 2 < ng) e = load_group(b + 2);
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if ((b << 2) + u < cnt) body(g.j[u], a[u]);
+    g = gn;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a[u] = an[u];
+  }
+}
+
+// Calls body(j, x_j) for every neighbor j of sorted particle s. The list is
+// used when the wave has one; otherwise (reservation exhausted) the words come
+// from the masks, or are recomputed from xn.
+template <bool kW, class Body>
+__device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const float4* __restrict__ xs,
+                                                    Body&& body) {
+  const int wave = s >> 6, lane = s & 63;
+  const long long lb = nv.list_base[wave];
+  if (lb >= 0) {
+    const int cnt = nv.nbr_cnt[s];
+    if (nv.list_mode[wave] == 0) {
+      const uint2* L = reinterpret_cast<const uint2*>(nv.list + lb) + lane;
+      const int jb = nv.list_jbase[wave];
+      list_loop<kW, uint2>(
+          cnt, xs, [&](int b) { return L[(long long)b << 6]; },
+          [&](uint2 e, int b) { return decode16(e, jb, b, cnt, s); }, body);
+    } else {
+      const int4* L = reinterpret_cast<const int4*>(nv.list + lb) + lane;
+      list_loop<kW, int4>(
+          cnt, xs, [&](int b) { return L[(long long)b << 6]; }, [&](int4 e, int b) { return decode32(e, b, cnt, s); },
+          body);
+    }
+    return;
+  }
+  const float4 xni = nv.xn[s];
+  int ci, cj, ck;
+  cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck);
+  const long long mb = nv.mask_base[wave];
+  int wofs = 0;
+  for (int r = 0; r < 9; ++r) {
+    int ra, rl;
+    run_range(nv.cell_start, nv.g, ci, cj, ck, r, ra, rl);
+    const int nw = (rl + 31) >> 5;
+    for (int w = 0; w < nw; ++w) {
+      const int jb = ra + (w << 5);
+      uint32_t bits = mb >= 0 ? nv.mask[mb + ((long long)(wofs + w) << 6) + lane]
+                              : test_word(nv.xn, xni, jb, min(32, rl - (w << 5)), nv.thr2);
+      while (bits) {
+        const int j = jb + __builtin_ctz(bits);
+        bits &= bits - 1u;
+        body(j, load_pos<kW>(xs, j));
+      }
+    }
+    wofs += nw;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Neighbor pass: K_CalNeighborSize + K_GetNeighborElements
+// (NeighborQuery.cu:245-276,337-367) in one traversal of the candidates.
+
+__global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
+                                                      unsigned long long* __restrict__ counters) {
+  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = s >> 6;
+  const bool active = s < nv.n;
+  float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
+  int ra[9], rl[9];
+  int words = 0, jlo = INT_MAX, jhi = -1;
+  if (active) {
+    xi = nv.xn[s];
+    int ci, cj, ck;
+    cell_of(nv.g, xi.x, xi.y, xi.z, ci, cj, ck);
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      run_range(nv.cell_start, nv.g, ci, cj, ck, r, ra[r], rl[r]);
+      words += (rl[r] + 31) >> 5;
+      if (rl[r] > 0) {
+        jlo = min(jlo, ra[r]);
+        jhi = max(jhi, ra[r] + rl[r] - 1);
+      }
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      ra[r] = 0;
+      rl[r] = 0;
+    }
+  }
+  const int wmax = wave_max(words);
+  const int wjlo = wave_min(jlo), wjhi = wave_max(jhi);
+  const int mode = (wjhi >= wjlo && wjhi - wjlo > 65535) ? 1 : 0;
+  long long mb = -1;
+  if (lane == 0) {
+    const unsigned long long need = 64ull * (unsigned long long)wmax;
+    const unsigned long long at = atomicAdd(&counters[C_MASK_WORDS], need);
+    if ((long long)(at + need) <= out.mask_cap)
+      mb = (long long)at;
+    else
+      atomicAdd(&counters[C_FALLBACK], 1ull);
+  }
+  mb = __shfl(mb, 0);
+
+  // phase 1: test every candidate, keep the words, count
+  uint32_t* mcol = out.mask + (mb >= 0 ? mb + lane : 0);
+  int cnt = 0, wofs = 0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int nw = (rl[r] + 31) >> 5;
+    for (int w = 0; w < nw; ++w) {
+      const uint32_t bits = test_word(nv.xn, xi, ra[r] + (w << 5), min(32, rl[r] - (w << 5)), nv.thr2);
+      if (mb >= 0) mcol[(long long)(wofs + w) << 6] = bits;
+      cnt += __popc(bits);
+    }
+    wofs += nw;
+  }
+  const int kmax = (wave_max(cnt) + 3) & ~3;
+  long long lb = -1;
+  if (lane == 0) {
+    const unsigned long long slots = 64ull * (unsigned long long)kmax * (unsigned long long)(mode + 1);
+    const unsigned long long at = atomicAdd(&counters[C_LIST_SLOTS], slots);
+    if (mb >= 0 && (long long)(at + slots) <= out.list_cap) lb = (long long)at;
+    out.mask_base[wave] = mb;
+    out.list_base[wave] = lb;
+    out.list_jbase[wave] = wjlo;
+    out.list_mode[wave] = mode;
+  }
+  lb = __shfl(lb, 0);
+  if (active) out.nbr_cnt[s] = cnt;
+
+  // phase 2: decode this lane's words (just written by this thread) into the list
+  if (active && lb >= 0) {
+    int k = 0;
+    wofs = 0;
+#pragma unroll
+    for (int r = 0; r < 9; ++r) {
+      const int nw = (rl[r] + 31) >> 5;
+      for (int w = 0; w < nw; ++w) {
+        uint32_t bits = mcol[(long long)(wofs + w) << 6];
+        const int jb = ra[r] + (w << 5);
+        while (bits) {
+          const int j = jb + __builtin_ctz(bits);
+          bits &= bits - 1u;
+          const long long slot = list_slot(lb, k, lane, mode);
+          if (mode == 0)
+            out.list[slot] = (uint16_t)(j - wjlo);
+          else
+            *reinterpret_cast<int*>(out.list + slot) = j;
+          ++k;
+        }
+      }
+      wofs += nw;
+    }
+  }
+  const int tot = wave_sum(cnt);
+  if (lane == 0 && tot) atomicAdd(&counters[C_PAIRS], (unsigned long long)tot);
+}
+
+void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st) {
+  k_neighbors<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, out, counters);
+}
+
+// Cells inside the stencil of an out-of-grid particle. Such a particle is not
+// hashed (GridHash.cu:85,194) yet queries, so an in-grid j it lists receives
+// -dp_kj from it (DensityPBD.cu:162) without listing it back.
+__global__ void k_oog_mark(NbrView nv, uint8_t* oog_flag) {
+  const int n_in = nv.cell_start[nv.g.ncell];
+  const int stride = gridDim.x * blockDim.x;
+  for (int k = n_in + blockIdx.x * blockDim.x + threadIdx.x; k < nv.n; k += stride) {
+    const float4 p = nv.xn[k];
+    int ci, cj, ck;
+    cell_of(nv.g, p.x, p.y, p.z, ci, cj, ck);
+    for (int dk = -1; dk <= 1; ++dk)
+      for (int dj = -1; dj <= 1; ++dj)
+        for (int di = -1; di <= 1; ++di) {
+          const int a = ci + di, b = cj + dj, c = ck + dk;
+          if (in_grid(nv.g, a, b, c)) oog_flag[a + nv.g.nx * (b + nv.g.ny * c)] = 1;
+        }
+  }
+}
+
+void launch_oog_mark(const NbrView& nv, uint8_t* oog_flag, hipStream_t st) {
+  k_oog_mark<<<64, kBlock, 0, st>>>(nv, oog_flag);
+}
+
+// ---------------------------------------------------------------------------
+// PBD passes (DensityPBD::takeOneIteration, DensityPBD.cu:306-365).
+
+// Pass A: K_ComputeDensity (SummationDensity.cu:13-36) and K_ComputeLambdas
+// (DensityPBD.cu:46-90) fused: one traversal gives rho_i, sum g and sum g.g;
+// lambda_i goes to xs[s].w for pass B.
+__global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* __restrict__ xs, PhysConst pc,
+                                                           float* __restrict__ rho_out,
+                                                           const int* __restrict__ perm) {
+  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= nv.n) return;
+  const float4 xi = xs[s];
+  float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
+  for_each_neighbor_x<false>(nv, s, xs, [&](int, float4 xj) {
+    const PairGeom p = pair_geom(xi, xj, pc.inv_h);
+    const float t2 = p.t * p.t;
+    rho += t2 * p.t;
+    const float gq = p.r > pc.pair_eps ? t2 * p.rinv : 0.0f;
+    const float gx = gq * p.dx, gy = gq * p.dy, gz = gq * p.dz;
+    sgx += gx;
+    sgy += gy;
+    sgz += gz;
+    sgg += gx * gx + gy * gy + gz * gz;
+  });
+  rho *= pc.cW;
+  const float S = (pc.cG * pc.cG) * (sgg + (sgx * sgx + sgy * sgy + sgz * sgz));
+  float lam = -(rho - pc.lam_rest) / (S + pc.lam_eps);
+  lam = lam > 0.0f ? 0.0f : lam;
+  reinterpret_cast<float*>(xs + s)[3] = lam;
+  if (rho_out) rho_out[perm[s]] = rho;
+}
+
+void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
+                                hipStream_t st) {
+  k_density_lambda<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, pc, rho_out, perm);
+}
+
+// Pass B: K_ComputeDisplacement + K_UpdatePosition (DensityPBD.cu:135-179,
+// 227-237) in gather form. The reference adds +dp_ij to i and -dp_ij to j from
+// both endpoints of every listed pair and dp_ji == -dp_ij, so a hashed particle
+// receives 2*sum_j dp_ij; an out-of-grid particle k is listed by nobody and
+// receives sum_j dp_kj once, while each hashed j it lists receives dp_jk once
+// more. The last iteration fuses DP_UpdateVelocity (DensityPBD.cu:368-378) and
+// the scatter back to original order.
+__global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float4* __restrict__ xs,
+                                                         float4* __restrict__ xd, PhysConst pc, int last,
+                                                         const float4* __restrict__ x0,
+                                                         const float4* __restrict__ vs,
+                                                         const int* __restrict__ perm, float* __restrict__ x_out,
+                                                         float* __restrict__ v_out) {
+  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= nv.n) return;
+  const float4 xi = xs[s];
+  const float li = xi.w;
+  // sum over the listed j of (lambda_i + lambda_j) t^2 / r (x_i - x_j)
+  auto pair = [&](float4 xj, float& cx, float& cy, float& cz) {
+    const PairGeom p = pair_geom(xi, xj, pc.inv_h);
+    const float c = p.r > pc.pair_eps ? (li + xj.w) * (p.t * p.t) * p.rinv : 0.0f;
+    cx += c * p.dx;
+    cy += c * p.dy;
+    cz += c * p.dz;
+  };
+  float ax = 0.f, ay = 0.f, az = 0.f;
+  for_each_neighbor_x<true>(nv, s, xs, [&](int, float4 xj) { pair(xj, ax, ay, az); });
+  const int n_in = nv.cell_start[nv.g.ncell];
+  float fa = 2.0f;
+  float bx = 0.f, by = 0.f, bz = 0.f;
+  if (n_in < nv.n) {  // out-of-grid particles exist
+    if (s >= n_in) {
+      fa = 1.0f;  // out-of-grid particle: its own pairs once
+    } else {
+      const float4 xni = nv.xn[s];
+      int ci, cj, ck;
+      cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck);
+      if (nv.oog_flag[ci + nv.g.nx * (cj + nv.g.ny * ck)]) {
+        // out-of-grid particles k that list s add -dp_ks = dp_sk to s once
+        for (int k = n_in; k < nv.n; ++k) {
+          const float4 xnk = nv.xn[k];
+          int ki, kj, kk;
+          cell_of(nv.g, xnk.x, xnk.y, xnk.z, ki, kj, kk);
+          if (abs(ki - ci) <= 1 && abs(kj - cj) <= 1 && abs(kk - ck) <= 1 && within(xnk, xni, nv.thr2))
+            pair(xs[k], bx, by, bz);
+        }
+      }
+    }
+  }
+  const float scale = pc.corr * pc.cG;
+  const float nx = xi.x + scale * (fa * ax + bx);
+  const float ny = xi.y + scale * (fa * ay + by);
+  const float nz = xi.z + scale * (fa * az + bz);
+  if (!last) {
+    xd[s] = make_float4(nx, ny, nz, 0.0f);
+    return;
+  }
+  const float4 xo = x0[s];
+  const float4 vv = vs[s];
+  const size_t i = (size_t)perm[s];
+  x_out[3 * i] = nx;
+  x_out[3 * i + 1] = ny;
+  x_out[3 * i + 2] = nz;
+  v_out[3 * i] = vv.x + (nx - xo.x) / pc.dt;
+  v_out[3 * i + 1] = vv.y + (ny - xo.y) / pc.dt;
+  v_out[3 * i + 2] = vv.z + (nz - xo.z) / pc.dt;
+}
+
+void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
+                              const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out,
+                              hipStream_t st) {
+  k_displacement<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm,
+                                                                 x_out, v_out);
+}
+
+// ---------------------------------------------------------------------------
+// CSR export (NeighborList, Framework/Topology/NeighborList.h:27-66) in
+// original ids.
+
+__global__ __launch_bounds__(kBlock) void k_csr_fill(NbrView nv, const int* __restrict__ perm,
+                                                     const int* __restrict__ offsets, int* __restrict__ elems,
+                                                     long long cap) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nv.n) return;
+  const int i = perm[s];
+  int o = offsets[i];
+  if ((long long)offsets[i + 1] > cap) return;
+  for_each_neighbor_x<false>(nv, s, nv.xn, [&](int j, float4) { elems[o++] = perm[j]; });
+}
+
+void launch_csr_fill(const NbrView& nv, const int* perm, const int* offsets, int* elems, long long cap,
+                     hipStream_t st) {
+  k_csr_fill<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, perm, offsets, elems, cap);
+}
+
+}  // namespace pbf

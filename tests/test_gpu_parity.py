@@ -247,22 +247,29 @@ def test_empty_input(fluid):
     assert nl.index.numel() == 1 and int(nl.index[0]) == 0
 
 
-def test_mask_fallback_is_identical(fluid, scene):
-    """A neighbor-mask reservation too small for the scene makes waves
-    re-test candidates; the results must be bit-identical."""
-    say("neighbor-mask fallback vs masks")
+@pytest.mark.parametrize("reserve", [{"mask_words": 1}, {"list_slots": 1}])
+def test_reservation_fallbacks_are_identical(fluid, scene, reserve):
+    """Reservations too small for the scene make waves iterate bit masks (list
+    full) or re-test candidates (masks full); the results must be
+    bit-identical to the list path, and the reservations grow for the next
+    call."""
+    say(f"neighbor storage fallback {reserve}")
     x = scene.dam_break_32k()
     v = np.zeros_like(x)
     a = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
     ra = a.step_arrays(x, v, 1)
-    b = fluid.PositionBasedFluidModel(0.0125, "cuda:0", mask_words=1)
+    b = fluid.PositionBasedFluidModel(0.0125, "cuda:0", **reserve)
     rb = b.step_arrays(x, v, 1)
     st = b.ctx.stats()
-    assert st["mask_fallback_waves"] > 0
+    if "mask_words" in reserve:
+        assert st["mask_fallback_waves"] > 0
+    else:
+        assert st["list_slots_needed"] > st["list_slots_reserved"]
     for u, w in zip(ra, rb):
         np.testing.assert_array_equal(u, w)
-    rb2 = b.step_arrays(x, v, 1)  # reservation grown after the first call
-    assert b.ctx.stats()["mask_fallback_waves"] == 0
+    rb2 = b.step_arrays(x, v, 1)  # reservations grown after the first call
+    st = b.ctx.stats()
+    assert st["mask_fallback_waves"] == 0 and st["list_slots_needed"] <= st["list_slots_reserved"]
     for u, w in zip(ra, rb2):
         np.testing.assert_array_equal(u, w)
 
