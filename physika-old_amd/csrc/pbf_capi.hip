@@ -286,7 +286,7 @@ int ensure_particles(pbf_ctx* c, int n) {
   PBF_HIP(c, hipMalloc(&c->inv, ni));
   PBF_HIP(c, hipMalloc(&c->nbr_cnt, ni));
   PBF_HIP(c, hipMalloc(&c->cnt_orig, ni));
-  PBF_HIP(c, hipMalloc(&c->xn, nf4));
+  PBF_HIP(c, hipMalloc(&c->xn, nf4 + sizeof(float4) * kXnPad));
   PBF_HIP(c, hipMalloc(&c->x0, nf4));
   PBF_HIP(c, hipMalloc(&c->xa, nf4));
   PBF_HIP(c, hipMalloc(&c->xb, nf4));

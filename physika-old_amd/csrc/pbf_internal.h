@@ -69,6 +69,9 @@ struct NbrOut {
 };
 
 constexpr int kBlock = 256;
+// Entries allocated past the end of the sorted position array xn, so that a
+// candidate group of 8 (or a 32-candidate word) may read past a run's end.
+constexpr int kXnPad = 32;
 
 size_t scan_scratch_ints(long long n);
 // out[0..n] = exclusive scan of in[0..n-1] (out[n] = total); in may equal out.

@@ -32,16 +32,17 @@ __device__ __forceinline__ void run_range(const int* __restrict__ cell_start, co
 }
 
 // Bits of the candidates [j0, j0+cnt) (1 <= cnt <= 32) within h of xi. Loads
-// go out eight at a time; indices past the run repeat its last candidate and
-// their bits are masked off.
+// go out eight at a time from one base address; the sorted position array
+// carries kXnPad far-away entries past its end, so a group may read past the
+// run (those bits are masked off) without index clamping.
 __device__ __forceinline__ uint32_t test_word(const float4* __restrict__ xn, float4 xi, int j0, int cnt,
                                               float thr2) {
   uint32_t bits = 0;
-  const int last = j0 + cnt - 1;
+  const float4* p = xn + j0;
   for (int k0 = 0; k0 < cnt; k0 += 8) {
     float4 c[8];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) c[u] = xn[min(j0 + k0 + u, last)];
+    for (int u = 0; u < 8; ++u) c[u] = p[k0 + u];
 #pragma unroll
     for (int u = 0; u < 8; ++u) bits |= (uint32_t)within(xi, c[u], thr2) << (k0 + u);
   }
@@ -169,8 +170,13 @@ __device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, co
 // Neighbor pass: K_CalNeighborSize + K_GetNeighborElements
 // (NeighborQuery.cu:245-276,337-367) in one traversal of the candidates.
 
+// Mask words per lane a wave keeps in LDS during the neighbor pass; waves that
+// need more keep them in the global mask reservation.
+constexpr int kNbLdsWords = 24;
+
 __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
                                                       unsigned long long* __restrict__ counters) {
+  __shared__ uint32_t lmask[kBlock / 64][kNbLdsWords * 64];
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63, wave = s >> 6;
   const bool active = s < nv.n;
@@ -200,8 +206,10 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
   const int wmax = wave_max(words);
   const int wjlo = wave_min(jlo), wjhi = wave_max(jhi);
   const int mode = (wjhi >= wjlo && wjhi - wjlo > 65535) ? 1 : 0;
+  // word storage: this wave's LDS block, else the global mask reservation
+  const bool in_lds = wmax <= kNbLdsWords;
   long long mb = -1;
-  if (lane == 0) {
+  if (!in_lds && lane == 0) {
     const unsigned long long need = 64ull * (unsigned long long)wmax;
     const unsigned long long at = atomicAdd(&counters[C_MASK_WORDS], need);
     if ((long long)(at + need) <= out.mask_cap)
@@ -210,16 +218,21 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
       atomicAdd(&counters[C_FALLBACK], 1ull);
   }
   mb = __shfl(mb, 0);
+  uint32_t* lcol = lmask[threadIdx.x >> 6] + lane;
+  uint32_t* gcol = out.mask + (mb >= 0 ? mb + lane : 0);
+  const bool keep = in_lds || mb >= 0;
 
   // phase 1: test every candidate, keep the words, count
-  uint32_t* mcol = out.mask + (mb >= 0 ? mb + lane : 0);
   int cnt = 0, wofs = 0;
 #pragma unroll
   for (int r = 0; r < 9; ++r) {
     const int nw = (rl[r] + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
       const uint32_t bits = test_word(nv.xn, xi, ra[r] + (w << 5), min(32, rl[r] - (w << 5)), nv.thr2);
-      if (mb >= 0) mcol[(long long)(wofs + w) << 6] = bits;
+      if (in_lds)
+        lcol[(wofs + w) << 6] = bits;
+      else if (mb >= 0)
+        gcol[(long long)(wofs + w) << 6] = bits;
       cnt += __popc(bits);
     }
     wofs += nw;
@@ -229,7 +242,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
   if (lane == 0) {
     const unsigned long long slots = 64ull * (unsigned long long)kmax * (unsigned long long)(mode + 1);
     const unsigned long long at = atomicAdd(&counters[C_LIST_SLOTS], slots);
-    if (mb >= 0 && (long long)(at + slots) <= out.list_cap) lb = (long long)at;
+    if (keep && (long long)(at + slots) <= out.list_cap) lb = (long long)at;
     out.mask_base[wave] = mb;
     out.list_base[wave] = lb;
     out.list_jbase[wave] = wjlo;
@@ -238,29 +251,60 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
   lb = __shfl(lb, 0);
   if (active) out.nbr_cnt[s] = cnt;
 
-  // phase 2: decode this lane's words (just written by this thread) into the list
+  // phase 2: decode this lane's words (written by this thread above) into the
+  // list, four entries per store
   if (active && lb >= 0) {
     int k = 0;
+    // the group being filled: e0..e3 (32-bit indices) or e0 = entries 0|1,
+    // e1 = entries 2|3 (16-bit deltas); entries past the lane's count are
+    // never read
+    uint32_t e0 = 0u, e1 = 0u, e2 = 0u, e3 = 0u;
+    auto flush = [&](int kk) {  // the group of entry kk-1 is complete (or final)
+      const long long slot = list_slot(lb, (kk - 1) & ~3, lane, mode);
+      if (mode == 0)
+        *reinterpret_cast<uint2*>(out.list + slot) = make_uint2(e0, e1);
+      else
+        *reinterpret_cast<uint4*>(out.list + slot) = make_uint4(e0, e1, e2, e3);
+    };
     wofs = 0;
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
       const int nw = (rl[r] + 31) >> 5;
       for (int w = 0; w < nw; ++w) {
-        uint32_t bits = mcol[(long long)(wofs + w) << 6];
+        uint32_t bits = in_lds ? lcol[(wofs + w) << 6] : gcol[(long long)(wofs + w) << 6];
         const int jb = ra[r] + (w << 5);
         while (bits) {
           const int j = jb + __builtin_ctz(bits);
           bits &= bits - 1u;
-          const long long slot = list_slot(lb, k, lane, mode);
-          if (mode == 0)
-            out.list[slot] = (uint16_t)(j - wjlo);
-          else
-            *reinterpret_cast<int*>(out.list + slot) = j;
+          const int q = k & 3;
+          if (mode == 0) {
+            const uint32_t d = (uint32_t)(j - wjlo);
+            if (q == 0)
+              e0 = d;
+            else if (q == 1)
+              e0 |= d << 16;
+            else if (q == 2)
+              e1 = d;
+            else
+              e1 |= d << 16;
+          } else {
+            const uint32_t u = (uint32_t)j;
+            if (q == 0)
+              e0 = u;
+            else if (q == 1)
+              e1 = u;
+            else if (q == 2)
+              e2 = u;
+            else
+              e3 = u;
+          }
           ++k;
+          if ((k & 3) == 0) flush(k);
         }
       }
       wofs += nw;
     }
+    if (k & 3) flush(k);
   }
   const int tot = wave_sum(cnt);
   if (lane == 0 && tot) atomicAdd(&counters[C_PAIRS], (unsigned long long)tot);

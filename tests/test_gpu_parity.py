@@ -247,24 +247,34 @@ def test_empty_input(fluid):
     assert nl.index.numel() == 1 and int(nl.index[0]) == 0
 
 
-@pytest.mark.parametrize("reserve", [{"mask_words": 1}, {"list_slots": 1}])
-def test_reservation_fallbacks_are_identical(fluid, scene, reserve):
-    """Reservations too small for the scene make waves iterate bit masks (list
-    full) or re-test candidates (masks full); the results must be
-    bit-identical to the list path, and the reservations grow for the next
-    call."""
-    say(f"neighbor storage fallback {reserve}")
-    x = scene.dam_break_32k()
+@pytest.mark.parametrize("scene_name,reserve", [
+    ("lattice", {"list_slots": 1}),                     # list full: words re-tested (kept in LDS only)
+    ("blob", {"list_slots": 1, "mask_words": 512}),     # list full: dense waves read global bit masks
+    ("blob", {"list_slots": 1, "mask_words": 1}),       # both full: every wave re-tests
+])
+def test_reservation_fallbacks_are_identical(fluid, scene, scene_name, reserve):
+    """Reservations too small for the scene make waves iterate global bit masks
+    or re-test their candidates in every pass; the results must be
+    bit-identical to the neighbor-list path, and the reservations grow for the
+    next call."""
+    say(f"neighbor storage fallback {scene_name} {reserve}")
+    if scene_name == "lattice":
+        x = scene.dam_break_32k()
+    else:
+        x = scene.gaussian_blobs(1, 20000, 0.02, seed=23)  # > 24 mask words per lane at the centre
     v = np.zeros_like(x)
     a = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
-    ra = a.step_arrays(x, v, 1)
+    a.step_arrays(x, v, 1)
+    ra = a.step_arrays(x, v, 1)  # second call: reservations sized for the scene, list path
+    assert a.ctx.stats()["list_slots_needed"] <= a.ctx.stats()["list_slots_reserved"]
     b = fluid.PositionBasedFluidModel(0.0125, "cuda:0", **reserve)
     rb = b.step_arrays(x, v, 1)
     st = b.ctx.stats()
-    if "mask_words" in reserve:
+    assert st["list_slots_needed"] > st["list_slots_reserved"]
+    if reserve.get("mask_words") == 1:
         assert st["mask_fallback_waves"] > 0
-    else:
-        assert st["list_slots_needed"] > st["list_slots_reserved"]
+    elif scene_name == "blob":
+        assert st["mask_words_needed"] > 0 and st["mask_fallback_waves"] == 0
     for u, w in zip(ra, rb):
         np.testing.assert_array_equal(u, w)
     rb2 = b.step_arrays(x, v, 1)  # reservations grown after the first call
