@@ -65,6 +65,59 @@ def cpu_baseline(x, steps, threads):
                       f"{dt:.1f} s"}
 
 
+def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
+    """256 independent lattice scenes, n_s log-uniform in [2048, 65536] (seed
+    4), each in its own [0,1]^3: all scenes in one pbf_batch_step call per
+    `steps` steps, against a loop of one pbf_step per scene and step."""
+    b = scene.batched_lattice_scenes(nscenes=256, nmin=2048, nmax=65536, seed=4)
+    dev = torch.device("cuda:0")
+    n = int(b.offsets[-1])
+    off = torch.from_numpy(b.offsets.astype(np.int32)).to(dev)
+    stream = torch.cuda.current_stream(dev)
+
+    def timed(fn, k):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        fn(k)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1)
+
+    xb = torch.from_numpy(b.pos).to(dev)
+    vb = torch.zeros_like(xb)
+    rb = torch.empty(n, dtype=torch.float32, device=dev)
+    cb = fluid.Context(dev)
+    cb.reserve(n, 256)
+    cb.batch_step(xb, vb, off, rb, 2)
+    ms_b = timed(lambda k: cb.batch_step(xb, vb, off, rb, k), steps)
+    cells = cb.stats()["cells_used"]
+
+    xs = torch.from_numpy(b.pos).to(dev)
+    vs = torch.zeros_like(xs)
+    rs = torch.empty(n, dtype=torch.float32, device=dev)
+    cs = fluid.Context(dev)
+    cs.reserve(int(b.counts.max()))
+    bounds = [(int(a), int(e)) for a, e in zip(b.offsets[:-1], b.offsets[1:])]
+
+    def serial(k):
+        for _ in range(k):
+            for a, e in bounds:
+                cs.step(xs[a:e], vs[a:e], rs[a:e], 1)
+
+    serial(1)
+    ms_s = timed(serial, serial_steps)
+    vb_ = n * steps / (ms_b / 1e3)
+    vs_ = n * serial_steps / (ms_s / 1e3)
+    return {"workload": "256 lattice scenes, n_s log-uniform [2048, 65536], seed 4", "scenes": 256,
+            "particles": n, "batched": {"value": round(vb_, 1), "unit": "particle-steps/s", "steps": steps,
+                                        "ms_per_step": round(ms_b / steps, 4), "cells_used": cells},
+            "serial": {"value": round(vs_, 1), "unit": "particle-steps/s", "steps": serial_steps,
+                       "ms_per_step": round(ms_s / serial_steps, 4)},
+            "speedup": round(vb_ / vs_, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +127,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workload", default="lattice_1m", choices=["lattice_1m", "blobs_1m"])
+    ap.add_argument("--batched-steps", type=int, default=20, help="steps of the 256-scene batched comparison")
+    ap.add_argument("--serial-steps", type=int, default=5, help="steps of its serial per-scene loop")
+    ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -126,7 +182,7 @@ def main():
 
     stats = ctx.stats()
     pairs = stats["neighbor_pairs"]
-    ncell = int(np.prod(stats["grid_dims"]))
+    ncell = int(stats["cells_used"])  # compact cells the sort touches (occupied window + bucket)
     iters = params["iterations"]
 
     # per-pass timing: the same number of steps again with HIP events around
@@ -174,6 +230,9 @@ def main():
         "step_roofline": whole_step,
         "wall_s": round(wall, 3),
     }
+    if not args.no_batched:
+        del x, v, rho, ctx
+        out["batched_vs_serial"] = batched_vs_serial(torch, fluid, pkg.scene, args.batched_steps, args.serial_steps)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(x_np, args.cpu_steps, threads)

@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 1
+#define PBF_ABI_VERSION 2
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -81,6 +81,9 @@ typedef struct pbf_stats {
                                     (bit-mask reservation full) */
   long long list_slots_needed;   /* neighbor-list slots the last search wanted */
   long long list_slots_reserved; /* neighbor-list slots reserved */
+  long long cells_used;          /* compact cells of the last search: the scenes'
+                                    occupied-cell windows plus one out-of-grid
+                                    bucket per scene */
   int grid_dims[3];              /* cells per axis (GridHash::setSpace) */
   float grid_lo[3];              /* grid origin = scene_lo - 2 cells */
   float cell_size;
@@ -114,6 +117,8 @@ int pbf_get_params(const pbf_ctx* ctx, pbf_params* out);
 
 /* Sizes the context's scratch for up to n particles. Blocking. */
 int pbf_reserve(pbf_ctx* ctx, int n);
+/* Same for a batch of up to nscenes scenes holding n particles in total. */
+int pbf_reserve_batch(pbf_ctx* ctx, int n, int nscenes);
 
 /* NeighborQuery::compute (NeighborQuery.cu:152-199): CSR neighbor lists of n
  * particles at d_x (original order, original ids, self included, strict
@@ -146,11 +151,41 @@ int pbf_pbd_constrain(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_offsets
  * integrate -> DensityPBD::constrain, fused. x and v (original order) are
  * updated in place, d_rho (may be NULL) receives the density of the last
  * iteration of the last step. Internally the particles are sorted by cell, the
- * neighbor sets are kept as bit masks over cell runs and the displacement is
- * gathered without atomics (deterministic: repeated runs give identical
- * bits). The neighbor-mask reservation grows between calls when a previous
+ * neighbor sets are kept as compact lists over cell runs and the displacement
+ * is gathered without atomics (deterministic: repeated runs give identical
+ * bits). The neighbor-storage reservations grow between calls when a previous
  * call needed more (read without blocking once that call has finished). */
 int pbf_step(pbf_ctx* ctx, float* d_x, float* d_v, float* d_rho, int n, int nsteps, void* stream);
+
+/* pbf_step on a batch of independent scenes that share the parameters (each
+ * scene in its own copy of the scene box; the reference runs one
+ * PositionBasedFluidModel per scene). Scene s holds particles
+ * d_scene_offsets[s] .. d_scene_offsets[s+1]-1 of x, v and rho
+ * (d_scene_offsets: nscenes+1 non-decreasing device ints, first 0, last n).
+ * Particles of different scenes never interact; every scene's result is
+ * bit-identical to a pbf_step of that scene alone. All scenes advance in one
+ * launch per pass. No counterpart in the reference, which has no batching. */
+int pbf_batch_step(pbf_ctx* ctx, float* d_x, float* d_v, float* d_rho, const int* d_scene_offsets,
+                   int nscenes, int n, int nsteps, void* stream);
+
+/* Reverse mode (not part of the reference, which has no derivatives).
+ *
+ * pbf_rollout_forward runs nsteps of pbf_batch_step (d_scene_offsets may be
+ * NULL for one scene, nscenes = 1) on x, v in place and keeps the state at
+ * every checkpoint_every-th step in the context. pbf_rollout_backward then
+ * takes the adjoint of the final state (d_xbar, d_vbar: dL/dx_N, dL/dv_N,
+ * original order) and overwrites it with the adjoint of the initial state
+ * (dL/dx_0, dL/dv_0). Each segment between checkpoints is recomputed, every
+ * step of it re-run with its PBD iterates kept and its neighbor sets rebuilt,
+ * and differentiated by hand-written adjoint kernels with the neighbor sets
+ * held fixed (the step is C1 across a set change: W and the kernel gradient
+ * vanish with their first derivative at r = h). The rollout stays valid for
+ * repeated backward calls until the next pbf_rollout_forward or a parameter
+ * change (PBF_ERR_INVALID). Returns PBF_ERR_UNSUPPORTED when a step had
+ * particles outside the grid. pbf_rollout_backward is blocking. */
+int pbf_rollout_forward(pbf_ctx* ctx, float* d_x, float* d_v, float* d_rho, const int* d_scene_offsets,
+                        int nscenes, int n, int nsteps, int checkpoint_every, void* stream);
+int pbf_rollout_backward(pbf_ctx* ctx, float* d_xbar, float* d_vbar, void* stream);
 
 /* CSR export of the neighbor lists built by the last pbf_step (its last step,
  * on the positions before that step's integration), original ids; same

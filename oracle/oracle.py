@@ -89,6 +89,8 @@ def lib():
             "orc_integrate_d": (None, [PD, dp, dp, dp, C.c_int]),
             "orc_step_f": (C.c_longlong, [PF, fp, fp, fp, C.c_int, C.c_int]),
             "orc_step_d": (C.c_longlong, [PD, dp, dp, dp, C.c_int, C.c_int]),
+            "orc_rollout_adjoint_f": (None, [PF, fp, fp, C.c_int, C.c_int, fp, fp]),
+            "orc_rollout_adjoint_d": (None, [PD, dp, dp, C.c_int, C.c_int, dp, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -210,3 +212,19 @@ def step(params, x, v, nsteps=1):
     fn = lib().orc_step_d if dbl else lib().orc_step_f
     pairs = fn(C.byref(params), _ptr(x, ct), _ptr(v, ct), _ptr(rho, ct), len(x), int(nsteps))
     return x, v, rho, pairs
+
+
+def rollout_adjoint(params, x0, v0, nsteps, xbar, vbar):
+    """Adjoint of nsteps steps from (x0, v0) with the neighbor sets held
+    fixed: given the adjoint (xbar, vbar) of the final state, returns the
+    adjoint of (x0, v0)."""
+    dbl = isinstance(params, ParamsD)
+    dt = np.float64 if dbl else np.float32
+    ct = C.c_double if dbl else C.c_float
+    x0 = np.array(x0, dtype=dt, copy=True, order="C")
+    v0 = np.array(v0, dtype=dt, copy=True, order="C")
+    xb = np.array(xbar, dtype=dt, copy=True, order="C")
+    vb = np.array(vbar, dtype=dt, copy=True, order="C")
+    fn = lib().orc_rollout_adjoint_d if dbl else lib().orc_rollout_adjoint_f
+    fn(C.byref(params), _ptr(x0, ct), _ptr(v0, ct), len(x0), int(nsteps), _ptr(xb, ct), _ptr(vb, ct))
+    return xb, vb

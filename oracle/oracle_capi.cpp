@@ -149,4 +149,17 @@ long long orc_step_d(const Params<double>* p, double* x, double* v, double* rho,
   return run_steps(p, x, v, rho, n, nsteps);
 }
 
+// Adjoint of nsteps steps from (x0, v0): xbar, vbar hold the adjoint of the
+// final state on entry and of the initial state on return.
+void orc_rollout_adjoint_d(const Params<double>* p, const double* x0, const double* v0, int n, int nsteps,
+                           double* xbar, double* vbar) {
+  rollout_adjoint(*p, reinterpret_cast<const V3<double>*>(x0), reinterpret_cast<const V3<double>*>(v0), n, nsteps,
+                  reinterpret_cast<V3<double>*>(xbar), reinterpret_cast<V3<double>*>(vbar));
+}
+void orc_rollout_adjoint_f(const Params<float>* p, const float* x0, const float* v0, int n, int nsteps,
+                           float* xbar, float* vbar) {
+  rollout_adjoint(*p, reinterpret_cast<const V3<float>*>(x0), reinterpret_cast<const V3<float>*>(v0), n, nsteps,
+                  reinterpret_cast<V3<float>*>(xbar), reinterpret_cast<V3<float>*>(vbar));
+}
+
 }  // extern "C"

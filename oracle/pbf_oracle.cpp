@@ -374,6 +374,169 @@ void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out) {
   pbd_constrain(q, x, v, n, nb, rho);
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of one step (not in the reference). Forward quantities of every
+// PBD iteration are recomputed in the order of pbd_constrain; the reverse
+// sweep treats each listed ordered pair (i, j) as the reference's scatter
+// does: dp_ij = corr (lambda_i + lambda_j) g(x_i - x_j) is added to i and
+// subtracted from j, with g(d) = Gradient(|d|) d / |d|.
+
+// g(d) = phi(r) d/r, phi = Gradient (Kernel.h:47-57); returns J(d) w with
+// J = dg/dd = phi'(r) dh dh^T + phi/r (I - dh dh^T), dh = d/r.
+template <class R>
+static inline V3<R> jac_g(const V3<R>& d, R r, R h, const V3<R>& w) {
+  const R q = r / h;
+  if (q > (R)1) return {0, 0, 0};
+  const R c = (R)-45 / ((R)kPi * h * h * h);
+  const R phi = c * (1 - q) * (1 - q);
+  const R dphi = c * (R)-2 * (1 - q) / h;
+  const R ir = (R)1 / r;
+  const V3<R> dh{d.x * ir, d.y * ir, d.z * ir};
+  const R dw = dh.x * w.x + dh.y * w.y + dh.z * w.z;
+  const R a = phi * ir, b = (dphi - phi * ir) * dw;
+  return {a * w.x + b * dh.x, a * w.y + b * dh.y, a * w.z + b * dh.z};
+}
+
+// d/dd of W(|d|) (Kernel.h:36-45): W'(r) d/r, zero at d = 0.
+template <class R> static inline V3<R> grad_w(const V3<R>& d, R r, R h) {
+  const R q = r / h;
+  if (q > (R)1 || !(r > (R)0)) return {0, 0, 0};
+  const R dw = (R)-45 / ((R)kPi * h * h * h * h) * (1 - q) * (1 - q);
+  return {dw * d.x / r, dw * d.y / r, dw * d.z / r};
+}
+
+template <class R>
+void step_adjoint(const Params<R>& p, const V3<R>* x, const V3<R>* v, int n, V3<R>* xbar, V3<R>* vbar) {
+  const Grid<R> g = grid_setup(p.h, p.lo, p.hi);
+  CSR nb;
+  neighbor_query(g, x, n, p.h, nb);
+  const int K = p.iters;
+  const R mass = scaling_factor(p.d, p.h) * (p.rho0 * (p.d * p.d * p.d));
+  // forward: integrate, then the PBD iterations with their per-iteration state
+  std::vector<V3<R>> xs(x, x + n), vs(v, v + n);
+  integrate<R>(p, xs.data(), vs.data(), nullptr, n);
+  std::vector<std::vector<V3<R>>> P(K + 1), G(K);
+  std::vector<std::vector<R>> rho(K), lamu(K), S(K);
+  P[0] = xs;
+  Params<R> q = p;
+  q.dx_mode = 2;  // the scatter form, valid for any list
+  for (int k = 0; k < K; ++k) {
+    const V3<R>* pk = P[k].data();
+    rho[k].assign(n, 0);
+    lamu[k].assign(n, 0);
+    S[k].assign(n, 0);
+    G[k].assign(n, {0, 0, 0});
+    density(pk, n, nb, p.h, mass, rho[k].data());
+    std::vector<R> lam(n);
+    for (int i = 0; i < n; ++i) {
+      R s = 0;
+      V3<R> gi{0, 0, 0};
+      for (int e = nb.offsets[i]; e < nb.offsets[i + 1]; ++e) {
+        const V3<R> d = sub(pk[i], pk[nb.elems[e]]);
+        const R r = norm(d);
+        if (!(r > p.pair_eps)) continue;
+        const R gr = spiky_grad(r, p.h);
+        const V3<R> gv{d.x * gr / r, d.y * gr / r, d.z * gr / r};
+        gi.x += gv.x;
+        gi.y += gv.y;
+        gi.z += gv.z;
+        s += dot(gv, gv);
+      }
+      s += dot(gi, gi);
+      S[k][i] = s;
+      G[k][i] = gi;
+      lamu[k][i] = -(rho[k][i] - p.lam_rest) / (s + p.lam_eps);
+      lam[i] = lamu[k][i] > 0 ? 0 : lamu[k][i];
+    }
+    std::vector<V3<R>> dpos(n);
+    displacement_atomic_order(q, lam.data(), pk, n, nb, dpos.data());
+    P[k + 1] = P[k];
+    for (int i = 0; i < n; ++i) {
+      P[k + 1][i].x += dpos[i].x;
+      P[k + 1][i].y += dpos[i].y;
+      P[k + 1][i].z += dpos[i].z;
+    }
+  }
+  // reverse: v' = v* + (P_K - x*)/dt, x' = P_K
+  std::vector<V3<R>> a(n), xsb(n), vsb(n);
+  for (int i = 0; i < n; ++i) {
+    a[i] = {xbar[i].x + vbar[i].x / p.dt, xbar[i].y + vbar[i].y / p.dt, xbar[i].z + vbar[i].z / p.dt};
+    xsb[i] = {-vbar[i].x / p.dt, -vbar[i].y / p.dt, -vbar[i].z / p.dt};
+    vsb[i] = vbar[i];
+  }
+  for (int k = K - 1; k >= 0; --k) {
+    const V3<R>* pk = P[k].data();
+    std::vector<R> lam(n), lamb(n, 0), rhob(n, 0), Sb(n, 0);
+    for (int i = 0; i < n; ++i) lam[i] = lamu[k][i] > 0 ? 0 : lamu[k][i];
+    // lambda adjoint: each pair's term (a_i - a_j) . dp_ij
+    for (int i = 0; i < n; ++i)
+      for (int e = nb.offsets[i]; e < nb.offsets[i + 1]; ++e) {
+        const int j = nb.elems[e];
+        const V3<R> d = sub(pk[i], pk[j]);
+        const R r = norm(d);
+        if (!(r > p.pair_eps)) continue;
+        const R gr = spiky_grad(r, p.h);
+        const V3<R> da = sub(a[i], a[j]);
+        const R t = p.corr * gr / r * dot(da, d);
+        lamb[i] += t;
+        lamb[j] += t;
+      }
+    for (int i = 0; i < n; ++i) {
+      if (!(lamu[k][i] < 0)) continue;  // lambda clamped to 0
+      const R den = S[k][i] + p.lam_eps;
+      rhob[i] = -lamb[i] / den;
+      Sb[i] = lamb[i] * (rho[k][i] - p.lam_rest) / (den * den);
+    }
+    std::vector<V3<R>> pb(a);
+    for (int i = 0; i < n; ++i)
+      for (int e = nb.offsets[i]; e < nb.offsets[i + 1]; ++e) {
+        const int j = nb.elems[e];
+        const V3<R> d = sub(pk[i], pk[j]);
+        const R r = norm(d);
+        const V3<R> gw = grad_w(d, r, p.h);
+        V3<R> db{rhob[i] * mass * gw.x, rhob[i] * mass * gw.y, rhob[i] * mass * gw.z};
+        if (r > p.pair_eps) {
+          const R gr = spiky_grad(r, p.h);
+          const V3<R> gv{d.x * gr / r, d.y * gr / r, d.z * gr / r};
+          const R cl = p.corr * (lam[i] + lam[j]);
+          const V3<R> w{cl * (a[i].x - a[j].x) + 2 * Sb[i] * (gv.x + G[k][i].x),
+                        cl * (a[i].y - a[j].y) + 2 * Sb[i] * (gv.y + G[k][i].y),
+                        cl * (a[i].z - a[j].z) + 2 * Sb[i] * (gv.z + G[k][i].z)};
+          const V3<R> jw = jac_g(d, r, p.h, w);
+          db.x += jw.x;
+          db.y += jw.y;
+          db.z += jw.z;
+        }
+        pb[i].x += db.x;
+        pb[i].y += db.y;
+        pb[i].z += db.z;
+        pb[j].x -= db.x;
+        pb[j].y -= db.y;
+        pb[j].z -= db.z;
+      }
+    a.swap(pb);
+  }
+  // x* = x + v* dt, v* = v + g dt
+  for (int i = 0; i < n; ++i) {
+    const V3<R> xb{xsb[i].x + a[i].x, xsb[i].y + a[i].y, xsb[i].z + a[i].z};
+    xbar[i] = xb;
+    vbar[i] = {vsb[i].x + xb.x * p.dt, vsb[i].y + xb.y * p.dt, vsb[i].z + xb.z * p.dt};
+  }
+}
+
+template <class R>
+void rollout_adjoint(const Params<R>& p, const V3<R>* x0, const V3<R>* v0, int n, int nsteps, V3<R>* xbar,
+                     V3<R>* vbar) {
+  std::vector<std::vector<V3<R>>> X(nsteps), Vv(nsteps);
+  std::vector<V3<R>> x(x0, x0 + n), v(v0, v0 + n);
+  for (int s = 0; s < nsteps; ++s) {
+    X[s] = x;
+    Vv[s] = v;
+    step(p, x.data(), v.data(), (R*)nullptr, n, (CSR*)nullptr);
+  }
+  for (int s = nsteps - 1; s >= 0; --s) step_adjoint(p, X[s].data(), Vv[s].data(), n, xbar, vbar);
+}
+
 #define PBF_ORACLE_INSTANTIATE(R)                                                                  \
   template Grid<R> grid_setup<R>(R, const R*, const R*);                                           \
   template void cell3<R>(const Grid<R>&, const V3<R>&, int*);                                      \
@@ -390,7 +553,10 @@ void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out) {
                                              const CSR&, V3<R>*);                                  \
   template void pbd_constrain<R>(const Params<R>&, V3<R>*, V3<R>*, int, const CSR&, R*);           \
   template void integrate<R>(const Params<R>&, V3<R>*, V3<R>*, const V3<R>*, int);                 \
-  template void step<R>(const Params<R>&, V3<R>*, V3<R>*, R*, int, CSR*);
+  template void step<R>(const Params<R>&, V3<R>*, V3<R>*, R*, int, CSR*);                         \
+  template void step_adjoint<R>(const Params<R>&, const V3<R>*, const V3<R>*, int, V3<R>*, V3<R>*); \
+  template void rollout_adjoint<R>(const Params<R>&, const V3<R>*, const V3<R>*, int, int, V3<R>*,  \
+                                   V3<R>*);
 
 PBF_ORACLE_INSTANTIATE(float)
 PBF_ORACLE_INSTANTIATE(double)

@@ -26,6 +26,9 @@
 //                     closed form
 //   integrator        free fall closed form (bit-exact)
 //   step              committed golden fixtures (regression pins only)
+//   step_adjoint,     FP64 central finite differences of step()/rollouts on
+//   rollout_adjoint   small lattices (compressed, so that lambda is active),
+//                     and the closed form of the free-fall adjoint
 #pragma once
 #include <cstdint>
 #include <vector>
@@ -94,5 +97,20 @@ template <class R>
 void integrate(const Params<R>& p, V3<R>* x, V3<R>* v, const V3<R>* f, int n);
 template <class R>
 void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out);
+
+// Reverse-mode derivative of step() with the neighbor sets held fixed (they
+// are piecewise constant in x; W and the kernel gradient vanish with their
+// first derivative at r = h, so the step is C1 across a set change). x, v:
+// the state before the step. xbar, vbar: in, the adjoint of the state after
+// the step; out, the adjoint of x, v. Pairwise scatter over the listed
+// ordered pairs, valid for symmetric and asymmetric lists alike. Not part of
+// the reference (which has no derivatives).
+template <class R>
+void step_adjoint(const Params<R>& p, const V3<R>* x, const V3<R>* v, int n, V3<R>* xbar, V3<R>* vbar);
+// nsteps of step() from (x0, v0) kept in memory, then step_adjoint in
+// reverse. xbar, vbar as for step_adjoint (final state in, initial out).
+template <class R>
+void rollout_adjoint(const Params<R>& p, const V3<R>* x0, const V3<R>* v0, int n, int nsteps, V3<R>* xbar,
+                     V3<R>* vbar);
 
 }  // namespace pbf_oracle

@@ -50,6 +50,7 @@ class pbf_stats(C.Structure):
         ("mask_fallback_waves", C.c_longlong),
         ("list_slots_needed", C.c_longlong),
         ("list_slots_reserved", C.c_longlong),
+        ("cells_used", C.c_longlong),
         ("grid_dims", C.c_int * 3),
         ("grid_lo", C.c_float * 3),
         ("cell_size", C.c_float),
@@ -78,11 +79,15 @@ SIGNATURES = {
     "pbf_set_params": (C.c_int, [_CTX, C.POINTER(pbf_params)]),
     "pbf_get_params": (C.c_int, [_CTX, C.POINTER(pbf_params)]),
     "pbf_reserve": (C.c_int, [_CTX, C.c_int]),
+    "pbf_reserve_batch": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "pbf_neighbor_query": (C.c_int, [_CTX, _VP, C.c_int, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
     "pbf_integrate": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, _VP]),
     "pbf_density": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, _VP]),
     "pbf_pbd_constrain": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
     "pbf_step": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, C.c_int, _VP]),
+    "pbf_batch_step": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP]),
+    "pbf_rollout_forward": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
+    "pbf_rollout_backward": (C.c_int, [_CTX, _VP, _VP, _VP]),
     "pbf_step_neighbors": (C.c_int, [_CTX, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
     "pbf_get_stats": (C.c_int, [_CTX, C.POINTER(pbf_stats)]),
     "pbf_set_profiling": (C.c_int, [_CTX, C.c_int]),

@@ -1,0 +1,152 @@
+// Adjoint (reverse-mode derivative) of one fluid step with the neighbor sets
+// held fixed. Not part of the reference, which has no derivatives; the
+// forward it differentiates is PositionBasedFluidModel::step without
+// viscosity (PositionBasedFluidModel.cpp:83-101): integrate, K Jacobi
+// iterations of DensityPBD (DensityPBD.cu:306-365), velocity update
+// (DensityPBD.cu:368-378).
+//
+// Forward in the terms of the gather passes (all particles in the grid, so
+// every listed pair is listed both ways):
+//   u_ij   = t_ij^2 / r_ij  (0 when r_ij <= eps),  g_ij = cG u_ij d_ij,
+//   rho_i  = cW sum_j t_ij^3,  G_i = sum_j g_ij,  S_i = sum_j |g_ij|^2 + |G_i|^2,
+//   lam_i  = min(0, -(rho_i - rho0) / (S_i + eps)),
+//   p'_i   = p_i + C sum_j (lam_i + lam_j) g_ij,   C = 2 corr.
+// Reverse of one iteration, a = adjoint of p':
+//   lam_bar_i = C sum_j (a_i - a_j) . g_ij
+//   rho_bar_i = -lam_bar_i / (S_i + eps),  S_bar_i = -lam_bar_i lam_i / (S_i + eps)
+//               (both 0 where lam_i was clamped)
+//   p_bar_i   = a_i + sum_j [ J(d_ij) w_ij + (rho_bar_i + rho_bar_j) psi(d_ij) ]
+//   w_ij      = C (lam_i + lam_j)(a_i - a_j) + 2 S_bar_i (g_ij + G_i)
+//               + 2 (S_bar_j g_ij - S_bar_j G_j)
+//   J(d) w    = cG [u w - (2 t/h + t^2/r) (d.w) d / r^2]   (Jacobian of g, symmetric)
+//   psi(d)    = -3 cW t^2 d / (h r)                         (gradient of cW t^3)
+// (g and psi are odd in d and J even, which turns every pair's scatter onto
+// j into the gathered terms above.) Integration and velocity update:
+//   x* = x + v* dt, v* = v + g dt, v' = v* + (p_K - x*)/dt, x' = p_K.
+#include "pbf_nbr.h"
+
+namespace pbf {
+
+struct F4x2 {
+  float4 a, b;
+};
+struct F4x3 {
+  float4 a, b, c;
+};
+
+// a = x_bar' + v_bar'/dt (adjoint of p_K), x*_bar = -v_bar'/dt, v*_bar = v_bar'.
+__global__ __launch_bounds__(kBlock) void k_adj_begin(int n, const int* __restrict__ perm,
+                                                      const float* __restrict__ xbar, const float* __restrict__ vbar,
+                                                      float dt, float4* __restrict__ a, float4* __restrict__ xsb,
+                                                      float4* __restrict__ vsb,
+                                                      const unsigned long long* __restrict__ counters,
+                                                      int* __restrict__ flag) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0 && counters[C_OOG] != 0) *flag = 1;
+  if (s >= n) return;
+  const size_t i = (size_t)perm[s];
+  const float4 xb = load3(xbar, (int)i), vb = load3(vbar, (int)i);
+  a[s] = make_float4(xb.x + vb.x / dt, xb.y + vb.y / dt, xb.z + vb.z / dt, 0.0f);
+  xsb[s] = make_float4(-vb.x / dt, -vb.y / dt, -vb.z / dt, 0.0f);
+  vsb[s] = vb;
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_lambda(NbrView nv, const float4* __restrict__ P,
+                                                       const float4* __restrict__ aux, const float4* __restrict__ a,
+                                                       float4* __restrict__ q1, float4* __restrict__ q2,
+                                                       PhysConst pc) {
+  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= nv.n) return;
+  const float4 ps = P[s], as = a[s];
+  float acc = 0.0f;
+  for_each_neighbor(
+      nv, s, [&](int j) { return F4x2{P[j], a[j]}; },
+      [&](int, const F4x2& q) {
+        const PairGeom g = pair_geom(ps, q.a, pc.inv_h);
+        const float u = g.r > pc.pair_eps ? g.t * g.t * g.rinv : 0.0f;
+        acc += u * ((as.x - q.b.x) * g.dx + (as.y - q.b.y) * g.dy + (as.z - q.b.z) * g.dz);
+      });
+  const float lam_bar = (2.0f * pc.corr * pc.cG) * acc;
+  const float4 ax = aux[s];
+  const bool active = ps.w < 0.0f;
+  const float rho_bar = active ? -lam_bar * ax.w : 0.0f;
+  const float s_bar = active ? -lam_bar * ps.w * ax.w : 0.0f;
+  q1[s] = make_float4(as.x, as.y, as.z, rho_bar);
+  q2[s] = make_float4(s_bar * pc.cG * ax.x, s_bar * pc.cG * ax.y, s_bar * pc.cG * ax.z, s_bar);
+}
+
+__global__ __launch_bounds__(kBlock) void k_adj_pos(NbrView nv, const float4* __restrict__ P,
+                                                    const float4* __restrict__ aux, const float4* __restrict__ q1,
+                                                    const float4* __restrict__ q2, float4* __restrict__ a_out,
+                                                    PhysConst pc) {
+  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s >= nv.n) return;
+  const float4 ps = P[s], q1s = q1[s], q2s = q2[s], ax = aux[s];
+  const float Gx = pc.cG * ax.x, Gy = pc.cG * ax.y, Gz = pc.cG * ax.z;
+  const float C = 2.0f * pc.corr;
+  const float cpsi = -3.0f * pc.cW * pc.inv_h;
+  float bx = q1s.x, by = q1s.y, bz = q1s.z;
+  for_each_neighbor(
+      nv, s, [&](int j) { return F4x3{P[j], q1[j], q2[j]}; },
+      [&](int, const F4x3& q) {
+        const PairGeom g = pair_geom(ps, q.a, pc.inv_h);
+        const float t2 = g.t * g.t;
+        // density term, every listed pair (zero for the self pair: d = 0)
+        const float cr = (q1s.w + q.b.w) * cpsi * t2 * g.rinv;
+        bx += cr * g.dx;
+        by += cr * g.dy;
+        bz += cr * g.dz;
+        if (g.r > pc.pair_eps) {
+          const float u = t2 * g.rinv;
+          const float cg = pc.cG * u;
+          const float gx = cg * g.dx, gy = cg * g.dy, gz = cg * g.dz;
+          const float cl = C * (ps.w + q.a.w);
+          const float wx = cl * (q1s.x - q.b.x) + 2.0f * q2s.w * (gx + Gx) + 2.0f * (q.c.w * gx - q.c.x);
+          const float wy = cl * (q1s.y - q.b.y) + 2.0f * q2s.w * (gy + Gy) + 2.0f * (q.c.w * gy - q.c.y);
+          const float wz = cl * (q1s.z - q.b.z) + 2.0f * q2s.w * (gz + Gz) + 2.0f * (q.c.w * gz - q.c.z);
+          const float dw = g.dx * wx + g.dy * wy + g.dz * wz;
+          const float k2 = (2.0f * g.t * pc.inv_h + u) * g.rinv * g.rinv * dw;
+          bx += pc.cG * (u * wx - k2 * g.dx);
+          by += pc.cG * (u * wy - k2 * g.dy);
+          bz += pc.cG * (u * wz - k2 * g.dz);
+        }
+      });
+  a_out[s] = make_float4(bx, by, bz, 0.0f);
+}
+
+// x_bar = x*_bar + a, v_bar = v*_bar + x_bar dt, back to original order.
+__global__ __launch_bounds__(kBlock) void k_adj_end(int n, const int* __restrict__ perm,
+                                                    const float4* __restrict__ a, const float4* __restrict__ xsb,
+                                                    const float4* __restrict__ vsb, float dt,
+                                                    float* __restrict__ xbar, float* __restrict__ vbar) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const size_t i = (size_t)perm[s];
+  const float4 aa = a[s], xb = xsb[s], vb = vsb[s];
+  const float x0 = xb.x + aa.x, x1 = xb.y + aa.y, x2 = xb.z + aa.z;
+  xbar[3 * i] = x0;
+  xbar[3 * i + 1] = x1;
+  xbar[3 * i + 2] = x2;
+  vbar[3 * i] = vb.x + x0 * dt;
+  vbar[3 * i + 1] = vb.y + x1 * dt;
+  vbar[3 * i + 2] = vb.z + x2 * dt;
+}
+
+void launch_adjoint_step(const NbrView& nv, float4* const* P, float4* const* aux, int K, const PhysConst& pc,
+                         const int* perm, float* xbar, float* vbar, const AdjBufs& ab, hipStream_t st) {
+  const int n = nv.n;
+  const int nb = (n + kBlock - 1) / kBlock;
+  k_adj_begin<<<nb, kBlock, 0, st>>>(n, perm, xbar, vbar, pc.dt, ab.a, ab.xsb, ab.vsb, nv.counters, ab.flag);
+  float4* a = ab.a;
+  float4* a2 = ab.a2;
+  for (int k = K - 1; k >= 0; --k) {
+    k_adj_lambda<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], a, ab.q1, ab.q2, pc);
+    k_adj_pos<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.q1, ab.q2, a2, pc);
+    float4* t = a;
+    a = a2;
+    a2 = t;
+  }
+  k_adj_end<<<nb, kBlock, 0, st>>>(n, perm, a, ab.xsb, ab.vsb, pc.dt, xbar, vbar);
+}
+
+}  // namespace pbf

@@ -18,19 +18,28 @@ struct pbf_ctx {
   pbf_params p{};
   GridDev g{};
   PhysConst pc{};
+  long long full_cells = 0;  // nx*ny*nz of one scene's grid
   float m_factor = 0.f, mass = 0.f;
   int device = 0;
-  hipStream_t last_stream = nullptr;
 
-  // grid arrays
-  int* cell_cnt = nullptr;    // ncell + 1
-  int* cell_start = nullptr;  // ncell + 2
+  // scenes
+  int scene_cap = 0;
+  int* off1 = nullptr;          // offsets {0, n} of a one-scene call
+  int* scene_minmax = nullptr;  // 6 per scene
+  SceneWin* win = nullptr;
+  int* total_cells = nullptr;
+  int* scan_part = nullptr;     // kScanDevBlocks + 1
+
+  // compact cell arrays
+  long long cell_cap = 0;
+  int* cell_cnt = nullptr;    // cell_cap + 1
+  int* cell_start = nullptr;  // cell_cap + 2
   uint8_t* oog_flag = nullptr;
-  int grid_cells = -1;  // ncell the grid arrays were sized for
 
   // particle arrays (capacity cap_n)
   int cap_n = 0;
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *inv = nullptr;
+  int *sid = nullptr, *ssid = nullptr;
   int *nbr_cnt = nullptr, *cnt_orig = nullptr;
   float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr;
   float* dpos = nullptr;
@@ -46,6 +55,27 @@ struct pbf_ctx {
 
   int* scan_scratch = nullptr;
   size_t scan_scratch_n = 0;
+
+  // rollout with checkpoints (reverse mode): states every `every` steps,
+  // the states of one segment, and the tape of one step
+  struct Rollout {
+    bool valid = false;
+    int n = 0, nscenes = 0, nsteps = 0, every = 0, iters = 0;
+    pbf_params p{};
+    int* off = nullptr;  // nscenes + 1
+    int off_cap = 0;
+    float* ckpt = nullptr;  // per checkpoint: x (3n) then v (3n)
+    size_t ckpt_cap = 0;    // floats
+    float* seg = nullptr;   // per step of a segment: x (3n) then v (3n)
+    size_t seg_cap = 0;
+    float4* tape = nullptr;  // K-1 iterates (iterate 0 is x0) then K aux arrays, cap_n each
+    size_t tape_cap = 0;     // float4
+    float4* adj = nullptr;   // q1, q2, xsb, vsb (cap_n each)
+    int adj_n = 0;
+    float* out = nullptr;    // x, v written by a taped step (discarded)
+    int out_n = 0;
+    int* flag = nullptr;
+  } roll;
 
   unsigned long long* counters = nullptr;     // device, C_COUNT
   unsigned long long* h_counters = nullptr;   // pinned host copy
@@ -86,7 +116,8 @@ int fail(pbf_ctx* c, int status, const std::string& msg) {
 #define PBF_LAUNCHED(ctx)                                                                            \
   do {                                                                                               \
     hipError_t e_ = hipGetLastError();                                                               \
-    if (e_ != hipSuccess) return fail(ctx, PBF_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_)); \
+    if (e_ != hipSuccess)                                                                            \
+      return fail(ctx, PBF_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e_));         \
   } while (0)
 
 template <class T> void dfree(T*& p) {
@@ -162,7 +193,8 @@ int validate(const pbf_params* p, std::string& why) {
   return PBF_OK;
 }
 
-// GridHash::setSpace (Framework/Topology/GridHash.cu:47-75).
+// GridHash::setSpace (Framework/Topology/GridHash.cu:47-75) and the constants
+// of the passes.
 int derive(pbf_ctx* c, const pbf_params* p) {
 #pragma clang fp contract(off)
   GridDev g{};
@@ -182,7 +214,6 @@ int derive(pbf_ctx* c, const pbf_params* p) {
   g.nx = (int)n[0];
   g.ny = (int)n[1];
   g.nz = (int)n[2];
-  g.ncell = (int)ncell;
 
   PhysConst pc{};
   const float h = p->smoothing_length;
@@ -205,6 +236,7 @@ int derive(pbf_ctx* c, const pbf_params* p) {
     pc.gdt[a] = (0.0f + p->gravity[a]) * p->dt;
   }
   c->g = g;
+  c->full_cells = ncell;
   c->pc = pc;
   c->m_factor = m_factor;
   c->mass = mass;
@@ -212,17 +244,31 @@ int derive(pbf_ctx* c, const pbf_params* p) {
   return PBF_OK;
 }
 
-int ensure_grid(pbf_ctx* c) {
-  if (c->grid_cells == c->g.ncell) return PBF_OK;
-  dfree(c->cell_cnt);
-  dfree(c->cell_start);
-  dfree(c->oog_flag);
-  c->grid_cells = -1;
-  PBF_HIP(c, hipMalloc(&c->cell_cnt, sizeof(int) * ((size_t)c->g.ncell + 1)));
-  PBF_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * ((size_t)c->g.ncell + 2)));
-  PBF_HIP(c, hipMalloc(&c->oog_flag, (size_t)c->g.ncell + 1));
-  c->grid_cells = c->g.ncell;
-  c->last_n = -1;
+// Scene arrays and the compact cell arrays for up to nscenes scenes (each
+// scene's window is at most the whole grid, plus one out-of-grid bucket).
+int ensure_scenes(pbf_ctx* c, int nscenes) {
+  const long long need_cells = (long long)nscenes * (c->full_cells + 1);
+  if (need_cells > (long long)INT32_MAX - 2)
+    return fail(c, PBF_ERR_UNSUPPORTED, "scenes x grid cells exceed 2^31");
+  if (nscenes > c->scene_cap) {
+    dfree(c->scene_minmax);
+    dfree(c->win);
+    c->scene_cap = 0;
+    PBF_HIP(c, hipMalloc(&c->scene_minmax, sizeof(int) * 6 * (size_t)nscenes));
+    PBF_HIP(c, hipMalloc(&c->win, sizeof(SceneWin) * (size_t)nscenes));
+    c->scene_cap = nscenes;
+  }
+  if (need_cells > c->cell_cap) {
+    dfree(c->cell_cnt);
+    dfree(c->cell_start);
+    dfree(c->oog_flag);
+    c->cell_cap = 0;
+    PBF_HIP(c, hipMalloc(&c->cell_cnt, sizeof(int) * ((size_t)need_cells + 1)));
+    PBF_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * ((size_t)need_cells + 2)));
+    PBF_HIP(c, hipMalloc(&c->oog_flag, (size_t)need_cells + 1));
+    c->cell_cap = need_cells;
+    c->last_n = -1;
+  }
   return PBF_OK;
 }
 
@@ -238,14 +284,12 @@ int ensure_scan_scratch(pbf_ctx* c, long long n) {
 
 long long mask_words_for(const pbf_ctx* c, long long n) {
   const long long per = c->p.mask_words > 0 ? c->p.mask_words : 32;
-  const long long waves = (n + 63) / 64;
-  return waves * 64 * per;
+  return (n + 63) / 64 * 64 * per;
 }
 
 long long list_slots_for(const pbf_ctx* c, long long n) {
   const long long per = c->p.list_slots > 0 ? c->p.list_slots : 96;
-  const long long waves = (n + 63) / 64;
-  return waves * 64 * per;
+  return (n + 63) / 64 * 64 * per;
 }
 
 int ensure_mask(pbf_ctx* c, long long words) {
@@ -266,15 +310,15 @@ int ensure_list(pbf_ctx* c, long long slots) {
   return PBF_OK;
 }
 
-int ensure_particles(pbf_ctx* c, int n) {
-  int st = ensure_grid(c);
+int ensure_particles(pbf_ctx* c, int n, int nscenes) {
+  int st = ensure_scenes(c, nscenes);
   if (st) return st;
-  st = ensure_scan_scratch(c, std::max<long long>((long long)c->g.ncell + 1, (long long)n + 1));
+  st = ensure_scan_scratch(c, (long long)n + 1);
   if (st) return st;
   if (n <= c->cap_n) return PBF_OK;
   const int cap = std::max(((n + 255) / 256) * 256, 256);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
-  dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->sid); dfree(c->ssid); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   c->cap_n = 0;
@@ -284,6 +328,8 @@ int ensure_particles(pbf_ctx* c, int n) {
   PBF_HIP(c, hipMalloc(&c->tmp, ni));
   PBF_HIP(c, hipMalloc(&c->perm, ni));
   PBF_HIP(c, hipMalloc(&c->inv, ni));
+  PBF_HIP(c, hipMalloc(&c->sid, ni));
+  PBF_HIP(c, hipMalloc(&c->ssid, ni));
   PBF_HIP(c, hipMalloc(&c->nbr_cnt, ni));
   PBF_HIP(c, hipMalloc(&c->cnt_orig, ni));
   PBF_HIP(c, hipMalloc(&c->xn, nf4 + sizeof(float4) * kXnPad));
@@ -308,8 +354,8 @@ int ensure_particles(pbf_ctx* c, int n) {
 }
 
 // Grows the mask and list reservations when the last finished call wanted
-// more; never blocks (the counters are copied asynchronously at the end of each
-// call).
+// more; never blocks (the counters are copied asynchronously at the end of
+// each call).
 int adapt_reservations(pbf_ctx* c) {
   if (!c->counters_pending) return PBF_OK;
   if (hipEventQuery(c->counters_ev) != hipSuccess) return PBF_OK;
@@ -325,6 +371,8 @@ int adapt_reservations(pbf_ctx* c) {
 NbrView view_of(const pbf_ctx* c, int n) {
   NbrView nv;
   nv.g = c->g;
+  nv.win = c->win;
+  nv.ssid = c->ssid;
   nv.cell_start = c->cell_start;
   nv.xn = c->xn;
   nv.nbr_cnt = c->nbr_cnt;
@@ -335,6 +383,7 @@ NbrView view_of(const pbf_ctx* c, int n) {
   nv.mask_base = c->mask_base;
   nv.mask = c->mask;
   nv.oog_flag = c->oog_flag;
+  nv.counters = c->counters;
   nv.thr2 = c->pc.thr2;
   nv.n = n;
   return nv;
@@ -351,7 +400,30 @@ NbrOut out_of(pbf_ctx* c) {
   o.list_jbase = c->list_jbase;
   o.list_mode = c->list_mode;
   o.nbr_cnt = c->nbr_cnt;
+  o.oog_flag = c->oog_flag;
   return o;
+}
+
+SortBufs sort_of(pbf_ctx* c, const int* scene_off, int nscenes) {
+  SortBufs sb;
+  sb.scene_off = scene_off;
+  sb.nscenes = nscenes;
+  sb.sid = c->sid;
+  sb.ssid = c->ssid;
+  sb.scene_minmax = c->scene_minmax;
+  sb.win = c->win;
+  sb.total_cells = c->total_cells;
+  sb.cell_cap = c->cell_cap;
+  sb.cell_cnt = c->cell_cnt;
+  sb.cell_start = c->cell_start;
+  sb.oog_flag = c->oog_flag;
+  sb.key = c->key;
+  sb.slot = c->slot;
+  sb.tmp = c->tmp;
+  sb.perm = c->perm;
+  sb.inv = c->inv;
+  sb.scan_part = c->scan_part;
+  return sb;
 }
 
 // HIP-event pair around one pass on the call's stream (profiling only).
@@ -400,25 +472,21 @@ int drain_marks(pbf_ctx* c) {
   return PBF_OK;
 }
 
-// NeighborQuery::compute on the positions x (GridHash::construct +
-// queryNeighborDynamic), leaving the cell-sorted state and neighbor masks in
-// the context; with `integrate` the integrator is fused into the reorder.
-int search(pbf_ctx* c, const float* x, const float* v, int n, bool integrate, hipStream_t st) {
+// NeighborQuery::compute on the positions x for a batch of scenes
+// (GridHash::construct + queryNeighborDynamic), leaving the cell-sorted state
+// and the neighbor lists in the context; with `integrate` the integrator is
+// fused into the reorder.
+int search(pbf_ctx* c, const float* x, const float* v, int n, const int* scene_off, int nscenes, bool integrate,
+           hipStream_t st) {
   {
     PassTimer t(c, PBF_PASS_SORT, st);
-    launch_clear(c->cell_cnt, c->oog_flag, c->g.ncell, c->counters, st);
-    launch_bin(x, n, c->g, c->key, c->slot, c->cell_cnt, c->counters, st);
-    scan_exclusive(c->cell_cnt, c->cell_start, c->g.ncell + 1, c->scan_scratch, st);
-    launch_scatter(c->key, c->slot, c->cell_start, c->tmp, n, st);
-    launch_reorder(c->key, c->cell_start, c->tmp, x, v, n, c->perm, c->inv, c->xn, c->x0, c->vs,
-                   integrate ? &c->pc : nullptr, st);
+    launch_sort(sort_of(c, scene_off, nscenes), c->g, x, v, n, c->xn, c->x0, c->vs, integrate ? &c->pc : nullptr,
+                c->counters, st);
   }
-  NbrView nv = view_of(c, n);
   {
     PassTimer t(c, PBF_PASS_NEIGHBORS, st);
-    launch_neighbors(nv, out_of(c), c->counters, st);
+    launch_neighbors(view_of(c, n), out_of(c), c->counters, st);
   }
-  launch_oog_mark(nv, c->oog_flag, st);
   PBF_LAUNCHED(c);
   c->last_n = n;
   return PBF_OK;
@@ -429,7 +497,6 @@ int record_counters(pbf_ctx* c, hipStream_t st) {
                             hipMemcpyDeviceToHost, st));
   PBF_HIP(c, hipEventRecord(c->counters_ev, st));
   c->counters_pending = true;
-  c->last_stream = st;
   return PBF_OK;
 }
 
@@ -453,6 +520,74 @@ int export_csr(pbf_ctx* c, int n, int* d_offsets, int* d_elems, long long cap, l
                                          std::to_string(tot));
   launch_csr_fill(view_of(c, n), c->perm, d_offsets, d_elems, cap, st);
   PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+// One fused step on a batch of scenes (scene_off: device offsets): the state
+// (x_in, v_in) advances into (x_out, v_out), which may alias the inputs. With
+// a tape (P: K iterate arrays, P[0] = the integrated positions; aux: K arrays)
+// every PBD iterate, its lambda and the adjoint's per-iteration sums are kept.
+int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, float* v_out, float* d_rho, int n,
+              const int* scene_off, int nscenes, float4* const* P, float4* const* aux, hipStream_t st) {
+  const int K = c->p.iterations;
+  int s = search(c, x_in, v_in, n, scene_off, nscenes, true, st);
+  if (s) return s;
+  NbrView nv = view_of(c, n);
+  float4* bufs[2] = {c->xa, c->xb};
+  if (K == 0) launch_write_back(c->x0, c->vs, c->perm, n, x_out, v_out, st);
+  for (int it = 0; it < K; ++it) {
+    const bool last = it + 1 == K;
+    float4* src = P ? P[it] : (it == 0 ? c->x0 : bufs[(it - 1) & 1]);
+    float4* dst = P ? (last ? nullptr : P[it + 1]) : bufs[it & 1];
+    {
+      PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st);
+      launch_pass_density_lambda(nv, src, c->pc, last ? d_rho : nullptr, c->perm, st, aux ? aux[it] : nullptr);
+    }
+    {
+      PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
+      launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st);
+    }
+  }
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+// nsteps fused steps in place; d_rho receives the last step's density.
+int run_steps(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, const int* scene_off, int nscenes, int nsteps,
+              hipStream_t st) {
+  for (int step = 0; step < nsteps; ++step) {
+    int s = step_once(c, d_x, d_v, d_x, d_v, step + 1 == nsteps ? d_rho : nullptr, n, scene_off, nscenes, nullptr,
+                      nullptr, st);
+    if (s) return s;
+  }
+  return record_counters(c, st);
+}
+
+template <class T> int ensure_buf(pbf_ctx* c, T*& p, size_t& cap, size_t need) {
+  if (need <= cap) return PBF_OK;
+  dfree(p);
+  cap = 0;
+  PBF_HIP(c, hipMalloc(&p, sizeof(T) * need));
+  cap = need;
+  return PBF_OK;
+}
+
+// Tape and adjoint arrays for the current particle capacity and iterations.
+int ensure_tape(pbf_ctx* c) {
+  auto& r = c->roll;
+  const int K = c->p.iterations;
+  const size_t cap = (size_t)c->cap_n;
+  int s = ensure_buf(c, r.tape, r.tape_cap, (size_t)std::max(2 * K - 1, 1) * cap);
+  if (s) return s;
+  if (r.adj_n < c->cap_n) {
+    dfree(r.adj);
+    dfree(r.out);
+    r.adj_n = r.out_n = 0;
+    PBF_HIP(c, hipMalloc(&r.adj, sizeof(float4) * 4 * cap));
+    PBF_HIP(c, hipMalloc(&r.out, sizeof(float) * 6 * cap));
+    r.adj_n = r.out_n = c->cap_n;
+  }
+  if (!r.flag) PBF_HIP(c, hipMalloc(&r.flag, sizeof(int)));
   return PBF_OK;
 }
 
@@ -521,15 +656,18 @@ int pbf_create(const pbf_params* p, pbf_ctx** out) {
     delete c;
     return st;
   }
-  if (hipGetDevice(&c->device) != hipSuccess || hipMalloc(&c->counters, sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
+  if (hipGetDevice(&c->device) != hipSuccess ||
+      hipMalloc(&c->counters, sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
       hipHostMalloc(&c->h_counters, sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
-      hipEventCreateWithFlags(&c->counters_ev, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->counters_ev, hipEventDisableTiming) != hipSuccess ||
+      hipMalloc(&c->off1, sizeof(int) * 2) != hipSuccess || hipMalloc(&c->total_cells, sizeof(int)) != hipSuccess ||
+      hipMalloc(&c->scan_part, sizeof(int) * (kScanDevBlocks + 1)) != hipSuccess) {
     pbf_destroy(c);
     return PBF_ERR_HIP;
   }
   std::memset(c->h_counters, 0, sizeof(unsigned long long) * C_COUNT);
   (void)hipMemset(c->counters, 0, sizeof(unsigned long long) * C_COUNT);
-  st = ensure_grid(c);
+  st = ensure_scenes(c, 1);
   if (st) {
     pbf_destroy(c);
     return st;
@@ -541,12 +679,15 @@ int pbf_create(const pbf_params* p, pbf_ctx** out) {
 int pbf_destroy(pbf_ctx* c) {
   if (!c) return PBF_ERR_INVALID;
   (void)hipDeviceSynchronize();
+  dfree(c->off1); dfree(c->scene_minmax); dfree(c->win); dfree(c->total_cells); dfree(c->scan_part);
   dfree(c->cell_cnt); dfree(c->cell_start); dfree(c->oog_flag);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
-  dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->sid); dfree(c->ssid); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->mask); dfree(c->list); dfree(c->scan_scratch);
+  dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.tape); dfree(c->roll.adj);
+  dfree(c->roll.out); dfree(c->roll.flag);
   dfree(c->counters);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->counters_ev) (void)hipEventDestroy(c->counters_ev);
@@ -565,10 +706,15 @@ int pbf_set_params(pbf_ctx* c, const pbf_params* p) {
   int st = validate(p, why);
   if (st) return fail(c, st, why);
   PBF_HIP(c, hipDeviceSynchronize());
+  const pbf_params old = c->p;
   st = derive(c, p);
   if (st) return st;
-  st = ensure_grid(c);
-  if (st) return st;
+  st = ensure_scenes(c, std::max(c->scene_cap, 1));
+  if (st) {
+    derive(c, &old);
+    return st;
+  }
+  c->last_n = -1;
   if (c->cap_n > 0) {
     st = ensure_mask(c, mask_words_for(c, c->cap_n));
     if (st) return st;
@@ -585,7 +731,12 @@ int pbf_get_params(const pbf_ctx* c, pbf_params* out) {
 
 int pbf_reserve(pbf_ctx* c, int n) {
   if (!c || bad_n(n)) return fail(c, PBF_ERR_INVALID, "bad particle count");
-  return ensure_particles(c, n);
+  return ensure_particles(c, n, 1);
+}
+
+int pbf_reserve_batch(pbf_ctx* c, int n, int nscenes) {
+  if (!c || bad_n(n) || nscenes < 1) return fail(c, PBF_ERR_INVALID, "bad particle or scene count");
+  return ensure_particles(c, n, nscenes);
 }
 
 int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_offsets, int* d_elems, long long cap,
@@ -593,7 +744,7 @@ int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_offsets, int*
   if (!c) return PBF_ERR_INVALID;
   if (bad_n(n) || !total || !d_offsets || (n > 0 && !d_x) || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
   hipStream_t st = (hipStream_t)stream;
-  int s = ensure_particles(c, n);
+  int s = ensure_particles(c, n, 1);
   if (s) return s;
   if (n == 0) {
     c->last_n = 0;
@@ -601,7 +752,8 @@ int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_offsets, int*
   }
   s = adapt_reservations(c);
   if (s) return s;
-  s = search(c, d_x, nullptr, n, false, st);
+  launch_set_offsets(c->off1, n, st);
+  s = search(c, d_x, nullptr, n, c->off1, 1, false, st);
   if (s) return s;
   s = record_counters(c, st);
   if (s) return s;
@@ -627,7 +779,8 @@ int pbf_integrate(pbf_ctx* c, float* d_x, float* d_v, const float* d_f, int n, v
 int pbf_density(pbf_ctx* c, const float* d_x, const int* d_offsets, const int* d_elems, float* d_rho, int n,
                 void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (bad_n(n) || (n > 0 && (!d_x || !d_offsets || !d_elems || !d_rho))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (bad_n(n) || (n > 0 && (!d_x || !d_offsets || !d_elems || !d_rho)))
+    return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0) return PBF_OK;
   launch_csr_density(d_x, d_offsets, d_elems, d_rho, n, c->pc, (hipStream_t)stream);
   PBF_LAUNCHED(c);
@@ -637,9 +790,10 @@ int pbf_density(pbf_ctx* c, const float* d_x, const int* d_offsets, const int* d
 int pbf_pbd_constrain(pbf_ctx* c, float* d_x, float* d_v, const int* d_offsets, const int* d_elems, float* d_rho,
                       int n, void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems)))
+    return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0) return PBF_OK;
-  int s = ensure_particles(c, n);
+  int s = ensure_particles(c, n, 1);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   launch_aos_to_f4(d_x, c->xa, n, st);
@@ -659,40 +813,146 @@ int pbf_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, int nsteps
   if (bad_n(n) || nsteps < 0 || (n > 0 && (!d_x || !d_v))) return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0 || nsteps == 0) return PBF_OK;
   hipStream_t st = (hipStream_t)stream;
-  int s = ensure_particles(c, n);
+  int s = ensure_particles(c, n, 1);
   if (s) return s;
   s = adapt_reservations(c);
   if (s) return s;
-  const int K = c->p.iterations;
-  for (int step = 0; step < nsteps; ++step) {
-    s = search(c, d_x, d_v, n, true, st);
-    if (s) return s;
-    NbrView nv = view_of(c, n);
-    float4* bufs[2] = {c->xa, c->xb};
-    if (K == 0) launch_write_back(c->x0, c->vs, c->perm, n, d_x, d_v, st);
-    for (int it = 0; it < K; ++it) {
-      const bool last = it + 1 == K;
-      float4* src = it == 0 ? c->x0 : bufs[(it - 1) & 1];
-      float4* dst = bufs[it & 1];
-      {
-        PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st);
-        launch_pass_density_lambda(nv, src, c->pc, (last && step + 1 == nsteps) ? d_rho : nullptr, c->perm, st);
-      }
-      {
-        PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
-        launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, d_x, d_v, st);
-      }
-    }
-    PBF_LAUNCHED(c);
+  launch_set_offsets(c->off1, n, st);
+  return run_steps(c, d_x, d_v, d_rho, n, c->off1, 1, nsteps, st);
+}
+
+int pbf_batch_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const int* d_scene_offsets, int nscenes, int n,
+                   int nsteps, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || nsteps < 0 || nscenes < 1 || !d_scene_offsets || (n > 0 && (!d_x || !d_v)))
+    return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n == 0 || nsteps == 0) return PBF_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int s = ensure_particles(c, n, nscenes);
+  if (s) return s;
+  s = adapt_reservations(c);
+  if (s) return s;
+  return run_steps(c, d_x, d_v, d_rho, n, d_scene_offsets, nscenes, nsteps, st);
+}
+
+int pbf_rollout_forward(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const int* d_scene_offsets, int nscenes,
+                        int n, int nsteps, int checkpoint_every, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || n == 0 || nsteps < 1 || checkpoint_every < 1 || nscenes < 1 || !d_x || !d_v ||
+      (!d_scene_offsets && nscenes != 1))
+    return fail(c, PBF_ERR_INVALID, "bad argument");
+  hipStream_t st = (hipStream_t)stream;
+  auto& r = c->roll;
+  r.valid = false;
+  int s = ensure_particles(c, n, nscenes);
+  if (s) return s;
+  s = adapt_reservations(c);
+  if (s) return s;
+  const size_t state = 6 * (size_t)n;
+  const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+  const int seg_steps = std::min(checkpoint_every, nsteps);
+  if ((s = ensure_buf(c, r.ckpt, r.ckpt_cap, (size_t)nck * state))) return s;
+  if ((s = ensure_buf(c, r.seg, r.seg_cap, (size_t)seg_steps * state))) return s;
+  if (nscenes + 1 > r.off_cap) {
+    dfree(r.off);
+    r.off_cap = 0;
+    PBF_HIP(c, hipMalloc(&r.off, sizeof(int) * (size_t)(nscenes + 1)));
+    r.off_cap = nscenes + 1;
   }
-  return record_counters(c, st);
+  if (d_scene_offsets)
+    PBF_HIP(c, hipMemcpyAsync(r.off, d_scene_offsets, sizeof(int) * (size_t)(nscenes + 1), hipMemcpyDeviceToDevice,
+                              st));
+  else
+    launch_set_offsets(r.off, n, st);
+  for (int k = 0; k < nsteps; ++k) {
+    if (k % checkpoint_every == 0) {
+      float* ck = r.ckpt + (size_t)(k / checkpoint_every) * state;
+      PBF_HIP(c, hipMemcpyAsync(ck, d_x, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st));
+      PBF_HIP(c, hipMemcpyAsync(ck + 3 * (size_t)n, d_v, sizeof(float) * 3 * (size_t)n, hipMemcpyDeviceToDevice,
+                                st));
+    }
+    s = step_once(c, d_x, d_v, d_x, d_v, k + 1 == nsteps ? d_rho : nullptr, n, r.off, nscenes, nullptr, nullptr, st);
+    if (s) return s;
+  }
+  s = record_counters(c, st);
+  if (s) return s;
+  r.n = n;
+  r.nscenes = nscenes;
+  r.nsteps = nsteps;
+  r.every = checkpoint_every;
+  r.iters = c->p.iterations;
+  r.p = c->p;
+  r.valid = true;
+  return PBF_OK;
+}
+
+int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (!d_xbar || !d_vbar) return fail(c, PBF_ERR_INVALID, "bad argument");
+  auto& r = c->roll;
+  if (!r.valid) return fail(c, PBF_ERR_INVALID, "no rollout has run on this context");
+  if (std::memcmp(&r.p, &c->p, sizeof(pbf_params)) != 0)
+    return fail(c, PBF_ERR_INVALID, "parameters changed since the rollout");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = r.n, K = r.iters;
+  int s = ensure_particles(c, n, r.nscenes);
+  if (s) return s;
+  if ((s = adapt_reservations(c))) return s;
+  if ((s = ensure_tape(c))) return s;
+  const size_t cap = (size_t)c->cap_n, state = 6 * (size_t)n;
+  std::vector<float4*> P(std::max(K, 1)), aux(std::max(K, 1));
+  P[0] = c->x0;
+  for (int k = 1; k < K; ++k) P[k] = r.tape + (size_t)(k - 1) * cap;
+  for (int k = 0; k < K; ++k) aux[k] = r.tape + (size_t)(K - 1 + k) * cap;
+  AdjBufs ab;
+  ab.a = c->xa;
+  ab.a2 = c->xb;
+  ab.q1 = r.adj;
+  ab.q2 = r.adj + cap;
+  ab.xsb = r.adj + 2 * cap;
+  ab.vsb = r.adj + 3 * cap;
+  ab.flag = r.flag;
+  PBF_HIP(c, hipMemsetAsync(r.flag, 0, sizeof(int), st));
+  float* xo = r.out;
+  float* vo = r.out + 3 * cap;
+  const int nck = (r.nsteps + r.every - 1) / r.every;
+  for (int ck = nck - 1; ck >= 0; --ck) {
+    const int s0 = ck * r.every, m = std::min(s0 + r.every, r.nsteps) - s0;
+    // states of the segment from its checkpoint
+    PBF_HIP(c, hipMemcpyAsync(r.seg, r.ckpt + (size_t)ck * state, sizeof(float) * state, hipMemcpyDeviceToDevice,
+                              st));
+    for (int q = 1; q < m; ++q) {
+      const float* a = r.seg + (size_t)(q - 1) * state;
+      float* b = r.seg + (size_t)q * state;
+      s = step_once(c, a, a + 3 * (size_t)n, b, b + 3 * (size_t)n, nullptr, n, r.off, r.nscenes, nullptr, nullptr,
+                    st);
+      if (s) return s;
+    }
+    // reverse over the segment: re-run each step with its tape, then its adjoint
+    for (int q = m - 1; q >= 0; --q) {
+      const float* a = r.seg + (size_t)q * state;
+      s = step_once(c, a, a + 3 * (size_t)n, xo, vo, nullptr, n, r.off, r.nscenes, P.data(), aux.data(), st);
+      if (s) return s;
+      launch_adjoint_step(view_of(c, n), P.data(), aux.data(), K, c->pc, c->perm, d_xbar, d_vbar, ab, st);
+      PBF_LAUNCHED(c);
+    }
+  }
+  s = record_counters(c, st);
+  if (s) return s;
+  int flag = 0;
+  PBF_HIP(c, hipMemcpyAsync(&flag, r.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  PBF_HIP(c, hipStreamSynchronize(st));
+  if (flag)
+    return fail(c, PBF_ERR_UNSUPPORTED,
+                "a step of the rollout had particles outside the grid; its adjoint is not implemented");
+  return PBF_OK;
 }
 
 int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
   if (!c || !out) return fail(c, PBF_ERR_INVALID, "null argument");
   std::memset(out, 0, sizeof(*out));
   // counters_pending stays set: the next call still sees the completed copy
-  // and grows the mask reservation if needed.
+  // and grows the reservations if needed.
   if (c->counters_pending) PBF_HIP(c, hipEventSynchronize(c->counters_ev));
   out->neighbor_pairs = (long long)c->h_counters[C_PAIRS];
   out->out_of_grid = (long long)c->h_counters[C_OOG];
@@ -702,6 +962,7 @@ int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
   out->mask_fallback_waves = (long long)c->h_counters[C_FALLBACK];
   out->list_slots_needed = (long long)c->h_counters[C_LIST_SLOTS];
   out->list_slots_reserved = c->list_cap;
+  out->cells_used = (long long)c->h_counters[C_CELLS];
   out->grid_dims[0] = c->g.nx;
   out->grid_dims[1] = c->g.ny;
   out->grid_dims[2] = c->g.nz;

@@ -8,12 +8,22 @@ namespace pbf {
 
 // Uniform grid of GridHash::setSpace (Framework/Topology/GridHash.cu:47-75):
 // cell = h, origin = scene_lo - 2 cells, n = ceil((hi - lo)/h) + 1 + 4 per
-// axis, linear id i + nx*(j + ny*k). Cell `ncell` is the bucket that holds the
-// particles outside the grid.
+// axis. Every scene of a batch uses this geometry, so a particle's cell
+// coordinates are exactly those of a one-scene run.
 struct GridDev {
   float lo[3];
   float ds;
   int nx, ny, nz;
+};
+
+// A scene's cells are stored in a window: the bounding box of the cells of its
+// in-grid particles (cells outside it are empty), indexed
+// cell_base + (i - lo0) + dim0*((j - lo1) + dim1*(k - lo2)). The window is
+// followed by the scene's bucket of out-of-grid particles at cell_base + ncell.
+struct SceneWin {
+  int lo[3];
+  int dim[3];
+  int cell_base;
   int ncell;
 };
 
@@ -32,25 +42,27 @@ struct PhysConst {
   float gdt[3];    // (0 + g) * dt: velocity increment of the integrator at zero force
 };
 
-enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_COUNT };
+enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_COUNT };
 
 // Cell-sorted view used by the neighbor pass and every pass over its lists.
 // Particles are processed in wavefronts of 64 consecutive sorted indices; a
-// wave's neighbor list holds entry k of lane l at list_base + 64*k + l (16-bit
-// j - list_jbase, or a 32-bit j in two slots when the wave's candidates span
-// more than 65536 indices), so the lanes of a wave read consecutive slots.
+// wave's neighbor list holds 16-bit j - list_jbase (or 32-bit j when the
+// wave's candidates span more than 65536 indices), in groups of four per lane.
 struct NbrView {
   GridDev g;
-  const int* cell_start;       // ncell + 2 entries; cell_start[ncell] = in-grid count
+  const SceneWin* win;         // per scene
+  const int* ssid;             // scene of each sorted particle
+  const int* cell_start;       // cell_start[c]..cell_start[c+1]: sorted range of compact cell c
   const float4* xn;            // sorted positions the neighbor sets are defined on
   const int* nbr_cnt;          // neighbors per sorted particle
   const long long* list_base;  // per wave: first 16-bit slot of its list, -1 = no list
   const int* list_jbase;       // per wave: index base of 16-bit entries
   const int* list_mode;        // per wave: 0 = 16-bit deltas, 1 = 32-bit indices
   const uint16_t* list;
-  const long long* mask_base;  // per wave: first word of its bit masks, -1 = none
+  const long long* mask_base;  // per wave: first word of its global bit masks, -1 = none
   const uint32_t* mask;        // wave-interleaved bit masks over the cell runs
-  const uint8_t* oog_flag;     // per cell: an out-of-grid particle lists this cell
+  const uint8_t* oog_flag;     // per compact cell: an out-of-grid particle lists it
+  const unsigned long long* counters;
   float thr2;
   int n;
 };
@@ -66,30 +78,50 @@ struct NbrOut {
   int* list_jbase;
   int* list_mode;
   int* nbr_cnt;
+  uint8_t* oog_flag;
+};
+
+// Sorting state of a step (device arrays).
+struct SortBufs {
+  const int* scene_off;   // nscenes + 1 particle offsets
+  int nscenes;
+  int* sid;               // scene of each particle, original order
+  int* ssid;              // scene of each sorted particle
+  int* scene_minmax;      // 6 per scene: min i,j,k then max i,j,k of in-grid cells
+  SceneWin* win;          // per scene
+  int* total_cells;       // device scalar: sum over scenes of ncell + 1
+  long long cell_cap;     // allocated compact cells
+  int* cell_cnt;          // cell_cap + 1
+  int* cell_start;        // cell_cap + 2
+  uint8_t* oog_flag;      // cell_cap + 1
+  int* key;
+  int* slot;
+  int* tmp;
+  int* perm;
+  int* inv;
+  int* scan_part;         // partial sums of the device-length scan
 };
 
 constexpr int kBlock = 256;
 // Entries allocated past the end of the sorted position array xn, so that a
-// candidate group of 8 (or a 32-candidate word) may read past a run's end.
+// candidate group of 8 may read past a run's end.
 constexpr int kXnPad = 32;
+// Blocks of the device-length scan.
+constexpr int kScanDevBlocks = 512;
 
 size_t scan_scratch_ints(long long n);
 // out[0..n] = exclusive scan of in[0..n-1] (out[n] = total); in may equal out.
 void scan_exclusive(const int* in, int* out, int n, int* scratch, hipStream_t st);
 
-void launch_clear(int* cell_cnt, uint8_t* oog_flag, int ncell, unsigned long long* counters,
-                  hipStream_t st);
-void launch_bin(const float* x, int n, GridDev g, int* key, int* slot, int* cell_cnt,
-                unsigned long long* counters, hipStream_t st);
-void launch_scatter(const int* key, const int* slot, const int* cell_start, int* tmp, int n,
-                    hipStream_t st);
-void launch_reorder(const int* key, const int* cell_start, const int* tmp, const float* x,
-                    const float* v, int n, int* perm, int* inv, float4* xn, float4* x0, float4* vs,
-                    const PhysConst* integrate, hipStream_t st);
+// Counting sort of a batch by (scene, window cell); integrate != nullptr fuses
+// the integrator into the reorder.
+void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const float* v, int n, float4* xn,
+                 float4* x0, float4* vs, const PhysConst* integrate, unsigned long long* counters,
+                 hipStream_t st);
+void launch_set_offsets(int* off, int n, hipStream_t st);
 void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st);
-void launch_oog_mark(const NbrView& nv, uint8_t* oog_flag, hipStream_t st);
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
-                                float* rho_out, const int* perm, hipStream_t st);
+                                float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr);
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc,
                               bool last, const float4* x0, const float4* vs, const int* perm,
                               float* x_out, float* v_out, hipStream_t st);
@@ -98,6 +130,21 @@ void launch_write_back(const float4* xs, const float4* vs, const int* perm, int 
 void launch_csr_count(const int* nbr_cnt, const int* inv, int n, int* cnt_orig, hipStream_t st);
 void launch_csr_fill(const NbrView& nv, const int* perm, const int* offsets, int* elems,
                      long long cap, hipStream_t st);
+
+// Adjoint of one taped step (sorted order inside, original order at the
+// boundary). P[k] (k < K): positions and lambda of PBD iteration k; aux[k]:
+// (sum_j t^2/r d_ij, 1/(S + eps)) of that iteration.
+struct AdjBufs {
+  float4* a;    // adjoint of the current iterate
+  float4* a2;   // next adjoint (ping-pong with a)
+  float4* q1;   // (a, rho_bar)
+  float4* q2;   // (S_bar * G, S_bar)
+  float4* xsb;  // adjoint of the integrated positions, velocity part
+  float4* vsb;  // adjoint of the integrated velocities
+  int* flag;    // set when a taped step had out-of-grid particles
+};
+void launch_adjoint_step(const NbrView& nv, float4* const* P, float4* const* aux, int K, const PhysConst& pc,
+                         const int* perm, float* xbar, float* vbar, const AdjBufs& ab, hipStream_t st);
 
 // Module-granular kernels, original particle order, caller CSR.
 void launch_integrate(float* x, float* v, const float* f, int n, const PhysConst& pc, hipStream_t st);

@@ -12,159 +12,9 @@
 // order, ascending index inside a run, on every path.
 #include <climits>
 
-#include "pbf_device.h"
+#include "pbf_nbr.h"
 
 namespace pbf {
-
-// Run r (0..8) of cell (ci,cj,ck): cells (ci-1..ci+1, cj+dj, ck+dk) with
-// dj = r/3-1, dk = r%3-1, clipped to the grid.
-__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const GridDev& g, int ci, int cj,
-                                          int ck, int r, int& a, int& len) {
-  int j = cj + (r / 3) - 1, k = ck + (r % 3) - 1;
-  a = 0;
-  len = 0;
-  if (j < 0 || j >= g.ny || k < 0 || k >= g.nz) return;
-  int i0 = max(ci - 1, 0), i1 = min(ci + 1, g.nx - 1);
-  if (i0 > i1) return;
-  int row = (j + k * g.ny) * g.nx;
-  a = cell_start[row + i0];
-  len = cell_start[row + i1 + 1] - a;
-}
-
-// Bits of the candidates [j0, j0+cnt) (1 <= cnt <= 32) within h of xi. Loads
-// go out eight at a time from one base address; the sorted position array
-// carries kXnPad far-away entries past its end, so a group may read past the
-// run (those bits are masked off) without index clamping.
-__device__ __forceinline__ uint32_t test_word(const float4* __restrict__ xn, float4 xi, int j0, int cnt,
-                                              float thr2) {
-  uint32_t bits = 0;
-  const float4* p = xn + j0;
-  for (int k0 = 0; k0 < cnt; k0 += 8) {
-    float4 c[8];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) c[u] = p[k0 + u];
-#pragma unroll
-    for (int u = 0; u < 8; ++u) bits |= (uint32_t)within(xi, c[u], thr2) << (k0 + u);
-  }
-  return cnt >= 32 ? bits : bits & ((1u << cnt) - 1u);
-}
-
-// Neighbor position load: all four components (pass B reads lambda from w),
-// or xyz only (a float4 load whose w is dead lets the register allocator
-// reuse that register while the load is in flight, forcing an early wait).
-template <bool kW>
-__device__ __forceinline__ float4 load_pos(const float4* __restrict__ xs, int j) {
-  if (kW) return xs[j];
-  const float* q = reinterpret_cast<const float*>(xs + j);
-  return make_float4(q[0], q[1], q[2], 0.0f);
-}
-
-// List layout: a lane's entries come in groups of four that sit together
-// (8 bytes as 16-bit deltas, 16 bytes as 32-bit indices), and group b of the
-// wave's 64 lanes is contiguous: group b of lane l at base + (64*b + l) groups.
-__device__ __forceinline__ long long list_slot(long long lb, int k, int lane, int mode) {
-  return lb + ((((long long)(k >> 2) << 6) + lane) << (2 + mode)) + ((long long)(k & 3) << mode);
-}
-
-struct Group4 {
-  int j[4];
-};
-
-// Indices of group b (entries 4b..4b+3); entries past cnt become s.
-__device__ __forceinline__ Group4 decode16(uint2 e, int jb, int b, int cnt, int s) {
-  const int k = b << 2;
-  Group4 g;
-  g.j[0] = k + 0 < cnt ? jb + (int)(e.x & 0xffffu) : s;
-  g.j[1] = k + 1 < cnt ? jb + (int)(e.x >> 16) : s;
-  g.j[2] = k + 2 < cnt ? jb + (int)(e.y & 0xffffu) : s;
-  g.j[3] = k + 3 < cnt ? jb + (int)(e.y >> 16) : s;
-  return g;
-}
-__device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int s) {
-  const int k = b << 2;
-  Group4 g;
-  g.j[0] = k + 0 < cnt ? e.x : s;
-  g.j[1] = k + 1 < cnt ? e.y : s;
-  g.j[2] = k + 2 < cnt ? e.z : s;
-  g.j[3] = k + 3 < cnt ? e.w : s;
-  return g;
-}
-
-// Counted loop over one lane's list, software-pipelined two groups deep:
-// while the bodies of group b run, the positions of group b+1 and the indices
-// of group b+2 are in flight (one wait per group).
-template <bool kW, class GroupT, class LoadFn, class DecodeFn, class Body>
-__device__ __forceinline__ void list_loop(int cnt, const float4* __restrict__ xs, LoadFn&& load_group,
-                                          DecodeFn&& decode, Body&& body) {
-  const int ng = (cnt + 3) >> 2;
-  if (ng == 0) return;
-  GroupT e = load_group(0);
-  Group4 g = decode(e, 0);
-  float4 a[4];
-#pragma unroll
-  for (int u = 0; u < 4; ++u) a[u] = load_pos<kW>(xs, g.j[u]);
-  if (ng > 1) e = load_group(1);
-  for (int b = 0; b < ng; ++b) {
-    const Group4 gn = decode(e, b + 1);
-    float4 an[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) an[u] = load_pos<kW>(xs, gn.j[u]);
-    if (b + 2 < ng) e = load_group(b + 2);
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if ((b << 2) + u < cnt) body(g.j[u], a[u]);
-    g = gn;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = an[u];
-  }
-}
-
-// Calls body(j, x_j) for every neighbor j of sorted particle s. The list is
-// used when the wave has one; otherwise (reservation exhausted) the words come
-// from the masks, or are recomputed from xn.
-template <bool kW, class Body>
-__device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const float4* __restrict__ xs,
-                                                    Body&& body) {
-  const int wave = s >> 6, lane = s & 63;
-  const long long lb = nv.list_base[wave];
-  if (lb >= 0) {
-    const int cnt = nv.nbr_cnt[s];
-    if (nv.list_mode[wave] == 0) {
-      const uint2* L = reinterpret_cast<const uint2*>(nv.list + lb) + lane;
-      const int jb = nv.list_jbase[wave];
-      list_loop<kW, uint2>(
-          cnt, xs, [&](int b) { return L[(long long)b << 6]; },
-          [&](uint2 e, int b) { return decode16(e, jb, b, cnt, s); }, body);
-    } else {
-      const int4* L = reinterpret_cast<const int4*>(nv.list + lb) + lane;
-      list_loop<kW, int4>(
-          cnt, xs, [&](int b) { return L[(long long)b << 6]; }, [&](int4 e, int b) { return decode32(e, b, cnt, s); },
-          body);
-    }
-    return;
-  }
-  const float4 xni = nv.xn[s];
-  int ci, cj, ck;
-  cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck);
-  const long long mb = nv.mask_base[wave];
-  int wofs = 0;
-  for (int r = 0; r < 9; ++r) {
-    int ra, rl;
-    run_range(nv.cell_start, nv.g, ci, cj, ck, r, ra, rl);
-    const int nw = (rl + 31) >> 5;
-    for (int w = 0; w < nw; ++w) {
-      const int jb = ra + (w << 5);
-      uint32_t bits = mb >= 0 ? nv.mask[mb + ((long long)(wofs + w) << 6) + lane]
-                              : test_word(nv.xn, xni, jb, min(32, rl - (w << 5)), nv.thr2);
-      while (bits) {
-        const int j = jb + __builtin_ctz(bits);
-        bits &= bits - 1u;
-        body(j, load_pos<kW>(xs, j));
-      }
-    }
-    wofs += nw;
-  }
-}
 
 // ---------------------------------------------------------------------------
 // Neighbor pass: K_CalNeighborSize + K_GetNeighborElements
@@ -187,14 +37,24 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
     xi = nv.xn[s];
     int ci, cj, ck;
     cell_of(nv.g, xi.x, xi.y, xi.z, ci, cj, ck);
+    const SceneWin win = nv.win[nv.ssid[s]];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-      run_range(nv.cell_start, nv.g, ci, cj, ck, r, ra[r], rl[r]);
+      run_range(nv.cell_start, win, ci, cj, ck, r, ra[r], rl[r]);
       words += (rl[r] + 31) >> 5;
       if (rl[r] > 0) {
         jlo = min(jlo, ra[r]);
         jhi = max(jhi, ra[r] + rl[r] - 1);
       }
+    }
+    // An out-of-grid particle is not hashed (GridHash.cu:85,194) yet queries,
+    // so each in-grid j it lists receives -dp_kj from it (DensityPBD.cu:162)
+    // without listing it back: flag the cells of its stencil for pass B.
+    if (!in_window(win, ci, cj, ck)) {  // the window holds every in-grid cell of the scene
+      for (int dk = -1; dk <= 1; ++dk)
+        for (int dj = -1; dj <= 1; ++dj)
+          for (int di = -1; di <= 1; ++di)
+            if (in_window(win, ci + di, cj + dj, ck + dk)) out.oog_flag[window_cell(win, ci + di, cj + dj, ck + dk)] = 1;
     }
   } else {
 #pragma unroll
@@ -314,38 +174,17 @@ void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* 
   k_neighbors<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, out, counters);
 }
 
-// Cells inside the stencil of an out-of-grid particle. Such a particle is not
-// hashed (GridHash.cu:85,194) yet queries, so an in-grid j it lists receives
-// -dp_kj from it (DensityPBD.cu:162) without listing it back.
-__global__ void k_oog_mark(NbrView nv, uint8_t* oog_flag) {
-  const int n_in = nv.cell_start[nv.g.ncell];
-  const int stride = gridDim.x * blockDim.x;
-  for (int k = n_in + blockIdx.x * blockDim.x + threadIdx.x; k < nv.n; k += stride) {
-    const float4 p = nv.xn[k];
-    int ci, cj, ck;
-    cell_of(nv.g, p.x, p.y, p.z, ci, cj, ck);
-    for (int dk = -1; dk <= 1; ++dk)
-      for (int dj = -1; dj <= 1; ++dj)
-        for (int di = -1; di <= 1; ++di) {
-          const int a = ci + di, b = cj + dj, c = ck + dk;
-          if (in_grid(nv.g, a, b, c)) oog_flag[a + nv.g.nx * (b + nv.g.ny * c)] = 1;
-        }
-  }
-}
-
-void launch_oog_mark(const NbrView& nv, uint8_t* oog_flag, hipStream_t st) {
-  k_oog_mark<<<64, kBlock, 0, st>>>(nv, oog_flag);
-}
-
 // ---------------------------------------------------------------------------
 // PBD passes (DensityPBD::takeOneIteration, DensityPBD.cu:306-365).
 
 // Pass A: K_ComputeDensity (SummationDensity.cu:13-36) and K_ComputeLambdas
 // (DensityPBD.cu:46-90) fused: one traversal gives rho_i, sum g and sum g.g;
-// lambda_i goes to xs[s].w for pass B.
+// lambda_i goes to xs[s].w for pass B. aux (taped steps only) receives
+// (sum_j t^2/r d_ij, 1/(S + eps)) for the adjoint.
 __global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* __restrict__ xs, PhysConst pc,
                                                            float* __restrict__ rho_out,
-                                                           const int* __restrict__ perm) {
+                                                           const int* __restrict__ perm,
+                                                           float4* __restrict__ aux) {
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= nv.n) return;
   const float4 xi = xs[s];
@@ -367,11 +206,12 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* _
   lam = lam > 0.0f ? 0.0f : lam;
   reinterpret_cast<float*>(xs + s)[3] = lam;
   if (rho_out) rho_out[perm[s]] = rho;
+  if (aux) aux[s] = make_float4(sgx, sgy, sgz, 1.0f / (S + pc.lam_eps));
 }
 
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                                hipStream_t st) {
-  k_density_lambda<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, pc, rho_out, perm);
+                                hipStream_t st, float4* aux) {
+  k_density_lambda<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
 }
 
 // Pass B: K_ComputeDisplacement + K_UpdatePosition (DensityPBD.cu:135-179,
@@ -401,19 +241,21 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
   };
   float ax = 0.f, ay = 0.f, az = 0.f;
   for_each_neighbor_x<true>(nv, s, xs, [&](int, float4 xj) { pair(xj, ax, ay, az); });
-  const int n_in = nv.cell_start[nv.g.ncell];
   float fa = 2.0f;
   float bx = 0.f, by = 0.f, bz = 0.f;
-  if (n_in < nv.n) {  // out-of-grid particles exist
-    if (s >= n_in) {
+  if (nv.counters[C_OOG] != 0) {  // some scene has out-of-grid particles
+    const SceneWin win = nv.win[nv.ssid[s]];
+    const int ob = win.cell_base + win.ncell;  // the scene's out-of-grid bucket
+    const int o0 = nv.cell_start[ob], o1 = nv.cell_start[ob + 1];
+    if (s >= o0) {
       fa = 1.0f;  // out-of-grid particle: its own pairs once
-    } else {
+    } else if (o1 > o0) {
       const float4 xni = nv.xn[s];
       int ci, cj, ck;
       cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck);
-      if (nv.oog_flag[ci + nv.g.nx * (cj + nv.g.ny * ck)]) {
+      if (nv.oog_flag[window_cell(win, ci, cj, ck)]) {
         // out-of-grid particles k that list s add -dp_ks = dp_sk to s once
-        for (int k = n_in; k < nv.n; ++k) {
+        for (int k = o0; k < o1; ++k) {
           const float4 xnk = nv.xn[k];
           int ki, kj, kk;
           cell_of(nv.g, xnk.x, xnk.y, xnk.z, ki, kj, kk);

@@ -1,29 +1,29 @@
-// Counting sort of the particles by grid cell (GridHash::construct,
-// Framework/Topology/GridHash.cu:298-323), the exclusive scan it needs
-// (Core/Utility/Scan.cu), the fused integrator, and the scatter back to the
-// caller's order.
+// Counting sort of the particles by (scene, grid cell) (GridHash::construct,
+// Framework/Topology/GridHash.cu:298-323, for a batch of independent scenes),
+// the exclusive scans it needs (Core/Utility/Scan.cu), the fused integrator and
+// the scatter back to the caller's order.
 //
-// HBM layout: caller arrays x, v are packed xyz float triples in original
-// order; the sorted arrays xn, x0, xa, xb, vs are float4 in cell order (w of the
-// position buffers carries lambda between passes); cell arrays cell_cnt[ncell+1],
-// cell_start[ncell+2].
+// HBM layout: caller arrays x, v are packed xyz float triples in original order
+// (scenes concatenated); the sorted arrays xn, x0, xa, xb, vs are float4 in
+// (scene, cell) order (w of the position buffers carries lambda between
+// passes). Cells are compact: each scene owns the window of cells spanned by
+// its in-grid particles plus one bucket for its out-of-grid particles.
+#include <climits>
+
 #include "pbf_device.h"
 
 namespace pbf {
 
 // ---------------------------------------------------------------------------
-// Exclusive scan: blocks of 1024 threads x 4 elements scanned with wave64
-// shuffles, recursion over the block sums.
+// Exclusive scan with a host-known length: blocks of 1024 threads x 4 elements
+// scanned with wave64 shuffles, recursion over the block sums.
 
 constexpr int kScanPerBlock = 4096;
 
-__global__ __launch_bounds__(1024) void k_scan_block(const int* in, int* out, int n, int* block_sums) {
+// Exclusive scan of a 4096-element tile held 4 per thread; returns this
+// thread's exclusive prefix of its first element and the tile total.
+__device__ __forceinline__ int block_scan4(int a0, int a1, int a2, int a3, int& total) {
   __shared__ int wsum[16];
-  const int base = blockIdx.x * kScanPerBlock + threadIdx.x * 4;
-  int a0 = base < n ? in[base] : 0;
-  int a1 = base + 1 < n ? in[base + 1] : 0;
-  int a2 = base + 2 < n ? in[base + 2] : 0;
-  int a3 = base + 3 < n ? in[base + 3] : 0;
   const int tsum = a0 + a1 + a2 + a3;
   const int incl = wave_inclusive_scan(tsum);
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -35,7 +35,13 @@ __global__ __launch_bounds__(1024) void k_scan_block(const int* in, int* out, in
     if (lane < 16) wsum[lane] = t;
   }
   __syncthreads();
-  int e = (wave > 0 ? wsum[wave - 1] : 0) + incl - tsum;
+  const int e = (wave > 0 ? wsum[wave - 1] : 0) + incl - tsum;
+  total = wsum[15];
+  __syncthreads();  // wsum is reused by the next call
+  return e;
+}
+
+__device__ __forceinline__ void store4(int* out, int base, int n, int e, int a0, int a1, int a2) {
   if (base < n) out[base] = e;
   e += a0;
   if (base + 1 < n) out[base + 1] = e;
@@ -43,9 +49,20 @@ __global__ __launch_bounds__(1024) void k_scan_block(const int* in, int* out, in
   if (base + 2 < n) out[base + 2] = e;
   e += a2;
   if (base + 3 < n) out[base + 3] = e;
-  if (threadIdx.x == 1023) {
-    block_sums[blockIdx.x] = wsum[15];
-    if (gridDim.x == 1) out[n] = wsum[15];
+}
+
+__global__ __launch_bounds__(1024) void k_scan_block(const int* in, int* out, int n, int* block_sums) {
+  const int base = blockIdx.x * kScanPerBlock + threadIdx.x * 4;
+  const int a0 = base < n ? in[base] : 0;
+  const int a1 = base + 1 < n ? in[base + 1] : 0;
+  const int a2 = base + 2 < n ? in[base + 2] : 0;
+  const int a3 = base + 3 < n ? in[base + 3] : 0;
+  int total;
+  const int e = block_scan4(a0, a1, a2, a3, total);
+  store4(out, base, n, e, a0, a1, a2);
+  if (threadIdx.x == 0) {
+    block_sums[blockIdx.x] = total;
+    if (gridDim.x == 1) out[n] = total;
   }
 }
 
@@ -80,53 +97,203 @@ void scan_exclusive(const int* in, int* out, int n, int* scratch, hipStream_t st
 }
 
 // ---------------------------------------------------------------------------
-// Hashing
+// Exclusive scan with a device-resident length (reduce, scan the block sums,
+// rescan): kScanDevBlocks blocks, each owning a contiguous chunk of tiles.
 
-__global__ void k_clear(int* cell_cnt, uint8_t* oog_flag, int ncell, unsigned long long* counters) {
-  const int stride = gridDim.x * blockDim.x;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c <= ncell; c += stride) {
-    cell_cnt[c] = 0;
-    if (c < ncell) oog_flag[c] = 0;
+__device__ __forceinline__ void dev_chunk(int n, int& lo, int& hi) {
+  const int tiles = (n + kScanPerBlock - 1) / kScanPerBlock;
+  const int per = (tiles + gridDim.x - 1) / gridDim.x;
+  lo = min(n, (int)blockIdx.x * per * kScanPerBlock);
+  hi = min(n, lo + per * kScanPerBlock);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_dev_reduce(const int* in, const int* n_ptr, int* part) {
+  __shared__ int wsum[16];
+  const int n = *n_ptr;
+  int lo, hi;
+  dev_chunk(n, lo, hi);
+  int acc = 0;
+  for (int t = lo + threadIdx.x; t < hi; t += 1024) acc += in[t];
+  acc = wave_sum(acc);
+  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int s = 0;
+    for (int w = 0; w < 16; ++w) s += wsum[w];
+    part[blockIdx.x] = s;
   }
-  if (blockIdx.x == 0 && threadIdx.x < C_COUNT) counters[threadIdx.x] = 0;
 }
 
-void launch_clear(int* cell_cnt, uint8_t* oog_flag, int ncell, unsigned long long* counters, hipStream_t st) {
-  int nb = min((ncell + 1 + kBlock - 1) / kBlock, 4096);
-  k_clear<<<nb, kBlock, 0, st>>>(cell_cnt, oog_flag, ncell, counters);
+__global__ __launch_bounds__(1024) void k_scan_dev_mid(int* part, int nb) {
+  // nb <= 1024 block sums, exclusive in place; part[nb] = total
+  const int i = threadIdx.x;
+  const int v = i < nb ? part[i] : 0;
+  int total;
+  const int e = block_scan4(v, 0, 0, 0, total);
+  if (i < nb) part[i] = e;
+  if (i == 0) part[nb] = total;
 }
 
-// K_CalculateParticleNumber (GridHash.cu:78-87): key = cell id or the
-// out-of-grid bucket; slot = arrival order inside the cell.
-__global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ x, int n, GridDev g,
-                                                int* __restrict__ key, int* __restrict__ slot,
-                                                int* __restrict__ cell_cnt,
-                                                unsigned long long* __restrict__ counters) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  bool oog = false, nonfinite = false;
-  if (i < n) {
-    const float px = x[3 * (size_t)i], py = x[3 * (size_t)i + 1], pz = x[3 * (size_t)i + 2];
-    nonfinite = !(isfinite(px) && isfinite(py) && isfinite(pz));
-    int ci, cj, ck;
-    cell_of(g, px, py, pz, ci, cj, ck);
-    int k = g.ncell;
-    if (!nonfinite && in_grid(g, ci, cj, ck))
-      k = ci + g.nx * (cj + g.ny * ck);
+__global__ __launch_bounds__(1024) void k_scan_dev_down(const int* in, int* out, const int* n_ptr, const int* part) {
+  const int n = *n_ptr;
+  int lo, hi;
+  dev_chunk(n, lo, hi);
+  int running = part[blockIdx.x];
+  for (int t0 = lo; t0 < hi; t0 += kScanPerBlock) {
+    const int base = t0 + threadIdx.x * 4;
+    const int a0 = base < hi ? in[base] : 0;
+    const int a1 = base + 1 < hi ? in[base + 1] : 0;
+    const int a2 = base + 2 < hi ? in[base + 2] : 0;
+    const int a3 = base + 3 < hi ? in[base + 3] : 0;
+    int total;
+    const int e = block_scan4(a0, a1, a2, a3, total);
+    store4(out, base, hi, running + e, a0, a1, a2);
+    running += total;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[gridDim.x];
+}
+
+// ---------------------------------------------------------------------------
+// Scene windows
+
+__global__ void k_scene_reset(int* minmax, int nscenes, unsigned long long* counters) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < 6 * nscenes) minmax[t] = (t % 6) < 3 ? INT_MAX : INT_MIN;
+  if (t < C_COUNT) counters[t] = 0;
+}
+
+__device__ __forceinline__ int scene_of(const int* __restrict__ off, int nscenes, int i) {
+  int lo = 0, hi = nscenes;  // off[lo] <= i < off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i)
+      lo = mid;
     else
-      oog = true;
-    key[i] = k;
-    slot[i] = atomicAdd(&cell_cnt[k], 1);
+      hi = mid;
   }
-  const unsigned long long b_oog = __ballot(oog), b_nf = __ballot(nonfinite);
+  return lo;
+}
+
+__device__ __forceinline__ bool grid_cell(const GridDev& g, const float* __restrict__ x, int i, int& ci, int& cj,
+                                          int& ck, bool& nonfinite) {
+  const float px = x[3 * (size_t)i], py = x[3 * (size_t)i + 1], pz = x[3 * (size_t)i + 2];
+  nonfinite = !(isfinite(px) && isfinite(py) && isfinite(pz));
+  cell_of(g, px, py, pz, ci, cj, ck);
+  return !nonfinite && in_grid(g, ci, cj, ck);
+}
+
+// Scene of every particle and the cell bounding box of each scene's in-grid
+// particles (wave-reduced when a wave lies in one scene).
+__global__ __launch_bounds__(kBlock) void k_scene_bounds(const float* __restrict__ x, int n, GridDev g,
+                                                         const int* __restrict__ off, int nscenes,
+                                                         int* __restrict__ sid, int* __restrict__ minmax,
+                                                         unsigned long long* __restrict__ counters) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = i < n;
+  int s = -1, ci = 0, cj = 0, ck = 0;
+  bool nonfinite = false, inside = false;
+  if (valid) {
+    s = scene_of(off, nscenes, i);
+    sid[i] = s;
+    inside = grid_cell(g, x, i, ci, cj, ck, nonfinite);
+  }
+  const unsigned long long b_oog = __ballot(valid && !inside), b_nf = __ballot(valid && nonfinite);
   if ((threadIdx.x & 63) == 0) {
     if (b_oog) atomicAdd(&counters[C_OOG], (unsigned long long)__popcll(b_oog));
     if (b_nf) atomicAdd(&counters[C_NONFINITE], (unsigned long long)__popcll(b_nf));
   }
+  const int s0 = __shfl(s, 0);  // lane 0 is valid whenever any lane is
+  if (__all(!valid || s == s0)) {
+    const int mi = wave_min(inside ? ci : INT_MAX), mj = wave_min(inside ? cj : INT_MAX);
+    const int mk = wave_min(inside ? ck : INT_MAX);
+    const int xi = wave_max(inside ? ci : INT_MIN), xj = wave_max(inside ? cj : INT_MIN);
+    const int xk = wave_max(inside ? ck : INT_MIN);
+    if ((threadIdx.x & 63) == 0 && s0 >= 0 && xi >= mi) {
+      int* m = minmax + 6 * s0;
+      atomicMin(m + 0, mi);
+      atomicMin(m + 1, mj);
+      atomicMin(m + 2, mk);
+      atomicMax(m + 3, xi);
+      atomicMax(m + 4, xj);
+      atomicMax(m + 5, xk);
+    }
+  } else if (inside) {
+    int* m = minmax + 6 * s;
+    atomicMin(m + 0, ci);
+    atomicMin(m + 1, cj);
+    atomicMin(m + 2, ck);
+    atomicMax(m + 3, ci);
+    atomicMax(m + 4, cj);
+    atomicMax(m + 5, ck);
+  }
 }
 
-void launch_bin(const float* x, int n, GridDev g, int* key, int* slot, int* cell_cnt, unsigned long long* counters,
-                hipStream_t st) {
-  k_bin<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(x, n, g, key, slot, cell_cnt, counters);
+// Windows and compact cell bases of all scenes (one block).
+__global__ __launch_bounds__(1024) void k_scene_windows(const int* __restrict__ minmax, int nscenes,
+                                                        SceneWin* __restrict__ win, int* __restrict__ total_cells,
+                                                        unsigned long long* __restrict__ counters) {
+  int running = 0;
+  for (int s0 = 0; s0 < nscenes; s0 += 1024) {
+    const int s = s0 + threadIdx.x;
+    SceneWin w{};
+    int cells = 0;
+    if (s < nscenes) {
+      const int* m = minmax + 6 * s;
+      if (m[3] >= m[0]) {
+        for (int a = 0; a < 3; ++a) {
+          w.lo[a] = m[a];
+          w.dim[a] = m[a + 3] - m[a] + 1;
+        }
+        w.ncell = w.dim[0] * w.dim[1] * w.dim[2];
+      } else {
+        for (int a = 0; a < 3; ++a) {
+          w.lo[a] = 0;
+          w.dim[a] = 0;
+        }
+        w.ncell = 0;
+      }
+      cells = w.ncell + 1;  // + out-of-grid bucket
+    }
+    int total;
+    const int e = block_scan4(cells, 0, 0, 0, total);
+    if (s < nscenes) {
+      w.cell_base = running + e;
+      win[s] = w;
+    }
+    running += total;
+  }
+  if (threadIdx.x == 0) {
+    *total_cells = running;
+    counters[C_CELLS] = (unsigned long long)running;
+  }
+}
+
+__global__ void k_clear_cells(int* __restrict__ cell_cnt, uint8_t* __restrict__ oog_flag,
+                              const int* __restrict__ total_cells) {
+  const int total = *total_cells;
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += stride) {
+    cell_cnt[c] = 0;
+    oog_flag[c] = 0;
+  }
+}
+
+// K_CalculateParticleNumber (GridHash.cu:78-87): key = compact cell or the
+// scene's out-of-grid bucket; slot = arrival order inside the cell.
+__global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ x, int n, GridDev g,
+                                                const int* __restrict__ sid, const SceneWin* __restrict__ win,
+                                                int* __restrict__ key, int* __restrict__ slot,
+                                                int* __restrict__ cell_cnt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int ci, cj, ck;
+  bool nonfinite;
+  const bool inside = grid_cell(g, x, i, ci, cj, ck, nonfinite);
+  const SceneWin w = win[sid[i]];
+  int k = w.cell_base + w.ncell;
+  if (inside) k = w.cell_base + (ci - w.lo[0]) + w.dim[0] * ((cj - w.lo[1]) + w.dim[1] * (ck - w.lo[2]));
+  key[i] = k;
+  slot[i] = atomicAdd(&cell_cnt[k], 1);
 }
 
 // K_ConstructHashTable (GridHash.cu:187-198): ids into their cell's slots.
@@ -135,10 +302,6 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int* __restrict__ key,
                                                     int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) tmp[cell_start[key[i]] + slot[i]] = i;
-}
-
-void launch_scatter(const int* key, const int* slot, const int* cell_start, int* tmp, int n, hipStream_t st) {
-  k_scatter<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(key, slot, cell_start, tmp, n);
 }
 
 // Final position inside the cell = rank of the particle id among the cell's
@@ -150,11 +313,12 @@ void launch_scatter(const int* key, const int* slot, const int* cell_start, int*
 constexpr int kRankMax = 1024;
 
 __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key, const int* __restrict__ cell_start,
-                                                    const int* __restrict__ tmp, const float* __restrict__ x,
-                                                    const float* __restrict__ v, int n, int* __restrict__ perm,
-                                                    int* __restrict__ inv, float4* __restrict__ xn,
-                                                    float4* __restrict__ x0, float4* __restrict__ vs,
-                                                    PhysConst pc, int integrate) {
+                                                    const int* __restrict__ tmp, const int* __restrict__ sid,
+                                                    const float* __restrict__ x, const float* __restrict__ v, int n,
+                                                    int* __restrict__ perm, int* __restrict__ inv,
+                                                    int* __restrict__ ssid, float4* __restrict__ xn,
+                                                    float4* __restrict__ x0, float4* __restrict__ vs, PhysConst pc,
+                                                    int integrate) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -172,6 +336,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
   }
   perm[s] = i;
   inv[i] = s;
+  ssid[s] = sid[i];
   const float px = x[3 * (size_t)i], py = x[3 * (size_t)i + 1], pz = x[3 * (size_t)i + 2];
   xn[s] = make_float4(px, py, pz, 0.0f);
   if (integrate) {
@@ -185,14 +350,34 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
   }
 }
 
-void launch_reorder(const int* key, const int* cell_start, const int* tmp, const float* x, const float* v, int n,
-                    int* perm, int* inv, float4* xn, float4* x0, float4* vs, const PhysConst* integrate,
-                    hipStream_t st) {
+void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const float* v, int n, float4* xn, float4* x0,
+                 float4* vs, const PhysConst* integrate, unsigned long long* counters, hipStream_t st) {
+  const int nb = (n + kBlock - 1) / kBlock;
+  k_scene_reset<<<(6 * sb.nscenes + C_COUNT + kBlock - 1) / kBlock, kBlock, 0, st>>>(sb.scene_minmax, sb.nscenes,
+                                                                                     counters);
+  k_scene_bounds<<<nb, kBlock, 0, st>>>(x, n, g, sb.scene_off, sb.nscenes, sb.sid, sb.scene_minmax, counters);
+  k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, counters);
+  const int clear_blocks = (int)std::min<long long>((sb.cell_cap + kBlock - 1) / kBlock, 2048);
+  k_clear_cells<<<clear_blocks, kBlock, 0, st>>>(sb.cell_cnt, sb.oog_flag, sb.total_cells);
+  k_bin<<<nb, kBlock, 0, st>>>(x, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
+  k_scan_dev_reduce<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.total_cells, sb.scan_part);
+  k_scan_dev_mid<<<1, 1024, 0, st>>>(sb.scan_part, kScanDevBlocks);
+  k_scan_dev_down<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.cell_start, sb.total_cells, sb.scan_part);
+  k_scatter<<<nb, kBlock, 0, st>>>(sb.key, sb.slot, sb.cell_start, sb.tmp, n);
   PhysConst pc{};
   if (integrate) pc = *integrate;
-  k_reorder<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(key, cell_start, tmp, x, v, n, perm, inv, xn, x0, vs,
-                                                          pc, integrate ? 1 : 0);
+  k_reorder<<<nb, kBlock, 0, st>>>(sb.key, sb.cell_start, sb.tmp, sb.sid, x, v, n, sb.perm, sb.inv, sb.ssid, xn, x0,
+                                   vs, pc, integrate ? 1 : 0);
 }
+
+__global__ void k_set_offsets(int* off, int n) {
+  if (threadIdx.x == 0) {
+    off[0] = 0;
+    off[1] = n;
+  }
+}
+
+void launch_set_offsets(int* off, int n, hipStream_t st) { k_set_offsets<<<1, 64, 0, st>>>(off, n); }
 
 __global__ __launch_bounds__(kBlock) void k_write_back(const float4* __restrict__ xs, const float4* __restrict__ vs,
                                                        const int* __restrict__ perm, int n, float* __restrict__ x_out,

@@ -74,9 +74,12 @@ class Context:
         with torch.cuda.device(self.device):
             _capi.check(self._lib.pbf_set_params(self._ctx, C.byref(p)), self._ctx)
 
-    def reserve(self, n: int) -> None:
+    def reserve(self, n: int, nscenes: int = 1) -> None:
         with torch.cuda.device(self.device):
-            _capi.check(self._lib.pbf_reserve(self._ctx, int(n)), self._ctx)
+            if nscenes == 1:
+                _capi.check(self._lib.pbf_reserve(self._ctx, int(n)), self._ctx)
+            else:
+                _capi.check(self._lib.pbf_reserve_batch(self._ctx, int(n), int(nscenes)), self._ctx)
 
     def stats(self) -> dict:
         s = _capi.pbf_stats()
@@ -165,6 +168,52 @@ class Context:
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_step(self._ctx, _ptr(x), _ptr(v), _ptr(rho), x.shape[0], int(nsteps),
                                            _stream(x.device)), self._ctx)
+
+    def batch_step(self, x, v, scene_offsets: torch.Tensor, rho: Optional[torch.Tensor] = None,
+                   nsteps: int = 1) -> None:
+        """nsteps of every scene of a batch; scene s holds rows
+        scene_offsets[s]:scene_offsets[s+1] (int32 device tensor, nscenes+1)."""
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        off = scene_offsets
+        if off.dtype != torch.int32 or off.dim() != 1 or off.device != x.device or off.shape[0] < 2:
+            raise ValueError("scene_offsets: expected an int32 device tensor of nscenes+1 offsets")
+        off = off.contiguous()
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_batch_step(self._ctx, _ptr(x), _ptr(v), _ptr(rho), _ptr(off),
+                                                 off.shape[0] - 1, x.shape[0], int(nsteps), _stream(x.device)),
+                        self._ctx)
+
+
+    # -- reverse mode -----------------------------------------------------------
+    def rollout_forward(self, x, v, rho: Optional[torch.Tensor] = None, nsteps: int = 1,
+                        checkpoint_every: int = 10, scene_offsets: Optional[torch.Tensor] = None) -> int:
+        """pbf_rollout_forward: nsteps in place, checkpoints kept in the
+        context. Returns a token that rollout_backward checks."""
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        off, nscenes = None, 1
+        if scene_offsets is not None:
+            if scene_offsets.dtype != torch.int32 or scene_offsets.device != x.device:
+                raise ValueError("scene_offsets: expected an int32 device tensor of nscenes+1 offsets")
+            off = scene_offsets.contiguous()
+            nscenes = off.shape[0] - 1
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_rollout_forward(self._ctx, _ptr(x), _ptr(v), _ptr(rho), _ptr(off), nscenes,
+                                                      x.shape[0], int(nsteps), int(checkpoint_every),
+                                                      _stream(x.device)), self._ctx)
+        self._rollout_token = getattr(self, "_rollout_token", 0) + 1
+        return self._rollout_token
+
+    def rollout_backward(self, xbar, vbar, token: Optional[int] = None) -> None:
+        """pbf_rollout_backward: (dL/dx_N, dL/dv_N) -> (dL/dx_0, dL/dv_0) in place."""
+        if token is not None and token != getattr(self, "_rollout_token", None):
+            raise RuntimeError("the context ran another rollout since this one; its checkpoints are gone")
+        xbar = _check_vec3(xbar, "xbar")
+        vbar = _check_vec3(vbar, "vbar")
+        with torch.cuda.device(xbar.device):
+            _capi.check(self._lib.pbf_rollout_backward(self._ctx, _ptr(xbar), _ptr(vbar), _stream(xbar.device)),
+                        self._ctx)
 
 
 class NeighborList:
@@ -363,3 +412,77 @@ class PositionBasedFluidModel:
         torch.cuda.synchronize(dev)
         return (self.m_position.getValue().cpu().numpy(), self.m_velocity.getValue().cpu().numpy(),
                 self.m_density.getValue().cpu().numpy())
+
+
+class BatchedPositionBasedFluidModel(PositionBasedFluidModel):
+    """Many independent PositionBasedFluidModel scenes with shared parameters,
+    stepped together: the scenes' particles are concatenated in m_position /
+    m_velocity / m_density and m_sceneOffsets (int32, nscenes+1) delimits them.
+    Each scene's result equals a PositionBasedFluidModel step of that scene
+    alone (the reference runs one model per scene)."""
+
+    def __init__(self, smoothing_length: float = 0.0125, device="cuda:0", **params):
+        super().__init__(smoothing_length, device, **params)
+        self.m_sceneOffsets = ArrayField()
+
+    def setScenes(self, positions, velocities=None) -> None:
+        """Concatenates per-scene (n_s, 3) arrays or tensors."""
+        dev = self.ctx.device
+        xs = [torch.as_tensor(np.asarray(p, np.float32) if not torch.is_tensor(p) else p, dtype=torch.float32)
+              for p in positions]
+        if velocities is None:
+            vs = [torch.zeros_like(t) for t in xs]
+        else:
+            vs = [torch.as_tensor(np.asarray(q, np.float32) if not torch.is_tensor(q) else q, dtype=torch.float32)
+                  for q in velocities]
+        counts = [int(t.shape[0]) for t in xs]
+        off = np.zeros(len(counts) + 1, np.int64)
+        off[1:] = np.cumsum(counts)
+        self.m_position.setValue(torch.cat([t.reshape(-1, 3) for t in xs]).to(dev).contiguous())
+        self.m_velocity.setValue(torch.cat([t.reshape(-1, 3) for t in vs]).to(dev).contiguous())
+        self.m_sceneOffsets.setValue(torch.from_numpy(off.astype(np.int32)).to(dev))
+        self.ctx.reserve(int(off[-1]), len(counts))
+
+    def scene(self, s: int):
+        off = self.m_sceneOffsets.getValue().cpu().numpy()
+        sl = slice(int(off[s]), int(off[s + 1]))
+        rho = self.m_density.getValue()
+        return (self.m_position.getValue()[sl], self.m_velocity.getValue()[sl],
+                None if rho is None else rho[sl])
+
+    def step(self, dt: Optional[float] = None, nsteps: int = 1) -> None:
+        self._set_dt(dt)
+        x = self.m_position.getValue()
+        rho = self.m_density.getValue()
+        if rho is None or rho.shape[0] != x.shape[0]:
+            rho = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+            self.m_density.setValue(rho)
+        self.ctx.batch_step(x, self.m_velocity.getValue(), self.m_sceneOffsets.getValue(), rho, nsteps)
+
+
+class _Rollout(torch.autograd.Function):
+    @staticmethod
+    def forward(fctx, x0, v0, ctx, nsteps, checkpoint_every, scene_offsets):
+        x = x0.detach().clone().contiguous()
+        v = v0.detach().clone().contiguous()
+        rho = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+        fctx.pbf = ctx
+        fctx.token = ctx.rollout_forward(x, v, rho, nsteps, checkpoint_every, scene_offsets)
+        fctx.mark_non_differentiable(rho)
+        return x, v, rho
+
+    @staticmethod
+    def backward(fctx, gx, gv, grho):
+        xb = torch.zeros_like(gv) if gx is None else gx.detach().clone().contiguous()
+        vb = torch.zeros_like(xb) if gv is None else gv.detach().clone().contiguous()
+        fctx.pbf.rollout_backward(xb, vb, fctx.token)
+        return xb, vb, None, None, None, None
+
+
+def rollout(ctx: Context, x0: torch.Tensor, v0: torch.Tensor, nsteps: int, checkpoint_every: int = 10,
+            scene_offsets: Optional[torch.Tensor] = None):
+    """Differentiable nsteps of the fluid step (one scene, or a batch delimited
+    by scene_offsets): returns (x_N, v_N, rho_N); gradients flow to x0 and v0
+    through the library's adjoint kernels (rho_N carries none). The context
+    keeps one rollout: run backward before the next rollout on it."""
+    return _Rollout.apply(x0, v0, ctx, int(nsteps), int(checkpoint_every), scene_offsets)

@@ -1,0 +1,218 @@
+"""Reverse mode of the fluid step on the GPU (pbf_rollout_forward /
+pbf_rollout_backward and the torch.autograd wrapper), against the oracle's
+FP64 adjoint (itself pinned by central differences, test_oracle_adjoint.py)
+and FP64 central differences of the oracle's forward step.
+
+Bars: gradients within 1e-3 relative (norm-wise, and per component against
+1e-3 of the largest component) of the FP64 references; batched gradients
+bit-identical to per-scene ones; free fall within fp32 rounding of the closed
+form. Every call goes through the C ABI."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from parity_util import dev, say  # noqa: E402
+
+GRAD_REL = 1e-3
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def compressed_block(scene, dims=(7, 7, 7), seed=11):
+    """Lattice at spacing 0.0036 (< the 0.005 rest spacing): density above
+    rest, so lambda is active."""
+    return scene.jittered_lattice(dims, (0.3, 0.5, 0.3), 0.0036, 0.1, seed=seed)
+
+
+def gpu_rollout_grad(fluid, x0, v0, nsteps, every, cx, cv, scene_offsets=None):
+    ctx = fluid.Context("cuda:0")
+    x, v = dev(x0), dev(v0)
+    off = None if scene_offsets is None else torch.from_numpy(np.asarray(scene_offsets, np.int32)).to("cuda:0")
+    ctx.rollout_forward(x, v, None, nsteps, every, off)
+    xb, vb = dev(cx), dev(cv)
+    ctx.rollout_backward(xb, vb)
+    return x.cpu().numpy(), v.cpu().numpy(), xb.cpu().numpy(), vb.cpu().numpy(), ctx
+
+
+def assert_grad_close(got, want, what):
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    comp = np.abs(got - want).max() / np.abs(want).max()
+    say(f"{what}: norm-wise rel err {rel:.3e}, max component err / max component {comp:.3e}")
+    assert rel <= GRAD_REL and comp <= GRAD_REL, (what, rel, comp)
+
+
+@pytest.mark.parametrize("nsteps,every", [(1, 1), (3, 2), (4, 10)])
+def test_rollout_adjoint_vs_oracle_fp64(fluid, oracle, scene, nsteps, every):
+    say(f"rollout adjoint vs oracle FP64 adjoint, 343 compressed particles, {nsteps} steps, checkpoint {every}")
+    x0 = compressed_block(scene)
+    n = len(x0)
+    v0 = (0.05 * (scene.uniform01(12, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)
+    cx = (scene.uniform01(13, 3 * n).reshape(n, 3) - 0.5).astype(np.float32)
+    cv = (1e-3 * (scene.uniform01(14, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)
+    xN, vN, xb, vb, _ = gpu_rollout_grad(fluid, x0, v0, nsteps, every, cx, cv)
+    P = oracle.make_params(double=True)
+    xo, vo, _, _ = oracle.step(P, x0.astype(np.float64), v0.astype(np.float64), nsteps)
+    assert np.abs(xN - xo).max() < 1e-5  # the rollout's forward is the step
+    xbo, vbo = oracle.rollout_adjoint(P, x0.astype(np.float64), v0.astype(np.float64), nsteps,
+                                      cx.astype(np.float64), cv.astype(np.float64))
+    assert_grad_close(xb, xbo, "dL/dx0")
+    assert_grad_close(vb, vbo, "dL/dv0")
+
+
+def test_rollout_adjoint_free_fall(fluid, scene):
+    say("rollout adjoint, isolated particles: closed form")
+    x0 = scene.jittered_lattice((4, 4, 4), (0.3, 0.5, 0.3), 0.05, 0.1, seed=3)
+    n = len(x0)
+    cx = (scene.uniform01(5, 3 * n).reshape(n, 3) - 0.5).astype(np.float32)
+    cv = (scene.uniform01(6, 3 * n).reshape(n, 3) - 0.5).astype(np.float32)
+    N = 7
+    _, _, xb, vb, _ = gpu_rollout_grad(fluid, x0, np.zeros_like(x0), N, 3, cx, cv)
+    np.testing.assert_allclose(xb, cx, rtol=1e-5, atol=1e-6)
+    np.testing.assert_allclose(vb, N * 1e-3 * cx + cv, rtol=1e-5, atol=1e-6)
+
+
+def test_rollout_backward_repeatable_and_token(fluid, scene):
+    say("rollout backward twice: identical; stale token rejected")
+    x0 = compressed_block(scene, seed=19)
+    n = len(x0)
+    ctx = fluid.Context("cuda:0")
+    x, v = dev(x0), dev(np.zeros_like(x0))
+    tok = ctx.rollout_forward(x, v, None, 5, 2)
+    c = dev(scene.uniform01(20, 3 * n).reshape(n, 3) - 0.5)
+    outs = []
+    for _ in range(2):
+        xb, vb = c.clone(), torch.zeros_like(c)
+        ctx.rollout_backward(xb, vb, tok)
+        outs.append((xb.cpu().numpy(), vb.cpu().numpy()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+    ctx.rollout_forward(dev(x0), dev(np.zeros_like(x0)), None, 1, 1)
+    with pytest.raises(RuntimeError):
+        ctx.rollout_backward(c.clone(), torch.zeros_like(c), tok)
+
+
+def test_batched_rollout_gradients_equal_per_scene(fluid, scene):
+    say("batched rollout adjoint vs per-scene rollouts, 3 scenes, 4 steps")
+    scenes = [compressed_block(scene, (5, 6, 7), 30), compressed_block(scene, (4, 4, 4), 31),
+              scene.jittered_lattice((9, 5, 6), (0.5, 0.4, 0.2), 0.004, 0.1, seed=32)]
+    counts = [len(s) for s in scenes]
+    off = np.zeros(4, np.int64)
+    off[1:] = np.cumsum(counts)
+    x0 = np.concatenate(scenes)
+    n = len(x0)
+    cx = (scene.uniform01(33, 3 * n).reshape(n, 3) - 0.5).astype(np.float32)
+    cv = (1e-3 * (scene.uniform01(34, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)
+    _, _, xb, vb, _ = gpu_rollout_grad(fluid, x0, np.zeros_like(x0), 4, 3, cx, cv, off)
+    for s in range(3):
+        a, b = off[s], off[s + 1]
+        _, _, xbs, vbs, _ = gpu_rollout_grad(fluid, x0[a:b], np.zeros_like(x0[a:b]), 4, 3, cx[a:b], cv[a:b])
+        np.testing.assert_array_equal(xb[a:b].view(np.uint32), xbs.view(np.uint32))
+        np.testing.assert_array_equal(vb[a:b].view(np.uint32), vbs.view(np.uint32))
+
+
+def test_rollout_out_of_grid_is_reported(fluid, capi, scene):
+    say("rollout adjoint with particles leaving the grid: PBF_ERR_UNSUPPORTED")
+    x0 = scene.jittered_lattice((6, 6, 6), (-0.03, 0.3, 0.3), 0.005, 0.1, seed=41)
+    ctx = fluid.Context("cuda:0")
+    x, v = dev(x0), dev(np.zeros_like(x0))
+    ctx.rollout_forward(x, v, None, 2, 1)
+    xb, vb = torch.ones_like(x), torch.zeros_like(x)
+    with pytest.raises(capi.PBFError) as e:
+        ctx.rollout_backward(xb, vb)
+    assert e.value.status == capi.PBF_ERR_UNSUPPORTED
+
+
+# ------------------------------------------------------------------ loss of the differentiable workload
+def node_density(x, cW, h, nodes=32, xp=np):
+    """rho_g = sum_j m m_factor W(|x_g - x_j|, h) on the cell-centre nodes
+    ((i + 0.5)/nodes per axis) of [0,1]^3; node spacing 1/32 > 2h, so a
+    particle reaches at most its nearest node."""
+    idx = xp.floor(x * nodes)
+    idx = idx.clip(0, nodes - 1)
+    g = (idx + 0.5) / nodes
+    d = x - g
+    if xp is np:
+        r = np.sqrt((d * d).sum(1))
+        t = np.maximum(1.0 - r / h, 0.0)
+        flat = (idx[:, 0] + nodes * (idx[:, 1] + nodes * idx[:, 2])).astype(np.int64)
+        out = np.zeros(nodes ** 3)
+        np.add.at(out, flat, cW * t ** 3)
+        return out
+    r = torch.sqrt((d * d).sum(1) + 1e-30)
+    t = torch.clamp(1.0 - r / h, min=0.0)
+    flat = (idx[:, 0] + nodes * (idx[:, 1] + nodes * idx[:, 2])).long()
+    out = torch.zeros(nodes ** 3, dtype=x.dtype, device=x.device)
+    return out.index_add(0, flat, cW * t ** 3)
+
+
+def test_differentiable_workload_vs_fd(fluid, oracle, scene):
+    """The differentiable workload at the FD size: a 16^3 = 4096-particle block,
+    50 steps, checkpoint every 10; L = |mean(x_50) - c*|^2 + sum_g (rho_g -
+    rho*_g)^2 with targets from a run with seeded random v0; gradient w.r.t.
+    v0 = 0 against FP64 central differences of the oracle on 24 seeded
+    components and against the oracle's FP64 adjoint on all of them."""
+    say("differentiable workload: 4096 particles, 50 steps, gradient vs FP64 FD and FP64 adjoint")
+    x0 = scene.jittered_lattice((16, 16, 16), (0.3, 0.5, 0.3), 0.005, 0.1, seed=5)
+    n = len(x0)
+    nsteps, every = 50, 10
+    Pd = oracle.make_params(double=True)
+    st = fluid.Context("cuda:0")
+    st.reserve(n)
+    cW = float(np.float32(st.stats()["density_factor"]) * np.float32(st.stats()["particle_mass"])) * 15.0 / (
+        np.pi * 0.0125 ** 3)
+    vt = 0.2 * (scene.uniform01(6, 3 * n).reshape(n, 3) - 0.5)
+    xt, _, _, _ = oracle.step(Pd, x0.astype(np.float64), vt, nsteps)
+    c_star = xt.mean(0)
+    rho_star = node_density(xt, cW, 0.0125)
+
+    def loss_np(x):
+        m = x.mean(0) - c_star
+        return float((m * m).sum() + ((node_density(x, cW, 0.0125) - rho_star) ** 2).sum())
+
+    # GPU gradient through the autograd wrapper
+    ctx = fluid.Context("cuda:0")
+    xg = dev(x0)
+    vg = torch.zeros_like(xg, requires_grad=True)
+    xN, vN, _ = fluid.rollout(ctx, xg, vg, nsteps, every)
+    xN64 = xN.double()
+    m = xN64.mean(0) - torch.from_numpy(c_star).to(xN.device)
+    rho_g = node_density(xN64, cW, 0.0125, xp=torch)
+    L = (m * m).sum() + ((rho_g - torch.from_numpy(rho_star).to(xN.device)) ** 2).sum()
+    L.backward()
+    g_gpu = vg.grad.cpu().numpy().astype(np.float64)
+
+    # FP64 adjoint of the oracle with the same loss gradient at its own x_N
+    xo, _, _, _ = oracle.step(Pd, x0.astype(np.float64), np.zeros((n, 3)), nsteps)
+    xt_ = torch.from_numpy(xo).requires_grad_(True)
+    m_ = xt_.mean(0) - torch.from_numpy(c_star)
+    L_ = (m_ * m_).sum() + ((node_density(xt_, cW, 0.0125, xp=torch) - torch.from_numpy(rho_star)) ** 2).sum()
+    L_.backward()
+    _, g_adj = oracle.rollout_adjoint(Pd, x0.astype(np.float64), np.zeros((n, 3)), nsteps, xt_.grad.numpy(),
+                                      np.zeros((n, 3)))
+    assert_grad_close(g_gpu, g_adj, "GPU vs oracle FP64 adjoint, dL/dv0")
+
+    # FP64 central differences on 24 seeded components
+    rng = np.random.default_rng(7)
+    comps = [(int(rng.integers(n)), int(rng.integers(3))) for _ in range(24)]
+    eps = 1e-6
+    fd = []
+    for i, c in comps:
+        vp = np.zeros((n, 3))
+        vp[i, c] = eps
+        Lp = loss_np(oracle.step(Pd, x0.astype(np.float64), vp, nsteps)[0])
+        Lm = loss_np(oracle.step(Pd, x0.astype(np.float64), -vp, nsteps)[0])
+        fd.append((Lp - Lm) / (2 * eps))
+    fd = np.array(fd)
+    ga = np.array([g_adj[i, c] for i, c in comps])
+    gg = np.array([g_gpu[i, c] for i, c in comps])
+    assert_grad_close(ga, fd, "oracle FP64 adjoint vs FP64 FD (24 components)")
+    assert_grad_close(gg, fd, "GPU vs FP64 FD (24 components)")
