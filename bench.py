@@ -118,6 +118,113 @@ def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
             "speedup": round(vb_ / vs_, 3)}
 
 
+def node_density(torch, x, cW, h, nodes=32, scene_of=None, nscenes=1):
+    """rho_g = sum_j m m_factor W(|x_g - x_j|, h) on the cell-centre nodes of
+    [0,1]^3 (spacing 1/32 > 2h: a particle reaches at most its nearest node),
+    one grid per scene."""
+    idx = torch.clamp(torch.floor(x * nodes), 0, nodes - 1)
+    d = x - (idx + 0.5) / nodes
+    r = torch.sqrt((d * d).sum(1) + 1e-30)
+    t = torch.clamp(1.0 - r / h, min=0.0)
+    flat = (idx[:, 0] + nodes * (idx[:, 1] + nodes * idx[:, 2])).long()
+    if scene_of is not None:
+        flat = flat + scene_of * nodes ** 3
+    out = torch.zeros(nscenes * nodes ** 3, dtype=x.dtype, device=x.device)
+    return out.index_add(0, flat, cW * t ** 3)
+
+
+def differentiable(torch, fluid, scene, nsteps=50, every=10):
+    """65,536-particle jittered block (64 x 32 x 32 at (0.3, 0.5, 0.3), seed 5),
+    50 steps, checkpoint every 10: L = |mean(x_50) - c*|^2 + sum_g (rho_g -
+    rho*_g)^2 on 32^3 nodes, targets from a run with seeded random v0 (seed 6);
+    gradient w.r.t. v0. Forward = rollout + loss, backward = loss backward +
+    rollout adjoint, each timed with HIP events on the stream the library runs
+    on."""
+    dev = torch.device("cuda:0")
+    x0 = torch.from_numpy(scene.block_64k()).to(dev)
+    n = x0.shape[0]
+    ctx = fluid.Context(dev)
+    ctx.reserve(n)
+    st = ctx.stats()
+    cW = float(st["density_factor"]) * float(st["particle_mass"]) * 15.0 / (np.pi * 0.0125 ** 3)
+    vt = torch.from_numpy((0.2 * (scene.uniform01(6, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)).to(dev)
+    xt, vt2 = x0.clone(), vt.clone()
+    ctx.step(xt, vt2, None, nsteps)
+    c_star = xt.mean(0)
+    rho_star = node_density(torch, xt, cW, 0.0125)
+    stream = torch.cuda.current_stream(dev)
+
+    def run():
+        v0 = torch.zeros_like(x0, requires_grad=True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        torch.cuda.synchronize()
+        e0.record(stream)
+        xN, vN, _ = fluid.rollout(ctx, x0, v0, nsteps, every)
+        m = xN.mean(0) - c_star
+        L = (m * m).sum() + ((node_density(torch, xN, cW, 0.0125) - rho_star) ** 2).sum()
+        e1.record(stream)
+        L.backward()
+        e2.record(stream)
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1), e1.elapsed_time(e2), float(L), v0.grad
+
+    run()
+    f_ms, b_ms, L, g = run()
+    return {"workload": "65,536-particle block, 50 steps, checkpoint every 10, loss centroid + 32^3 density L2",
+            "particles": n, "steps": nsteps, "checkpoint_every": every, "fwd_ms": round(f_ms, 3),
+            "bwd_ms": round(b_ms, 3), "bwd_fwd_ratio": round(b_ms / f_ms, 3), "loss": L,
+            "grad_norm": float(g.norm())}
+
+
+def batched_inverse(torch, fluid, scene, adam_steps=20, nsteps=30, every=10, lr=0.05):
+    """64 scenes, n_s log-uniform in [8192, 32768] (seed 8), per-scene initial
+    velocity field optimised by Adam against per-scene targets (centroid and
+    32^3 density field of a run from a seeded random v0); every Adam step is a
+    30-step batched rollout and its adjoint."""
+    dev = torch.device("cuda:0")
+    b = scene.batched_lattice_scenes(nscenes=64, nmin=8192, nmax=32768, seed=8)
+    n = int(b.offsets[-1])
+    off = torch.from_numpy(b.offsets.astype(np.int32)).to(dev)
+    sid = torch.from_numpy(np.repeat(np.arange(64), b.counts)).to(dev)
+    counts = torch.from_numpy(b.counts.astype(np.float32)).to(dev)
+    x0 = torch.from_numpy(b.pos).to(dev)
+    ctx = fluid.Context(dev)
+    ctx.reserve(n, 64)
+    st = ctx.stats()
+    cW = float(st["density_factor"]) * float(st["particle_mass"]) * 15.0 / (np.pi * 0.0125 ** 3)
+
+    def centroids(x):
+        return torch.zeros(64, 3, device=dev).index_add(0, sid, x) / counts[:, None]
+
+    vt = torch.from_numpy((0.2 * (scene.uniform01(9, 3 * n).reshape(n, 3) - 0.5)).astype(np.float32)).to(dev)
+    xt = x0.clone()
+    ctx.batch_step(xt, vt, off, None, nsteps)
+    c_star = centroids(xt)
+    rho_star = node_density(torch, xt, cW, 0.0125, scene_of=sid, nscenes=64)
+    v0 = torch.zeros_like(x0, requires_grad=True)
+    opt = torch.optim.Adam([v0], lr=lr)
+    losses = []
+    stream = torch.cuda.current_stream(dev)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for it in range(adam_steps + 1):
+        if it == 1:  # the first step also sizes the context's reservations
+            torch.cuda.synchronize()
+            e0.record(stream)
+        opt.zero_grad()
+        xN, _, _ = fluid.rollout(ctx, x0, v0, nsteps, every, off)
+        m = centroids(xN) - c_star
+        L = (m * m).sum() + ((node_density(torch, xN, cW, 0.0125, scene_of=sid, nscenes=64) - rho_star) ** 2).sum()
+        L.backward()
+        opt.step()
+        losses.append(float(L))
+    e1.record(stream)
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / adam_steps
+    return {"workload": "64 scenes, n_s log-uniform [8192, 32768], seed 8; 30-step rollout + adjoint per Adam step",
+            "scenes": 64, "particles": n, "adam_steps": adam_steps, "ms_per_adam_step": round(ms, 3),
+            "loss_first": losses[0], "loss_last": losses[-1]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -130,6 +237,7 @@ def main():
     ap.add_argument("--batched-steps", type=int, default=20, help="steps of the 256-scene batched comparison")
     ap.add_argument("--serial-steps", type=int, default=5, help="steps of its serial per-scene loop")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-diff", action="store_true", help="skip the differentiable and batched-inverse workloads")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -233,6 +341,9 @@ def main():
     if not args.no_batched:
         del x, v, rho, ctx
         out["batched_vs_serial"] = batched_vs_serial(torch, fluid, pkg.scene, args.batched_steps, args.serial_steps)
+    if not args.no_diff:
+        out["differentiable"] = differentiable(torch, fluid, pkg.scene)
+        out["batched_inverse"] = batched_inverse(torch, fluid, pkg.scene)
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(x_np, args.cpu_steps, threads)

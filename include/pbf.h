@@ -67,8 +67,26 @@ typedef struct pbf_params {
   int list_slots;          /* neighbor-list reservation, 16-bit slots per particle
                               (0 = 96). Both reservations grow automatically
                               between calls, see pbf_step */
-  unsigned flags;          /* reserved, must be 0 */
+  float viscosity;         /* ImplicitViscosity m_viscosity; the model sets 1
+                              (PositionBasedFluidModel.cpp:70) */
+  int viscosity_iterations;/* ImplicitViscosity m_maxInteration (5,
+                              ImplicitViscosity.cu:71-76) */
+  float box_lo[3];         /* box of StaticBoundary::loadCube (fluid inside,
+                              StaticBoundary.cpp:72-77) */
+  float box_hi[3];
+  float box_spacing;       /* loadCube distance: sampling of the box field
+                              (BoundaryConstraint.cu:109-117) */
+  float normal_friction;   /* BoundaryConstraint m_normal_friction (0.95) */
+  float tangent_friction;  /* BoundaryConstraint m_tangent_friction (0) */
+  unsigned flags;          /* PBF_FLAG_* (0: the plain PBD step) */
 } pbf_params;
+
+/* pbf_params.flags */
+#define PBF_FLAG_VISCOSITY 1u /* ImplicitViscosity after the PBD solve, as the
+                                 reference model's step does */
+#define PBF_FLAG_BOX 2u       /* box boundary on x, v before every step (the
+                                 StaticBoundary root node advances before the
+                                 fluid, SceneGraph.cpp:101-114) */
 
 /* Diagnostics of the last neighbor search of a context. */
 typedef struct pbf_stats {
@@ -99,7 +117,9 @@ const char* pbf_status_string(int status);
 
 /* Reference defaults: h 0.0125, d 0.005, rho0 1000, dt 0.001, g (0,-9.8,0),
  * 5 iterations, lambda constants 1000 / 0.1, correction 10, pair eps 1e-6,
- * scene [0,1]^3. */
+ * scene [0,1]^3; viscosity 1 with 5 iterations, box [0,1]^3 sampled at 0.005
+ * with normal friction 0.95 and tangential friction 0; flags 0 (the
+ * viscosity and the box are off until their flag is set). */
 void pbf_params_default(pbf_params* p);
 
 /* Creates a context on the current HIP device. Replaces module construction
@@ -196,6 +216,19 @@ int pbf_step_neighbors(pbf_ctx* ctx, int* d_offsets, int* d_elems, long long cap
 /* Diagnostics of the last neighbor search. Blocking (synchronises the stream
  * of the last call). */
 int pbf_get_stats(pbf_ctx* ctx, pbf_stats* out);
+
+/* ImplicitViscosity::constrain (ImplicitViscosity.cu:99-128) over a caller
+ * CSR list: viscosity_iterations Jacobi rounds on v (in place) with weights
+ * from the positions d_x. */
+int pbf_viscosity(pbf_ctx* ctx, const float* d_x, float* d_v, const int* d_offsets, const int* d_elems, int n,
+                  void* stream);
+
+/* BoundaryConstraint::constrain (BoundaryConstraint.cu:30-84) with the box of
+ * box_lo/box_hi/box_spacing: particles at or outside the walls are projected
+ * back inside (plus a jitter of at most 1e-4 per particle id, drawn by this
+ * library's own counter-based generator rather than curand), their normal
+ * velocity reflected and damped by the normal friction. x, v in place. */
+int pbf_box_constrain(pbf_ctx* ctx, float* d_x, float* d_v, int n, void* stream);
 
 /* Per-pass GPU timing (HIP events recorded on the call's stream around each
  * pass of pbf_step; the reference's GTimer, Core/Utility/GTimer.cpp:21-27, has

@@ -28,7 +28,9 @@ def _params_struct(real):
     class P(C.Structure):
         _fields_ = [("h", real), ("d", real), ("rho0", real), ("dt", real), ("g", real * 3),
                     ("iters", C.c_int), ("lam_rest", real), ("lam_eps", real), ("corr", real),
-                    ("pair_eps", real), ("lo", real * 3), ("hi", real * 3), ("dx_mode", C.c_int)]
+                    ("pair_eps", real), ("lo", real * 3), ("hi", real * 3), ("dx_mode", C.c_int),
+                    ("mu", real), ("vis_iters", C.c_int), ("box_lo", real * 3), ("box_hi", real * 3),
+                    ("box_spacing", real), ("nfric", real), ("tfric", real), ("flags", C.c_int)]
     return P
 
 
@@ -37,7 +39,12 @@ ParamsD = _params_struct(C.c_double)
 
 DEFAULTS = dict(h=0.0125, d=0.005, rho0=1000.0, dt=0.001, g=(0.0, -9.8, 0.0), iters=5,
                 lam_rest=1000.0, lam_eps=0.1, corr=10.0, pair_eps=1e-6,
-                lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dx_mode=0)
+                lo=(0.0, 0.0, 0.0), hi=(1.0, 1.0, 1.0), dx_mode=0,
+                mu=1.0, vis_iters=5, box_lo=(0.0, 0.0, 0.0), box_hi=(1.0, 1.0, 1.0), box_spacing=0.005,
+                nfric=0.95, tfric=0.0, flags=0)
+
+FLAG_VISCOSITY = 1
+FLAG_BOX = 2
 
 
 def make_params(double: bool = False, **kw):
@@ -46,7 +53,7 @@ def make_params(double: bool = False, **kw):
     P = ParamsD if double else ParamsF
     p = P()
     for k, val in v.items():
-        if k in ("g", "lo", "hi"):
+        if k in ("g", "lo", "hi", "box_lo", "box_hi"):
             arr = getattr(p, k)
             for i in range(3):
                 arr[i] = val[i]
@@ -89,6 +96,12 @@ def lib():
             "orc_integrate_d": (None, [PD, dp, dp, dp, C.c_int]),
             "orc_step_f": (C.c_longlong, [PF, fp, fp, fp, C.c_int, C.c_int]),
             "orc_step_d": (C.c_longlong, [PD, dp, dp, dp, C.c_int, C.c_int]),
+            "orc_viscosity_f": (None, [PF, fp, fp, C.c_int, ip, ip]),
+            "orc_vis_weight_f": (C.c_float, [C.c_float, C.c_float]),
+            "orc_box_sdf_f": (C.c_longlong, [fp, fp, C.c_float, C.c_int, ip, fp, fp, fp, C.c_longlong]),
+            "orc_sdf_distance_f": (None, [fp, fp, C.c_float, fp, C.c_int, fp, fp]),
+            "orc_box_constrain_f": (None, [PF, fp, fp, C.c_int]),
+            "orc_box_jitter": (C.c_float, [C.c_int]),
             "orc_rollout_adjoint_f": (None, [PF, fp, fp, C.c_int, C.c_int, fp, fp]),
             "orc_rollout_adjoint_d": (None, [PD, dp, dp, C.c_int, C.c_int, dp, dp]),
         }
@@ -228,3 +241,55 @@ def rollout_adjoint(params, x0, v0, nsteps, xbar, vbar):
     fn = lib().orc_rollout_adjoint_d if dbl else lib().orc_rollout_adjoint_f
     fn(C.byref(params), _ptr(x0, ct), _ptr(v0, ct), len(x0), int(nsteps), _ptr(xb, ct), _ptr(vb, ct))
     return xb, vb
+
+
+def viscosity(params, x, v, off, el):
+    """ImplicitViscosity::constrain on the list (off, el); returns the new v."""
+    x = _f32(x)
+    v = np.array(v, dtype=np.float32, copy=True, order="C")
+    lib().orc_viscosity_f(C.byref(params), _ptr(x, C.c_float), _ptr(v, C.c_float), len(x),
+                          _ptr(np.ascontiguousarray(off, np.int32), C.c_int),
+                          _ptr(np.ascontiguousarray(el, np.int32), C.c_int))
+    return v
+
+
+def vis_weight(r, h):
+    return lib().orc_vis_weight_f(float(r), float(h))
+
+
+def box_sdf(lo, hi, spacing, inverted=True):
+    """(dims, left, h, samples) of the sampled box field of setCube."""
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    dims = np.zeros(3, np.int32)
+    left = np.zeros(3, np.float32)
+    h = np.zeros(3, np.float32)
+    total = lib().orc_box_sdf_f(_ptr(lo, C.c_float), _ptr(hi, C.c_float), float(spacing), int(inverted),
+                                _ptr(dims, C.c_int), _ptr(left, C.c_float), _ptr(h, C.c_float), None, 0)
+    samples = np.zeros(total, np.float32)
+    lib().orc_box_sdf_f(_ptr(lo, C.c_float), _ptr(hi, C.c_float), float(spacing), int(inverted),
+                        _ptr(dims, C.c_int), _ptr(left, C.c_float), _ptr(h, C.c_float),
+                        _ptr(samples, C.c_float), total)
+    return dims, left, h, samples.reshape(dims[2], dims[1], dims[0])
+
+
+def sdf_distance(lo, hi, spacing, x):
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    x = _f32(x)
+    d = np.zeros(len(x), np.float32)
+    nrm = np.zeros((len(x), 3), np.float32)
+    lib().orc_sdf_distance_f(_ptr(lo, C.c_float), _ptr(hi, C.c_float), float(spacing), _ptr(x, C.c_float), len(x),
+                             _ptr(d, C.c_float), _ptr(nrm, C.c_float))
+    return d, nrm
+
+
+def box_constrain(params, x, v):
+    x = np.array(x, dtype=np.float32, copy=True, order="C")
+    v = np.array(v, dtype=np.float32, copy=True, order="C")
+    lib().orc_box_constrain_f(C.byref(params), _ptr(x, C.c_float), _ptr(v, C.c_float), len(x))
+    return x, v
+
+
+def box_jitter(i):
+    return lib().orc_box_jitter(int(i))

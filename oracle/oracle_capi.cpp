@@ -40,8 +40,10 @@ long long nq(const Params<R>* p, const R* x, int n, int* offsets, int* elems, lo
 template <class R>
 long long run_steps(const Params<R>* p, R* x, R* v, R* rho, int n, int nsteps) {
   CSR c;
+  SDF<R> sdf;
+  if (p->flags & 2) sdf = box_sdf(p->box_lo, p->box_hi, p->box_spacing, true);
   for (int s = 0; s < nsteps; ++s)
-    step(*p, reinterpret_cast<V3<R>*>(x), reinterpret_cast<V3<R>*>(v), rho, n, &c);
+    step(*p, reinterpret_cast<V3<R>*>(x), reinterpret_cast<V3<R>*>(v), rho, n, &c, (p->flags & 2) ? &sdf : nullptr);
   return nsteps > 0 ? (long long)c.offsets[n] : 0;
 }
 
@@ -161,5 +163,49 @@ void orc_rollout_adjoint_f(const Params<float>* p, const float* x0, const float*
   rollout_adjoint(*p, reinterpret_cast<const V3<float>*>(x0), reinterpret_cast<const V3<float>*>(v0), n, nsteps,
                   reinterpret_cast<V3<float>*>(xbar), reinterpret_cast<V3<float>*>(vbar));
 }
+
+// ImplicitViscosity::constrain on a caller list.
+void orc_viscosity_f(const Params<float>* p, const float* x, float* v, int n, const int* off, const int* el) {
+  CSR c = to_csr<float>(off, el, n);
+  viscosity(*p, reinterpret_cast<const V3<float>*>(x), reinterpret_cast<V3<float>*>(v), n, c);
+}
+float orc_vis_weight_f(float r, float h) { return vis_weight(r, h); }
+
+// Box SDF of setCube(lo, hi, spacing, inverted): dims (3 ints), left, h (3
+// floats each); samples copied when `samples` is non-null and holds cap.
+long long orc_box_sdf_f(const float* lo, const float* hi, float spacing, int inverted, int* dims, float* left,
+                        float* h, float* samples, long long cap) {
+  SDF<float> f = box_sdf(lo, hi, spacing, inverted != 0);
+  dims[0] = f.nx;
+  dims[1] = f.ny;
+  dims[2] = f.nz;
+  for (int a = 0; a < 3; ++a) {
+    left[a] = f.left[a];
+    h[a] = f.hs[a];
+  }
+  const long long total = (long long)f.d.size();
+  if (samples && cap >= total) std::memcpy(samples, f.d.data(), sizeof(float) * (size_t)total);
+  return total;
+}
+
+// Distance and normal at points of the box field (inverted: fluid inside).
+void orc_sdf_distance_f(const float* lo, const float* hi, float spacing, const float* x, int n, float* d,
+                        float* normal) {
+  SDF<float> f = box_sdf(lo, hi, spacing, true);
+  for (int i = 0; i < n; ++i) {
+    V3<float> nrm;
+    sdf_distance(f, reinterpret_cast<const V3<float>*>(x)[i], d[i], nrm);
+    normal[3 * i] = nrm.x;
+    normal[3 * i + 1] = nrm.y;
+    normal[3 * i + 2] = nrm.z;
+  }
+}
+
+void orc_box_constrain_f(const Params<float>* p, float* x, float* v, int n) {
+  SDF<float> f = box_sdf(p->box_lo, p->box_hi, p->box_spacing, true);
+  box_constrain(*p, f, reinterpret_cast<V3<float>*>(x), reinterpret_cast<V3<float>*>(v), n);
+}
+
+float orc_box_jitter(int id) { return box_jitter(id); }
 
 }  // extern "C"

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 
 namespace pbf_oracle {
 
@@ -347,11 +348,22 @@ void integrate(const Params<R>& p, V3<R>* x, V3<R>* v, const V3<R>* f, int n) {
   }
 }
 
-// PositionBasedFluidModel::step (PositionBasedFluidModel.cpp:83-101) without
-// the viscosity module: begin (force := 0) -> NeighborQuery::compute on x^n ->
-// integrate -> DensityPBD::constrain.
+// One frame of a scene holding StaticBoundary (the box, flags & 2) and the
+// fluid: the boundary node advances first (SceneGraph::takeOneFrame traverses
+// top-down, Framework/Framework/SceneGraph.cpp:101-114, Node.cpp:393-405),
+// then PositionBasedFluidModel::step (PositionBasedFluidModel.cpp:83-101):
+// begin (force := 0) -> NeighborQuery::compute on x^n -> integrate ->
+// DensityPBD::constrain -> ImplicitViscosity::constrain (flags & 1).
 template <class R>
-void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out) {
+void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out, const SDF<R>* sdf) {
+  if (p.flags & 2) {
+    SDF<R> local_sdf;
+    if (!sdf) {
+      local_sdf = box_sdf(p.box_lo, p.box_hi, p.box_spacing, true);
+      sdf = &local_sdf;
+    }
+    box_constrain(p, *sdf, x, v, n);
+  }
   Grid<R> g = grid_setup(p.h, p.lo, p.hi);
   CSR local;
   CSR& nb = nb_out ? *nb_out : local;
@@ -372,6 +384,227 @@ void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out) {
   }
   integrate<R>(q, x, v, nullptr, n);
   pbd_constrain(q, x, v, n, nb, rho);
+  if (p.flags & 1) viscosity(p, x, v, n, nb);
+}
+
+// ---------------------------------------------------------------------------
+// ImplicitViscosity (Dynamics/ParticleSystem/ImplicitViscosity.cu).
+
+// VB_VisWeight, ImplicitViscosity.cu:9-18: 45/(13 pi h^3) (1 - r/h), 0 beyond h.
+template <class R> R vis_weight(R r, R h) {
+  const R q = r / h;
+  if (q > (R)1) return (R)0;
+  const R d = (R)1 - q;
+  const R RR = h * h;
+  return (R)45 / ((R)13 * (R)kPi * RR * h) * d;
+}
+
+// ImplicitViscosity::constrain (:99-128) with K_ApplyViscosity (:20-62):
+// Jacobi iterations v_i = v_old,i/(1+b) + (sum_j w_ij v_j / sum_j w_ij) b/(1+b),
+// b = dt mu / h (0 when sum_j w_ij < EPSILON), pairs with r > EPSILON.
+template <class R>
+void viscosity(const Params<R>& p, const V3<R>* x, V3<R>* v, int n, const CSR& nb) {
+  std::vector<V3<R>> v_old(v, v + n), v_buf(n);
+  for (int t = 0; t < p.vis_iters; ++t) {
+    std::copy(v, v + n, v_buf.begin());
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; ++i) {
+      V3<R> dv{0, 0, 0};
+      R tw = (R)0;
+      for (int e = nb.offsets[i]; e < nb.offsets[i + 1]; ++e) {
+        const int j = nb.elems[e];
+        const R r = norm(sub(x[i], x[j]));
+        if (r > p.pair_eps) {
+          const R w = vis_weight(r, p.h);
+          tw += w;
+          dv.x += w * v_buf[j].x;
+          dv.y += w * v_buf[j].y;
+          dv.z += w * v_buf[j].z;
+        }
+      }
+      R b = p.dt * p.mu / p.h;
+      b = tw < p.pair_eps ? (R)0 : b;
+      tw = tw < p.pair_eps ? (R)1 : tw;
+      dv = {dv.x / tw, dv.y / tw, dv.z / tw};
+      const R ob = (R)1 + b;
+      v[i] = {v_old[i].x / ob + dv.x * b / ob, v_old[i].y / ob + dv.y * b / ob, v_old[i].z / ob + dv.z * b / ob};
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Box boundary: DistanceField3D (Framework/Topology/DistanceField3D.cu/.h) and
+// K_ConstrainSDF (Dynamics/ParticleSystem/BoundaryConstraint.cu:30-67).
+// CUDA's min/max on floats return the non-NaN operand, as std::fmin/fmax do.
+
+template <class R> static inline R seg_distance(const V3<R>& pos, const V3<R>& lo, const V3<R>& hi) {
+  // DistanceToSegment, DistanceField3D.cu:14-32
+  V3<R> seg = sub(hi, lo);
+  const V3<R> e1 = sub(pos, lo), e2 = sub(pos, hi);
+  if (dot(e1, seg) < (R)0) return norm(e1);
+  const V3<R> nseg{-seg.x, -seg.y, -seg.z};
+  if (dot(e2, nseg) < (R)0) return norm(e2);
+  const R length1 = dot(e1, e1);
+  // Vector::normalize (Core/Vector/vector_3d.inl:220-224): glm::normalize =
+  // x * (1/sqrt(dot(x, x))) when |x| > FLT_EPSILON, else 0
+  const R len = norm(seg);
+  if (len > std::numeric_limits<R>::epsilon()) {
+    const R is = (R)1 / std::sqrt(dot(seg, seg));
+    seg = {seg.x * is, seg.y * is, seg.z * is};
+  } else {
+    seg = {0, 0, 0};
+  }
+  const R length2 = dot(e1, seg);
+  return std::sqrt(length1 - length2 * length2);
+}
+
+template <class R>
+static inline R square_distance(const V3<R>& pos, const V3<R>& lo, const V3<R>& hi, int axis) {
+  // DistanceToSqure, DistanceField3D.cu:34-88
+  V3<R> n, c1, c2, c3, c4;
+  R loc[2], hic[2], pp[2];
+  if (axis == 0) {
+    c1 = {lo.x, lo.y, lo.z}; c2 = {lo.x, hi.y, lo.z}; c3 = {lo.x, hi.y, hi.z}; c4 = {lo.x, lo.y, hi.z};
+    n = {1, 0, 0};
+    loc[0] = lo.y; loc[1] = lo.z; hic[0] = hi.y; hic[1] = hi.z; pp[0] = pos.y; pp[1] = pos.z;
+  } else if (axis == 1) {
+    c1 = {lo.x, lo.y, lo.z}; c2 = {lo.x, lo.y, hi.z}; c3 = {hi.x, lo.y, hi.z}; c4 = {hi.x, lo.y, lo.z};
+    n = {0, 1, 0};
+    loc[0] = lo.x; loc[1] = lo.z; hic[0] = hi.x; hic[1] = hi.z; pp[0] = pos.x; pp[1] = pos.z;
+  } else {
+    c1 = {lo.x, lo.y, lo.z}; c2 = {hi.x, lo.y, lo.z}; c3 = {hi.x, hi.y, lo.z}; c4 = {lo.x, hi.y, lo.z};
+    n = {0, 0, 1};
+    loc[0] = lo.x; loc[1] = lo.y; hic[0] = hi.x; hic[1] = hi.y; pp[0] = pos.x; pp[1] = pos.y;
+  }
+  const R d1 = seg_distance(pos, c1, c2), d2 = seg_distance(pos, c2, c3);
+  const R d3 = seg_distance(pos, c3, c4), d4 = seg_distance(pos, c4, c1);
+  const R d5 = std::fabs(dot(n, sub(pos, c1)));
+  if (pp[0] < hic[0] && pp[0] > loc[0] && pp[1] < hic[1] && pp[1] > loc[1]) return d5;
+  return std::fmin(std::fmin(d1, d2), std::fmin(d3, d4));
+}
+
+template <class R> static inline R box_distance(const V3<R>& pos, const R lo[3], const R hi[3]) {
+  // DistanceToBox, DistanceField3D.cu:90-129 (the corner norms it computes
+  // are unused there)
+  const V3<R> c0{lo[0], lo[1], lo[2]}, c1{hi[0], lo[1], lo[2]}, c2{hi[0], hi[1], lo[2]}, c3{lo[0], hi[1], lo[2]};
+  const V3<R> c4{lo[0], lo[1], hi[2]}, c5{hi[0], lo[1], hi[2]}, c6{hi[0], hi[1], hi[2]}, c7{lo[0], hi[1], hi[2]};
+  if (pos.x < hi[0] && pos.x > lo[0] && pos.y < hi[1] && pos.y > lo[1] && pos.z < hi[2] && pos.z > lo[2]) {
+    const R dx = std::fmin(std::fabs(pos.x - hi[0]), std::fabs(pos.x - lo[0]));
+    const R dy = std::fmin(std::fabs(pos.y - hi[1]), std::fabs(pos.y - lo[1]));
+    const R dz = std::fmin(std::fabs(pos.z - hi[2]), std::fabs(pos.z - lo[2]));
+    return std::fmin(std::fmin(dx, dy), dz);
+  }
+  const R x1 = square_distance(pos, c0, c7, 0), x2 = square_distance(pos, c1, c6, 0);
+  const R y1 = square_distance(pos, c0, c5, 1), y2 = square_distance(pos, c3, c6, 1);
+  const R z1 = square_distance(pos, c0, c2, 2), z2 = square_distance(pos, c4, c6, 2);
+  return -std::fmin(std::fmin(std::fmin(x1, x2), std::fmin(y1, y2)), std::fmin(z1, z2));
+}
+
+template <class R> SDF<R> box_sdf(const R lo[3], const R hi[3], R spacing, bool inverted) {
+  SDF<R> f;
+  // setCube: n = floor((hi - lo)/distance); setSpace(lo - 5d, hi + 5d, n + 10)
+  int nb[3];
+  R p0[3], p1[3];
+  for (int a = 0; a < 3; ++a) {
+    nb[a] = (int)std::floor((hi[a] - lo[a]) / spacing) + 10;
+    const R pad = (R)5 * spacing;
+    p0[a] = lo[a] - pad;
+    p1[a] = hi[a] + pad;
+  }
+  // setSpace: h = (p1 - p0) * (1.0/(n+1)) with the reciprocal in double,
+  // n + 1 samples per axis
+  for (int a = 0; a < 3; ++a) {
+    f.left[a] = p0[a];
+    f.hs[a] = (p1[a] - p0[a]) * (R)(1.0 / (double)(R)(nb[a] + 1));
+    f.inv_h[a] = (R)(1.0 / (double)f.hs[a]);
+  }
+  f.nx = nb[0] + 1;
+  f.ny = nb[1] + 1;
+  f.nz = nb[2] + 1;
+  f.inverted = inverted;
+  f.d.assign((size_t)f.nx * f.ny * f.nz, (R)0);
+  const R sign = inverted ? (R)1 : (R)-1;
+  // K_DistanceFieldToBox: p = start + (i,j,k)*h
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < f.nz; ++k)
+    for (int j = 0; j < f.ny; ++j)
+      for (int i = 0; i < f.nx; ++i) {
+        const V3<R> q{f.left[0] + (R)i * f.hs[0], f.left[1] + (R)j * f.hs[1], f.left[2] + (R)k * f.hs[2]};
+        f.d[(size_t)i + (size_t)f.nx * ((size_t)j + (size_t)f.ny * k)] = sign * box_distance(q, lo, hi);
+      }
+  return f;
+}
+
+template <class R> void sdf_distance(const SDF<R>& f, const V3<R>& p, R& d, V3<R>& normal) {
+  const R fx = (p.x - f.left[0]) * f.inv_h[0];
+  const R fy = (p.y - f.left[1]) * f.inv_h[1];
+  const R fz = (p.z - f.left[2]) * f.inv_h[2];
+  const R ffx = std::floor(fx), ffy = std::floor(fy), ffz = std::floor(fz);
+  // out of the field (also NaN): +-100000 and no normal
+  if (!(ffx >= (R)0 && ffx < (R)(f.nx - 1) && ffy >= (R)0 && ffy < (R)(f.ny - 1) && ffz >= (R)0 &&
+        ffz < (R)(f.nz - 1))) {
+    d = f.inverted ? (R)-100000 : (R)100000;
+    normal = {0, 0, 0};
+    return;
+  }
+  const int i = (int)ffx, j = (int)ffy, k = (int)ffz;
+  const R alpha = fx - (R)i, beta = fy - (R)j, gamma = fz - (R)k;
+  auto D = [&](int a, int b, int c) { return f.d[(size_t)a + (size_t)f.nx * ((size_t)b + (size_t)f.ny * c)]; };
+  auto lerp = [](R a, R b, R t) { return ((R)1 - t) * a + t * b; };
+  const R d000 = D(i, j, k), d100 = D(i + 1, j, k), d010 = D(i, j + 1, k), d110 = D(i + 1, j + 1, k);
+  const R d001 = D(i, j, k + 1), d101 = D(i + 1, j, k + 1), d011 = D(i, j + 1, k + 1), d111 = D(i + 1, j + 1, k + 1);
+  const R dx00 = lerp(d000, d100, alpha), dx10 = lerp(d010, d110, alpha), dxy0 = lerp(dx00, dx10, beta);
+  const R dx01 = lerp(d001, d101, alpha), dx11 = lerp(d011, d111, alpha), dxy1 = lerp(dx01, dx11, beta);
+  const R d0y0 = lerp(d000, d010, beta), d0y1 = lerp(d001, d011, beta), d0yz = lerp(d0y0, d0y1, gamma);
+  const R d1y0 = lerp(d100, d110, beta), d1y1 = lerp(d101, d111, beta), d1yz = lerp(d1y0, d1y1, gamma);
+  const R dx0z = lerp(dx00, dx01, gamma), dx1z = lerp(dx10, dx11, gamma);
+  V3<R> nrm{d0yz - d1yz, dx0z - dx1z, dxy0 - dxy1};
+  const R l = norm(nrm);
+  if (l < (R)0.0001f) {
+    nrm = {0, 0, 0};
+  } else if (l > std::numeric_limits<R>::epsilon()) {
+    const R is = (R)1 / std::sqrt(dot(nrm, nrm));
+    nrm = {nrm.x * is, nrm.y * is, nrm.z * is};
+  }
+  normal = nrm;
+  d = ((R)1 - gamma) * dxy0 + gamma * dxy1;
+}
+
+float box_jitter(int id) {
+  uint64_t z = ((uint64_t)(uint32_t)id + 1u) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z ^= z >> 31;
+  const float u = (float)((z >> 40) + 1) * (1.0f / 16777216.0f);  // (0, 1]
+  return 0.0001f * u;
+}
+
+template <class R>
+void box_constrain(const Params<R>& p, const SDF<R>& f, V3<R>* x, V3<R>* v, int n) {
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < n; ++i) {
+    V3<R> pos = x[i], vec = v[i];
+    R dist;
+    V3<R> nrm;
+    sdf_distance(f, pos, dist, nrm);
+    if (dist <= (R)0) {
+      const R olddist = -dist;
+      dist = (R)box_jitter(i);
+      const R s = olddist + dist;
+      pos = {pos.x - s * nrm.x, pos.y - s * nrm.y, pos.z - s * nrm.z};
+      const R vn = dot(vec, nrm);
+      V3<R> vnor{vn * nrm.x, vn * nrm.y, vn * nrm.z};
+      const V3<R> vtan{vec.x - vnor.x, vec.y - vnor.y, vec.z - vnor.z};
+      if (vn > (R)0) vnor = {-vnor.x, -vnor.y, -vnor.z};
+      const R keep = (R)1 - p.nfric;
+      vnor = {vnor.x * keep, vnor.y * keep, vnor.z * keep};
+      vec = {vnor.x + vtan.x, vnor.y + vtan.y, vnor.z + vtan.z};
+      const R damp = std::pow((R)2.71828182845904523536, -p.dt * p.tfric);
+      vec = {vec.x * damp, vec.y * damp, vec.z * damp};
+    }
+    x[i] = pos;
+    v[i] = vec;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -553,7 +786,12 @@ void rollout_adjoint(const Params<R>& p, const V3<R>* x0, const V3<R>* v0, int n
                                              const CSR&, V3<R>*);                                  \
   template void pbd_constrain<R>(const Params<R>&, V3<R>*, V3<R>*, int, const CSR&, R*);           \
   template void integrate<R>(const Params<R>&, V3<R>*, V3<R>*, const V3<R>*, int);                 \
-  template void step<R>(const Params<R>&, V3<R>*, V3<R>*, R*, int, CSR*);                         \
+  template void step<R>(const Params<R>&, V3<R>*, V3<R>*, R*, int, CSR*, const SDF<R>*);         \
+  template R vis_weight<R>(R, R);                                                                  \
+  template void viscosity<R>(const Params<R>&, const V3<R>*, V3<R>*, int, const CSR&);             \
+  template SDF<R> box_sdf<R>(const R*, const R*, R, bool);                                         \
+  template void sdf_distance<R>(const SDF<R>&, const V3<R>&, R&, V3<R>&);                          \
+  template void box_constrain<R>(const Params<R>&, const SDF<R>&, V3<R>*, V3<R>*, int);            \
   template void step_adjoint<R>(const Params<R>&, const V3<R>*, const V3<R>*, int, V3<R>*, V3<R>*); \
   template void rollout_adjoint<R>(const Params<R>&, const V3<R>*, const V3<R>*, int, int, V3<R>*,  \
                                    V3<R>*);

@@ -60,6 +60,26 @@ template <class R> struct Params {
   // Δx accumulation: 0 = gather when the list is symmetric, else the
   // atomic-order emulation; 1 = gather; 2 = atomic-order emulation.
   int dx_mode;
+  // ImplicitViscosity (ImplicitViscosity.cu:71-81; the model sets
+  // viscosity 1, PositionBasedFluidModel.cpp:70) and the box boundary of
+  // StaticBoundary::loadCube (StaticBoundary.cpp:72-77; normal friction 0.95,
+  // tangential 0, BoundaryConstraint.h:32-33). flags: 1 = viscosity after the
+  // PBD solve, 2 = box boundary before every step.
+  R mu;
+  int vis_iters;
+  R box_lo[3], box_hi[3];
+  R box_spacing;
+  R nfric, tfric;
+  int flags;
+};
+
+// Sampled signed distance field of DistanceField3D (Framework/Topology/
+// DistanceField3D.h:17-110): samples at left + (i,j,k)*h, i < n.
+template <class R> struct SDF {
+  R left[3], hs[3], inv_h[3];
+  int nx, ny, nz;
+  bool inverted;
+  std::vector<R> d;  // x fastest
 };
 
 struct CSR {
@@ -95,8 +115,30 @@ template <class R>
 void pbd_constrain(const Params<R>& p, V3<R>* x, V3<R>* v, int n, const CSR& nb, R* rho);
 template <class R>
 void integrate(const Params<R>& p, V3<R>* x, V3<R>* v, const V3<R>* f, int n);
+// sdf: the box field when flags & 2 (built here when null).
 template <class R>
-void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out);
+void step(const Params<R>& p, V3<R>* x, V3<R>* v, R* rho, int n, CSR* nb_out, const SDF<R>* sdf = nullptr);
+
+// VB_VisWeight (ImplicitViscosity.cu:9-18) and ImplicitViscosity::constrain
+// (ImplicitViscosity.cu:99-128) over the list nb on positions x.
+template <class R> R vis_weight(R r, R h);
+template <class R>
+void viscosity(const Params<R>& p, const V3<R>* x, V3<R>* v, int n, const CSR& nb);
+// BoundaryConstraint::setCube (BoundaryConstraint.cu:109-117) with
+// DistanceField3D::setSpace / loadBox (DistanceField3D.cu:194-201,283-291).
+template <class R> SDF<R> box_sdf(const R lo[3], const R hi[3], R spacing, bool inverted);
+// DistanceField3D::getDistance (DistanceField3D.h:112-170).
+template <class R> void sdf_distance(const SDF<R>& f, const V3<R>& p, R& d, V3<R>& normal);
+// The jitter of K_ConstrainSDF: 1e-4 times a uniform in (0, 1] drawn per
+// particle id. The reference draws it with curand (RandNumber(pId),
+// Core/Utility/CudaRand.h); this build uses its own counter-based draw
+// (SplitMix64 of the id, top 24 bits), so jittered positions differ from the
+// reference's by at most 1e-4 m along the wall normal.
+float box_jitter(int id);
+// K_ConstrainSDF (BoundaryConstraint.cu:30-67); ids are the particles'
+// indices in x (the reference's pId).
+template <class R>
+void box_constrain(const Params<R>& p, const SDF<R>& f, V3<R>* x, V3<R>* v, int n);
 
 // Reverse-mode derivative of step() with the neighbor sets held fixed (they
 // are piecewise constant in x; W and the kernel gradient vanish with their

@@ -19,6 +19,9 @@ PBF_ERR_CAPACITY = -3
 PBF_ERR_ALLOC = -4
 PBF_ERR_UNSUPPORTED = -5
 
+PBF_FLAG_VISCOSITY = 1
+PBF_FLAG_BOX = 2
+
 
 class pbf_params(C.Structure):
     _fields_ = [
@@ -36,6 +39,13 @@ class pbf_params(C.Structure):
         ("scene_hi", C.c_float * 3),
         ("mask_words", C.c_int),
         ("list_slots", C.c_int),
+        ("viscosity", C.c_float),
+        ("viscosity_iterations", C.c_int),
+        ("box_lo", C.c_float * 3),
+        ("box_hi", C.c_float * 3),
+        ("box_spacing", C.c_float),
+        ("normal_friction", C.c_float),
+        ("tangent_friction", C.c_float),
         ("flags", C.c_uint),
     ]
 
@@ -89,6 +99,8 @@ SIGNATURES = {
     "pbf_rollout_forward": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
     "pbf_rollout_backward": (C.c_int, [_CTX, _VP, _VP, _VP]),
     "pbf_step_neighbors": (C.c_int, [_CTX, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
+    "pbf_viscosity": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+    "pbf_box_constrain": (C.c_int, [_CTX, _VP, _VP, C.c_int, _VP]),
     "pbf_get_stats": (C.c_int, [_CTX, C.POINTER(pbf_stats)]),
     "pbf_set_profiling": (C.c_int, [_CTX, C.c_int]),
     "pbf_get_profile": (C.c_int, [_CTX, C.POINTER(pbf_profile)]),
@@ -140,7 +152,7 @@ def params_from(**kw) -> pbf_params:
     """Default parameters with keyword overrides (vector fields take 3-sequences)."""
     p = default_params()
     for k, v in kw.items():
-        if k in ("gravity", "scene_lo", "scene_hi"):
+        if k in ("gravity", "scene_lo", "scene_hi", "box_lo", "box_hi"):
             arr = getattr(p, k)
             for i in range(3):
                 arr[i] = float(v[i])

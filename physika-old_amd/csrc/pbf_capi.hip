@@ -41,7 +41,7 @@ struct pbf_ctx {
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *inv = nullptr;
   int *sid = nullptr, *ssid = nullptr;
   int *nbr_cnt = nullptr, *cnt_orig = nullptr;
-  float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr;
+  float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr, *vv = nullptr;
   float* dpos = nullptr;
   // neighbor pass output: per-wave metadata, bit masks (scratch), lists
   long long* mask_base = nullptr;
@@ -55,6 +55,13 @@ struct pbf_ctx {
 
   int* scan_scratch = nullptr;
   size_t scan_scratch_n = 0;
+
+  // viscosity constants and the sampled box field
+  VisConst vc{};
+  BoxField bf{};
+  size_t bf_cap = 0;     // allocated samples
+  bool bf_valid = false; // samples match the parameters
+  float keep_normal = 0.f, damp = 1.f;
 
   // rollout with checkpoints (reverse mode): states every `every` steps,
   // the states of one segment, and the tape of one step
@@ -181,7 +188,16 @@ int validate(const pbf_params* p, std::string& why) {
   if (p->iterations < 0 || p->iterations > 1000) return why = "iterations must be in [0, 1000]", PBF_ERR_INVALID;
   if (p->mask_words < 0) return why = "mask_words must be >= 0", PBF_ERR_INVALID;
   if (p->list_slots < 0) return why = "list_slots must be >= 0", PBF_ERR_INVALID;
-  if (p->flags != 0) return why = "flags must be 0", PBF_ERR_INVALID;
+  if (p->flags & ~(PBF_FLAG_VISCOSITY | PBF_FLAG_BOX)) return why = "unknown flag bits", PBF_ERR_INVALID;
+  if (!std::isfinite(p->viscosity) || p->viscosity < 0.0f) return why = "viscosity must be finite and >= 0", PBF_ERR_INVALID;
+  if (p->viscosity_iterations < 0 || p->viscosity_iterations > 1000)
+    return why = "viscosity_iterations must be in [0, 1000]", PBF_ERR_INVALID;
+  if (!pos(p->box_spacing)) return why = "box_spacing must be finite and > 0", PBF_ERR_INVALID;
+  if (!std::isfinite(p->normal_friction) || !std::isfinite(p->tangent_friction))
+    return why = "frictions must be finite", PBF_ERR_INVALID;
+  for (int c = 0; c < 3; ++c)
+    if (!std::isfinite(p->box_lo[c]) || !std::isfinite(p->box_hi[c]) || !(p->box_hi[c] > p->box_lo[c]))
+      return why = "box bounds must be finite with hi > lo", PBF_ERR_INVALID;
   for (int c = 0; c < 3; ++c) {
     if (!std::isfinite(p->scene_lo[c]) || !std::isfinite(p->scene_hi[c]) || !(p->scene_hi[c] > p->scene_lo[c]))
       return why = "scene bounds must be finite with hi > lo", PBF_ERR_INVALID;
@@ -235,6 +251,39 @@ int derive(pbf_ctx* c, const pbf_params* p) {
     pc.g[a] = p->gravity[a];
     pc.gdt[a] = (0.0f + p->gravity[a]) * p->dt;
   }
+  // ImplicitViscosity (ImplicitViscosity.cu:9-18,52): w = 45/(13 pi h^3)(1 - r/h)
+  VisConst vc{};
+  vc.inv_h = pc.inv_h;
+  vc.cw = 45.0f / (13.0f * (float)kPi * hh * h);
+  vc.b = p->dt * p->viscosity / h;
+  vc.eps = p->pair_eps;
+  // box field of BoundaryConstraint::setCube (BoundaryConstraint.cu:109-117)
+  // and DistanceField3D::setSpace (DistanceField3D.cu:194-201)
+  BoxField bf{};
+  long long nsamp = 1;
+  for (int a = 0; a < 3; ++a) {
+    const float nseg = std::floor((p->box_hi[a] - p->box_lo[a]) / p->box_spacing);
+    if (!(nseg < 1.0e5f)) return fail(c, PBF_ERR_UNSUPPORTED, "box field too fine along one axis");
+    const int nb = (int)nseg + 10;
+    const float pad = 5.0f * p->box_spacing;
+    const float p0 = p->box_lo[a] - pad, p1 = p->box_hi[a] + pad;
+    bf.left[a] = p0;
+    bf.h[a] = (p1 - p0) * (float)(1.0 / (double)(float)(nb + 1));
+    bf.inv_h[a] = (float)(1.0 / (double)bf.h[a]);
+    (a == 0 ? bf.nx : a == 1 ? bf.ny : bf.nz) = nb + 1;
+    nsamp *= nb + 1;
+  }
+  if ((p->flags & PBF_FLAG_BOX) && nsamp > (1ll << 31))
+    return fail(c, PBF_ERR_UNSUPPORTED, "box field has more than 2^31 samples");
+  const bool box_same = c->bf_valid && std::memcmp(c->p.box_lo, p->box_lo, sizeof(p->box_lo)) == 0 &&
+                        std::memcmp(c->p.box_hi, p->box_hi, sizeof(p->box_hi)) == 0 &&
+                        c->p.box_spacing == p->box_spacing;
+  bf.d = c->bf.d;
+  c->bf = bf;
+  c->bf_valid = box_same;
+  c->vc = vc;
+  c->keep_normal = 1.0f - p->normal_friction;
+  c->damp = std::pow(2.71828182845904523536f, -p->dt * p->tangent_friction);
   c->g = g;
   c->full_cells = ncell;
   c->pc = pc;
@@ -319,7 +368,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   const int cap = std::max(((n + 255) / 256) * 256, 256);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->nbr_cnt); dfree(c->cnt_orig);
-  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
+  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   c->cap_n = 0;
   const size_t ni = sizeof(int) * (size_t)cap, nf4 = sizeof(float4) * (size_t)cap;
@@ -337,6 +386,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   PBF_HIP(c, hipMalloc(&c->xa, nf4));
   PBF_HIP(c, hipMalloc(&c->xb, nf4));
   PBF_HIP(c, hipMalloc(&c->vs, nf4));
+  PBF_HIP(c, hipMalloc(&c->vv, nf4));
   PBF_HIP(c, hipMalloc(&c->dpos, 3 * sizeof(float) * (size_t)cap));
   PBF_HIP(c, hipMemset(c->dpos, 0, 3 * sizeof(float) * (size_t)cap));
   const size_t nwv = (size_t)(cap / 64 + 4);
@@ -523,18 +573,46 @@ int export_csr(pbf_ctx* c, int n, int* d_offsets, int* d_elems, long long cap, l
   return PBF_OK;
 }
 
+// Samples of the box field (K_DistanceFieldToBox, computed on the device once
+// per box).
+int ensure_box(pbf_ctx* c, hipStream_t st) {
+  if (c->bf_valid) return PBF_OK;
+  const size_t need = (size_t)c->bf.nx * c->bf.ny * c->bf.nz;
+  if (need > c->bf_cap) {
+    dfree(c->bf.d);
+    c->bf_cap = 0;
+    PBF_HIP(c, hipMalloc(&c->bf.d, sizeof(float) * need));
+    c->bf_cap = need;
+  }
+  launch_box_sdf(c->bf, c->p.box_lo, c->p.box_hi, st);
+  PBF_LAUNCHED(c);
+  c->bf_valid = true;
+  return PBF_OK;
+}
+
 // One fused step on a batch of scenes (scene_off: device offsets): the state
 // (x_in, v_in) advances into (x_out, v_out), which may alias the inputs. With
 // a tape (P: K iterate arrays, P[0] = the integrated positions; aux: K arrays)
 // every PBD iterate, its lambda and the adjoint's per-iteration sums are kept.
+// The box boundary runs first and the viscosity last when their flags are set
+// (never with a tape).
 int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, float* v_out, float* d_rho, int n,
               const int* scene_off, int nscenes, float4* const* P, float4* const* aux, hipStream_t st) {
   const int K = c->p.iterations;
-  int s = search(c, x_in, v_in, n, scene_off, nscenes, true, st);
+  const bool box = (c->p.flags & PBF_FLAG_BOX) != 0;
+  const bool visc = (c->p.flags & PBF_FLAG_VISCOSITY) != 0 && c->p.viscosity_iterations > 0;
+  int s;
+  if (box) {
+    if ((s = ensure_box(c, st))) return s;
+    launch_box_constrain(c->bf, x_in, v_in, x_out, v_out, n, scene_off, nscenes, c->keep_normal, c->damp, st);
+    x_in = x_out;
+    v_in = v_out;
+  }
+  s = search(c, x_in, v_in, n, scene_off, nscenes, true, st);
   if (s) return s;
   NbrView nv = view_of(c, n);
   float4* bufs[2] = {c->xa, c->xb};
-  if (K == 0) launch_write_back(c->x0, c->vs, c->perm, n, x_out, v_out, st);
+  if (K == 0 && !visc) launch_write_back(c->x0, c->vs, c->perm, n, x_out, v_out, st);
   for (int it = 0; it < K; ++it) {
     const bool last = it + 1 == K;
     float4* src = P ? P[it] : (it == 0 ? c->x0 : bufs[(it - 1) & 1]);
@@ -545,7 +623,21 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
     }
     {
       PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
-      launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st);
+      launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st,
+                               (last && visc) ? c->vv : nullptr);
+    }
+  }
+  if (visc) {
+    // ImplicitViscosity::constrain on the constrained positions and velocities
+    const float4* xk = K == 0 ? c->x0 : bufs[(K - 1) & 1];
+    const float4* v_old = K == 0 ? c->vs : c->vv;
+    float4* V[2] = {K == 0 ? c->xa : bufs[K & 1], K == 0 ? c->xb : c->x0};
+    const int T = c->p.viscosity_iterations;
+    for (int t = 0; t < T; ++t) {
+      const bool last = t + 1 == T;
+      const float4* vin = t == 0 ? v_old : V[(t - 1) & 1];
+      launch_viscosity(nv, xk, v_old, vin, last ? nullptr : V[t & 1], c->vc, c->perm, last ? x_out : nullptr,
+                       last ? v_out : nullptr, st);
     }
   }
   PBF_LAUNCHED(c);
@@ -632,6 +724,15 @@ void pbf_params_default(pbf_params* p) {
   }
   p->mask_words = 32;
   p->list_slots = 96;
+  p->viscosity = 1.0f;
+  p->viscosity_iterations = 5;
+  for (int a = 0; a < 3; ++a) {
+    p->box_lo[a] = 0.0f;
+    p->box_hi[a] = 1.0f;
+  }
+  p->box_spacing = 0.005f;
+  p->normal_friction = 0.95f;
+  p->tangent_friction = 0.0f;
   p->flags = 0;
 }
 
@@ -683,9 +784,9 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->cell_cnt); dfree(c->cell_start); dfree(c->oog_flag);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->nbr_cnt); dfree(c->cnt_orig);
-  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs);
+  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
-  dfree(c->mask); dfree(c->list); dfree(c->scan_scratch);
+  dfree(c->mask); dfree(c->list); dfree(c->scan_scratch); dfree(c->bf.d);
   dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.tape); dfree(c->roll.adj);
   dfree(c->roll.out); dfree(c->roll.flag);
   dfree(c->counters);
@@ -841,6 +942,8 @@ int pbf_rollout_forward(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const 
   if (bad_n(n) || n == 0 || nsteps < 1 || checkpoint_every < 1 || nscenes < 1 || !d_x || !d_v ||
       (!d_scene_offsets && nscenes != 1))
     return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (c->p.flags != 0)
+    return fail(c, PBF_ERR_UNSUPPORTED, "reverse mode covers the plain PBD step (no viscosity, no box)");
   hipStream_t st = (hipStream_t)stream;
   auto& r = c->roll;
   r.valid = false;
@@ -945,6 +1048,42 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
   if (flag)
     return fail(c, PBF_ERR_UNSUPPORTED,
                 "a step of the rollout had particles outside the grid; its adjoint is not implemented");
+  return PBF_OK;
+}
+
+int pbf_viscosity(pbf_ctx* c, const float* d_x, float* d_v, const int* d_offsets, const int* d_elems, int n,
+                  void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  const int T = c->p.viscosity_iterations;
+  if (n == 0 || T == 0) return PBF_OK;
+  int s = ensure_particles(c, n, 1);
+  if (s) return s;
+  hipStream_t st = (hipStream_t)stream;
+  launch_aos_to_f4(d_x, c->xa, n, st);
+  launch_aos_to_f4(d_v, c->vv, n, st);
+  float4* V[2] = {c->xb, c->x0};
+  for (int t = 0; t < T; ++t) {
+    const bool last = t + 1 == T;
+    const float4* vin = t == 0 ? c->vv : V[(t - 1) & 1];
+    launch_csr_viscosity(c->xa, d_offsets, d_elems, c->vv, vin, last ? nullptr : V[t & 1], last ? d_v : nullptr, n,
+                         c->vc, st);
+  }
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+int pbf_box_constrain(pbf_ctx* c, float* d_x, float* d_v, int n, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n == 0) return PBF_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int s = ensure_particles(c, n, 1);
+  if (s) return s;
+  if ((s = ensure_box(c, st))) return s;
+  launch_set_offsets(c->off1, n, st);
+  launch_box_constrain(c->bf, d_x, d_v, d_x, d_v, n, c->off1, 1, c->keep_normal, c->damp, st);
+  PBF_LAUNCHED(c);
   return PBF_OK;
 }
 

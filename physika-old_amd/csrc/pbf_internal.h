@@ -42,6 +42,20 @@ struct PhysConst {
   float gdt[3];    // (0 + g) * dt: velocity increment of the integrator at zero force
 };
 
+// Sampled box field (DistanceField3D, Framework/Topology/DistanceField3D.h):
+// nx*ny*nz samples at left + (i,j,k)*h, x fastest; inverted (positive
+// inside the box).
+struct BoxField {
+  float* d;
+  float left[3], h[3], inv_h[3];
+  int nx, ny, nz;
+};
+
+// ImplicitViscosity constants: w = cw (1 - r/h), b = dt mu / h.
+struct VisConst {
+  float inv_h, cw, b, eps;
+};
+
 enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_COUNT };
 
 // Cell-sorted view used by the neighbor pass and every pass over its lists.
@@ -122,9 +136,11 @@ void launch_set_offsets(int* off, int n, hipStream_t st);
 void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st);
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
                                 float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr);
+// last: write x/v to original order, or with v_sorted the positions to xd and
+// the velocities to v_sorted (sorted order, for the viscosity passes).
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc,
                               bool last, const float4* x0, const float4* vs, const int* perm,
-                              float* x_out, float* v_out, hipStream_t st);
+                              float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr);
 void launch_write_back(const float4* xs, const float4* vs, const int* perm, int n, float* x_out,
                        float* v_out, hipStream_t st);
 void launch_csr_count(const int* nbr_cnt, const int* inv, int n, int* cnt_orig, hipStream_t st);
@@ -145,6 +161,15 @@ struct AdjBufs {
 };
 void launch_adjoint_step(const NbrView& nv, float4* const* P, float4* const* aux, int K, const PhysConst& pc,
                          const int* perm, float* xbar, float* vbar, const AdjBufs& ab, hipStream_t st);
+
+// Box boundary and viscosity.
+void launch_box_sdf(const BoxField& f, const float lo[3], const float hi[3], hipStream_t st);
+void launch_box_constrain(const BoxField& f, const float* x_in, const float* v_in, float* x_out, float* v_out, int n,
+                          const int* scene_off, int nscenes, float keep_normal, float damp, hipStream_t st);
+void launch_viscosity(const NbrView& nv, const float4* xs, const float4* v_old, const float4* v_in, float4* v_next,
+                      const VisConst& vc, const int* perm, float* x_out, float* v_out, hipStream_t st);
+void launch_csr_viscosity(const float4* xa, const int* off, const int* el, const float4* v_old, const float4* v_in,
+                          float4* v_next, float* v_out, int n, const VisConst& vc, hipStream_t st);
 
 // Module-granular kernels, original particle order, caller CSR.
 void launch_integrate(float* x, float* v, const float* f, int n, const PhysConst& pc, hipStream_t st);

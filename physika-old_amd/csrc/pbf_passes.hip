@@ -226,7 +226,7 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
                                                          const float4* __restrict__ x0,
                                                          const float4* __restrict__ vs,
                                                          const int* __restrict__ perm, float* __restrict__ x_out,
-                                                         float* __restrict__ v_out) {
+                                                         float* __restrict__ v_out, float4* __restrict__ v_sorted) {
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= nv.n) return;
   const float4 xi = xs[s];
@@ -275,6 +275,12 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
   }
   const float4 xo = x0[s];
   const float4 vv = vs[s];
+  if (v_sorted) {  // the viscosity passes follow
+    xd[s] = make_float4(nx, ny, nz, 0.0f);
+    v_sorted[s] = make_float4(vv.x + (nx - xo.x) / pc.dt, vv.y + (ny - xo.y) / pc.dt, vv.z + (nz - xo.z) / pc.dt,
+                              0.0f);
+    return;
+  }
   const size_t i = (size_t)perm[s];
   x_out[3 * i] = nx;
   x_out[3 * i + 1] = ny;
@@ -286,9 +292,9 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
 
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                               const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out,
-                              hipStream_t st) {
+                              hipStream_t st, float4* v_sorted) {
   k_displacement<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm,
-                                                                 x_out, v_out);
+                                                                 x_out, v_out, v_sorted);
 }
 
 // ---------------------------------------------------------------------------

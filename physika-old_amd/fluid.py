@@ -185,6 +185,22 @@ class Context:
                         self._ctx)
 
 
+    def viscosity(self, x, v, nbr: "NeighborList") -> None:
+        """pbf_viscosity: ImplicitViscosity::constrain on v (in place)."""
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_viscosity(self._ctx, _ptr(x), _ptr(v), _ptr(nbr.index), _ptr(nbr.elements),
+                                                x.shape[0], _stream(x.device)), self._ctx)
+
+    def box_constrain(self, x, v) -> None:
+        """pbf_box_constrain: the box boundary on x, v (in place)."""
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_box_constrain(self._ctx, _ptr(x), _ptr(v), x.shape[0], _stream(x.device)),
+                        self._ctx)
+
     # -- reverse mode -----------------------------------------------------------
     def rollout_forward(self, x, v, rho: Optional[torch.Tensor] = None, nsteps: int = 1,
                         checkpoint_every: int = 10, scene_offsets: Optional[torch.Tensor] = None) -> int:
@@ -348,17 +364,104 @@ class DensityPBD:
         return True
 
 
+class ImplicitViscosity:
+    """XSPH-like implicit viscosity (ImplicitViscosity.h:10-44,
+    ImplicitViscosity.cu:71-128): m_viscosity, m_smoothingLength, m_position,
+    m_velocity, m_neighborhood; constrain()."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.m_viscosity = VarField(ctx, "viscosity")
+        self.m_smoothingLength = VarField(ctx, "smoothing_length")
+        self.m_position = ArrayField()
+        self.m_velocity = ArrayField()
+        self.m_neighborhood = ArrayField()
+
+    def setViscosity(self, mu: float) -> None:
+        self.m_viscosity.setValue(mu)
+
+    def setIterationNumber(self, n: int) -> None:
+        self.ctx.set_params(viscosity_iterations=int(n))
+
+    def constrain(self) -> bool:
+        if not self.m_position.isEmpty():
+            self.ctx.viscosity(self.m_position.getValue(), self.m_velocity.getValue(), self.m_neighborhood.getValue())
+        return True
+
+
+class BoundaryConstraint:
+    """Particles kept inside a box (BoundaryConstraint.h:12-40): setCube with
+    inverted=True (fluid inside), m_normal_friction, m_tangent_friction,
+    constrain()."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.m_position = ArrayField()
+        self.m_velocity = ArrayField()
+        self.m_normal_friction = VarField(ctx, "normal_friction")
+        self.m_tangent_friction = VarField(ctx, "tangent_friction")
+
+    def setCube(self, lo, hi, distance: float = 0.005, inverted: bool = True) -> None:
+        if not inverted:
+            raise NotImplementedError("only the container box (particles inside) is supported")
+        self.ctx.set_params(box_lo=list(lo), box_hi=list(hi), box_spacing=float(distance))
+
+    def constrain(self, position=None, velocity=None) -> bool:
+        x = self.m_position.getValue() if position is None else position
+        v = self.m_velocity.getValue() if velocity is None else velocity
+        if x is not None and x.shape[0]:
+            self.ctx.box_constrain(x, v)
+        return True
+
+
+class StaticBoundary:
+    """Root node holding box obstacles (StaticBoundary.cpp:45-77): advance(dt)
+    constrains every registered particle system before it steps."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.m_obstacles = []
+        self.m_particleSystems = []
+
+    def loadCube(self, lo, hi, distance: float = 0.005, bOutBoundary: bool = True) -> BoundaryConstraint:
+        b = BoundaryConstraint(self.ctx)
+        b.setCube(lo, hi, distance, bOutBoundary)
+        self.m_obstacles.append(b)
+        return b
+
+    def addParticleSystem(self, model: "PositionBasedFluidModel") -> None:
+        self.m_particleSystems.append(model)
+
+    def advance(self, dt: Optional[float] = None) -> None:
+        for b in self.m_obstacles:
+            for m in self.m_particleSystems:
+                b.constrain(m.m_position.getValue(), m.m_velocity.getValue())
+
+
 class PositionBasedFluidModel:
-    """NumericalModel of the PBD fluid (PositionBasedFluidModel.cpp:41-101)
-    without the viscosity module.
+    """NumericalModel of the PBD fluid (PositionBasedFluidModel.cpp:41-101).
+
+    As in the reference, the model runs ImplicitViscosity (viscosity 1) after
+    the PBD solve; viscosity=False gives the plain PBD step. box=(lo, hi,
+    spacing) adds the container box of a StaticBoundary root node, applied
+    before every step.
 
     step(dt) runs the fused MI355X step (one library call);
     step_modules(dt) runs the module sequence begin -> NeighborQuery::compute ->
-    integrate -> DensityPBD::constrain -> end through the module-granular
-    entry points, as the reference's step does."""
+    integrate -> DensityPBD::constrain -> ImplicitViscosity::constrain -> end
+    through the module-granular entry points, as the reference's step does
+    (the box, when set, first)."""
 
-    def __init__(self, smoothing_length: float = 0.0125, device="cuda:0", **params):
-        self.ctx = Context(device, smoothing_length=smoothing_length, **params)
+    def __init__(self, smoothing_length: float = 0.0125, device="cuda:0", viscosity: bool = True, box=None,
+                 **params):
+        flags = params.pop("flags", 0)
+        if viscosity:
+            flags |= _capi.PBF_FLAG_VISCOSITY
+        if box is not None:
+            flags |= _capi.PBF_FLAG_BOX
+            lo, hi, spacing = box
+            params.update(box_lo=list(lo), box_hi=list(hi), box_spacing=float(spacing))
+        self.ctx = Context(device, smoothing_length=smoothing_length, flags=flags, **params)
         self.m_smoothingLength = VarField(self.ctx, "smoothing_length")
         self.m_position = ArrayField()
         self.m_velocity = ArrayField()
@@ -367,6 +470,12 @@ class PositionBasedFluidModel:
         self.nbrQuery = NeighborQuery(self.ctx)
         self.pbdModule = DensityPBD(self.ctx)
         self.integrator = ParticleIntegrator(self.ctx)
+        self.visModule = ImplicitViscosity(self.ctx) if viscosity else None
+        self.boundary = None
+        if box is not None:
+            self.boundary = BoundaryConstraint(self.ctx)
+            self.m_position.connect(self.boundary.m_position)
+            self.m_velocity.connect(self.boundary.m_velocity)
         self.m_position.connect(self.nbrQuery.inPosition)
         self.m_position.connect(self.pbdModule.inPosition)
         self.m_velocity.connect(self.pbdModule.inVelocity)
@@ -375,6 +484,10 @@ class PositionBasedFluidModel:
         self.m_forceDensity.connect(self.integrator.inForceDensity)
         self.nbrQuery.outNeighborhood.connect(self.pbdModule.inNeighborIndex)
         self.pbdModule.outDensity.connect(self.m_density)
+        if self.visModule is not None:
+            self.m_position.connect(self.visModule.m_position)
+            self.m_velocity.connect(self.visModule.m_velocity)
+            self.nbrQuery.outNeighborhood.connect(self.visModule.m_neighborhood)
 
     def _set_dt(self, dt: Optional[float]) -> None:
         if dt is not None and abs(self.ctx.params()["dt"] - float(np.float32(dt))) > 0:
@@ -391,10 +504,14 @@ class PositionBasedFluidModel:
 
     def step_modules(self, dt: Optional[float] = None) -> None:
         self._set_dt(dt)
+        if self.boundary is not None:
+            self.boundary.constrain()
         self.integrator.begin()
         self.nbrQuery.compute()
         self.integrator.integrate()
         self.pbdModule.constrain()
+        if self.visModule is not None:
+            self.visModule.constrain()
         self.integrator.end()
 
     def step_arrays(self, x: np.ndarray, v: np.ndarray, nsteps: int = 1, modules: bool = False):

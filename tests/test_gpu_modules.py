@@ -1,0 +1,128 @@
+"""ImplicitViscosity and the box boundary on the GPU against the oracle.
+
+Bars: the box constraint bit-exact (the same IEEE operations in the same
+order, the sampled field included); viscosity velocities within 1e-5 m/s of
+the FP32 restatement (the pair distance comes from a hardware reciprocal
+square root, the restatement's from sqrtf); full steps with both modules
+within the single-step bars of the plain step; the 100-step dam break in the
+box within 0.5% on mean density and kinetic energy; batched scenes in the box
+bit-identical to serial runs (the jitter's particle id is the index inside the
+scene). Every call goes through the C ABI."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from parity_util import assert_step_parity, dev, oracle_both, say  # noqa: E402
+
+V_VIS = 1e-5
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def box_scene(scene, n_side=(20, 16, 20), origin=(-0.004, 0.002, 0.3), seed=9):
+    """A block whose first layers lie beyond the walls x = 0 and y = 0."""
+    return scene.jittered_lattice(n_side, origin, 0.005, 0.1, seed=seed)
+
+
+def test_box_constrain_bit_exact(fluid, oracle, scene):
+    say("pbf_box_constrain vs oracle, particles on both sides of every wall")
+    u = scene.uniform01(71, 3 * 20000).reshape(-1, 3)
+    x = (-0.02 + 1.04 * u).astype(np.float32)  # the box [0,1]^3 and 0.02 beyond
+    v = (2.0 * (scene.uniform01(72, 3 * 20000).reshape(-1, 3) - 0.5)).astype(np.float32)
+    ctx = fluid.Context("cuda:0")
+    xd, vd = dev(x), dev(v)
+    ctx.box_constrain(xd, vd)
+    xo, vo = oracle.box_constrain(oracle.make_params(), x, v)
+    moved = (xo != x).any(1)
+    assert moved.sum() > 100
+    np.testing.assert_array_equal(xd.cpu().numpy(), xo)
+    np.testing.assert_array_equal(vd.cpu().numpy(), vo)
+
+
+def test_viscosity_module_vs_oracle(fluid, oracle, scene):
+    say("pbf_viscosity on the dam-break neighbor list vs oracle")
+    x = scene.dam_break_32k()
+    v = (0.5 * (scene.uniform01(73, 3 * len(x)).reshape(-1, 3) - 0.5)).astype(np.float32)
+    ctx = fluid.Context("cuda:0")
+    xd, vd = dev(x), dev(v)
+    nl = ctx.neighbor_query(xd)
+    ctx.viscosity(xd, vd, nl)
+    off, el = oracle.neighbor_query(oracle.make_params(), x)
+    want = oracle.viscosity(oracle.make_params(), x, v, off, el)
+    err = np.abs(vd.cpu().numpy() - want).max()
+    say(f"viscosity max |dv| {err:.3e}")
+    assert err <= V_VIS
+
+
+@pytest.mark.parametrize("flags", [1, 2, 3])
+def test_fused_step_with_modules(fluid, oracle, scene, flags):
+    say(f"fused step with flags {flags} (1 viscosity, 2 box) vs oracle, 1 step")
+    x = box_scene(scene)
+    v = np.zeros_like(x)
+    v[:, 0] = -0.3
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=bool(flags & 1),
+                                          box=((0, 0, 0), (1, 1, 1), 0.005) if flags & 2 else None)
+    gpu = model.step_arrays(x, v, 1)
+    assert_step_parity(gpu, *oracle_both(oracle, x, v, 1, flags=flags))
+
+
+def test_module_path_with_modules(fluid, oracle, scene):
+    say("module path with box and viscosity vs oracle, 1 step")
+    x = box_scene(scene)
+    v = np.zeros_like(x)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=True, box=((0, 0, 0), (1, 1, 1), 0.005))
+    gpu = model.step_arrays(x, v, 1, modules=True)
+    assert_step_parity(gpu, *oracle_both(oracle, x, v, 1, flags=3))
+
+
+def test_dam_break_in_box_hundred_steps(fluid, oracle, scene):
+    say("dam break in the box with viscosity, 32768 particles at (0.02, 0.02, 0.02), 100 steps")
+    x = scene.jittered_lattice((32, 32, 32), (0.02, 0.02, 0.02), 0.005, 0.1, seed=1)
+    v = np.zeros_like(x)
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=True, box=((0, 0, 0), (1, 1, 1), 0.005))
+    xs, vs, rho = model.step_arrays(x, v, 100)
+    xo, vo, ro, _ = oracle.step(oracle.make_params(flags=3), x, v, 100)
+    assert xs.min() >= 0.0 and xs.max() <= 1.0  # the box holds
+    ke, keo = (vs.astype(np.float64) ** 2).sum(), (vo.astype(np.float64) ** 2).sum()
+    say(f"mean rho {rho.mean():.4f} vs {ro.mean():.4f}; KE {ke:.5e} vs {keo:.5e}")
+    assert abs(rho.mean() / ro.mean() - 1) <= 5e-3
+    assert abs(ke / keo - 1) <= 5e-3
+
+
+def test_batched_box_scenes_equal_serial(fluid, scene):
+    say("batched scenes in the box with viscosity vs serial, 3 steps")
+    scenes = [box_scene(scene, (8, 6, 7), (-0.004, 0.002, 0.3), 80 + k) for k in range(4)]
+    scenes.append(scene.jittered_lattice((9, 9, 9), (0.4, 0.4, 0.4), 0.005, 0.1, seed=85))
+    counts = [len(s) for s in scenes]
+    off = np.zeros(len(scenes) + 1, np.int32)
+    off[1:] = np.cumsum(counts)
+    x = np.concatenate(scenes)
+    v = np.zeros_like(x)
+    v[:, 1] = -0.5
+    kw = dict(flags=3)
+    ctx = fluid.Context("cuda:0", **kw)
+    xb, vb = dev(x), dev(v)
+    ctx.batch_step(xb, vb, torch.from_numpy(off).to("cuda:0"), None, 3)
+    cs = fluid.Context("cuda:0", **kw)
+    for s in range(len(scenes)):
+        a, b = off[s], off[s + 1]
+        xs_, vs_ = dev(x[a:b]), dev(v[a:b])
+        cs.step(xs_, vs_, None, 3)
+        np.testing.assert_array_equal(xb[a:b].cpu().numpy(), xs_.cpu().numpy(), err_msg=f"scene {s} x")
+        np.testing.assert_array_equal(vb[a:b].cpu().numpy(), vs_.cpu().numpy(), err_msg=f"scene {s} v")
+
+
+def test_rollout_rejects_modules(fluid, capi, scene):
+    say("reverse mode with viscosity: PBF_ERR_UNSUPPORTED")
+    ctx = fluid.Context("cuda:0", flags=1)
+    x = dev(scene.jittered_lattice((4, 4, 4), (0.3, 0.3, 0.3), 0.005, 0.1, seed=3))
+    with pytest.raises(capi.PBFError) as e:
+        ctx.rollout_forward(x, torch.zeros_like(x), None, 2, 1)
+    assert e.value.status == capi.PBF_ERR_UNSUPPORTED

@@ -90,7 +90,7 @@ def test_one_step_parity_dam_break(fluid, oracle, scene):
     say("fused step, 32768-particle dam break, 1 step")
     x = scene.dam_break_32k()
     v = np.zeros_like(x)
-    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
     gpu = model.step_arrays(x, v, 1)
     assert_step_parity(gpu, *oracle_both(oracle, x, v, 1))
 
@@ -99,7 +99,7 @@ def test_module_path_parity_dam_break(fluid, oracle, scene):
     say("module path NeighborQuery -> integrate -> DensityPBD, 1 step")
     x = scene.dam_break_32k()
     v = np.zeros_like(x)
-    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
     gpu = model.step_arrays(x, v, 1, modules=True)
     assert_step_parity(gpu, *oracle_both(oracle, x, v, 1))
 
@@ -119,7 +119,7 @@ def test_hundred_step_aggregates(fluid, oracle, scene):
     say("fused step, 32768 particles, 100 steps, mean density and kinetic energy")
     x = scene.dam_break_32k()
     v = np.zeros_like(x)
-    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
     xs, vs, rho = model.step_arrays(x, v, 100)
     xo, vo, ro, _ = oracle.step(oracle.make_params(), x, v, 100)
     assert abs(rho.mean() / ro.mean() - 1) <= 5e-3
@@ -132,7 +132,7 @@ def test_step_is_bitwise_reproducible(fluid, scene):
     x = scene.dam_break_32k()
     outs = []
     for _ in range(2):
-        model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+        model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
         outs.append(model.step_arrays(x, np.zeros_like(x), 3))
     for a, b in zip(*outs):
         np.testing.assert_array_equal(a, b)
@@ -142,7 +142,7 @@ def test_free_fall_bit_exact(fluid):
     say("free fall, 20 steps")
     x0 = np.array([[0.2, 0.5, 0.2], [0.5, 0.6, 0.5], [0.8, 0.7, 0.3]], np.float32)
     v0 = np.array([[0.1, 0.0, -0.2], [0.0, 0.3, 0.0], [-0.05, 0.0, 0.07]], np.float32)
-    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
     xs, vs, rho = model.step_arrays(x0, v0, 20)
     ex, ev = x0.copy(), v0.copy()
     g = np.array([0.0, -9.8, 0.0], np.float32)
@@ -158,7 +158,7 @@ def test_ragged_sizes(fluid, oracle, scene, n):
     say(f"fused step, ragged n={n}")
     x = scene.jittered_lattice((17, 17, 17), (0.3, 0.5, 0.3), 0.005, 0.1, seed=n, count=n)
     v = np.zeros_like(x)
-    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
     gpu = model.step_arrays(x, v, 2)
     assert_step_parity(gpu, *oracle_both(oracle, x, v, 2))
 
@@ -188,11 +188,11 @@ def test_reservation_fallbacks_are_identical(fluid, scene, scene_name, reserve):
     else:
         x = scene.gaussian_blobs(1, 20000, 0.02, seed=23)  # > 24 mask words per lane at the centre
     v = np.zeros_like(x)
-    a = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    a = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
     a.step_arrays(x, v, 1)
     ra = a.step_arrays(x, v, 1)  # second call: reservations sized for the scene, list path
     assert a.ctx.stats()["list_slots_needed"] <= a.ctx.stats()["list_slots_reserved"]
-    b = fluid.PositionBasedFluidModel(0.0125, "cuda:0", **reserve)
+    b = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False, **reserve)
     rb = b.step_arrays(x, v, 1)
     st = b.ctx.stats()
     assert st["list_slots_needed"] > st["list_slots_reserved"]
@@ -216,7 +216,7 @@ def test_out_of_grid_particles(fluid, oracle, scene):
     x = scene.jittered_lattice((8, 8, 8), (-0.047, 0.5, 0.5), 0.005, 0.1, seed=12)
     v = np.zeros_like(x)
     p = oracle.make_params(g=(0.0, 0.0, 0.0))
-    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", gravity=(0.0, 0.0, 0.0))
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False, gravity=(0.0, 0.0, 0.0))
     gpu = model.step_arrays(x, v, 1)
     c3, cid = oracle.cell3(p, x)
     n_oog = int((cid < 0).sum())
@@ -316,6 +316,6 @@ def test_million_particle_one_step(fluid, oracle, scene):
     say("1,048,576-particle lattice: one fused step vs oracle")
     x = scene.lattice_1m()
     v = np.zeros_like(x)
-    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0")
+    model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
     gpu = model.step_arrays(x, v, 1)
     assert_step_parity(gpu, *oracle_both(oracle, x, v, 1))
