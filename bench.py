@@ -48,6 +48,11 @@ def algorithmic_flops(n, pairs, iters):
     return (9 + 57 * iters) * pairs + 30 * n
 
 
+def progress(msg):
+    """One line per finished phase on stderr (the JSON result stays alone on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def cpu_baseline(x, steps, threads):
     """The oracle (C++ restatement of the reference path, OpenMP over
     particles) on this host: one warm-up step, then `steps` timed steps."""
@@ -57,9 +62,11 @@ def cpu_baseline(x, steps, threads):
     p = O.make_params()
     v = np.zeros_like(x)
     xw, vw, _, _ = O.step(p, x, v, 1)
+    progress("cpu baseline: warm-up step done")
     t0 = time.perf_counter()
     O.step(p, xw, vw, steps)
     dt = time.perf_counter() - t0
+    progress(f"cpu baseline: {steps} steps in {dt:.1f} s")
     return {"value": len(x) * steps / dt, "unit": "particle-steps/s", "cores": used, "kind": "port",
             "sample": f"{len(x)}-particle lattice (the benchmark workload), {steps} steps after 1 warm-up step, "
                       f"{dt:.1f} s"}
@@ -93,6 +100,7 @@ def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
     cb.batch_step(xb, vb, off, rb, 2)
     ms_b = timed(lambda k: cb.batch_step(xb, vb, off, rb, k), steps)
     cells = cb.stats()["cells_used"]
+    progress(f"batched: {steps} steps of {n} particles in 256 scenes, {ms_b:.1f} ms")
 
     xs = torch.from_numpy(b.pos).to(dev)
     vs = torch.zeros_like(xs)
@@ -108,6 +116,7 @@ def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
 
     serial(1)
     ms_s = timed(serial, serial_steps)
+    progress(f"serial: {serial_steps} steps, {ms_s:.1f} ms")
     vb_ = n * steps / (ms_b / 1e3)
     vs_ = n * serial_steps / (ms_s / 1e3)
     return {"workload": "256 lattice scenes, n_s log-uniform [2048, 65536], seed 4", "scenes": 256,
@@ -170,6 +179,7 @@ def differentiable(torch, fluid, scene, nsteps=50, every=10):
 
     run()
     f_ms, b_ms, L, g = run()
+    progress(f"differentiable: fwd {f_ms:.2f} ms, bwd {b_ms:.2f} ms")
     return {"workload": "65,536-particle block, 50 steps, checkpoint every 10, loss centroid + 32^3 density L2",
             "particles": n, "steps": nsteps, "checkpoint_every": every, "fwd_ms": round(f_ms, 3),
             "bwd_ms": round(b_ms, 3), "bwd_fwd_ratio": round(b_ms / f_ms, 3), "loss": L,
@@ -217,6 +227,8 @@ def batched_inverse(torch, fluid, scene, adam_steps=20, nsteps=30, every=10, lr=
         L.backward()
         opt.step()
         losses.append(float(L))
+        if it % 5 == 0:
+            progress(f"batched inverse: Adam step {it}, loss {losses[-1]:.6g}")
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / adam_steps
@@ -254,6 +266,7 @@ def main():
 
     x_np = getattr(pkg.scene, args.workload)()
     n = len(x_np)
+    progress(f"{args.workload}: {n} particles generated")
     dev = torch.device("cuda:0")
     if args.copy_calibration:
         src = torch.ones(64 << 20, dtype=torch.float32, device=dev)  # 256 MiB
@@ -273,6 +286,7 @@ def main():
     torch.cuda.synchronize()
     ctx.step(x, v, rho, 1)
     torch.cuda.synchronize()
+    progress("warm-up done")
 
     stream = torch.cuda.current_stream(dev)
     e0 = torch.cuda.Event(enable_timing=True)
@@ -286,6 +300,7 @@ def main():
     wall = time.perf_counter() - t0
     ms_total = e0.elapsed_time(e1)
     ms_step = ms_total / args.steps
+    progress(f"{args.workload}: {args.steps} steps, {ms_step:.4f} ms/step")
     value = n * args.steps / (ms_total / 1e3)
 
     stats = ctx.stats()
@@ -293,12 +308,13 @@ def main():
     ncell = int(stats["cells_used"])  # compact cells the sort touches (occupied window + bucket)
     iters = params["iterations"]
 
-    # per-pass timing: the same number of steps again with HIP events around
-    # every pass (recorded on the kernels' stream)
+    # per-pass timing: up to 50 more steps with HIP events around every pass
+    # (recorded on the kernels' stream)
     ctx.set_profiling(True)
-    ctx.step(x, v, rho, args.steps)
+    ctx.step(x, v, rho, min(args.steps, 50))
     prof = ctx.profile()
     ctx.set_profiling(False)
+    progress("per-pass profile done")
     pass_ms = {k: (ms / max(cnt, 1), cnt) for k, (ms, cnt) in prof.items()}
     dom = max(("density_lambda", "displacement", "neighbors", "sort"), key=lambda k: pass_ms[k][0] * pass_ms[k][1])
     t_dom = pass_ms[dom][0] / 1e3

@@ -63,10 +63,13 @@ typedef struct pbf_params {
   float scene_lo[3];       /* SceneGraph lower bound (SceneGraph.h:77) */
   float scene_hi[3];       /* SceneGraph upper bound (SceneGraph.h:78) */
   int mask_words;          /* neighbor bit-mask scratch reservation, 32-bit words
-                              per particle (0 = 32) */
-  int list_slots;          /* neighbor-list reservation, 16-bit slots per particle
-                              (0 = 96). Both reservations grow automatically
-                              between calls, see pbf_step */
+                              per particle; 0 (default) = 32 to start with,
+                              grown automatically between calls (pbf_step) */
+  int list_slots;          /* neighbor-list reservation, 16-bit slots per particle;
+                              0 (default) = 96 to start with, grown automatically.
+                              A value > 0 fixes the reservation: waves that do
+                              not fit fall back to bit masks or re-testing,
+                              with identical results */
   float viscosity;         /* ImplicitViscosity m_viscosity; the model sets 1
                               (PositionBasedFluidModel.cpp:70) */
   int viscosity_iterations;/* ImplicitViscosity m_maxInteration (5,

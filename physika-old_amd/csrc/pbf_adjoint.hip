@@ -23,6 +23,12 @@
 // (g and psi are odd in d and J even, which turns every pair's scatter onto
 // j into the gathered terms above.) Integration and velocity update:
 //   x* = x + v* dt, v* = v + g dt, v' = v* + (p_K - x*)/dt, x' = p_K.
+// The adjoint of p_K is x_bar' + c with c = v_bar'/dt, and that of x* is
+// p_bar_0 - c. c can exceed the rest by orders of magnitude, so the iterates
+// are carried as e = p_bar - c (e_K = x_bar', e_k = e_{k+1} + J_k^T (e_{k+1}
+// + c), x*_bar = e_0) and f = e + c enters only the pair terms.
+// Arithmetic is not contracted, so every neighbor-storage path of the list
+// traversal gives the same bits.
 #include "pbf_nbr.h"
 
 namespace pbf {
@@ -34,33 +40,37 @@ struct F4x3 {
   float4 a, b, c;
 };
 
-// a = x_bar' + v_bar'/dt (adjoint of p_K), x*_bar = -v_bar'/dt, v*_bar = v_bar'.
+// e = x_bar', c = v_bar'/dt, f = e + c, v*_bar = v_bar'.
 __global__ __launch_bounds__(kBlock) void k_adj_begin(int n, const int* __restrict__ perm,
                                                       const float* __restrict__ xbar, const float* __restrict__ vbar,
-                                                      float dt, float4* __restrict__ a, float4* __restrict__ xsb,
-                                                      float4* __restrict__ vsb,
+                                                      float dt, float4* __restrict__ e, float4* __restrict__ c,
+                                                      float4* __restrict__ f, float4* __restrict__ vsb,
                                                       const unsigned long long* __restrict__ counters,
                                                       int* __restrict__ flag) {
+#pragma clang fp contract(off)
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s == 0 && counters[C_OOG] != 0) *flag = 1;
   if (s >= n) return;
   const size_t i = (size_t)perm[s];
   const float4 xb = load3(xbar, (int)i), vb = load3(vbar, (int)i);
-  a[s] = make_float4(xb.x + vb.x / dt, xb.y + vb.y / dt, xb.z + vb.z / dt, 0.0f);
-  xsb[s] = make_float4(-vb.x / dt, -vb.y / dt, -vb.z / dt, 0.0f);
+  const float cx = vb.x / dt, cy = vb.y / dt, cz = vb.z / dt;
+  e[s] = xb;
+  c[s] = make_float4(cx, cy, cz, 0.0f);
+  f[s] = make_float4(xb.x + cx, xb.y + cy, xb.z + cz, 0.0f);
   vsb[s] = vb;
 }
 
 __global__ __launch_bounds__(kBlock) void k_adj_lambda(NbrView nv, const float4* __restrict__ P,
-                                                       const float4* __restrict__ aux, const float4* __restrict__ a,
+                                                       const float4* __restrict__ aux, const float4* __restrict__ f,
                                                        float4* __restrict__ q1, float4* __restrict__ q2,
                                                        PhysConst pc) {
+#pragma clang fp contract(off)
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= nv.n) return;
-  const float4 ps = P[s], as = a[s];
+  const float4 ps = P[s], as = f[s];
   float acc = 0.0f;
   for_each_neighbor(
-      nv, s, [&](int j) { return F4x2{P[j], a[j]}; },
+      nv, s, [&](int j) { return F4x2{P[j], f[j]}; },
       [&](int, const F4x2& q) {
         const PairGeom g = pair_geom(ps, q.a, pc.inv_h);
         const float u = g.r > pc.pair_eps ? g.t * g.t * g.rinv : 0.0f;
@@ -77,15 +87,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_lambda(NbrView nv, const float4*
 
 __global__ __launch_bounds__(kBlock) void k_adj_pos(NbrView nv, const float4* __restrict__ P,
                                                     const float4* __restrict__ aux, const float4* __restrict__ q1,
-                                                    const float4* __restrict__ q2, float4* __restrict__ a_out,
-                                                    PhysConst pc) {
+                                                    const float4* __restrict__ q2, const float4* __restrict__ e_in,
+                                                    const float4* __restrict__ c, float4* __restrict__ e_out,
+                                                    float4* __restrict__ f_out, PhysConst pc) {
+#pragma clang fp contract(off)
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= nv.n) return;
   const float4 ps = P[s], q1s = q1[s], q2s = q2[s], ax = aux[s];
   const float Gx = pc.cG * ax.x, Gy = pc.cG * ax.y, Gz = pc.cG * ax.z;
   const float C = 2.0f * pc.corr;
   const float cpsi = -3.0f * pc.cW * pc.inv_h;
-  float bx = q1s.x, by = q1s.y, bz = q1s.z;
+  float bx = 0.0f, by = 0.0f, bz = 0.0f;
   for_each_neighbor(
       nv, s, [&](int j) { return F4x3{P[j], q1[j], q2[j]}; },
       [&](int, const F4x3& q) {
@@ -111,42 +123,44 @@ __global__ __launch_bounds__(kBlock) void k_adj_pos(NbrView nv, const float4* __
           bz += pc.cG * (u * wz - k2 * g.dz);
         }
       });
-  a_out[s] = make_float4(bx, by, bz, 0.0f);
+  const float4 es = e_in[s], cs = c[s];
+  const float ex = es.x + bx, ey = es.y + by, ez = es.z + bz;
+  e_out[s] = make_float4(ex, ey, ez, 0.0f);
+  f_out[s] = make_float4(ex + cs.x, ey + cs.y, ez + cs.z, 0.0f);
 }
 
-// x_bar = x*_bar + a, v_bar = v*_bar + x_bar dt, back to original order.
+// x_bar = e_0 (the adjoint of x*), v_bar = v*_bar + x_bar dt, original order.
 __global__ __launch_bounds__(kBlock) void k_adj_end(int n, const int* __restrict__ perm,
-                                                    const float4* __restrict__ a, const float4* __restrict__ xsb,
-                                                    const float4* __restrict__ vsb, float dt,
-                                                    float* __restrict__ xbar, float* __restrict__ vbar) {
+                                                    const float4* __restrict__ e, const float4* __restrict__ vsb,
+                                                    float dt, float* __restrict__ xbar, float* __restrict__ vbar) {
+#pragma clang fp contract(off)
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= n) return;
   const size_t i = (size_t)perm[s];
-  const float4 aa = a[s], xb = xsb[s], vb = vsb[s];
-  const float x0 = xb.x + aa.x, x1 = xb.y + aa.y, x2 = xb.z + aa.z;
-  xbar[3 * i] = x0;
-  xbar[3 * i + 1] = x1;
-  xbar[3 * i + 2] = x2;
-  vbar[3 * i] = vb.x + x0 * dt;
-  vbar[3 * i + 1] = vb.y + x1 * dt;
-  vbar[3 * i + 2] = vb.z + x2 * dt;
+  const float4 ee = e[s], vb = vsb[s];
+  xbar[3 * i] = ee.x;
+  xbar[3 * i + 1] = ee.y;
+  xbar[3 * i + 2] = ee.z;
+  vbar[3 * i] = vb.x + ee.x * dt;
+  vbar[3 * i + 1] = vb.y + ee.y * dt;
+  vbar[3 * i + 2] = vb.z + ee.z * dt;
 }
 
 void launch_adjoint_step(const NbrView& nv, float4* const* P, float4* const* aux, int K, const PhysConst& pc,
                          const int* perm, float* xbar, float* vbar, const AdjBufs& ab, hipStream_t st) {
   const int n = nv.n;
   const int nb = (n + kBlock - 1) / kBlock;
-  k_adj_begin<<<nb, kBlock, 0, st>>>(n, perm, xbar, vbar, pc.dt, ab.a, ab.xsb, ab.vsb, nv.counters, ab.flag);
-  float4* a = ab.a;
-  float4* a2 = ab.a2;
+  k_adj_begin<<<nb, kBlock, 0, st>>>(n, perm, xbar, vbar, pc.dt, ab.e, ab.c, ab.f, ab.vsb, nv.counters, ab.flag);
+  float4* e = ab.e;
+  float4* e2 = ab.e2;
   for (int k = K - 1; k >= 0; --k) {
-    k_adj_lambda<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], a, ab.q1, ab.q2, pc);
-    k_adj_pos<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.q1, ab.q2, a2, pc);
-    float4* t = a;
-    a = a2;
-    a2 = t;
+    k_adj_lambda<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.f, ab.q1, ab.q2, pc);
+    k_adj_pos<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.q1, ab.q2, e, ab.c, e2, ab.f, pc);
+    float4* t = e;
+    e = e2;
+    e2 = t;
   }
-  k_adj_end<<<nb, kBlock, 0, st>>>(n, perm, a, ab.xsb, ab.vsb, pc.dt, xbar, vbar);
+  k_adj_end<<<nb, kBlock, 0, st>>>(n, perm, e, ab.vsb, pc.dt, xbar, vbar);
 }
 
 }  // namespace pbf

@@ -39,7 +39,7 @@ struct pbf_ctx {
   // particle arrays (capacity cap_n)
   int cap_n = 0;
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *inv = nullptr;
-  int *sid = nullptr, *ssid = nullptr;
+  int *sid = nullptr, *ssid = nullptr, *bpart = nullptr;
   int *nbr_cnt = nullptr, *cnt_orig = nullptr;
   float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr, *vv = nullptr;
   float* dpos = nullptr;
@@ -367,7 +367,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   if (n <= c->cap_n) return PBF_OK;
   const int cap = std::max(((n + 255) / 256) * 256, 256);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
-  dfree(c->sid); dfree(c->ssid); dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   c->cap_n = 0;
@@ -379,6 +379,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   PBF_HIP(c, hipMalloc(&c->inv, ni));
   PBF_HIP(c, hipMalloc(&c->sid, ni));
   PBF_HIP(c, hipMalloc(&c->ssid, ni));
+  PBF_HIP(c, hipMalloc(&c->bpart, sizeof(int) * bounds_part_ints(cap)));
   PBF_HIP(c, hipMalloc(&c->nbr_cnt, ni));
   PBF_HIP(c, hipMalloc(&c->cnt_orig, ni));
   PBF_HIP(c, hipMalloc(&c->xn, nf4 + sizeof(float4) * kXnPad));
@@ -413,8 +414,9 @@ int adapt_reservations(pbf_ctx* c) {
   const long long mw = (long long)c->h_counters[C_MASK_WORDS];
   const long long ls = (long long)c->h_counters[C_LIST_SLOTS];
   int st = PBF_OK;
-  if (mw > c->mask_cap) st = ensure_mask(c, mw + mw / 8);
-  if (!st && ls > c->list_cap) st = ensure_list(c, ls + ls / 8);
+  // a reservation given explicitly (> 0 per particle) stays as given
+  if (c->p.mask_words == 0 && mw > c->mask_cap) st = ensure_mask(c, mw + mw / 8);
+  if (!st && c->p.list_slots == 0 && ls > c->list_cap) st = ensure_list(c, ls + ls / 8);
   return st;
 }
 
@@ -473,6 +475,7 @@ SortBufs sort_of(pbf_ctx* c, const int* scene_off, int nscenes) {
   sb.perm = c->perm;
   sb.inv = c->inv;
   sb.scan_part = c->scan_part;
+  sb.bpart = c->bpart;
   return sb;
 }
 
@@ -487,6 +490,8 @@ hipEvent_t take_event(pbf_ctx* c) {
   if (hipEventCreate(&e) != hipSuccess) return nullptr;
   return e;
 }
+
+int drain_marks(pbf_ctx* c);
 
 struct PassTimer {
   pbf_ctx* c;
@@ -505,6 +510,8 @@ struct PassTimer {
     }
     (void)hipEventRecord(b, st);
     c->marks.push_back({pass, a, b});
+    // bound the events in flight: sum the finished ones every 512 passes
+    if (c->marks.size() >= 512) (void)drain_marks(c);
   }
 };
 
@@ -722,8 +729,8 @@ void pbf_params_default(pbf_params* p) {
     p->scene_lo[a] = 0.0f;
     p->scene_hi[a] = 1.0f;
   }
-  p->mask_words = 32;
-  p->list_slots = 96;
+  p->mask_words = 0;
+  p->list_slots = 0;
   p->viscosity = 1.0f;
   p->viscosity_iterations = 5;
   for (int a = 0; a < 3; ++a) {
@@ -783,7 +790,7 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->off1); dfree(c->scene_minmax); dfree(c->win); dfree(c->total_cells); dfree(c->scan_part);
   dfree(c->cell_cnt); dfree(c->cell_start); dfree(c->oog_flag);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
-  dfree(c->sid); dfree(c->ssid); dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->mask); dfree(c->list); dfree(c->scan_scratch); dfree(c->bf.d);
@@ -1008,11 +1015,12 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
   for (int k = 1; k < K; ++k) P[k] = r.tape + (size_t)(k - 1) * cap;
   for (int k = 0; k < K; ++k) aux[k] = r.tape + (size_t)(K - 1 + k) * cap;
   AdjBufs ab;
-  ab.a = c->xa;
-  ab.a2 = c->xb;
+  ab.e = c->xa;
+  ab.e2 = c->xb;
+  ab.f = c->vv;
   ab.q1 = r.adj;
   ab.q2 = r.adj + cap;
-  ab.xsb = r.adj + 2 * cap;
+  ab.c = r.adj + 2 * cap;
   ab.vsb = r.adj + 3 * cap;
   ab.flag = r.flag;
   PBF_HIP(c, hipMemsetAsync(r.flag, 0, sizeof(int), st));

@@ -114,6 +114,7 @@ struct SortBufs {
   int* perm;
   int* inv;
   int* scan_part;         // partial sums of the device-length scan
+  int* bpart;             // per block of the bounds pass: scene and cell box (bounds_part_ints)
 };
 
 constexpr int kBlock = 256;
@@ -124,6 +125,7 @@ constexpr int kXnPad = 32;
 constexpr int kScanDevBlocks = 512;
 
 size_t scan_scratch_ints(long long n);
+size_t bounds_part_ints(long long n);
 // out[0..n] = exclusive scan of in[0..n-1] (out[n] = total); in may equal out.
 void scan_exclusive(const int* in, int* out, int n, int* scratch, hipStream_t st);
 
@@ -151,11 +153,12 @@ void launch_csr_fill(const NbrView& nv, const int* perm, const int* offsets, int
 // boundary). P[k] (k < K): positions and lambda of PBD iteration k; aux[k]:
 // (sum_j t^2/r d_ij, 1/(S + eps)) of that iteration.
 struct AdjBufs {
-  float4* a;    // adjoint of the current iterate
-  float4* a2;   // next adjoint (ping-pong with a)
-  float4* q1;   // (a, rho_bar)
+  float4* e;    // adjoint of the current iterate minus c
+  float4* e2;   // next e (ping-pong)
+  float4* f;    // e + c, gathered by the pair terms
+  float4* c;    // v_bar' / dt
+  float4* q1;   // (f, rho_bar)
   float4* q2;   // (S_bar * G, S_bar)
-  float4* xsb;  // adjoint of the integrated positions, velocity part
   float4* vsb;  // adjoint of the integrated velocities
   int* flag;    // set when a taped step had out-of-grid particles
 };

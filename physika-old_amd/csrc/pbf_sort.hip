@@ -183,12 +183,26 @@ __device__ __forceinline__ bool grid_cell(const GridDev& g, const float* __restr
 }
 
 // Scene of every particle and the cell bounding box of each scene's in-grid
-// particles (wave-reduced when a wave lies in one scene).
-__global__ __launch_bounds__(kBlock) void k_scene_bounds(const float* __restrict__ x, int n, GridDev g,
-                                                         const int* __restrict__ off, int nscenes,
-                                                         int* __restrict__ sid, int* __restrict__ minmax,
-                                                         unsigned long long* __restrict__ counters) {
+// particles. A block whose particles all belong to one scene writes its
+// box to bpart (8 ints: scene, min i j k, max i j k, -); blocks that straddle
+// scenes mark bpart -1 and update the scenes' boxes with atomics (per wave,
+// per lane where a wave straddles). Same-address atomics from every block
+// would serialize in L2, so the single-scene blocks are reduced per scene by
+// k_scene_reduce.
+constexpr int kBoundsBlock = 1024;
+
+size_t bounds_part_ints(long long n) { return 8 * (size_t)((n + kBoundsBlock - 1) / kBoundsBlock + 1); }
+
+__global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const float* __restrict__ x, int n, GridDev g,
+                                                               const int* __restrict__ off, int nscenes,
+                                                               int* __restrict__ sid, int* __restrict__ minmax,
+                                                               int* __restrict__ bpart,
+                                                               unsigned long long* __restrict__ counters) {
+  __shared__ int red[kBoundsBlock / 64][6];
+  __shared__ unsigned cnt[kBoundsBlock / 64][2];
+  __shared__ int bscene[2];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const bool valid = i < n;
   int s = -1, ci = 0, cj = 0, ck = 0;
   bool nonfinite = false, inside = false;
@@ -197,18 +211,49 @@ __global__ __launch_bounds__(kBlock) void k_scene_bounds(const float* __restrict
     sid[i] = s;
     inside = grid_cell(g, x, i, ci, cj, ck, nonfinite);
   }
+  const int last = min(n, (int)(blockIdx.x * blockDim.x + blockDim.x)) - 1 - (int)(blockIdx.x * blockDim.x);
+  if (threadIdx.x == 0) bscene[0] = s;
+  if ((int)threadIdx.x == last) bscene[1] = s;
   const unsigned long long b_oog = __ballot(valid && !inside), b_nf = __ballot(valid && nonfinite);
-  if ((threadIdx.x & 63) == 0) {
-    if (b_oog) atomicAdd(&counters[C_OOG], (unsigned long long)__popcll(b_oog));
-    if (b_nf) atomicAdd(&counters[C_NONFINITE], (unsigned long long)__popcll(b_nf));
+  if (lane == 0) {
+    cnt[wave][0] = (unsigned)__popcll(b_oog);
+    cnt[wave][1] = (unsigned)__popcll(b_nf);
   }
+  const int mi = wave_min(inside ? ci : INT_MAX), mj = wave_min(inside ? cj : INT_MAX);
+  const int mk = wave_min(inside ? ck : INT_MAX);
+  const int xi = wave_max(inside ? ci : INT_MIN), xj = wave_max(inside ? cj : INT_MIN);
+  const int xk = wave_max(inside ? ck : INT_MIN);
+  if (lane == 0) {
+    red[wave][0] = mi;
+    red[wave][1] = mj;
+    red[wave][2] = mk;
+    red[wave][3] = xi;
+    red[wave][4] = xj;
+    red[wave][5] = xk;
+  }
+  __syncthreads();
+  const bool one_scene = bscene[0] == bscene[1];
+  if (threadIdx.x == 0) {
+    unsigned o = 0, f = 0;
+    for (int w = 0; w < kBoundsBlock / 64; ++w) {
+      o += cnt[w][0];
+      f += cnt[w][1];
+    }
+    if (o) atomicAdd(&counters[C_OOG], (unsigned long long)o);
+    if (f) atomicAdd(&counters[C_NONFINITE], (unsigned long long)f);
+    int r[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    for (int w = 0; w < kBoundsBlock / 64; ++w) {
+      for (int a = 0; a < 3; ++a) r[a] = min(r[a], red[w][a]);
+      for (int a = 3; a < 6; ++a) r[a] = max(r[a], red[w][a]);
+    }
+    int* bp = bpart + 8 * (size_t)blockIdx.x;
+    bp[0] = one_scene ? bscene[0] : -1;
+    for (int a = 0; a < 6; ++a) bp[1 + a] = r[a];
+  }
+  if (one_scene) return;  // block-uniform
   const int s0 = __shfl(s, 0);  // lane 0 is valid whenever any lane is
   if (__all(!valid || s == s0)) {
-    const int mi = wave_min(inside ? ci : INT_MAX), mj = wave_min(inside ? cj : INT_MAX);
-    const int mk = wave_min(inside ? ck : INT_MAX);
-    const int xi = wave_max(inside ? ci : INT_MIN), xj = wave_max(inside ? cj : INT_MIN);
-    const int xk = wave_max(inside ? ck : INT_MIN);
-    if ((threadIdx.x & 63) == 0 && s0 >= 0 && xi >= mi) {
+    if (lane == 0 && s0 >= 0 && xi >= mi) {
       int* m = minmax + 6 * s0;
       atomicMin(m + 0, mi);
       atomicMin(m + 1, mj);
@@ -225,6 +270,44 @@ __global__ __launch_bounds__(kBlock) void k_scene_bounds(const float* __restrict
     atomicMax(m + 3, ci);
     atomicMax(m + 4, cj);
     atomicMax(m + 5, ck);
+  }
+}
+
+// One block per scene: reduce the boxes of the single-scene blocks of the
+// scene's particle range and merge them into the scene's box.
+__global__ __launch_bounds__(256) void k_scene_reduce(const int* __restrict__ off, int nscenes,
+                                                      const int* __restrict__ bpart, int* __restrict__ minmax) {
+  __shared__ int red[4][6];
+  const int s = blockIdx.x;
+  const int b0 = off[s] / kBoundsBlock;
+  const int b1 = off[s + 1] > off[s] ? (off[s + 1] - 1) / kBoundsBlock + 1 : b0;
+  int r[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+  for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
+    const int* bp = bpart + 8 * (size_t)b;
+    if (bp[0] != s) continue;
+    for (int a = 0; a < 3; ++a) r[a] = min(r[a], bp[1 + a]);
+    for (int a = 3; a < 6; ++a) r[a] = max(r[a], bp[1 + a]);
+  }
+  for (int a = 0; a < 3; ++a) r[a] = wave_min(r[a]);
+  for (int a = 3; a < 6; ++a) r[a] = wave_max(r[a]);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  if (lane == 0)
+    for (int a = 0; a < 6; ++a) red[wave][a] = r[a];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < 4; ++w) {
+      for (int a = 0; a < 3; ++a) r[a] = min(r[a], red[w][a]);
+      for (int a = 3; a < 6; ++a) r[a] = max(r[a], red[w][a]);
+    }
+    if (r[3] >= r[0]) {
+      int* m = minmax + 6 * s;
+      atomicMin(m + 0, r[0]);
+      atomicMin(m + 1, r[1]);
+      atomicMin(m + 2, r[2]);
+      atomicMax(m + 3, r[3]);
+      atomicMax(m + 4, r[4]);
+      atomicMax(m + 5, r[5]);
+    }
   }
 }
 
@@ -355,7 +438,10 @@ void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const flo
   const int nb = (n + kBlock - 1) / kBlock;
   k_scene_reset<<<(6 * sb.nscenes + C_COUNT + kBlock - 1) / kBlock, kBlock, 0, st>>>(sb.scene_minmax, sb.nscenes,
                                                                                      counters);
-  k_scene_bounds<<<nb, kBlock, 0, st>>>(x, n, g, sb.scene_off, sb.nscenes, sb.sid, sb.scene_minmax, counters);
+  k_scene_bounds<<<(n + kBoundsBlock - 1) / kBoundsBlock, kBoundsBlock, 0, st>>>(x, n, g, sb.scene_off, sb.nscenes,
+                                                                                 sb.sid, sb.scene_minmax, sb.bpart,
+                                                                                 counters);
+  k_scene_reduce<<<sb.nscenes, 256, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
   k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, counters);
   const int clear_blocks = (int)std::min<long long>((sb.cell_cap + kBlock - 1) / kBlock, 2048);
   k_clear_cells<<<clear_blocks, kBlock, 0, st>>>(sb.cell_cnt, sb.oog_flag, sb.total_cells);

@@ -92,6 +92,7 @@ def test_batch_equals_serial_mixed(fluid, scene):
     scenes = mixed_scenes(scene)
     vels = [velocities(scene, len(s), 60 + k) for k, s in enumerate(scenes)]
     vels[8][:] = np.array([-2.0, 0.0, 0.0], np.float32)  # the out-of-grid block keeps leaving
+    vels[10] = vels[9].copy()  # twins: same state
     got, ctx = batched(fluid, scenes, vels, 3)
     assert_bit_identical(got, serial(fluid, scenes, vels, 3))
     st = ctx.stats()

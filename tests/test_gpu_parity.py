@@ -180,8 +180,8 @@ def test_empty_input(fluid):
 def test_reservation_fallbacks_are_identical(fluid, scene, scene_name, reserve):
     """Reservations too small for the scene make waves iterate global bit masks
     or re-test their candidates in every pass; the results must be
-    bit-identical to the neighbor-list path, and the reservations grow for the
-    next call."""
+    bit-identical to the neighbor-list path. Automatic reservations (the
+    default) grow for the next call; explicit ones stay as given."""
     say(f"neighbor storage fallback {scene_name} {reserve}")
     if scene_name == "lattice":
         x = scene.dam_break_32k()
@@ -202,9 +202,9 @@ def test_reservation_fallbacks_are_identical(fluid, scene, scene_name, reserve):
         assert st["mask_words_needed"] > 0 and st["mask_fallback_waves"] == 0
     for u, w in zip(ra, rb):
         np.testing.assert_array_equal(u, w)
-    rb2 = b.step_arrays(x, v, 1)  # reservations grown after the first call
+    rb2 = b.step_arrays(x, v, 1)  # explicit reservations: the same fallbacks again
     st = b.ctx.stats()
-    assert st["mask_fallback_waves"] == 0 and st["list_slots_needed"] <= st["list_slots_reserved"]
+    assert st["list_slots_needed"] > st["list_slots_reserved"]
     for u, w in zip(ra, rb2):
         np.testing.assert_array_equal(u, w)
 
