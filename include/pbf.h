@@ -153,6 +153,15 @@ int pbf_reserve_batch(pbf_ctx* ctx, int n, int nscenes);
 int pbf_neighbor_query(pbf_ctx* ctx, const float* d_x, int n, int* d_offsets, int* d_elems,
                        long long cap, long long* total, void* stream);
 
+/* NeighborQuery::compute with a neighbor limit (NeighborList::isLimited;
+ * queryNeighborFixed, NeighborQuery.cu:514-581): row i keeps the `limit`
+ * nearest of its neighbors (1 <= limit <= 32, else PBF_ERR_UNSUPPORTED):
+ * d_counts[i] = kept, d_elems[limit*i + k] their ids sorted by distance
+ * (ties: smaller id first; the reference's tie order follows its atomic
+ * visit order). d_elems holds n*limit ints. */
+int pbf_neighbor_query_fixed(pbf_ctx* ctx, const float* d_x, int n, int limit, int* d_counts, int* d_elems,
+                             void* stream);
+
 /* ParticleIntegrator::begin + integrate (ParticleIntegrator.cu:20-34,129-138):
  * v += dt*(f + g), x += dt*v. d_f may be NULL (the zeroed force of begin()). */
 int pbf_integrate(pbf_ctx* ctx, float* d_x, float* d_v, const float* d_f, int n, void* stream);
@@ -168,6 +177,13 @@ int pbf_density(pbf_ctx* ctx, const float* d_x, const int* d_offsets, const int*
  * receives the last iteration's density. */
 int pbf_pbd_constrain(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_offsets,
                       const int* d_elems, float* d_rho, int n, void* stream);
+
+/* DensityPBD::constrain with DensityPBD::m_massInv set (the massInv variants
+ * of K_ComputeLambdas and K_ComputeDisplacement, DensityPBD.cu:93-131,182-224):
+ * d_mass_inv holds one inverse mass per particle. Otherwise as
+ * pbf_pbd_constrain. */
+int pbf_pbd_constrain_mass(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_offsets, const int* d_elems,
+                           const float* d_mass_inv, float* d_rho, int n, void* stream);
 
 /* nsteps of PositionBasedFluidModel::step without viscosity
  * (PositionBasedFluidModel.cpp:83-101): begin -> NeighborQuery on x^n ->

@@ -96,6 +96,9 @@ def lib():
             "orc_integrate_d": (None, [PD, dp, dp, dp, C.c_int]),
             "orc_step_f": (C.c_longlong, [PF, fp, fp, fp, C.c_int, C.c_int]),
             "orc_step_d": (C.c_longlong, [PD, dp, dp, dp, C.c_int, C.c_int]),
+            "orc_pbd_constrain_minv_f": (None, [PF, fp, fp, C.c_int, ip, ip, fp, fp]),
+            "orc_pbd_constrain_minv_d": (None, [PD, dp, dp, C.c_int, ip, ip, dp, dp]),
+            "orc_neighbor_query_fixed_f": (None, [PF, fp, C.c_int, C.c_int, ip, ip]),
             "orc_viscosity_f": (None, [PF, fp, fp, C.c_int, ip, ip]),
             "orc_vis_weight_f": (C.c_float, [C.c_float, C.c_float]),
             "orc_box_sdf_f": (C.c_longlong, [fp, fp, C.c_float, C.c_int, ip, fp, fp, fp, C.c_longlong]),
@@ -293,3 +296,29 @@ def box_constrain(params, x, v):
 
 def box_jitter(i):
     return lib().orc_box_jitter(int(i))
+
+
+def pbd_constrain_minv(params, x, v, off, el, minv):
+    """DensityPBD::constrain with inverse masses (the massInv variant);
+    returns (x, v, rho)."""
+    dbl = isinstance(params, ParamsD)
+    dt = np.float64 if dbl else np.float32
+    ct = C.c_double if dbl else C.c_float
+    x = np.array(x, dtype=dt, copy=True, order="C")
+    v = np.array(v, dtype=dt, copy=True, order="C")
+    minv = np.ascontiguousarray(minv, dtype=dt)
+    rho = np.zeros(len(x), dtype=dt)
+    fn = lib().orc_pbd_constrain_minv_d if dbl else lib().orc_pbd_constrain_minv_f
+    fn(C.byref(params), _ptr(x, ct), _ptr(v, ct), len(x), _ptr(np.ascontiguousarray(off, np.int32), C.c_int),
+       _ptr(np.ascontiguousarray(el, np.int32), C.c_int), _ptr(minv, ct), _ptr(rho, ct))
+    return x, v, rho
+
+
+def neighbor_query_fixed(params, x, limit):
+    """Fixed-capacity neighbor rows: (counts (n,), elems (n, limit))."""
+    x = _f32(x)
+    counts = np.zeros(len(x), np.int32)
+    elems = np.full((len(x), limit), -1, np.int32)
+    lib().orc_neighbor_query_fixed_f(C.byref(params), _ptr(x, C.c_float), len(x), int(limit),
+                                     _ptr(counts, C.c_int), _ptr(elems, C.c_int))
+    return counts, elems

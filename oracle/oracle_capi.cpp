@@ -208,4 +208,20 @@ void orc_box_constrain_f(const Params<float>* p, float* x, float* v, int n) {
 
 float orc_box_jitter(int id) { return box_jitter(id); }
 
+void orc_pbd_constrain_minv_f(const Params<float>* p, float* x, float* v, int n, const int* off, const int* el,
+                              const float* minv, float* rho) {
+  CSR c = to_csr<float>(off, el, n);
+  pbd_constrain(*p, reinterpret_cast<V3<float>*>(x), reinterpret_cast<V3<float>*>(v), n, c, rho, minv);
+}
+void orc_pbd_constrain_minv_d(const Params<double>* p, double* x, double* v, int n, const int* off, const int* el,
+                              const double* minv, double* rho) {
+  CSR c = to_csr<double>(off, el, n);
+  pbd_constrain(*p, reinterpret_cast<V3<double>*>(x), reinterpret_cast<V3<double>*>(v), n, c, rho, minv);
+}
+
+void orc_neighbor_query_fixed_f(const Params<float>* p, const float* x, int n, int limit, int* counts, int* elems) {
+  Grid<float> g = grid_setup(p->h, p->lo, p->hi);
+  neighbor_query_fixed(g, reinterpret_cast<const V3<float>*>(x), n, p->h, limit, counts, elems);
+}
+
 }  // extern "C"

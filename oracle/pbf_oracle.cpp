@@ -176,6 +176,45 @@ void neighbor_query(const Grid<R>& g, const V3<R>* x, int n, R h, CSR& out) {
   }
 }
 
+// NeighborQuery::queryNeighborFixed with K_ComputeNeighborFixed
+// (NeighborQuery.cu:514-581): candidates visited in the 27-offset order,
+// the first `limit` neighbors fill the row, each later one replaces the
+// farthest kept one when closer (first farthest on ties). counts[i] = kept,
+// elems[limit*i + k] in the row's final order.
+template <class R>
+void neighbor_query_fixed(const Grid<R>& g, const V3<R>* x, int n, R h, int limit, int* counts, int* elems) {
+  CSR nb;
+  neighbor_query(g, x, n, h, nb);
+#pragma omp parallel for schedule(dynamic, 256)
+  for (int i = 0; i < n; ++i) {
+    std::vector<R> dist((size_t)limit);
+    int* ids = elems + (size_t)limit * i;
+    int counter = 0;
+    for (int e = nb.offsets[i]; e < nb.offsets[i + 1]; ++e) {
+      const int j = nb.elems[e];
+      const R d = (R)(float)norm(sub(x[i], x[j]));  // float d_ij (NeighborQuery.cu:547)
+      if (counter < limit) {
+        ids[counter] = j;
+        dist[counter] = d;
+        ++counter;
+      } else {
+        int max_id = 0;
+        R max_d = dist[0];
+        for (int k = 1; k < limit; ++k)
+          if (max_d < dist[k]) {
+            max_d = dist[k];
+            max_id = k;
+          }
+        if (d < dist[max_id]) {
+          dist[max_id] = d;
+          ids[max_id] = j;
+        }
+      }
+    }
+    counts[i] = counter;
+  }
+}
+
 // K_ComputeDensity, SummationDensity.cu:13-36 (mass = m_factor * particle
 // mass, SummationDensity.cu:104-109).
 template <class R>
@@ -192,9 +231,10 @@ void density(const V3<R>* x, int n, const CSR& nb, R h, R mass, R* rho) {
   }
 }
 
-// K_ComputeLambdas, DensityPBD.cu:46-90.
+// K_ComputeLambdas, DensityPBD.cu:46-90; with minv (inverse masses) the
+// massInv variant, DensityPBD.cu:93-131: sum_j |g|^2 m_j^-1 + |sum g|^2 m_i^-1.
 template <class R>
-void lambdas(const Params<R>& p, const R* rho, const V3<R>* x, int n, const CSR& nb, R* lam) {
+void lambdas(const Params<R>& p, const R* rho, const V3<R>* x, int n, const CSR& nb, R* lam, const R* minv) {
 #pragma omp parallel for schedule(static)
   for (int i = 0; i < n; ++i) {
     R lamda_i = (R)0;
@@ -210,10 +250,10 @@ void lambdas(const Params<R>& p, const R* rho, const V3<R>* x, int n, const CSR&
         grad_ci.x += gv.x;
         grad_ci.y += gv.y;
         grad_ci.z += gv.z;
-        lamda_i += dot(gv, gv);
+        lamda_i += minv ? dot(gv, gv) * minv[j] : dot(gv, gv);
       }
     }
-    lamda_i += dot(grad_ci, grad_ci);
+    lamda_i += minv ? dot(grad_ci, grad_ci) * minv[i] : dot(grad_ci, grad_ci);
     R rho_i = rho[i];
     lamda_i = -(rho_i - p.lam_rest) / (lamda_i + p.lam_eps);
     lam[i] = lamda_i > (R)0 ? (R)0 : lamda_i;
@@ -262,22 +302,29 @@ void displacement_gather(const Params<R>& p, const R* lam, const V3<R>* x, int n
 // Literal sequential emulation of the reference's pairwise atomic scatter
 // (DensityPBD.cu:152-176): for every i in order, for every listed j,
 // dPos[i] += dp_ij and dPos[j] -= dp_ij. Exact for any list, including the
-// asymmetric lists produced when particles leave the grid.
+// asymmetric lists produced when particles leave the grid. With minv, the
+// massInv variant (DensityPBD.cu:182-224): dp_ji = -dp_ij m_j^-1 to j and
+// dp_ij m_i^-1 to i.
 template <class R>
 void displacement_atomic_order(const Params<R>& p, const R* lam, const V3<R>* x, int n, const CSR& nb,
-                               V3<R>* dpos) {
+                               V3<R>* dpos, const R* minv) {
   for (int i = 0; i < n; ++i) dpos[i] = {0, 0, 0};  // m_deltaPos.reset(), :315
   for (int i = 0; i < n; ++i) {
     for (int k = nb.offsets[i]; k < nb.offsets[i + 1]; ++k) {
       int j = nb.elems[k];
       V3<R> dp;
       if (pair_dp(p, x[i], x[j], lam[i], lam[j], dp)) {
+        V3<R> dj{-dp.x, -dp.y, -dp.z};
+        if (minv) {
+          dj = {dj.x * minv[j], dj.y * minv[j], dj.z * minv[j]};
+          dp = {dp.x * minv[i], dp.y * minv[i], dp.z * minv[i]};
+        }
         dpos[i].x += dp.x;
-        dpos[j].x += -dp.x;
+        dpos[j].x += dj.x;
         dpos[i].y += dp.y;
-        dpos[j].y += -dp.y;
+        dpos[j].y += dj.y;
         dpos[i].z += dp.z;
-        dpos[j].z += -dp.z;
+        dpos[j].z += dj.z;
       }
     }
   }
@@ -301,21 +348,21 @@ template <class R> static bool list_symmetric(const CSR& nb, int n) {
 // rho receives the density of the last iteration, computed before its Δx
 // (outDensity is SummationDensity's output, DensityPBD.cu:259).
 template <class R>
-void pbd_constrain(const Params<R>& p, V3<R>* x, V3<R>* v, int n, const CSR& nb, R* rho) {
+void pbd_constrain(const Params<R>& p, V3<R>* x, V3<R>* v, int n, const CSR& nb, R* rho, const R* minv) {
   std::vector<V3<R>> x_old(x, x + n);
   std::vector<R> lam(n);
   std::vector<V3<R>> dpos(n);
   std::vector<R> rho_tmp(n);
   R* rho_w = rho ? rho : rho_tmp.data();
   const R mass = scaling_factor(p.d, p.h) * (p.rho0 * (p.d * p.d * p.d));
-  const bool gather = p.dx_mode == 1 || (p.dx_mode == 0 && list_symmetric<R>(nb, n));
+  const bool gather = !minv && (p.dx_mode == 1 || (p.dx_mode == 0 && list_symmetric<R>(nb, n)));
   for (int it = 0; it < p.iters; ++it) {
     density(x, n, nb, p.h, mass, rho_w);
-    lambdas(p, rho_w, x, n, nb, lam.data());
+    lambdas(p, rho_w, x, n, nb, lam.data(), minv);
     if (gather)
       displacement_gather(p, lam.data(), x, n, nb, dpos.data());
     else
-      displacement_atomic_order(p, lam.data(), x, n, nb, dpos.data());
+      displacement_atomic_order(p, lam.data(), x, n, nb, dpos.data(), minv);
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
       x[i].x += dpos[i].x;
@@ -779,12 +826,13 @@ void rollout_adjoint(const Params<R>& p, const V3<R>* x0, const V3<R>* v0, int n
   template R scaling_factor<R>(R, R);                                                              \
   template void neighbor_query<R>(const Grid<R>&, const V3<R>*, int, R, CSR&);                     \
   template void density<R>(const V3<R>*, int, const CSR&, R, R, R*);                               \
-  template void lambdas<R>(const Params<R>&, const R*, const V3<R>*, int, const CSR&, R*);         \
+  template void lambdas<R>(const Params<R>&, const R*, const V3<R>*, int, const CSR&, R*, const R*); \
+  template void neighbor_query_fixed<R>(const Grid<R>&, const V3<R>*, int, R, int, int*, int*);     \
   template void displacement_gather<R>(const Params<R>&, const R*, const V3<R>*, int, const CSR&,  \
                                        V3<R>*);                                                    \
   template void displacement_atomic_order<R>(const Params<R>&, const R*, const V3<R>*, int,        \
-                                             const CSR&, V3<R>*);                                  \
-  template void pbd_constrain<R>(const Params<R>&, V3<R>*, V3<R>*, int, const CSR&, R*);           \
+                                             const CSR&, V3<R>*, const R*);                        \
+  template void pbd_constrain<R>(const Params<R>&, V3<R>*, V3<R>*, int, const CSR&, R*, const R*); \
   template void integrate<R>(const Params<R>&, V3<R>*, V3<R>*, const V3<R>*, int);                 \
   template void step<R>(const Params<R>&, V3<R>*, V3<R>*, R*, int, CSR*, const SDF<R>*);         \
   template R vis_weight<R>(R, R);                                                                  \

@@ -25,6 +25,10 @@
 //                     the reference's pairwise scatter (factor 2); 2-particle
 //                     closed form
 //   integrator        free fall closed form (bit-exact)
+//   massInv variant   unit inverse masses reproduce the plain path; mass-
+//                     weighted momentum conservation; two-particle closed form
+//   fixed neighbors   the replace-farthest rows hold the `limit` nearest of the
+//                     dynamic set (sorted selection on tie-free inputs)
 //   step              committed golden fixtures (regression pins only)
 //   step_adjoint,     FP64 central finite differences of step()/rollouts on
 //   rollout_adjoint   small lattices (compressed, so that lambda is active),
@@ -103,16 +107,23 @@ template <class R>
 void neighbor_query(const Grid<R>& g, const V3<R>* x, int n, R h, CSR& out);
 template <class R>
 void density(const V3<R>* x, int n, const CSR& nb, R h, R mass, R* rho);
+// minv: inverse masses (the massInv variant) or null.
 template <class R>
-void lambdas(const Params<R>& p, const R* rho, const V3<R>* x, int n, const CSR& nb, R* lam);
+void lambdas(const Params<R>& p, const R* rho, const V3<R>* x, int n, const CSR& nb, R* lam,
+             const R* minv = nullptr);
 template <class R>
 void displacement_gather(const Params<R>& p, const R* lam, const V3<R>* x, int n,
                          const CSR& nb, V3<R>* dpos);
 template <class R>
 void displacement_atomic_order(const Params<R>& p, const R* lam, const V3<R>* x, int n,
-                               const CSR& nb, V3<R>* dpos);
+                               const CSR& nb, V3<R>* dpos, const R* minv = nullptr);
 template <class R>
-void pbd_constrain(const Params<R>& p, V3<R>* x, V3<R>* v, int n, const CSR& nb, R* rho);
+void pbd_constrain(const Params<R>& p, V3<R>* x, V3<R>* v, int n, const CSR& nb, R* rho,
+                   const R* minv = nullptr);
+// NeighborQuery with a neighbor limit (NeighborList::isLimited, the
+// fixed-capacity layout elems[limit*i + k]).
+template <class R>
+void neighbor_query_fixed(const Grid<R>& g, const V3<R>* x, int n, R h, int limit, int* counts, int* elems);
 template <class R>
 void integrate(const Params<R>& p, V3<R>* x, V3<R>* v, const V3<R>* f, int n);
 // sdf: the box field when flags & 2 (built here when null).

@@ -91,9 +91,11 @@ SIGNATURES = {
     "pbf_reserve": (C.c_int, [_CTX, C.c_int]),
     "pbf_reserve_batch": (C.c_int, [_CTX, C.c_int, C.c_int]),
     "pbf_neighbor_query": (C.c_int, [_CTX, _VP, C.c_int, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
+    "pbf_neighbor_query_fixed": (C.c_int, [_CTX, _VP, C.c_int, C.c_int, _VP, _VP, _VP]),
     "pbf_integrate": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, _VP]),
     "pbf_density": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, _VP]),
     "pbf_pbd_constrain": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+    "pbf_pbd_constrain_mass": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
     "pbf_step": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, C.c_int, _VP]),
     "pbf_batch_step": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP]),
     "pbf_rollout_forward": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),

@@ -1059,6 +1059,47 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
   return PBF_OK;
 }
 
+int pbf_neighbor_query_fixed(pbf_ctx* c, const float* d_x, int n, int limit, int* d_counts, int* d_elems,
+                             void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_counts || !d_elems))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (limit < 1 || limit > kFixedMax)
+    return fail(c, PBF_ERR_UNSUPPORTED, "neighbor limit must be in [1, " + std::to_string(kFixedMax) + "]");
+  if (n == 0) return PBF_OK;
+  hipStream_t st = (hipStream_t)stream;
+  int s = ensure_particles(c, n, 1);
+  if (s) return s;
+  s = adapt_reservations(c);
+  if (s) return s;
+  launch_set_offsets(c->off1, n, st);
+  s = search(c, d_x, nullptr, n, c->off1, 1, false, st);
+  if (s) return s;
+  launch_nbr_fixed(view_of(c, n), c->perm, limit, d_counts, d_elems, st);
+  PBF_LAUNCHED(c);
+  return record_counters(c, st);
+}
+
+int pbf_pbd_constrain_mass(pbf_ctx* c, float* d_x, float* d_v, const int* d_offsets, const int* d_elems,
+                           const float* d_mass_inv, float* d_rho, int n, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems || !d_mass_inv)))
+    return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n == 0) return PBF_OK;
+  int s = ensure_particles(c, n, 1);
+  if (s) return s;
+  hipStream_t st = (hipStream_t)stream;
+  launch_aos_to_f4(d_x, c->xa, n, st);
+  for (int it = 0; it < c->p.iterations; ++it) {
+    const bool last = it + 1 == c->p.iterations;
+    launch_csr_lambda(c->xa, d_offsets, d_elems, last ? d_rho : nullptr, n, c->pc, st, d_mass_inv);
+    launch_csr_dx_atomic(c->xa, d_offsets, d_elems, c->dpos, n, c->pc, st, d_mass_inv);
+    launch_csr_apply(c->xa, c->dpos, n, st);
+  }
+  launch_csr_finish(c->xa, d_x, d_v, n, c->pc.dt, st);
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
 int pbf_viscosity(pbf_ctx* c, const float* d_x, float* d_v, const int* d_offsets, const int* d_elems, int n,
                   void* stream) {
   if (!c) return PBF_ERR_INVALID;

@@ -180,9 +180,13 @@ void launch_csr_density(const float* x, const int* off, const int* el, float* rh
                         const PhysConst& pc, hipStream_t st);
 void launch_aos_to_f4(const float* x, float4* xa, int n, hipStream_t st);
 void launch_csr_lambda(float4* xa, const int* off, const int* el, float* rho, int n,
-                       const PhysConst& pc, hipStream_t st);
+                       const PhysConst& pc, hipStream_t st, const float* minv = nullptr);
 void launch_csr_dx_atomic(const float4* xa, const int* off, const int* el, float* dpos, int n,
-                          const PhysConst& pc, hipStream_t st);
+                          const PhysConst& pc, hipStream_t st, const float* minv = nullptr);
+// Fixed-capacity rows (the `limit` nearest of each neighbor set, by distance
+// then id) in original ids; limit <= kFixedMax.
+constexpr int kFixedMax = 32;
+void launch_nbr_fixed(const NbrView& nv, const int* perm, int limit, int* counts, int* elems, hipStream_t st);
 void launch_csr_apply(float4* xa, float* dpos, int n, hipStream_t st);
 void launch_csr_finish(const float4* xa, float* x, float* v, int n, float dt, hipStream_t st);
 

@@ -66,16 +66,18 @@ void launch_aos_to_f4(const float* x, float4* xa, int n, hipStream_t st) {
 }
 
 // Density and K_ComputeLambdas (DensityPBD.cu:46-90) over a CSR list; lambda
-// goes to xa[i].w.
+// goes to xa[i].w. With minv (inverse masses) the massInv variant
+// (DensityPBD.cu:93-131): S = sum_j |g|^2 m_j^-1 + |sum g|^2 m_i^-1.
 __global__ __launch_bounds__(kBlock) void k_csr_lambda(float4* __restrict__ xa, const int* __restrict__ off,
                                                        const int* __restrict__ el, float* __restrict__ rho, int n,
-                                                       PhysConst pc) {
+                                                       PhysConst pc, const float* __restrict__ minv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 xi = xa[i];
   float acc = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
   for (int k = off[i]; k < off[i + 1]; ++k) {
-    const PairGeom p = pair_geom(xi, xa[el[k]], pc.inv_h);
+    const int j = el[k];
+    const PairGeom p = pair_geom(xi, xa[j], pc.inv_h);
     const float t2 = p.t * p.t;
     acc += t2 * p.t;
     const float gq = p.r > pc.pair_eps ? t2 * p.rinv : 0.0f;
@@ -83,10 +85,12 @@ __global__ __launch_bounds__(kBlock) void k_csr_lambda(float4* __restrict__ xa, 
     sgx += gx;
     sgy += gy;
     sgz += gz;
-    sgg += gx * gx + gy * gy + gz * gz;
+    const float gg = gx * gx + gy * gy + gz * gz;
+    sgg += minv ? gg * minv[j] : gg;
   }
   acc *= pc.cW;
-  const float S = (pc.cG * pc.cG) * (sgg + (sgx * sgx + sgy * sgy + sgz * sgz));
+  const float G2 = sgx * sgx + sgy * sgy + sgz * sgz;
+  const float S = (pc.cG * pc.cG) * (sgg + (minv ? G2 * minv[i] : G2));
   float lam = -(acc - pc.lam_rest) / (S + pc.lam_eps);
   lam = lam > 0.0f ? 0.0f : lam;
   reinterpret_cast<float*>(xa + i)[3] = lam;
@@ -94,20 +98,22 @@ __global__ __launch_bounds__(kBlock) void k_csr_lambda(float4* __restrict__ xa, 
 }
 
 void launch_csr_lambda(float4* xa, const int* off, const int* el, float* rho, int n, const PhysConst& pc,
-                       hipStream_t st) {
-  k_csr_lambda<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, off, el, rho, n, pc);
+                       hipStream_t st, const float* minv) {
+  k_csr_lambda<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, off, el, rho, n, pc, minv);
 }
 
 // K_ComputeDisplacement (DensityPBD.cu:135-179): pairwise scatter with float
 // atomics, valid for any list (symmetric or not), order nondeterministic as in
-// the reference.
+// the reference. With minv the massInv variant (DensityPBD.cu:182-224):
+// dp_ij m_i^-1 to i, -dp_ij m_j^-1 to j.
 __global__ __launch_bounds__(kBlock) void k_csr_dx_atomic(const float4* __restrict__ xa, const int* __restrict__ off,
                                                           const int* __restrict__ el, float* __restrict__ dpos, int n,
-                                                          PhysConst pc) {
+                                                          PhysConst pc, const float* __restrict__ minv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float4 xi = xa[i];
   const float scale = pc.corr * pc.cG;
+  const float mi = minv ? minv[i] : 1.0f;
   float ax = 0.f, ay = 0.f, az = 0.f;
   for (int k = off[i]; k < off[i + 1]; ++k) {
     const int j = el[k];
@@ -116,12 +122,13 @@ __global__ __launch_bounds__(kBlock) void k_csr_dx_atomic(const float4* __restri
     if (!(p.r > pc.pair_eps)) continue;
     const float c = scale * (xi.w + xj.w) * (p.t * p.t) * p.rinv;
     const float px = c * p.dx, py = c * p.dy, pz = c * p.dz;
-    ax += px;
-    ay += py;
-    az += pz;
-    atomicAdd(&dpos[3 * (size_t)j], -px);
-    atomicAdd(&dpos[3 * (size_t)j + 1], -py);
-    atomicAdd(&dpos[3 * (size_t)j + 2], -pz);
+    const float mj = minv ? minv[j] : 1.0f;
+    ax += px * mi;
+    ay += py * mi;
+    az += pz * mi;
+    atomicAdd(&dpos[3 * (size_t)j], -px * mj);
+    atomicAdd(&dpos[3 * (size_t)j + 1], -py * mj);
+    atomicAdd(&dpos[3 * (size_t)j + 2], -pz * mj);
   }
   atomicAdd(&dpos[3 * (size_t)i], ax);
   atomicAdd(&dpos[3 * (size_t)i + 1], ay);
@@ -129,8 +136,8 @@ __global__ __launch_bounds__(kBlock) void k_csr_dx_atomic(const float4* __restri
 }
 
 void launch_csr_dx_atomic(const float4* xa, const int* off, const int* el, float* dpos, int n, const PhysConst& pc,
-                          hipStream_t st) {
-  k_csr_dx_atomic<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, off, el, dpos, n, pc);
+                          hipStream_t st, const float* minv) {
+  k_csr_dx_atomic<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, off, el, dpos, n, pc, minv);
 }
 
 // K_UpdatePosition (DensityPBD.cu:227-237); clears dpos for the next round

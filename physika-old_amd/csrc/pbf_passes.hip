@@ -298,6 +298,56 @@ void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, c
 }
 
 // ---------------------------------------------------------------------------
+// NeighborQuery::queryNeighborFixed (NeighborQuery.cu:514-581): rows of at most
+// `limit` neighbors, the nearest ones (the reference keeps them by replacing
+// the farthest kept neighbor; ties at the limit follow its visit order, here
+// the smaller id wins). Distances are |x_i - x_j| as the reference forms them
+// (glm's dot, then sqrtf). Rows sorted by (distance, id), in original ids:
+// counts[i], elems[limit*i + k].
+constexpr int kFixedBlock = 64;
+
+__global__ __launch_bounds__(kFixedBlock) void k_nbr_fixed(NbrView nv, const int* __restrict__ perm, int limit,
+                                                           int* __restrict__ counts, int* __restrict__ elems) {
+  __shared__ float sd[kFixedMax][kFixedBlock];
+  __shared__ int sid[kFixedMax][kFixedBlock];
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nv.n) return;
+  const int t = threadIdx.x;
+  const float4 xi = nv.xn[s];
+  int cnt = 0;
+  for_each_neighbor(
+      nv, s, [&](int j) { return nv.xn[j]; },
+      [&](int j, float4 xj) {
+#pragma clang fp contract(off)
+        const float dx = xi.x - xj.x, dy = xi.y - xj.y, dz = xi.z - xj.z;
+        const float d = sqrtf((dx * dx + dy * dy) + dz * dz);
+        const int id = perm[j];
+        int k;
+        if (cnt < limit) {
+          k = cnt++;
+        } else {
+          k = limit - 1;
+          if (!(d < sd[k][t] || (d == sd[k][t] && id < sid[k][t]))) return;
+        }
+        // insertion into the sorted row
+        while (k > 0 && (d < sd[k - 1][t] || (d == sd[k - 1][t] && id < sid[k - 1][t]))) {
+          sd[k][t] = sd[k - 1][t];
+          sid[k][t] = sid[k - 1][t];
+          --k;
+        }
+        sd[k][t] = d;
+        sid[k][t] = id;
+      });
+  const size_t i = (size_t)perm[s];
+  counts[i] = cnt;
+  for (int k = 0; k < cnt; ++k) elems[(size_t)limit * i + k] = sid[k][t];
+}
+
+void launch_nbr_fixed(const NbrView& nv, const int* perm, int limit, int* counts, int* elems, hipStream_t st) {
+  k_nbr_fixed<<<(nv.n + kFixedBlock - 1) / kFixedBlock, kFixedBlock, 0, st>>>(nv, perm, limit, counts, elems);
+}
+
+// ---------------------------------------------------------------------------
 // CSR export (NeighborList, Framework/Topology/NeighborList.h:27-66) in
 // original ids.
 

@@ -139,6 +139,18 @@ class Context:
                                                      _stream(self.device)), self._ctx)
         return NeighborList(off, el[: total.value])
 
+    def neighbor_query_fixed(self, x: torch.Tensor, limit: int) -> "NeighborList":
+        """pbf_neighbor_query_fixed: rows of at most `limit` nearest neighbors
+        (the reference's limited NeighborList layout)."""
+        x = _check_vec3(x, "position")
+        n = x.shape[0]
+        counts = torch.zeros(n, dtype=torch.int32, device=x.device)
+        elems = torch.full((n * limit,), -1, dtype=torch.int32, device=x.device)
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_neighbor_query_fixed(self._ctx, _ptr(x), n, int(limit), _ptr(counts),
+                                                           _ptr(elems), _stream(x.device)), self._ctx)
+        return NeighborList(counts, elems, limit=int(limit))
+
     def integrate(self, x, v, f=None) -> None:
         x = _check_vec3(x, "position")
         v = _check_vec3(v, "velocity")
@@ -153,6 +165,18 @@ class Context:
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_density(self._ctx, _ptr(x), _ptr(nbr.index), _ptr(nbr.elements), _ptr(rho),
                                               x.shape[0], _stream(x.device)), self._ctx)
+
+    def pbd_constrain_mass(self, x, v, nbr: "NeighborList", mass_inv: torch.Tensor,
+                           rho: Optional[torch.Tensor]) -> None:
+        """pbf_pbd_constrain_mass: DensityPBD::constrain with inverse masses."""
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity")
+        if mass_inv.dtype != torch.float32 or mass_inv.shape != (x.shape[0],) or mass_inv.device != x.device:
+            raise ValueError("mass_inv: expected an (n,) float32 tensor on the positions' device")
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_pbd_constrain_mass(self._ctx, _ptr(x), _ptr(v), _ptr(nbr.index),
+                                                         _ptr(nbr.elements), _ptr(mass_inv.contiguous()), _ptr(rho),
+                                                         x.shape[0], _stream(x.device)), self._ctx)
 
     def pbd_constrain(self, x, v, nbr: "NeighborList", rho: Optional[torch.Tensor]) -> None:
         x = _check_vec3(x, "position")
@@ -233,28 +257,42 @@ class Context:
 
 
 class NeighborList:
-    """CSR neighbor lists: ``index`` holds n+1 exclusive offsets (the
-    reference's m_index plus the total), ``elements`` the neighbor ids
-    (NeighborList.h:27-66)."""
+    """Neighbor lists (NeighborList.h:9-125). Dynamic (CSR): ``index`` holds
+    n+1 exclusive offsets (the reference's m_index plus the total) and
+    ``elements`` the neighbor ids. Limited (``limit`` > 0, isLimited()):
+    ``index`` holds the n row sizes and row i sits at elements[limit*i:]."""
 
-    def __init__(self, index: torch.Tensor, elements: torch.Tensor):
+    def __init__(self, index: torch.Tensor, elements: torch.Tensor, limit: int = 0):
         self.index = index
         self.elements = elements
+        self.limit = limit
+
+    def isLimited(self) -> bool:
+        return self.limit > 0
+
+    def getNeighborLimit(self) -> int:
+        return self.limit
 
     def size(self) -> int:
-        return self.index.numel() - 1
+        return self.index.numel() if self.limit else self.index.numel() - 1
 
     def getNeighborSize(self, i: int) -> int:
+        if self.limit:
+            return int(self.index[i])
         return int(self.index[i + 1] - self.index[i])
 
     def getElement(self, i: int, j: int) -> int:
+        if self.limit:
+            return int(self.elements[self.limit * i + j])
         return int(self.elements[int(self.index[i]) + j])
 
     def rows(self):
-        """Host copy as a list of sorted numpy rows."""
-        off = self.index.cpu().numpy()
+        """Host copy as a list of numpy arrays."""
+        idx = self.index.cpu().numpy()
         el = self.elements.cpu().numpy()
-        return [np.sort(el[off[i]:off[i + 1]]) for i in range(len(off) - 1)]
+        if self.limit:
+            return [el[self.limit * i:self.limit * i + idx[i]] for i in range(len(idx))]
+        return [el[idx[i]:idx[i + 1]] for i in range(len(idx) - 1)]
 
 
 class VarField:
@@ -313,8 +351,19 @@ class NeighborQuery:
         self.inPosition = ArrayField()
         self.outNeighborhood = ArrayField()
 
+        self.m_maxNum = 0
+
+    def setNeighborSizeLimit(self, num: int) -> None:
+        """NeighborQuery::setNeighborSizeLimit (NeighborQuery.h:34): rows keep
+        at most `num` nearest neighbors."""
+        self.m_maxNum = int(num)
+
     def compute(self) -> None:
-        self.outNeighborhood.setValue(self.ctx.neighbor_query(self.inPosition.getValue()))
+        x = self.inPosition.getValue()
+        if self.m_maxNum > 0:
+            self.outNeighborhood.setValue(self.ctx.neighbor_query_fixed(x, self.m_maxNum))
+        else:
+            self.outNeighborhood.setValue(self.ctx.neighbor_query(x))
 
 
 class ParticleIntegrator:
@@ -353,6 +402,7 @@ class DensityPBD:
         self.inVelocity = ArrayField()
         self.inNeighborIndex = ArrayField()
         self.outDensity = ArrayField()
+        self.m_massInv = ArrayField()  # optional inverse masses (DensityPBD.h:77)
 
     def constrain(self) -> bool:
         x = self.inPosition.getValue()
@@ -360,7 +410,11 @@ class DensityPBD:
         if rho is None or rho.shape[0] != x.shape[0]:
             rho = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
             self.outDensity.setValue(rho)
-        self.ctx.pbd_constrain(x, self.inVelocity.getValue(), self.inNeighborIndex.getValue(), rho)
+        if self.m_massInv.isEmpty():
+            self.ctx.pbd_constrain(x, self.inVelocity.getValue(), self.inNeighborIndex.getValue(), rho)
+        else:
+            self.ctx.pbd_constrain_mass(x, self.inVelocity.getValue(), self.inNeighborIndex.getValue(),
+                                        self.m_massInv.getValue(), rho)
         return True
 
 
