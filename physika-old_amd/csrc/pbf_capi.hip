@@ -75,14 +75,32 @@ struct pbf_ctx {
     size_t ckpt_cap = 0;    // floats
     float* seg = nullptr;   // per step of a segment: x (3n) then v (3n)
     size_t seg_cap = 0;
-    float4* tape = nullptr;  // K-1 iterates (iterate 0 is x0) then K aux arrays, cap_n each
-    size_t tape_cap = 0;     // float4
-    float4* adj = nullptr;   // q1, q2, xsb, vsb (cap_n each)
+    float4* adj = nullptr;   // q1, q2, c, vsb (cap_n each)
     int adj_n = 0;
     float* out = nullptr;    // x, v written by a taped step (discarded)
     int out_n = 0;
     int* flag = nullptr;
   } roll;
+
+  // Tape of one step of a segment: the arrays its neighbor search wrote
+  // (swapped in for the step), copies of its cell starts, windows and
+  // counters, and its PBD iterates and per-iteration sums.
+  struct TapeSlot {
+    int *perm = nullptr, *ssid = nullptr, *nbr_cnt = nullptr;
+    float4 *xn = nullptr, *x0 = nullptr;
+    long long *list_base = nullptr, *mask_base = nullptr;
+    int *list_jbase = nullptr, *list_mode = nullptr;
+    uint16_t* list = nullptr;
+    uint32_t* mask = nullptr;
+    int* cell_start = nullptr;
+    SceneWin* win = nullptr;
+    unsigned long long* counters = nullptr;
+    float4* it = nullptr;  // iterates 1..K-1, then the K aux arrays (cap_n each)
+  };
+  std::vector<TapeSlot> slots;
+  size_t slot_n = 0, slot_list = 0, slot_mask = 0;
+  long long slot_cells = 0;
+  int slot_scenes = 0, slot_K = -1;
 
   unsigned long long* counters = nullptr;     // device, C_COUNT
   unsigned long long* h_counters = nullptr;   // pinned host copy
@@ -671,13 +689,53 @@ template <class T> int ensure_buf(pbf_ctx* c, T*& p, size_t& cap, size_t need) {
   return PBF_OK;
 }
 
-// Tape and adjoint arrays for the current particle capacity and iterations.
-int ensure_tape(pbf_ctx* c) {
+void free_slot(pbf_ctx::TapeSlot& t) {
+  dfree(t.perm); dfree(t.ssid); dfree(t.nbr_cnt); dfree(t.xn); dfree(t.x0);
+  dfree(t.list_base); dfree(t.mask_base); dfree(t.list_jbase); dfree(t.list_mode);
+  dfree(t.list); dfree(t.mask); dfree(t.cell_start); dfree(t.win); dfree(t.counters); dfree(t.it);
+}
+
+void free_slots(pbf_ctx* c) {
+  for (auto& t : c->slots) free_slot(t);
+  c->slots.clear();
+}
+
+// m tape slots and the adjoint arrays for the current capacities and
+// iterations.
+int ensure_tape(pbf_ctx* c, int m) {
   auto& r = c->roll;
   const int K = c->p.iterations;
   const size_t cap = (size_t)c->cap_n;
-  int s = ensure_buf(c, r.tape, r.tape_cap, (size_t)std::max(2 * K - 1, 1) * cap);
-  if (s) return s;
+  if (c->slot_n != cap || c->slot_list != (size_t)c->list_cap || c->slot_mask != (size_t)c->mask_cap ||
+      c->slot_cells != c->cell_cap || c->slot_scenes != c->scene_cap || c->slot_K != K) {
+    free_slots(c);
+    c->slot_n = cap;
+    c->slot_list = (size_t)c->list_cap;
+    c->slot_mask = (size_t)c->mask_cap;
+    c->slot_cells = c->cell_cap;
+    c->slot_scenes = c->scene_cap;
+    c->slot_K = K;
+  }
+  const size_t nwv = cap / 64 + 4;
+  while ((int)c->slots.size() < m) {
+    c->slots.emplace_back();
+    auto& t = c->slots.back();
+    PBF_HIP(c, hipMalloc(&t.perm, sizeof(int) * cap));
+    PBF_HIP(c, hipMalloc(&t.ssid, sizeof(int) * cap));
+    PBF_HIP(c, hipMalloc(&t.nbr_cnt, sizeof(int) * cap));
+    PBF_HIP(c, hipMalloc(&t.xn, sizeof(float4) * (cap + kXnPad)));
+    PBF_HIP(c, hipMalloc(&t.x0, sizeof(float4) * cap));
+    PBF_HIP(c, hipMalloc(&t.list_base, sizeof(long long) * nwv));
+    PBF_HIP(c, hipMalloc(&t.mask_base, sizeof(long long) * nwv));
+    PBF_HIP(c, hipMalloc(&t.list_jbase, sizeof(int) * nwv));
+    PBF_HIP(c, hipMalloc(&t.list_mode, sizeof(int) * nwv));
+    PBF_HIP(c, hipMalloc(&t.list, sizeof(uint16_t) * std::max<size_t>(c->slot_list, 1)));
+    PBF_HIP(c, hipMalloc(&t.mask, sizeof(uint32_t) * std::max<size_t>(c->slot_mask, 1)));
+    PBF_HIP(c, hipMalloc(&t.cell_start, sizeof(int) * ((size_t)c->cell_cap + 2)));
+    PBF_HIP(c, hipMalloc(&t.win, sizeof(SceneWin) * (size_t)std::max(c->scene_cap, 1)));
+    PBF_HIP(c, hipMalloc(&t.counters, sizeof(unsigned long long) * C_COUNT));
+    PBF_HIP(c, hipMalloc(&t.it, sizeof(float4) * cap * (size_t)std::max(2 * K - 1, 1)));
+  }
   if (r.adj_n < c->cap_n) {
     dfree(r.adj);
     dfree(r.out);
@@ -688,6 +746,39 @@ int ensure_tape(pbf_ctx* c) {
   }
   if (!r.flag) PBF_HIP(c, hipMalloc(&r.flag, sizeof(int)));
   return PBF_OK;
+}
+
+// Exchanges the context's search outputs with a slot's arrays: swapped in
+// before a taped step, the step writes into the slot; swapped back after it.
+void swap_slot(pbf_ctx* c, pbf_ctx::TapeSlot& t) {
+  std::swap(c->perm, t.perm);
+  std::swap(c->ssid, t.ssid);
+  std::swap(c->nbr_cnt, t.nbr_cnt);
+  std::swap(c->xn, t.xn);
+  std::swap(c->x0, t.x0);
+  std::swap(c->list_base, t.list_base);
+  std::swap(c->mask_base, t.mask_base);
+  std::swap(c->list_jbase, t.list_jbase);
+  std::swap(c->list_mode, t.list_mode);
+  std::swap(c->list, t.list);
+  std::swap(c->mask, t.mask);
+}
+
+NbrView slot_view(const pbf_ctx* c, const pbf_ctx::TapeSlot& t, int n) {
+  NbrView nv = view_of(c, n);
+  nv.win = t.win;
+  nv.ssid = t.ssid;
+  nv.cell_start = t.cell_start;
+  nv.xn = t.xn;
+  nv.nbr_cnt = t.nbr_cnt;
+  nv.list_base = t.list_base;
+  nv.list_jbase = t.list_jbase;
+  nv.list_mode = t.list_mode;
+  nv.list = t.list;
+  nv.mask_base = t.mask_base;
+  nv.mask = t.mask;
+  nv.counters = t.counters;
+  return nv;
 }
 
 bool bad_n(int n) { return n < 0; }
@@ -794,7 +885,8 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->mask); dfree(c->list); dfree(c->scan_scratch); dfree(c->bf.d);
-  dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.tape); dfree(c->roll.adj);
+  free_slots(c);
+  dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.adj);
   dfree(c->roll.out); dfree(c->roll.flag);
   dfree(c->counters);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
@@ -960,9 +1052,8 @@ int pbf_rollout_forward(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const 
   if (s) return s;
   const size_t state = 6 * (size_t)n;
   const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
-  const int seg_steps = std::min(checkpoint_every, nsteps);
   if ((s = ensure_buf(c, r.ckpt, r.ckpt_cap, (size_t)nck * state))) return s;
-  if ((s = ensure_buf(c, r.seg, r.seg_cap, (size_t)seg_steps * state))) return s;
+  if ((s = ensure_buf(c, r.seg, r.seg_cap, 2 * state))) return s;
   if (nscenes + 1 > r.off_cap) {
     dfree(r.off);
     r.off_cap = 0;
@@ -1008,12 +1099,8 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
   int s = ensure_particles(c, n, r.nscenes);
   if (s) return s;
   if ((s = adapt_reservations(c))) return s;
-  if ((s = ensure_tape(c))) return s;
+  if ((s = ensure_tape(c, std::min(r.every, r.nsteps)))) return s;
   const size_t cap = (size_t)c->cap_n, state = 6 * (size_t)n;
-  std::vector<float4*> P(std::max(K, 1)), aux(std::max(K, 1));
-  P[0] = c->x0;
-  for (int k = 1; k < K; ++k) P[k] = r.tape + (size_t)(k - 1) * cap;
-  for (int k = 0; k < K; ++k) aux[k] = r.tape + (size_t)(K - 1 + k) * cap;
   AdjBufs ab;
   ab.e = c->xa;
   ab.e2 = c->xb;
@@ -1024,30 +1111,48 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
   ab.vsb = r.adj + 3 * cap;
   ab.flag = r.flag;
   PBF_HIP(c, hipMemsetAsync(r.flag, 0, sizeof(int), st));
-  float* xo = r.out;
-  float* vo = r.out + 3 * cap;
+  std::vector<float4*> P(std::max(K, 1)), aux(std::max(K, 1));
+  auto slot_arrays = [&](pbf_ctx::TapeSlot& t, float4* x0) {
+    P[0] = x0;
+    for (int k = 1; k < K; ++k) P[k] = t.it + (size_t)(k - 1) * cap;
+    for (int k = 0; k < K; ++k) aux[k] = t.it + (size_t)(K - 1 + k) * cap;
+  };
+  float* sa = r.seg;
+  float* sb = r.seg + state;
   const int nck = (r.nsteps + r.every - 1) / r.every;
   for (int ck = nck - 1; ck >= 0; --ck) {
     const int s0 = ck * r.every, m = std::min(s0 + r.every, r.nsteps) - s0;
-    // states of the segment from its checkpoint
-    PBF_HIP(c, hipMemcpyAsync(r.seg, r.ckpt + (size_t)ck * state, sizeof(float) * state, hipMemcpyDeviceToDevice,
-                              st));
-    for (int q = 1; q < m; ++q) {
-      const float* a = r.seg + (size_t)(q - 1) * state;
-      float* b = r.seg + (size_t)q * state;
-      s = step_once(c, a, a + 3 * (size_t)n, b, b + 3 * (size_t)n, nullptr, n, r.off, r.nscenes, nullptr, nullptr,
-                    st);
+    // the segment from its checkpoint, every step taped into its slot
+    PBF_HIP(c, hipMemcpyAsync(sa, r.ckpt + (size_t)ck * state, sizeof(float) * state, hipMemcpyDeviceToDevice, st));
+    for (int q = 0; q < m; ++q) {
+      const float* in = (q & 1) ? sb : sa;
+      float* xout = q + 1 == m ? r.out : ((q & 1) ? sa : sb);
+      float* vout = q + 1 == m ? r.out + 3 * cap : xout + 3 * (size_t)n;
+      auto& t = c->slots[q];
+      swap_slot(c, t);
+      slot_arrays(t, c->x0);
+      s = step_once(c, in, in + 3 * (size_t)n, xout, vout, nullptr, n, r.off, r.nscenes, P.data(), aux.data(), st);
+      if (!s) {
+        launch_copy_cells(c->cell_start, t.cell_start, c->total_cells, c->cell_cap, st);
+        if (hipMemcpyAsync(t.win, c->win, sizeof(SceneWin) * (size_t)r.nscenes, hipMemcpyDeviceToDevice, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(t.counters, c->counters, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToDevice,
+                           st) != hipSuccess)
+          s = fail(c, PBF_ERR_HIP, "tape copy failed");
+      }
+      swap_slot(c, t);
       if (s) return s;
     }
-    // reverse over the segment: re-run each step with its tape, then its adjoint
+    // reverse over the segment
     for (int q = m - 1; q >= 0; --q) {
-      const float* a = r.seg + (size_t)q * state;
-      s = step_once(c, a, a + 3 * (size_t)n, xo, vo, nullptr, n, r.off, r.nscenes, P.data(), aux.data(), st);
-      if (s) return s;
-      launch_adjoint_step(view_of(c, n), P.data(), aux.data(), K, c->pc, c->perm, d_xbar, d_vbar, ab, st);
+      auto& t = c->slots[q];
+      slot_arrays(t, t.x0);
+      launch_adjoint_step(slot_view(c, t, n), P.data(), aux.data(), K, c->pc, t.perm, d_xbar, d_vbar, ab, st);
       PBF_LAUNCHED(c);
     }
   }
+  // the context's own search arrays no longer describe one search
+  c->last_n = -1;
   s = record_counters(c, st);
   if (s) return s;
   int flag = 0;

@@ -135,6 +135,8 @@ void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const flo
                  float4* x0, float4* vs, const PhysConst* integrate, unsigned long long* counters,
                  hipStream_t st);
 void launch_set_offsets(int* off, int n, hipStream_t st);
+// cell_start[0..*total_cells] from src to dst (cap: allocated cells).
+void launch_copy_cells(const int* src, int* dst, const int* total_cells, long long cap, hipStream_t st);
 void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st);
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
                                 float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr);

@@ -456,6 +456,18 @@ void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const flo
                                    vs, pc, integrate ? 1 : 0);
 }
 
+// Copy of cell_start[0..total] (the device-side cell count of the last sort).
+__global__ void k_copy_cells(const int* __restrict__ src, int* __restrict__ dst, const int* __restrict__ total_cells) {
+  const int total = *total_cells + 1;
+  const int stride = gridDim.x * blockDim.x;
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += stride) dst[c] = src[c];
+}
+
+void launch_copy_cells(const int* src, int* dst, const int* total_cells, long long cap, hipStream_t st) {
+  const int blocks = (int)std::min<long long>((cap + 2 + kBlock - 1) / kBlock, 1024);
+  k_copy_cells<<<blocks, kBlock, 0, st>>>(src, dst, total_cells);
+}
+
 __global__ void k_set_offsets(int* off, int n) {
   if (threadIdx.x == 0) {
     off[0] = 0;
