@@ -32,8 +32,9 @@ FP32_PEAK_TFLOPS = 157.3
 # structure of the reference; neighbor data is on-chip):
 #   sort/bin 60 B, integrate 48 B, per iteration density+lambda 32 B and
 #   displacement 40 B, velocity update 48 B; grid arrays 12 B per cell.
-#   Flops per neighbor pair: 9 in the neighbor test, 33 in density+lambda,
-#   24 in displacement; 30 per particle.
+#   Flops: 9 per candidate pair in the neighbor test (every particle of the
+#   27 stencil cells), 33 per neighbor pair in density+lambda, 24 in
+#   displacement; 30 per particle.
 BYTES_PASS_A = 32
 BYTES_PASS_B = 40
 FLOPS_PAIR_A = 33
@@ -44,8 +45,8 @@ def algorithmic_bytes(n, ncell, iters):
     return (60 + 48 + 72 * iters + 48) * n + 12 * ncell
 
 
-def algorithmic_flops(n, pairs, iters):
-    return (9 + 57 * iters) * pairs + 30 * n
+def algorithmic_flops(n, pairs, cand, iters):
+    return 9 * cand + 57 * iters * pairs + 30 * n
 
 
 def progress(msg):
@@ -305,6 +306,7 @@ def main():
 
     stats = ctx.stats()
     pairs = stats["neighbor_pairs"]
+    cand = stats["candidates"]
     ncell = int(stats["cells_used"])  # compact cells the sort touches (occupied window + bucket)
     iters = params["iterations"]
 
@@ -321,7 +323,7 @@ def main():
     per_launch_bytes = {"density_lambda": BYTES_PASS_A * n, "displacement": BYTES_PASS_B * n,
                         "neighbors": 28 * n, "sort": 60 * n + 12 * ncell}[dom]
     per_launch_flops = {"density_lambda": FLOPS_PAIR_A * pairs, "displacement": FLOPS_PAIR_B * pairs,
-                        "neighbors": 9 * pairs, "sort": 0}[dom]
+                        "neighbors": 9 * cand, "sort": 0}[dom]
     achieved_tflops = per_launch_flops / t_dom / 1e12
     achieved_gbs = per_launch_bytes / t_dom / 1e9
     roofline = {
@@ -336,7 +338,7 @@ def main():
         "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2),
     }
     b_alg = algorithmic_bytes(n, ncell, iters)
-    f_alg = algorithmic_flops(n, pairs, iters)
+    f_alg = algorithmic_flops(n, pairs, cand, iters)
     step_s = ms_step / 1e3
     whole_step = {"bytes_alg": b_alg, "flops_alg": f_alg,
                   "hbm_fraction": round(b_alg / step_s / (HBM_PEAK_GBS * 1e9), 4),
@@ -348,7 +350,7 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "particles": n, "smoothing_length": params["smoothing_length"],
                    "spacing": params["sampling_distance"], "iterations": iters, "dt": params["dt"],
-                   "grid_cells": ncell, "neighbor_pairs": pairs},
+                   "grid_cells": ncell, "neighbor_pairs": pairs, "candidate_pairs": cand},
         "roofline": roofline,
         "passes_ms": {k: round(v[0], 4) for k, v in pass_ms.items()},
         "step_roofline": whole_step,

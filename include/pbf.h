@@ -105,6 +105,8 @@ typedef struct pbf_stats {
   long long cells_used;          /* compact cells of the last search: the scenes'
                                     occupied-cell windows plus one out-of-grid
                                     bucket per scene */
+  long long candidates;          /* candidate pairs the last search tested (every
+                                    particle of the 27 stencil cells) */
   int grid_dims[3];              /* cells per axis (GridHash::setSpace) */
   float grid_lo[3];              /* grid origin = scene_lo - 2 cells */
   float cell_size;

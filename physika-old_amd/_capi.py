@@ -61,6 +61,7 @@ class pbf_stats(C.Structure):
         ("list_slots_needed", C.c_longlong),
         ("list_slots_reserved", C.c_longlong),
         ("cells_used", C.c_longlong),
+        ("candidates", C.c_longlong),
         ("grid_dims", C.c_int * 3),
         ("grid_lo", C.c_float * 3),
         ("cell_size", C.c_float),

@@ -1256,6 +1256,7 @@ int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
   out->list_slots_needed = (long long)c->h_counters[C_LIST_SLOTS];
   out->list_slots_reserved = c->list_cap;
   out->cells_used = (long long)c->h_counters[C_CELLS];
+  out->candidates = (long long)c->h_counters[C_CANDIDATES];
   out->grid_dims[0] = c->g.nx;
   out->grid_dims[1] = c->g.ny;
   out->grid_dims[2] = c->g.nz;

@@ -56,7 +56,7 @@ struct VisConst {
   float inv_h, cw, b, eps;
 };
 
-enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_COUNT };
+enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_CANDIDATES, C_COUNT };
 
 // Cell-sorted view used by the neighbor pass and every pass over its lists.
 // Particles are processed in wavefronts of 64 consecutive sorted indices; a

@@ -32,7 +32,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
   const bool active = s < nv.n;
   float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
   int ra[9], rl[9];
-  int words = 0, jlo = INT_MAX, jhi = -1;
+  int words = 0, cand = 0, jlo = INT_MAX, jhi = -1;
   if (active) {
     xi = nv.xn[s];
     int ci, cj, ck;
@@ -42,6 +42,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
     for (int r = 0; r < 9; ++r) {
       run_range(nv.cell_start, win, ci, cj, ck, r, ra[r], rl[r]);
       words += (rl[r] + 31) >> 5;
+      cand += rl[r];
       if (rl[r] > 0) {
         jlo = min(jlo, ra[r]);
         jhi = max(jhi, ra[r] + rl[r] - 1);
@@ -166,8 +167,9 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
     }
     if (k & 3) flush(k);
   }
-  const int tot = wave_sum(cnt);
+  const int tot = wave_sum(cnt), tcand = wave_sum(cand);
   if (lane == 0 && tot) atomicAdd(&counters[C_PAIRS], (unsigned long long)tot);
+  if (lane == 0 && tcand) atomicAdd(&counters[C_CANDIDATES], (unsigned long long)tcand);
 }
 
 void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st) {
