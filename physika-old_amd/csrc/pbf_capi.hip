@@ -58,6 +58,7 @@ struct pbf_ctx {
 
   // viscosity constants and the sampled box field
   VisConst vc{};
+  float cull_margin = 0.f, cull_cut2 = 0.f;  // run culling of the neighbor stencil
   BoxField bf{};
   size_t bf_cap = 0;     // allocated samples
   bool bf_valid = false; // samples match the parameters
@@ -302,6 +303,19 @@ int derive(pbf_ctx* c, const pbf_params* p) {
   c->vc = vc;
   c->keep_normal = 1.0f - p->normal_friction;
   c->damp = std::pow(2.71828182845904523536f, -p->dt * p->tangent_friction);
+  // Run culling: a cell is left out when the distance from a particle to its
+  // box, shortened by a margin of 64 ulp of the grid's largest coordinate,
+  // is at least sqrt(cut2) with cut2 = thr2 (1 + 1e-5): face positions, the
+  // particle's cell and the squared norm all round well inside these slacks.
+  {
+    float ext = 0.0f;
+    for (int a = 0; a < 3; ++a) {
+      const int na = a == 0 ? g.nx : a == 1 ? g.ny : g.nz;
+      ext = std::max(ext, std::max(std::fabs(g.lo[a]), std::fabs(g.lo[a] + (float)na * ds)));
+    }
+    c->cull_margin = 64.0f * ext * 1.1920928955078125e-07f;
+    c->cull_cut2 = pc.thr2 * (1.0f + 1e-5f);
+  }
   c->g = g;
   c->full_cells = ncell;
   c->pc = pc;
@@ -455,6 +469,8 @@ NbrView view_of(const pbf_ctx* c, int n) {
   nv.oog_flag = c->oog_flag;
   nv.counters = c->counters;
   nv.thr2 = c->pc.thr2;
+  nv.cull_margin = c->cull_margin;
+  nv.cull_cut2 = c->cull_cut2;
   nv.n = n;
   return nv;
 }

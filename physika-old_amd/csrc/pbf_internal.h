@@ -78,6 +78,8 @@ struct NbrView {
   const uint8_t* oog_flag;     // per compact cell: an out-of-grid particle lists it
   const unsigned long long* counters;
   float thr2;
+  float cull_margin;  // run culling: face-distance slack
+  float cull_cut2;    // run culling: squared-distance cut (>= thr2)
   int n;
 };
 

@@ -16,13 +16,33 @@ namespace pbf {
 // dk = r/3-1, dj = r%3-1 (so runs come in ascending sorted index), clipped to
 // the scene's window (cells outside it hold no particle, and the window lies
 // inside the grid, so this is the reference's in-grid stencil).
-__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, int ci, int cj,
-                                          int ck, int r, int& a, int& len) {
-  const int lj = cj + (r % 3) - 1 - w.lo[1], lk = ck + (r / 3) - 1 - w.lo[2];
+//
+// Cells that cannot hold a neighbor of p are left out: the run is dropped when
+// its row lies at least h away in y and z, and its left or right cell when that
+// cell does. The bound is the distance from p to the cell's box (cell faces at
+// lo + c*ds), shortened by `margin` (covers the rounding of the faces and of p's
+// cell) and compared with cut2 >= the squared-distance threshold (covers the
+// rounding of the squared norm), so no neighbor is ever left out and the
+// neighbor lists are unchanged. A particle outside its clamped cell only makes
+// the bounds on its own side smaller (clamped at 0).
+__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, const GridDev& g,
+                                          float4 p, float margin, float cut2, int ci, int cj, int ck, int r, int& a,
+                                          int& len) {
+  const int dj = (r % 3) - 1, dk = (r / 3) - 1;
+  const int lj = cj + dj - w.lo[1], lk = ck + dk - w.lo[2];
   a = 0;
   len = 0;
   if (lj < 0 || lj >= w.dim[1] || lk < 0 || lk >= w.dim[2]) return;
-  const int i0 = max(ci - 1, w.lo[0]), i1 = min(ci + 1, w.lo[0] + w.dim[0] - 1);
+  float by = 0.0f, bz = 0.0f;
+  if (dj != 0) by = fmaxf((dj < 0 ? p.y - (g.lo[1] + (float)cj * g.ds) : (g.lo[1] + (float)(cj + 1) * g.ds) - p.y) - margin, 0.0f);
+  if (dk != 0) bz = fmaxf((dk < 0 ? p.z - (g.lo[2] + (float)ck * g.ds) : (g.lo[2] + (float)(ck + 1) * g.ds) - p.z) - margin, 0.0f);
+  const float base = by * by + bz * bz;
+  if (base >= cut2) return;
+  int i0 = max(ci - 1, w.lo[0]), i1 = min(ci + 1, w.lo[0] + w.dim[0] - 1);
+  const float bl = fmaxf(p.x - (g.lo[0] + (float)ci * g.ds) - margin, 0.0f);
+  const float br = fmaxf((g.lo[0] + (float)(ci + 1) * g.ds) - p.x - margin, 0.0f);
+  if (base + bl * bl >= cut2) i0 = max(i0, ci);
+  if (base + br * br >= cut2) i1 = min(i1, ci);
   if (i0 > i1) return;
   const int row = w.cell_base + w.dim[0] * (lj + w.dim[1] * lk) - w.lo[0];
   a = cell_start[row + i0];
@@ -159,7 +179,7 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
   int wofs = 0;
   for (int r = 0; r < 9; ++r) {
     int ra, rl;
-    run_range(nv.cell_start, win, ci, cj, ck, r, ra, rl);
+    run_range(nv.cell_start, win, nv.g, xni, nv.cull_margin, nv.cull_cut2, ci, cj, ck, r, ra, rl);
     const int nw = (rl + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
       const int jb = ra + (w << 5);

@@ -40,7 +40,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
     const SceneWin win = nv.win[nv.ssid[s]];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-      run_range(nv.cell_start, win, ci, cj, ck, r, ra[r], rl[r]);
+      run_range(nv.cell_start, win, nv.g, xi, nv.cull_margin, nv.cull_cut2, ci, cj, ck, r, ra[r], rl[r]);
       words += (rl[r] + 31) >> 5;
       cand += rl[r];
       if (rl[r] > 0) {
