@@ -396,8 +396,8 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   if (st) return st;
   st = ensure_scan_scratch(c, (long long)n + 1);
   if (st) return st;
-  if (n <= c->cap_n) return PBF_OK;
-  const int cap = std::max(((n + 255) / 256) * 256, 256);
+  if (n < c->cap_n) return PBF_OK;  // index n holds the padding particle
+  const int cap = std::max(((n + 1 + 255) / 256) * 256, 256);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
@@ -472,6 +472,7 @@ NbrView view_of(const pbf_ctx* c, int n) {
   nv.cull_margin = c->cull_margin;
   nv.cull_cut2 = c->cull_cut2;
   nv.n = n;
+  nv.pad = n;
   return nv;
 }
 

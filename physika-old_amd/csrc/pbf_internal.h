@@ -81,7 +81,13 @@ struct NbrView {
   float cull_margin;  // run culling: face-distance slack
   float cull_cut2;    // run culling: squared-distance cut (>= thr2)
   int n;
+  int pad;            // padding particle of the position buffers (index n)
 };
+
+// Position of the padding particle (index n of every sorted position buffer):
+// far outside every stencil, so a pass's terms for it are exact zeros, and
+// small enough that its squared distances stay finite.
+constexpr float kPadCoord = 1.0e9f;
 
 // Output arrays of the neighbor pass.
 struct NbrOut {

@@ -120,8 +120,11 @@ __device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int s) {
 // Counted loop over one lane's list, software-pipelined two groups deep:
 // while the bodies of group b run, the gathered data of group b+1 and the
 // indices of group b+2 are in flight (one wait per group). gather(j) loads
-// what the body needs of neighbor j.
-template <class GroupT, class LoadFn, class DecodeFn, class GatherFn, class Body>
+// what the body needs of neighbor j. The group load is issued every iteration
+// (the last group again at the end) so that the wait counts stay static.
+// kPadded: entries past the count were decoded to a padding particle whose
+// terms are exactly zero, and the body runs on them unguarded.
+template <bool kPadded, class GroupT, class LoadFn, class DecodeFn, class GatherFn, class Body>
 __device__ __forceinline__ void list_loop(int cnt, LoadFn&& load_group, DecodeFn&& decode, GatherFn&& gather,
                                           Body&& body) {
   using T = decltype(gather(0));
@@ -132,16 +135,20 @@ __device__ __forceinline__ void list_loop(int cnt, LoadFn&& load_group, DecodeFn
   T a[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) a[u] = gather(g.j[u]);
-  if (ng > 1) e = load_group(1);
+  e = load_group(min(1, ng - 1));
   for (int b = 0; b < ng; ++b) {
     const Group4 gn = decode(e, b + 1);
     T an[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) an[u] = gather(gn.j[u]);
-    if (b + 2 < ng) e = load_group(b + 2);
+    e = load_group(min(b + 2, ng - 1));
+    // keep the loads above in this iteration: without the fence the compiler
+    // sinks them to their uses in the next one and every group waits on its
+    // own gathers
+    asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if ((b << 2) + u < cnt) body(g.j[u], a[u]);
+      if (kPadded || (b << 2) + u < cnt) body(g.j[u], a[u]);
     g = gn;
 #pragma unroll
     for (int u = 0; u < 4; ++u) a[u] = an[u];
@@ -151,22 +158,27 @@ __device__ __forceinline__ void list_loop(int cnt, LoadFn&& load_group, DecodeFn
 // Calls body(j, gather(j)) for every neighbor j of sorted particle s, in
 // ascending j. The list is used when the wave has one; otherwise (reservation
 // exhausted) the words come from the masks, or are recomputed from xn.
-template <class GatherFn, class Body>
+// kPadded (list path only): the body also runs on up to three entries of the
+// padding particle nv.pad, whose position lies far outside every stencil; a
+// body whose terms vanish with the kernel's t = 0 (and are finite) adds exact
+// zeros for them.
+template <bool kPadded = false, class GatherFn, class Body>
 __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, GatherFn&& gather, Body&& body) {
   const int wave = s >> 6, lane = s & 63;
   const long long lb = nv.list_base[wave];
   if (lb >= 0) {
     const int cnt = nv.nbr_cnt[s];
+    const int past = kPadded ? nv.pad : s;
     if (nv.list_mode[wave] == 0) {
       const uint2* L = reinterpret_cast<const uint2*>(nv.list + lb) + lane;
       const int jb = nv.list_jbase[wave];
-      list_loop<uint2>(
+      list_loop<kPadded, uint2>(
           cnt, [&](int b) { return L[(long long)b << 6]; },
-          [&](uint2 e, int b) { return decode16(e, jb, b, cnt, s); }, gather, body);
+          [&](uint2 e, int b) { return decode16(e, jb, b, cnt, past); }, gather, body);
     } else {
       const int4* L = reinterpret_cast<const int4*>(nv.list + lb) + lane;
-      list_loop<int4>(
-          cnt, [&](int b) { return L[(long long)b << 6]; }, [&](int4 e, int b) { return decode32(e, b, cnt, s); },
+      list_loop<kPadded, int4>(
+          cnt, [&](int b) { return L[(long long)b << 6]; }, [&](int4 e, int b) { return decode32(e, b, cnt, past); },
           gather, body);
     }
     return;
@@ -196,10 +208,10 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
 }
 
 // for_each_neighbor with the neighbor's position (kW: with its w component).
-template <bool kW, class Body>
+template <bool kW, bool kPadded = false, class Body>
 __device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const float4* __restrict__ xs,
                                                     Body&& body) {
-  for_each_neighbor(nv, s, [&](int j) { return load_pos<kW>(xs, j); }, body);
+  for_each_neighbor<kPadded>(nv, s, [&](int j) { return load_pos<kW>(xs, j); }, body);
 }
 
 }  // namespace pbf

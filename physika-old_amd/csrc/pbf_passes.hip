@@ -191,7 +191,7 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* _
   if (s >= nv.n) return;
   const float4 xi = xs[s];
   float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
-  for_each_neighbor_x<false>(nv, s, xs, [&](int, float4 xj) {
+  for_each_neighbor_x<false, true>(nv, s, xs, [&](int, float4 xj) {
     const PairGeom p = pair_geom(xi, xj, pc.inv_h);
     const float t2 = p.t * p.t;
     rho += t2 * p.t;
@@ -230,6 +230,7 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
                                                          const int* __restrict__ perm, float* __restrict__ x_out,
                                                          float* __restrict__ v_out, float4* __restrict__ v_sorted) {
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  if (s == 0 && (!last || v_sorted)) xd[nv.pad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
   if (s >= nv.n) return;
   const float4 xi = xs[s];
   const float li = xi.w;
@@ -242,7 +243,7 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
     cz += c * p.dz;
   };
   float ax = 0.f, ay = 0.f, az = 0.f;
-  for_each_neighbor_x<true>(nv, s, xs, [&](int, float4 xj) { pair(xj, ax, ay, az); });
+  for_each_neighbor_x<true, true>(nv, s, xs, [&](int, float4 xj) { pair(xj, ax, ay, az); });
   float fa = 2.0f;
   float bx = 0.f, by = 0.f, bz = 0.f;
   if (nv.counters[C_OOG] != 0) {  // some scene has out-of-grid particles

@@ -404,6 +404,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
                                                     int integrate) {
 #pragma clang fp contract(off)
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) x0[n] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);  // the padding particle
   if (i >= n) return;
   const int k = key[i];
   const int a = cell_start[k], b = cell_start[k + 1];
