@@ -65,7 +65,10 @@ typedef struct pbf_params {
   int mask_words;          /* neighbor bit-mask scratch reservation, 32-bit words
                               per particle; 0 (default) = 32 to start with,
                               grown automatically between calls (pbf_step) */
-  int list_slots;          /* neighbor-list reservation, 16-bit slots per particle;
+  int list_slots;          /* neighbor-list reservation, 16-bit slots per particle
+                              (each 64-particle wavefront owns 64 x this; an
+                              entry takes 1 slot, or 2 when the wavefront's
+                              candidates span more than 65536 indices);
                               0 (default) = 96 to start with, grown automatically.
                               A value > 0 fixes the reservation: waves that do
                               not fit fall back to bit masks or re-testing,
@@ -100,7 +103,9 @@ typedef struct pbf_stats {
   long long mask_words_reserved; /* neighbor bit-mask words reserved */
   long long mask_fallback_waves; /* wavefronts that re-test candidates in every pass
                                     (bit-mask reservation full) */
-  long long list_slots_needed;   /* neighbor-list slots the last search wanted */
+  long long list_slots_needed;   /* neighbor-list slots the last search wanted: its
+                                    wavefronts x 64 x the slots per particle of the
+                                    fullest wavefront */
   long long list_slots_reserved; /* neighbor-list slots reserved */
   long long cells_used;          /* compact cells of the last search: the scenes'
                                     occupied-cell windows plus one out-of-grid

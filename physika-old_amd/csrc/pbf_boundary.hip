@@ -251,13 +251,14 @@ __global__ __launch_bounds__(kBlock) void k_viscosity(NbrView nv, const float4* 
   for_each_neighbor(
       nv, s, [&](int j) { return VisPair{xs[j], v_in[j]}; },
       [&](int, const VisPair& q) {
+#pragma clang fp contract(off)
         const PairGeom g = pair_geom(xi, q.x, vc.inv_h);
         if (g.r > vc.eps) {
           const float w = vc.cw * g.t;
           tw += w;
-          dx += w * q.v.x;
-          dy += w * q.v.y;
-          dz += w * q.v.z;
+          dx = fmaf(w, q.v.x, dx);
+          dy = fmaf(w, q.v.y, dy);
+          dz = fmaf(w, q.v.z, dz);
         }
       });
   const float b = tw < vc.eps ? 0.0f : vc.b;

@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -52,13 +53,15 @@ struct pbf_ctx {
   long long mask_cap = 0;  // words
   uint16_t* list = nullptr;
   long long list_cap = 0;  // 16-bit slots
+  long long auto_list_per = 96;  // automatic list reservation, 16-bit slots per particle
+  int4* wave_stat = nullptr;     // per wave of the neighbor pass: pairs, candidates, need
+  int stats_waves = 0;           // waves of the last search whose statistics are not yet reduced
 
   int* scan_scratch = nullptr;
   size_t scan_scratch_n = 0;
 
   // viscosity constants and the sampled box field
   VisConst vc{};
-  float cull_margin = 0.f, cull_cut2 = 0.f;  // run culling of the neighbor stencil
   BoxField bf{};
   size_t bf_cap = 0;     // allocated samples
   bool bf_valid = false; // samples match the parameters
@@ -303,19 +306,6 @@ int derive(pbf_ctx* c, const pbf_params* p) {
   c->vc = vc;
   c->keep_normal = 1.0f - p->normal_friction;
   c->damp = std::pow(2.71828182845904523536f, -p->dt * p->tangent_friction);
-  // Run culling: a cell is left out when the distance from a particle to its
-  // box, shortened by a margin of 64 ulp of the grid's largest coordinate,
-  // is at least sqrt(cut2) with cut2 = thr2 (1 + 1e-5): face positions, the
-  // particle's cell and the squared norm all round well inside these slacks.
-  {
-    float ext = 0.0f;
-    for (int a = 0; a < 3; ++a) {
-      const int na = a == 0 ? g.nx : a == 1 ? g.ny : g.nz;
-      ext = std::max(ext, std::max(std::fabs(g.lo[a]), std::fabs(g.lo[a] + (float)na * ds)));
-    }
-    c->cull_margin = 64.0f * ext * 1.1920928955078125e-07f;
-    c->cull_cut2 = pc.thr2 * (1.0f + 1e-5f);
-  }
   c->g = g;
   c->full_cells = ncell;
   c->pc = pc;
@@ -369,8 +359,14 @@ long long mask_words_for(const pbf_ctx* c, long long n) {
 }
 
 long long list_slots_for(const pbf_ctx* c, long long n) {
-  const long long per = c->p.list_slots > 0 ? c->p.list_slots : 96;
+  const long long per = c->p.list_slots > 0 ? c->p.list_slots : c->auto_list_per;
   return (n + 63) / 64 * 64 * per;
+}
+
+// 16-bit list slots per particle of every wave's fixed list region.
+int list_per_of(const pbf_ctx* c) {
+  const long long waves = ((long long)c->cap_n + 63) / 64;
+  return waves > 0 ? (int)std::min<long long>(c->list_cap / (64 * waves), INT_MAX) : 0;
 }
 
 int ensure_mask(pbf_ctx* c, long long words) {
@@ -394,6 +390,7 @@ int ensure_list(pbf_ctx* c, long long slots) {
 int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   int st = ensure_scenes(c, nscenes);
   if (st) return st;
+  if (n >= kMaxParticles) return fail(c, PBF_ERR_UNSUPPORTED, "more than 2^27 - 1 particles in one call");
   st = ensure_scan_scratch(c, (long long)n + 1);
   if (st) return st;
   if (n < c->cap_n) return PBF_OK;  // index n holds the padding particle
@@ -402,6 +399,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
+  dfree(c->wave_stat);
   c->cap_n = 0;
   const size_t ni = sizeof(int) * (size_t)cap, nf4 = sizeof(float4) * (size_t)cap;
   PBF_HIP(c, hipMalloc(&c->key, ni));
@@ -427,6 +425,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   PBF_HIP(c, hipMalloc(&c->list_base, sizeof(long long) * nwv));
   PBF_HIP(c, hipMalloc(&c->list_jbase, sizeof(int) * nwv));
   PBF_HIP(c, hipMalloc(&c->list_mode, sizeof(int) * nwv));
+  PBF_HIP(c, hipMalloc(&c->wave_stat, sizeof(int4) * nwv));
   c->cap_n = cap;
   st = ensure_mask(c, mask_words_for(c, cap));
   if (st) return st;
@@ -444,11 +443,14 @@ int adapt_reservations(pbf_ctx* c) {
   if (hipEventQuery(c->counters_ev) != hipSuccess) return PBF_OK;
   c->counters_pending = false;
   const long long mw = (long long)c->h_counters[C_MASK_WORDS];
-  const long long ls = (long long)c->h_counters[C_LIST_SLOTS];
+  const long long need = (long long)c->h_counters[C_LIST_NEED];  // slots per particle of the fullest wave
   int st = PBF_OK;
   // a reservation given explicitly (> 0 per particle) stays as given
   if (c->p.mask_words == 0 && mw > c->mask_cap) st = ensure_mask(c, mw + mw / 8);
-  if (!st && c->p.list_slots == 0 && ls > c->list_cap) st = ensure_list(c, ls + ls / 8);
+  if (!st && c->p.list_slots == 0 && need > list_per_of(c)) {
+    c->auto_list_per = std::max(c->auto_list_per, (need + need / 8 + 3) / 4 * 4);
+    st = ensure_list(c, list_slots_for(c, c->cap_n));
+  }
   return st;
 }
 
@@ -469,8 +471,6 @@ NbrView view_of(const pbf_ctx* c, int n) {
   nv.oog_flag = c->oog_flag;
   nv.counters = c->counters;
   nv.thr2 = c->pc.thr2;
-  nv.cull_margin = c->cull_margin;
-  nv.cull_cut2 = c->cull_cut2;
   nv.n = n;
   nv.pad = n;
   return nv;
@@ -481,13 +481,14 @@ NbrOut out_of(pbf_ctx* c) {
   o.mask = c->mask;
   o.mask_cap = c->mask_cap;
   o.list = c->list;
-  o.list_cap = c->list_cap;
+  o.list_per = list_per_of(c);
   o.mask_base = c->mask_base;
   o.list_base = c->list_base;
   o.list_jbase = c->list_jbase;
   o.list_mode = c->list_mode;
   o.nbr_cnt = c->nbr_cnt;
   o.oog_flag = c->oog_flag;
+  o.wave_stat = c->wave_stat;
   return o;
 }
 
@@ -579,12 +580,15 @@ int search(pbf_ctx* c, const float* x, const float* v, int n, const int* scene_o
     PassTimer t(c, PBF_PASS_NEIGHBORS, st);
     launch_neighbors(view_of(c, n), out_of(c), c->counters, st);
   }
+  c->stats_waves = (n + 63) / 64;
   PBF_LAUNCHED(c);
   c->last_n = n;
   return PBF_OK;
 }
 
 int record_counters(pbf_ctx* c, hipStream_t st) {
+  launch_wave_stats(c->wave_stat, c->stats_waves, c->counters, st);
+  c->stats_waves = 0;
   PBF_HIP(c, hipMemcpyAsync(c->h_counters, c->counters, sizeof(unsigned long long) * C_COUNT,
                             hipMemcpyDeviceToHost, st));
   PBF_HIP(c, hipEventRecord(c->counters_ev, st));
@@ -901,6 +905,7 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
+  dfree(c->wave_stat);
   dfree(c->mask); dfree(c->list); dfree(c->scan_scratch); dfree(c->bf.d);
   free_slots(c);
   dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.adj);

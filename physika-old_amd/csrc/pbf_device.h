@@ -86,15 +86,19 @@ struct PairGeom {
   float dx, dy, dz, r, rinv, t;
 };
 
+// Every multiply-add that is fused is written as fmaf and nothing else is
+// contracted, so each copy of a pair body (list, mask and re-test traversals)
+// rounds identically and all paths give bit-identical sums.
 __device__ __forceinline__ PairGeom pair_geom(float4 xi, float4 xj, float inv_h) {
+#pragma clang fp contract(off)
   PairGeom p;
   p.dx = xi.x - xj.x;
   p.dy = xi.y - xj.y;
   p.dz = xi.z - xj.z;
-  const float r2 = p.dx * p.dx + p.dy * p.dy + p.dz * p.dz;
+  const float r2 = fmaf(p.dz, p.dz, fmaf(p.dy, p.dy, p.dx * p.dx));
   p.rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
   p.r = r2 * p.rinv;
-  p.t = fmaxf(1.0f - p.r * inv_h, 0.0f);
+  p.t = fmaxf(fmaf(-p.r, inv_h, 1.0f), 0.0f);
   return p;
 }
 

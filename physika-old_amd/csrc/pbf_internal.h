@@ -56,7 +56,10 @@ struct VisConst {
   float inv_h, cw, b, eps;
 };
 
-enum Counter { C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_CANDIDATES, C_COUNT };
+enum Counter {
+  C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_CANDIDATES, C_LIST_NEED,
+  C_COUNT
+};
 
 // Cell-sorted view used by the neighbor pass and every pass over its lists.
 // Particles are processed in wavefronts of 64 consecutive sorted indices; a
@@ -71,15 +74,14 @@ struct NbrView {
   const int* nbr_cnt;          // neighbors per sorted particle
   const long long* list_base;  // per wave: first 16-bit slot of its list, -1 = no list
   const int* list_jbase;       // per wave: index base of 16-bit entries
-  const int* list_mode;        // per wave: 0 = 16-bit deltas, 1 = 32-bit indices
+  const int* list_mode;        // per wave: bit 0: 0 = 16-bit deltas, 1 = 32-bit indices;
+                               // bits 1..: groups of four that every lane of the wave fills
   const uint16_t* list;
   const long long* mask_base;  // per wave: first word of its global bit masks, -1 = none
   const uint32_t* mask;        // wave-interleaved bit masks over the cell runs
   const uint8_t* oog_flag;     // per compact cell: an out-of-grid particle lists it
   const unsigned long long* counters;
   float thr2;
-  float cull_margin;  // run culling: face-distance slack
-  float cull_cut2;    // run culling: squared-distance cut (>= thr2)
   int n;
   int pad;            // padding particle of the position buffers (index n)
 };
@@ -94,13 +96,14 @@ struct NbrOut {
   uint32_t* mask;
   long long mask_cap;   // words
   uint16_t* list;
-  long long list_cap;   // 16-bit slots
+  int list_per;         // 16-bit slots per particle: wave w owns [64 w list_per, 64 (w+1) list_per)
   long long* mask_base;
   long long* list_base;
   int* list_jbase;
   int* list_mode;
   int* nbr_cnt;
   uint8_t* oog_flag;
+  int4* wave_stat;      // per wave: neighbor pairs, candidates, list slots per lane it needs
 };
 
 // Sorting state of a step (device arrays).
@@ -126,6 +129,9 @@ struct SortBufs {
 };
 
 constexpr int kBlock = 256;
+// Particles per call: sorted arrays are indexed with 32-bit byte offsets
+// (16 bytes per particle, plus the padding entries).
+constexpr int kMaxParticles = 1 << 27;
 // Entries allocated past the end of the sorted position array xn, so that a
 // candidate group of 8 may read past a run's end.
 constexpr int kXnPad = 32;
@@ -146,6 +152,9 @@ void launch_set_offsets(int* off, int n, hipStream_t st);
 // cell_start[0..*total_cells] from src to dst (cap: allocated cells).
 void launch_copy_cells(const int* src, int* dst, const int* total_cells, long long cap, hipStream_t st);
 void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st);
+// Sums the per-wave statistics of the last neighbor pass (nwaves waves) into
+// counters[C_PAIRS], [C_CANDIDATES], [C_LIST_SLOTS] and [C_LIST_NEED].
+void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* counters, hipStream_t st);
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
                                 float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr);
 // last: write x/v to original order, or with v_sorted the positions to xd and

@@ -8,6 +8,8 @@
 // re-test of the candidates as fallbacks when the reservations are full.
 // Neighbor order is ascending sorted index on every path.
 #pragma once
+#include <type_traits>
+
 #include "pbf_device.h"
 
 namespace pbf {
@@ -16,33 +18,13 @@ namespace pbf {
 // dk = r/3-1, dj = r%3-1 (so runs come in ascending sorted index), clipped to
 // the scene's window (cells outside it hold no particle, and the window lies
 // inside the grid, so this is the reference's in-grid stencil).
-//
-// Cells that cannot hold a neighbor of p are left out: the run is dropped when
-// its row lies at least h away in y and z, and its left or right cell when that
-// cell does. The bound is the distance from p to the cell's box (cell faces at
-// lo + c*ds), shortened by `margin` (covers the rounding of the faces and of p's
-// cell) and compared with cut2 >= the squared-distance threshold (covers the
-// rounding of the squared norm), so no neighbor is ever left out and the
-// neighbor lists are unchanged. A particle outside its clamped cell only makes
-// the bounds on its own side smaller (clamped at 0).
-__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, const GridDev& g,
-                                          float4 p, float margin, float cut2, int ci, int cj, int ck, int r, int& a,
-                                          int& len) {
-  const int dj = (r % 3) - 1, dk = (r / 3) - 1;
-  const int lj = cj + dj - w.lo[1], lk = ck + dk - w.lo[2];
+__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, int ci, int cj,
+                                          int ck, int r, int& a, int& len) {
+  const int lj = cj + (r % 3) - 1 - w.lo[1], lk = ck + (r / 3) - 1 - w.lo[2];
   a = 0;
   len = 0;
   if (lj < 0 || lj >= w.dim[1] || lk < 0 || lk >= w.dim[2]) return;
-  float by = 0.0f, bz = 0.0f;
-  if (dj != 0) by = fmaxf((dj < 0 ? p.y - (g.lo[1] + (float)cj * g.ds) : (g.lo[1] + (float)(cj + 1) * g.ds) - p.y) - margin, 0.0f);
-  if (dk != 0) bz = fmaxf((dk < 0 ? p.z - (g.lo[2] + (float)ck * g.ds) : (g.lo[2] + (float)(ck + 1) * g.ds) - p.z) - margin, 0.0f);
-  const float base = by * by + bz * bz;
-  if (base >= cut2) return;
-  int i0 = max(ci - 1, w.lo[0]), i1 = min(ci + 1, w.lo[0] + w.dim[0] - 1);
-  const float bl = fmaxf(p.x - (g.lo[0] + (float)ci * g.ds) - margin, 0.0f);
-  const float br = fmaxf((g.lo[0] + (float)(ci + 1) * g.ds) - p.x - margin, 0.0f);
-  if (base + bl * bl >= cut2) i0 = max(i0, ci);
-  if (base + br * br >= cut2) i1 = min(i1, ci);
+  const int i0 = max(ci - 1, w.lo[0]), i1 = min(ci + 1, w.lo[0] + w.dim[0] - 1);
   if (i0 > i1) return;
   const int row = w.cell_base + w.dim[0] * (lj + w.dim[1] * lk) - w.lo[0];
   a = cell_start[row + i0];
@@ -58,31 +40,60 @@ __device__ __forceinline__ int window_cell(const SceneWin& w, int ci, int cj, in
   return w.cell_base + (ci - w.lo[0]) + w.dim[0] * ((cj - w.lo[1]) + w.dim[1] * (ck - w.lo[2]));
 }
 
-// Bits of the candidates [j0, j0+cnt) (1 <= cnt <= 32) within h of xi. Loads
-// go out eight at a time from one base address; the sorted position array
-// carries kXnPad far-away entries past its end, so a group may read past the
-// run (those bits are masked off) without index clamping.
-__device__ __forceinline__ uint32_t test_word(const float4* __restrict__ xn, float4 xi, int j0, int cnt,
-                                              float thr2) {
+// bits * 2 + (s < thr2): the squared distance of one candidate shifted into a
+// word (v_cmp into VCC, then an add with carry-in; NaN compares false as in the
+// reference's test).
+__device__ __forceinline__ uint32_t push_bit(uint32_t bits, float s, float thr2) {
+  uint32_t out;
+  asm("v_cmp_lt_f32_e64 vcc, %2, %3\n\tv_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
+      : "=v"(out)
+      : "v"(bits), "v"(s), "v"(thr2)
+      : "vcc");
+  return out;
+}
+
+// The squared norm of xi - c as the reference evaluates it (see within()).
+__device__ __forceinline__ float sq_dist(float4 a, float4 b) {
+#pragma clang fp contract(off)
+  const float dx = a.x - b.x, dy = a.y - b.y, dz = a.z - b.z;
+  return (dx * dx + dy * dy) + dz * dz;
+}
+
+// Word of the candidates p[0..cnt) (1 <= cnt <= 32), most significant bit
+// first: bit 31-k is set when candidate k lies within h of xi. Candidates are
+// read eight at a time from one base address (p may point into global memory
+// or LDS); the arrays are padded so that a group may read past the run's end,
+// and those bits are cleared.
+__device__ __forceinline__ uint32_t test_word(const float4* p, float4 xi, int cnt, float thr2) {
   uint32_t bits = 0;
-  const float4* p = xn + j0;
-  for (int k0 = 0; k0 < cnt; k0 += 8) {
+  int k0 = 0;
+  for (; k0 < cnt; k0 += 8) {
     float4 c[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) c[u] = p[k0 + u];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) bits |= (uint32_t)within(xi, c[u], thr2) << (k0 + u);
+    for (int u = 0; u < 8; ++u) bits = push_bit(bits, sq_dist(xi, c[u]), thr2);
   }
-  return cnt >= 32 ? bits : bits & ((1u << cnt) - 1u);
+  return (bits << (32 - k0)) & (~0u << (32 - cnt));
+}
+
+// Pops the first candidate (most significant set bit) of a non-zero word.
+__device__ __forceinline__ int pop_first(uint32_t& bits) {
+  const int c = __builtin_clz(bits);
+  bits ^= 0x80000000u >> c;
+  return c;
 }
 
 // Neighbor position load: all four components (pass B reads lambda from w),
 // or xyz only (a float4 load whose w is dead lets the register allocator
 // reuse that register while the load is in flight, forcing an early wait).
+// The byte offset is 32-bit (fewer than kMaxParticles entries), so the load
+// takes the scalar-base + vector-offset form.
 template <bool kW>
 __device__ __forceinline__ float4 load_pos(const float4* __restrict__ xs, int j) {
-  if (kW) return xs[j];
-  const float* q = reinterpret_cast<const float*>(xs + j);
+  const char* b = reinterpret_cast<const char*>(xs) + ((uint32_t)j << 4);
+  if (kW) return *reinterpret_cast<const float4*>(b);
+  const float* q = reinterpret_cast<const float*>(b);
   return make_float4(q[0], q[1], q[2], 0.0f);
 }
 
@@ -97,23 +108,34 @@ struct Group4 {
   int j[4];
 };
 
-// Indices of group b (entries 4b..4b+3); entries past cnt become s.
-__device__ __forceinline__ Group4 decode16(uint2 e, int jb, int b, int cnt, int s) {
+// Indices of group b (entries 4b..4b+3). kFull: the group is complete;
+// otherwise entries past cnt become `past`.
+template <bool kFull>
+__device__ __forceinline__ Group4 decode16(uint2 e, int jb, int b, int cnt, int past) {
   const int k = b << 2;
   Group4 g;
-  g.j[0] = k + 0 < cnt ? jb + (int)(e.x & 0xffffu) : s;
-  g.j[1] = k + 1 < cnt ? jb + (int)(e.x >> 16) : s;
-  g.j[2] = k + 2 < cnt ? jb + (int)(e.y & 0xffffu) : s;
-  g.j[3] = k + 3 < cnt ? jb + (int)(e.y >> 16) : s;
+  g.j[0] = jb + (int)(e.x & 0xffffu);
+  g.j[1] = jb + (int)(e.x >> 16);
+  g.j[2] = jb + (int)(e.y & 0xffffu);
+  g.j[3] = jb + (int)(e.y >> 16);
+  if (!kFull) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g.j[u] = k + u < cnt ? g.j[u] : past;
+  }
   return g;
 }
-__device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int s) {
+template <bool kFull>
+__device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int past) {
   const int k = b << 2;
   Group4 g;
-  g.j[0] = k + 0 < cnt ? e.x : s;
-  g.j[1] = k + 1 < cnt ? e.y : s;
-  g.j[2] = k + 2 < cnt ? e.z : s;
-  g.j[3] = k + 3 < cnt ? e.w : s;
+  g.j[0] = e.x;
+  g.j[1] = e.y;
+  g.j[2] = e.z;
+  g.j[3] = e.w;
+  if (!kFull) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g.j[u] = k + u < cnt ? g.j[u] : past;
+  }
   return g;
 }
 
@@ -122,36 +144,49 @@ __device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int s) {
 // indices of group b+2 are in flight (one wait per group). gather(j) loads
 // what the body needs of neighbor j. The group load is issued every iteration
 // (the last group again at the end) so that the wait counts stay static.
-// kPadded: entries past the count were decoded to a padding particle whose
-// terms are exactly zero, and the body runs on them unguarded.
+// Groups below nfull are complete on every lane of the wave and run without
+// per-entry checks (a loop with a uniform trip count); the rest check each
+// entry against the count. kPadded: entries past the count were decoded to a
+// padding particle whose terms are exactly zero, and the body runs on them
+// unguarded. decode(e, b, full) returns the indices of group b.
 template <bool kPadded, class GroupT, class LoadFn, class DecodeFn, class GatherFn, class Body>
-__device__ __forceinline__ void list_loop(int cnt, LoadFn&& load_group, DecodeFn&& decode, GatherFn&& gather,
-                                          Body&& body) {
+__device__ __forceinline__ void list_loop(int cnt, int nfull, LoadFn&& load_group, DecodeFn&& decode,
+                                          GatherFn&& gather, Body&& body) {
   using T = decltype(gather(0));
   const int ng = (cnt + 3) >> 2;
   if (ng == 0) return;
+  // two group buffers that trade roles every step (no register copies)
+  Group4 g0, g1;
+  T a0[4], a1[4];
   GroupT e = load_group(0);
-  Group4 g = decode(e, 0);
-  T a[4];
+  g0 = nfull > 0 ? decode(e, 0, std::true_type{}) : decode(e, 0, std::false_type{});
 #pragma unroll
-  for (int u = 0; u < 4; ++u) a[u] = gather(g.j[u]);
+  for (int u = 0; u < 4; ++u) a0[u] = gather(g0.j[u]);
   e = load_group(min(1, ng - 1));
-  for (int b = 0; b < ng; ++b) {
-    const Group4 gn = decode(e, b + 1);
-    T an[4];
+  // runs the bodies of group b (gc, ac) while group b+1 is gathered into
+  // (gx, ax) and the indices of group b+2 are loaded
+  auto step = [&](int b, auto full, const Group4& gc, const T(&ac)[4], Group4& gx, T(&ax)[4]) {
+    gx = decode(e, b + 1, full);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) an[u] = gather(gn.j[u]);
+    for (int u = 0; u < 4; ++u) ax[u] = gather(gx.j[u]);
     e = load_group(min(b + 2, ng - 1));
-    // keep the loads above in this iteration: without the fence the compiler
-    // sinks them to their uses in the next one and every group waits on its
-    // own gathers
+    // keep the loads above in this step: without the fence the compiler sinks
+    // them to their uses in the next one and every group waits on its own
+    // gathers
     asm volatile("" ::: "memory");
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (kPadded || (b << 2) + u < cnt) body(g.j[u], a[u]);
-    g = gn;
-#pragma unroll
-    for (int u = 0; u < 4; ++u) a[u] = an[u];
+      if (decltype(full)::value || kPadded || (b << 2) + u < cnt) body(gc.j[u], ac[u]);
+  };
+  int b = 0;
+  for (; b + 2 < nfull; b += 2) {
+    step(b, std::true_type{}, g0, a0, g1, a1);
+    step(b + 1, std::true_type{}, g1, a1, g0, a0);
+  }
+  for (; b < ng; b += 2) {
+    step(b, std::false_type{}, g0, a0, g1, a1);
+    if (b + 1 >= ng) break;
+    step(b + 1, std::false_type{}, g1, a1, g0, a0);
   }
 }
 
@@ -164,22 +199,26 @@ __device__ __forceinline__ void list_loop(int cnt, LoadFn&& load_group, DecodeFn
 // zeros for them.
 template <bool kPadded = false, class GatherFn, class Body>
 __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, GatherFn&& gather, Body&& body) {
-  const int wave = s >> 6, lane = s & 63;
+  // per-wave values, uniform over the wave's lanes (scalar registers)
+  const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
   const long long lb = nv.list_base[wave];
   if (lb >= 0) {
     const int cnt = nv.nbr_cnt[s];
     const int past = kPadded ? nv.pad : s;
-    if (nv.list_mode[wave] == 0) {
+    const int lm = nv.list_mode[wave];
+    const int nfull = lm >> 1;
+    if ((lm & 1) == 0) {
       const uint2* L = reinterpret_cast<const uint2*>(nv.list + lb) + lane;
       const int jb = nv.list_jbase[wave];
       list_loop<kPadded, uint2>(
-          cnt, [&](int b) { return L[(long long)b << 6]; },
-          [&](uint2 e, int b) { return decode16(e, jb, b, cnt, past); }, gather, body);
+          cnt, nfull, [&](int b) { return L[(long long)b << 6]; },
+          [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cnt, past); }, gather,
+          body);
     } else {
       const int4* L = reinterpret_cast<const int4*>(nv.list + lb) + lane;
       list_loop<kPadded, int4>(
-          cnt, [&](int b) { return L[(long long)b << 6]; }, [&](int4 e, int b) { return decode32(e, b, cnt, past); },
-          gather, body);
+          cnt, nfull, [&](int b) { return L[(long long)b << 6]; },
+          [&](int4 e, int b, auto full) { return decode32<decltype(full)::value>(e, b, cnt, past); }, gather, body);
     }
     return;
   }
@@ -191,15 +230,14 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
   int wofs = 0;
   for (int r = 0; r < 9; ++r) {
     int ra, rl;
-    run_range(nv.cell_start, win, nv.g, xni, nv.cull_margin, nv.cull_cut2, ci, cj, ck, r, ra, rl);
+    run_range(nv.cell_start, win, ci, cj, ck, r, ra, rl);
     const int nw = (rl + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
       const int jb = ra + (w << 5);
       uint32_t bits = mb >= 0 ? nv.mask[mb + ((long long)(wofs + w) << 6) + lane]
-                              : test_word(nv.xn, xni, jb, min(32, rl - (w << 5)), nv.thr2);
+                              : test_word(nv.xn + jb, xni, min(32, rl - (w << 5)), nv.thr2);
       while (bits) {
-        const int j = jb + __builtin_ctz(bits);
-        bits &= bits - 1u;
+        const int j = jb + pop_first(bits);
         body(j, gather(j));
       }
     }

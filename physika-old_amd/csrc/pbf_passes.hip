@@ -23,16 +23,99 @@ namespace pbf {
 // Mask words per lane a wave keeps in LDS during the neighbor pass; waves that
 // need more keep them in the global mask reservation.
 constexpr int kNbLdsWords = 24;
+// Candidates of one dk-plane (the union of the wave's three runs there) staged
+// in LDS; a plane whose union is larger is read from global memory.
+constexpr int kNbStage = 512;
 
-__global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
-                                                      unsigned long long* __restrict__ counters) {
-  __shared__ uint32_t lmask[kBlock / 64][kNbLdsWords * 64];
-  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  const int lane = threadIdx.x & 63, wave = s >> 6;
+// Appends neighbor j to a lane's group of four and stores complete groups:
+// 16-bit offsets from the wave base (mode 0: a 64-bit shift register, entry 0
+// in the low half of e0) or 32-bit indices (mode 1).
+template <int kMode>
+struct ListWriter {
+  uint16_t* list;
+  long long lb;
+  int lane, k = 0;
+  uint32_t e0 = 0u, e1 = 0u, e2 = 0u, e3 = 0u;
+
+  __device__ __forceinline__ void store(int group) {
+    const long long slot = list_slot(lb, group << 2, lane, kMode);
+    if (kMode == 0)
+      *reinterpret_cast<uint2*>(list + slot) = make_uint2(e0, e1);
+    else
+      *reinterpret_cast<uint4*>(list + slot) = make_uint4(e0, e1, e2, e3);
+  }
+  __device__ __forceinline__ void push(uint32_t d) {
+    if (kMode == 0) {
+      e0 = __builtin_amdgcn_alignbit(e1, e0, 16);
+      e1 = __builtin_amdgcn_alignbit(d, e1, 16);
+    } else {
+      e0 = e1;
+      e1 = e2;
+      e2 = e3;
+      e3 = d;
+    }
+    ++k;
+    if ((k & 3) == 0) store((k >> 2) - 1);
+  }
+  // the last, partial group: entries moved to its start (the slots past the
+  // count hold stale offsets of this wave, which no pass reads)
+  __device__ __forceinline__ void finish() {
+    const int m = k & 3;
+    if (m == 0) return;
+    if (kMode == 0) {
+      const int sh = 16 * (4 - m);
+      const unsigned long long v = (((unsigned long long)e1 << 32) | e0) >> sh;
+      e0 = (uint32_t)v;
+      e1 = (uint32_t)(v >> 32);
+    } else {
+      for (int t = m; t < 4; ++t) {
+        e0 = e1;
+        e1 = e2;
+        e2 = e3;
+      }
+    }
+    store(k >> 2);
+  }
+};
+
+// Phase 2: the words of this lane (written by this thread in phase 1) decoded
+// into its list in ascending index.
+template <int kMode>
+__device__ __forceinline__ void write_list(uint16_t* list, long long lb, int lane, const int (&ra)[9],
+                                           const int (&rl)[9], const uint32_t* lcol, const uint32_t* gcol,
+                                           bool in_lds, int jbase) {
+  ListWriter<kMode> wr{list, lb, lane};
+  int wofs = 0;
+#pragma unroll
+  for (int r = 0; r < 9; ++r) {
+    const int nw = (rl[r] + 31) >> 5;
+    for (int w = 0; w < nw; ++w) {
+      uint32_t bits = in_lds ? lcol[(wofs + w) << 6] : gcol[(long long)(wofs + w) << 6];
+      const int jb = ra[r] + (w << 5) - jbase;
+      while (bits) wr.push((uint32_t)(jb + pop_first(bits)));
+    }
+    wofs += nw;
+  }
+  wr.finish();
+}
+
+// One wave per block. Per dk-plane the three runs of all 64 lanes are merged
+// into one index range per run (for lanes in window cells, run starts and ends
+// are non-decreasing in sorted order, so the first and last lanes with a
+// non-empty run bound it; a wave holding an out-of-grid particle takes the
+// minimum and maximum over its lanes), staged in LDS with coalesced loads and
+// tested from there.
+__global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsigned long long* __restrict__ counters) {
+  __shared__ float4 stage[kNbStage + 8];
+  __shared__ uint32_t lmask[kNbLdsWords * 64];
+  const int wave = xcd_block(blockIdx.x, gridDim.x);
+  const int lane = threadIdx.x;
+  const int s = (wave << 6) + lane;
   const bool active = s < nv.n;
   float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
   int ra[9], rl[9];
   int words = 0, cand = 0, jlo = INT_MAX, jhi = -1;
+  bool oog = false;
   if (active) {
     xi = nv.xn[s];
     int ci, cj, ck;
@@ -40,7 +123,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
     const SceneWin win = nv.win[nv.ssid[s]];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-      run_range(nv.cell_start, win, nv.g, xi, nv.cull_margin, nv.cull_cut2, ci, cj, ck, r, ra[r], rl[r]);
+      run_range(nv.cell_start, win, ci, cj, ck, r, ra[r], rl[r]);
       words += (rl[r] + 31) >> 5;
       cand += rl[r];
       if (rl[r] > 0) {
@@ -51,7 +134,8 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
     // An out-of-grid particle is not hashed (GridHash.cu:85,194) yet queries,
     // so each in-grid j it lists receives -dp_kj from it (DensityPBD.cu:162)
     // without listing it back: flag the cells of its stencil for pass B.
-    if (!in_window(win, ci, cj, ck)) {  // the window holds every in-grid cell of the scene
+    oog = !in_window(win, ci, cj, ck);  // the window holds every in-grid cell of the scene
+    if (oog) {
       for (int dk = -1; dk <= 1; ++dk)
         for (int dj = -1; dj <= 1; ++dj)
           for (int di = -1; di <= 1; ++di)
@@ -64,6 +148,7 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
       rl[r] = 0;
     }
   }
+  const bool mono = __ballot(oog) == 0ull;
   const int wmax = wave_max(words);
   const int wjlo = wave_min(jlo), wjhi = wave_max(jhi);
   const int mode = (wjhi >= wjlo && wjhi - wjlo > 65535) ? 1 : 0;
@@ -79,101 +164,145 @@ __global__ __launch_bounds__(kBlock) void k_neighbors(NbrView nv, NbrOut out,
       atomicAdd(&counters[C_FALLBACK], 1ull);
   }
   mb = __shfl(mb, 0);
-  uint32_t* lcol = lmask[threadIdx.x >> 6] + lane;
+  uint32_t* lcol = lmask + lane;
   uint32_t* gcol = out.mask + (mb >= 0 ? mb + lane : 0);
   const bool keep = in_lds || mb >= 0;
 
   // phase 1: test every candidate, keep the words, count
   int cnt = 0, wofs = 0;
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int nw = (rl[r] + 31) >> 5;
-    for (int w = 0; w < nw; ++w) {
-      const uint32_t bits = test_word(nv.xn, xi, ra[r] + (w << 5), min(32, rl[r] - (w << 5)), nv.thr2);
-      if (in_lds)
-        lcol[(wofs + w) << 6] = bits;
-      else if (mb >= 0)
-        gcol[(long long)(wofs + w) << 6] = bits;
-      cnt += __popc(bits);
+  for (int p = 0; p < 3; ++p) {
+    int ulo[3], ubase[3], tot = 0;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const int r = 3 * p + q;
+      int lo = 0, hi = 0;
+      if (mono) {
+        const unsigned long long m = __ballot(rl[r] > 0);
+        if (m) {
+          lo = __builtin_amdgcn_readlane(ra[r], __builtin_ctzll(m));
+          hi = __builtin_amdgcn_readlane(ra[r] + rl[r], 63 - __builtin_clzll(m));
+        }
+      } else {
+        lo = wave_min(rl[r] > 0 ? ra[r] : INT_MAX);
+        hi = wave_max(rl[r] > 0 ? ra[r] + rl[r] : INT_MIN);
+        if (hi < lo) lo = hi = 0;
+      }
+      ulo[q] = lo;
+      ubase[q] = tot;
+      tot += hi - lo;
     }
-    wofs += nw;
+    if (tot <= kNbStage) {
+      __syncthreads();  // the previous plane's reads are done
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int len = (q < 2 ? ubase[q + 1] : tot) - ubase[q];
+        for (int k = lane; k < len; k += 64) stage[ubase[q] + k] = nv.xn[ulo[q] + k];
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int r = 3 * p + q;
+        const int nw = (rl[r] + 31) >> 5;
+        const float4* src = stage + ubase[q] + (ra[r] - ulo[q]);
+        for (int w = 0; w < nw; ++w) {
+          const uint32_t bits = test_word(src + (w << 5), xi, min(32, rl[r] - (w << 5)), nv.thr2);
+          if (in_lds)
+            lcol[(wofs + w) << 6] = bits;
+          else if (mb >= 0)
+            gcol[(long long)(wofs + w) << 6] = bits;
+          cnt += __popc(bits);
+        }
+        wofs += nw;
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < 3; ++q) {
+        const int r = 3 * p + q;
+        const int nw = (rl[r] + 31) >> 5;
+        for (int w = 0; w < nw; ++w) {
+          const uint32_t bits = test_word(nv.xn + ra[r] + (w << 5), xi, min(32, rl[r] - (w << 5)), nv.thr2);
+          if (in_lds)
+            lcol[(wofs + w) << 6] = bits;
+          else if (mb >= 0)
+            gcol[(long long)(wofs + w) << 6] = bits;
+          cnt += __popc(bits);
+        }
+        wofs += nw;
+      }
+    }
   }
   const int kmax = (wave_max(cnt) + 3) & ~3;
-  long long lb = -1;
+  const int nfull = wave_min(active ? cnt : INT_MAX) >> 2;  // groups every active lane fills
+  // The wave's list lives in its own fixed region (no allocation atomic: a
+  // same-address atomic per wave serializes in L2); a wave that needs more
+  // than the region takes the mask or re-test path and reports its need.
+  const int need = kmax << mode;  // 16-bit slots per lane
+  const long long lb = (keep && need <= out.list_per) ? (long long)wave * 64 * out.list_per : -1;
+  const int tot = wave_sum(cnt), tcand = wave_sum(cand);
   if (lane == 0) {
-    const unsigned long long slots = 64ull * (unsigned long long)kmax * (unsigned long long)(mode + 1);
-    const unsigned long long at = atomicAdd(&counters[C_LIST_SLOTS], slots);
-    if (keep && (long long)(at + slots) <= out.list_cap) lb = (long long)at;
     out.mask_base[wave] = mb;
     out.list_base[wave] = lb;
     out.list_jbase[wave] = wjlo;
-    out.list_mode[wave] = mode;
+    out.list_mode[wave] = mode | (nfull << 1);
+    out.wave_stat[wave] = make_int4(tot, tcand, need, 0);
   }
-  lb = __shfl(lb, 0);
   if (active) out.nbr_cnt[s] = cnt;
 
-  // phase 2: decode this lane's words (written by this thread above) into the
-  // list, four entries per store
+  // phase 2: decode this lane's words into the list, four entries per store
   if (active && lb >= 0) {
-    int k = 0;
-    // the group being filled: e0..e3 (32-bit indices) or e0 = entries 0|1,
-    // e1 = entries 2|3 (16-bit deltas); entries past the lane's count are
-    // never read
-    uint32_t e0 = 0u, e1 = 0u, e2 = 0u, e3 = 0u;
-    auto flush = [&](int kk) {  // the group of entry kk-1 is complete (or final)
-      const long long slot = list_slot(lb, (kk - 1) & ~3, lane, mode);
-      if (mode == 0)
-        *reinterpret_cast<uint2*>(out.list + slot) = make_uint2(e0, e1);
-      else
-        *reinterpret_cast<uint4*>(out.list + slot) = make_uint4(e0, e1, e2, e3);
-    };
-    wofs = 0;
-#pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      const int nw = (rl[r] + 31) >> 5;
-      for (int w = 0; w < nw; ++w) {
-        uint32_t bits = in_lds ? lcol[(wofs + w) << 6] : gcol[(long long)(wofs + w) << 6];
-        const int jb = ra[r] + (w << 5);
-        while (bits) {
-          const int j = jb + __builtin_ctz(bits);
-          bits &= bits - 1u;
-          const int q = k & 3;
-          if (mode == 0) {
-            const uint32_t d = (uint32_t)(j - wjlo);
-            if (q == 0)
-              e0 = d;
-            else if (q == 1)
-              e0 |= d << 16;
-            else if (q == 2)
-              e1 = d;
-            else
-              e1 |= d << 16;
-          } else {
-            const uint32_t u = (uint32_t)j;
-            if (q == 0)
-              e0 = u;
-            else if (q == 1)
-              e1 = u;
-            else if (q == 2)
-              e2 = u;
-            else
-              e3 = u;
-          }
-          ++k;
-          if ((k & 3) == 0) flush(k);
-        }
-      }
-      wofs += nw;
-    }
-    if (k & 3) flush(k);
+    if (mode == 0)
+      write_list<0>(out.list, lb, lane, ra, rl, lcol, gcol, in_lds, wjlo);
+    else
+      write_list<1>(out.list, lb, lane, ra, rl, lcol, gcol, in_lds, 0);
   }
-  const int tot = wave_sum(cnt), tcand = wave_sum(cand);
-  if (lane == 0 && tot) atomicAdd(&counters[C_PAIRS], (unsigned long long)tot);
-  if (lane == 0 && tcand) atomicAdd(&counters[C_CANDIDATES], (unsigned long long)tcand);
 }
 
 void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st) {
-  k_neighbors<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, out, counters);
+  k_neighbors<<<(nv.n + 63) / 64, 64, 0, st>>>(nv, out, counters);
+}
+
+// One block: sums and maxima over the per-wave statistics.
+__global__ __launch_bounds__(1024) void k_wave_stats(const int4* __restrict__ ws, int nwaves,
+                                                     unsigned long long* __restrict__ counters) {
+  __shared__ unsigned long long sp[16], sc[16];
+  __shared__ int sn[16];
+  unsigned long long pairs = 0, cand = 0;
+  int need = 0;
+  for (int w = threadIdx.x; w < nwaves; w += blockDim.x) {
+    const int4 v = ws[w];
+    pairs += (unsigned long long)v.x;
+    cand += (unsigned long long)v.y;
+    need = max(need, v.z);
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    pairs += __shfl_xor(pairs, d);
+    cand += __shfl_xor(cand, d);
+    need = max(need, __shfl_xor(need, d));
+  }
+  const int wv = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    sp[wv] = pairs;
+    sc[wv] = cand;
+    sn[wv] = need;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < (int)(blockDim.x >> 6); ++k) {
+      pairs += sp[k];
+      cand += sc[k];
+      need = max(need, sn[k]);
+    }
+    counters[C_PAIRS] = pairs;
+    counters[C_CANDIDATES] = cand;
+    counters[C_LIST_NEED] = (unsigned long long)need;
+    counters[C_LIST_SLOTS] = 64ull * (unsigned long long)nwaves * (unsigned long long)need;
+  }
+}
+
+void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* counters, hipStream_t st) {
+  if (nwaves > 0) k_wave_stats<<<1, 1024, 0, st>>>(wave_stat, nwaves, counters);
 }
 
 // ---------------------------------------------------------------------------
@@ -192,15 +321,16 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* _
   const float4 xi = xs[s];
   float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
   for_each_neighbor_x<false, true>(nv, s, xs, [&](int, float4 xj) {
+#pragma clang fp contract(off)
     const PairGeom p = pair_geom(xi, xj, pc.inv_h);
     const float t2 = p.t * p.t;
-    rho += t2 * p.t;
+    rho = fmaf(t2, p.t, rho);
     const float gq = p.r > pc.pair_eps ? t2 * p.rinv : 0.0f;
-    const float gx = gq * p.dx, gy = gq * p.dy, gz = gq * p.dz;
-    sgx += gx;
-    sgy += gy;
-    sgz += gz;
-    sgg += gx * gx + gy * gy + gz * gz;
+    sgx = fmaf(gq, p.dx, sgx);
+    sgy = fmaf(gq, p.dy, sgy);
+    sgz = fmaf(gq, p.dz, sgz);
+    const float gr = gq * p.r;  // |g| / |cG| = t^2
+    sgg = fmaf(gr, gr, sgg);
   });
   rho *= pc.cW;
   const float S = (pc.cG * pc.cG) * (sgg + (sgx * sgx + sgy * sgy + sgz * sgz));
@@ -236,11 +366,12 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
   const float li = xi.w;
   // sum over the listed j of (lambda_i + lambda_j) t^2 / r (x_i - x_j)
   auto pair = [&](float4 xj, float& cx, float& cy, float& cz) {
+#pragma clang fp contract(off)
     const PairGeom p = pair_geom(xi, xj, pc.inv_h);
     const float c = p.r > pc.pair_eps ? (li + xj.w) * (p.t * p.t) * p.rinv : 0.0f;
-    cx += c * p.dx;
-    cy += c * p.dy;
-    cz += c * p.dz;
+    cx = fmaf(c, p.dx, cx);
+    cy = fmaf(c, p.dy, cy);
+    cz = fmaf(c, p.dz, cz);
   };
   float ax = 0.f, ay = 0.f, az = 0.f;
   for_each_neighbor_x<true, true>(nv, s, xs, [&](int, float4 xj) { pair(xj, ax, ay, az); });
