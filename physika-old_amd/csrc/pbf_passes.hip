@@ -23,9 +23,6 @@ namespace pbf {
 // Mask words per lane a wave keeps in LDS during the neighbor pass; waves that
 // need more keep them in the global mask reservation.
 constexpr int kNbLdsWords = 24;
-// Candidates of one dk-plane (the union of the wave's three runs there) staged
-// in LDS; a plane whose union is larger is read from global memory.
-constexpr int kNbStage = 512;
 
 // Appends neighbor j to a lane's group of four and stores complete groups:
 // 16-bit offsets from the wave base (mode 0: a 64-bit shift register, entry 0
@@ -78,19 +75,38 @@ struct ListWriter {
   }
 };
 
-// Phase 2: the words of this lane (written by this thread in phase 1) decoded
-// into its list in ascending index.
+// Phase 2 over LDS words: each word sits with the list value of its first
+// candidate ({bits, base}); one flat loop over the lane's cnt entries pops the
+// next set bit and moves to the next word when one runs empty, so the wave
+// runs as many iterations as its largest count.
 template <int kMode>
-__device__ __forceinline__ void write_list(uint16_t* list, long long lb, int lane, const int (&ra)[9],
-                                           const int (&rl)[9], const uint32_t* lcol, const uint32_t* gcol,
-                                           bool in_lds, int jbase) {
+__device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int lane, const uint2* lw, int cnt) {
+  ListWriter<kMode> wr{list, lb, lane};
+  uint32_t bits = 0u;
+  int jb = 0, wi = 0;
+  for (int k = 0; k < cnt; ++k) {
+    while (bits == 0u) {  // the remaining words hold cnt - k > 0 set bits
+      const uint2 e = lw[wi << 6];
+      bits = e.x;
+      jb = (int)e.y;
+      ++wi;
+    }
+    wr.push((uint32_t)(jb + pop_first(bits)));
+  }
+  wr.finish();
+}
+
+// Phase 2 over words in the global mask reservation, run by run.
+template <int kMode>
+__device__ __forceinline__ void write_list_global(uint16_t* list, long long lb, int lane, const int (&ra)[9],
+                                                  const int (&rl)[9], const uint32_t* gcol, int jbase) {
   ListWriter<kMode> wr{list, lb, lane};
   int wofs = 0;
 #pragma unroll
   for (int r = 0; r < 9; ++r) {
     const int nw = (rl[r] + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
-      uint32_t bits = in_lds ? lcol[(wofs + w) << 6] : gcol[(long long)(wofs + w) << 6];
+      uint32_t bits = gcol[(long long)(wofs + w) << 6];
       const int jb = ra[r] + (w << 5) - jbase;
       while (bits) wr.push((uint32_t)(jb + pop_first(bits)));
     }
@@ -99,15 +115,11 @@ __device__ __forceinline__ void write_list(uint16_t* list, long long lb, int lan
   wr.finish();
 }
 
-// One wave per block. Per dk-plane the three runs of all 64 lanes are merged
-// into one index range per run (for lanes in window cells, run starts and ends
-// are non-decreasing in sorted order, so the first and last lanes with a
-// non-empty run bound it; a wave holding an out-of-grid particle takes the
-// minimum and maximum over its lanes), staged in LDS with coalesced loads and
-// tested from there.
+// One wave per block: phase 1 tests the candidates of the 9 runs (a lane's
+// words in LDS, or in the global mask reservation when they do not fit),
+// phase 2 writes the list into the wave's fixed region.
 __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsigned long long* __restrict__ counters) {
-  __shared__ float4 stage[kNbStage + 8];
-  __shared__ uint32_t lmask[kNbLdsWords * 64];
+  __shared__ uint2 lword[kNbLdsWords * 64];
   const int wave = xcd_block(blockIdx.x, gridDim.x);
   const int lane = threadIdx.x;
   const int s = (wave << 6) + lane;
@@ -115,7 +127,6 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
   int ra[9], rl[9];
   int words = 0, cand = 0, jlo = INT_MAX, jhi = -1;
-  bool oog = false;
   if (active) {
     xi = nv.xn[s];
     int ci, cj, ck;
@@ -134,8 +145,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     // An out-of-grid particle is not hashed (GridHash.cu:85,194) yet queries,
     // so each in-grid j it lists receives -dp_kj from it (DensityPBD.cu:162)
     // without listing it back: flag the cells of its stencil for pass B.
-    oog = !in_window(win, ci, cj, ck);  // the window holds every in-grid cell of the scene
-    if (oog) {
+    if (!in_window(win, ci, cj, ck)) {  // the window holds every in-grid cell of the scene
       for (int dk = -1; dk <= 1; ++dk)
         for (int dj = -1; dj <= 1; ++dj)
           for (int di = -1; di <= 1; ++di)
@@ -148,10 +158,10 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
       rl[r] = 0;
     }
   }
-  const bool mono = __ballot(oog) == 0ull;
   const int wmax = wave_max(words);
   const int wjlo = wave_min(jlo), wjhi = wave_max(jhi);
   const int mode = (wjhi >= wjlo && wjhi - wjlo > 65535) ? 1 : 0;
+  const int jbase = mode == 0 ? wjlo : 0;  // list values are j - jbase
   // word storage: this wave's LDS block, else the global mask reservation
   const bool in_lds = wmax <= kNbLdsWords;
   long long mb = -1;
@@ -164,73 +174,25 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
       atomicAdd(&counters[C_FALLBACK], 1ull);
   }
   mb = __shfl(mb, 0);
-  uint32_t* lcol = lmask + lane;
+  uint2* lcol = lword + lane;
   uint32_t* gcol = out.mask + (mb >= 0 ? mb + lane : 0);
   const bool keep = in_lds || mb >= 0;
 
   // phase 1: test every candidate, keep the words, count
   int cnt = 0, wofs = 0;
 #pragma unroll
-  for (int p = 0; p < 3; ++p) {
-    int ulo[3], ubase[3], tot = 0;
-#pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const int r = 3 * p + q;
-      int lo = 0, hi = 0;
-      if (mono) {
-        const unsigned long long m = __ballot(rl[r] > 0);
-        if (m) {
-          lo = __builtin_amdgcn_readlane(ra[r], __builtin_ctzll(m));
-          hi = __builtin_amdgcn_readlane(ra[r] + rl[r], 63 - __builtin_clzll(m));
-        }
-      } else {
-        lo = wave_min(rl[r] > 0 ? ra[r] : INT_MAX);
-        hi = wave_max(rl[r] > 0 ? ra[r] + rl[r] : INT_MIN);
-        if (hi < lo) lo = hi = 0;
-      }
-      ulo[q] = lo;
-      ubase[q] = tot;
-      tot += hi - lo;
+  for (int r = 0; r < 9; ++r) {
+    const int nw = (rl[r] + 31) >> 5;
+    for (int w = 0; w < nw; ++w) {
+      const int jb = ra[r] + (w << 5);
+      const uint32_t bits = test_word(nv.xn + jb, xi, min(32, rl[r] - (w << 5)), nv.thr2);
+      if (in_lds)
+        lcol[(wofs + w) << 6] = make_uint2(bits, (uint32_t)(jb - jbase));
+      else if (mb >= 0)
+        gcol[(long long)(wofs + w) << 6] = bits;
+      cnt += __popc(bits);
     }
-    if (tot <= kNbStage) {
-      __syncthreads();  // the previous plane's reads are done
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int len = (q < 2 ? ubase[q + 1] : tot) - ubase[q];
-        for (int k = lane; k < len; k += 64) stage[ubase[q] + k] = nv.xn[ulo[q] + k];
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int r = 3 * p + q;
-        const int nw = (rl[r] + 31) >> 5;
-        const float4* src = stage + ubase[q] + (ra[r] - ulo[q]);
-        for (int w = 0; w < nw; ++w) {
-          const uint32_t bits = test_word(src + (w << 5), xi, min(32, rl[r] - (w << 5)), nv.thr2);
-          if (in_lds)
-            lcol[(wofs + w) << 6] = bits;
-          else if (mb >= 0)
-            gcol[(long long)(wofs + w) << 6] = bits;
-          cnt += __popc(bits);
-        }
-        wofs += nw;
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < 3; ++q) {
-        const int r = 3 * p + q;
-        const int nw = (rl[r] + 31) >> 5;
-        for (int w = 0; w < nw; ++w) {
-          const uint32_t bits = test_word(nv.xn + ra[r] + (w << 5), xi, min(32, rl[r] - (w << 5)), nv.thr2);
-          if (in_lds)
-            lcol[(wofs + w) << 6] = bits;
-          else if (mb >= 0)
-            gcol[(long long)(wofs + w) << 6] = bits;
-          cnt += __popc(bits);
-        }
-        wofs += nw;
-      }
-    }
+    wofs += nw;
   }
   const int kmax = (wave_max(cnt) + 3) & ~3;
   const int nfull = wave_min(active ? cnt : INT_MAX) >> 2;  // groups every active lane fills
@@ -251,10 +213,17 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
 
   // phase 2: decode this lane's words into the list, four entries per store
   if (active && lb >= 0) {
-    if (mode == 0)
-      write_list<0>(out.list, lb, lane, ra, rl, lcol, gcol, in_lds, wjlo);
-    else
-      write_list<1>(out.list, lb, lane, ra, rl, lcol, gcol, in_lds, 0);
+    if (in_lds) {
+      if (mode == 0)
+        write_list_lds<0>(out.list, lb, lane, lcol, cnt);
+      else
+        write_list_lds<1>(out.list, lb, lane, lcol, cnt);
+    } else {
+      if (mode == 0)
+        write_list_global<0>(out.list, lb, lane, ra, rl, gcol, jbase);
+      else
+        write_list_global<1>(out.list, lb, lane, ra, rl, gcol, jbase);
+    }
   }
 }
 
