@@ -28,21 +28,37 @@ METRIC = "particle-steps/s at 1M PBD fluid; batched vs serial; bwd/fwd time rati
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3
 
-# Algorithmic traffic per particle and flops per pair (fp32, the pass
-# structure of the reference; neighbor data is on-chip):
+# Algorithmic traffic per particle and per neighbor pair, and flops per pair
+# (fp32, the pass structure of the reference):
 #   sort/bin 60 B, integrate 48 B, per iteration density+lambda 32 B and
-#   displacement 40 B, velocity update 48 B; grid arrays 12 B per cell.
+#   displacement 40 B, velocity update 48 B per particle; grid arrays 12 B per
+#   cell; each pass reads the neighbor list, 2 B per pair (16-bit entries; the
+#   reference's int32 list is 4 B), and the neighbor pass writes it.
 #   Flops: 9 per candidate pair in the neighbor test (every particle of the
 #   27 stencil cells), 33 per neighbor pair in density+lambda, 24 in
 #   displacement; 30 per particle.
 BYTES_PASS_A = 32
 BYTES_PASS_B = 40
+BYTES_LIST = 2
 FLOPS_PAIR_A = 33
 FLOPS_PAIR_B = 24
+TRAFFIC_PROFILE = ROOT / "profiles" / "pbf_step_lattice_1m_hbm_traffic.md"
 
 
-def algorithmic_bytes(n, ncell, iters):
-    return (60 + 48 + 72 * iters + 48) * n + 12 * ncell
+def algorithmic_bytes(n, ncell, pairs, iters):
+    return (60 + 48 + 72 * iters + 48) * n + 12 * ncell + BYTES_LIST * pairs * (2 * iters + 1)
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+    try:
+        for line in TRAFFIC_PROFILE.read_text().splitlines():
+            if line.startswith("traffic_json: "):
+                return json.loads(line[len("traffic_json: "):]).get(kernel, {}).get("bytes")
+    except OSError:
+        return None
+    return None
 
 
 def algorithmic_flops(n, pairs, cand, iters):
@@ -320,24 +336,28 @@ def main():
     pass_ms = {k: (ms / max(cnt, 1), cnt) for k, (ms, cnt) in prof.items()}
     dom = max(("density_lambda", "displacement", "neighbors", "sort"), key=lambda k: pass_ms[k][0] * pass_ms[k][1])
     t_dom = pass_ms[dom][0] / 1e3
-    per_launch_bytes = {"density_lambda": BYTES_PASS_A * n, "displacement": BYTES_PASS_B * n,
-                        "neighbors": 28 * n, "sort": 60 * n + 12 * ncell}[dom]
+    per_launch_bytes = {"density_lambda": BYTES_PASS_A * n + BYTES_LIST * pairs,
+                        "displacement": BYTES_PASS_B * n + BYTES_LIST * pairs,
+                        "neighbors": 28 * n + BYTES_LIST * pairs, "sort": 60 * n + 12 * ncell}[dom]
     per_launch_flops = {"density_lambda": FLOPS_PAIR_A * pairs, "displacement": FLOPS_PAIR_B * pairs,
                         "neighbors": 9 * cand, "sort": 0}[dom]
+    kname = {"density_lambda": "k_density_lambda", "displacement": "k_displacement",
+             "neighbors": "k_neighbors", "sort": "sort passes"}[dom]
     achieved_tflops = per_launch_flops / t_dom / 1e12
     achieved_gbs = per_launch_bytes / t_dom / 1e9
     roofline = {
-        "bound": "valu",
-        "kernel": {"density_lambda": "k_density_lambda", "displacement": "k_displacement",
-                   "neighbors": "k_neighbors", "sort": "sort passes"}[dom],
-        "achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-        "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
-        "traffic": None,
-        "hbm": {"achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved_gbs / HBM_PEAK_GBS, 4)},
+        "bound": "hbm",
+        "kernel": kname,
+        "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
+        "traffic": profiled_traffic(kname),
+        "traffic_source": TRAFFIC_PROFILE.name,
+        "algorithmic_bytes": per_launch_bytes,
+        "valu": {"achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                 "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)},
         "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2),
     }
-    b_alg = algorithmic_bytes(n, ncell, iters)
+    b_alg = algorithmic_bytes(n, ncell, pairs, iters)
     f_alg = algorithmic_flops(n, pairs, cand, iters)
     step_s = ms_step / 1e3
     whole_step = {"bytes_alg": b_alg, "flops_alg": f_alg,
