@@ -97,11 +97,16 @@ __device__ __forceinline__ float4 load_pos(const float4* __restrict__ xs, int j)
   return make_float4(q[0], q[1], q[2], 0.0f);
 }
 
-// List layout: a lane's entries come in groups of four that sit together
-// (8 bytes as 16-bit deltas, 16 bytes as 32-bit indices), and group b of the
-// wave's 64 lanes is contiguous: group b of lane l at base + (64*b + l) groups.
+// List layout: a lane's entries come in groups of four. 32-bit indices: a
+// group is 16 bytes and group g of the wave's 64 lanes is contiguous (group g
+// of lane l at base + (64*g + l) * 16 B). 16-bit deltas: a group is 8 bytes and
+// groups 2p and 2p+1 of a lane sit together, so one 16-byte load fetches both
+// (pair p of lane l at base + (64*p + l) * 16 B). Offsets in 16-bit slots.
 __device__ __forceinline__ long long list_slot(long long lb, int k, int lane, int mode) {
-  return lb + ((((long long)(k >> 2) << 6) + lane) << (2 + mode)) + ((long long)(k & 3) << mode);
+  const int g = k >> 2;
+  if (mode == 0)
+    return lb + ((((long long)(g >> 1) << 6) + lane) << 3) + ((g & 1) << 2) + (k & 3);
+  return lb + ((((long long)g << 6) + lane) << 3) + ((long long)(k & 3) << 1);
 }
 
 struct Group4 {
@@ -190,6 +195,58 @@ __device__ __forceinline__ void list_loop(int cnt, int nfull, LoadFn&& load_grou
   }
 }
 
+// list_loop for 16-bit lists: one 16-byte load brings two groups. An even
+// step runs group b while group b+1 (the pair's second half) is gathered and
+// the next pair is loaded; an odd step runs group b+1 while group b+2 (the
+// first half of that pair) is gathered. decode(e, b, full) takes the 8 bytes of
+// group b.
+template <bool kPadded, class DecodeFn, class GatherFn, class Body>
+__device__ __forceinline__ void list_loop_pairs(int cnt, int nfull, const uint4* __restrict__ L, DecodeFn&& decode,
+                                                GatherFn&& gather, Body&& body) {
+  using T = decltype(gather(0));
+  const int ng = (cnt + 3) >> 2;
+  if (ng == 0) return;
+  const int np = (ng + 1) >> 1;
+  Group4 g0, g1;
+  T a0[4], a1[4];
+  uint4 pc = L[0], pn;
+  g0 = nfull > 0 ? decode(make_uint2(pc.x, pc.y), 0, std::true_type{})
+                 : decode(make_uint2(pc.x, pc.y), 0, std::false_type{});
+#pragma unroll
+  for (int u = 0; u < 4; ++u) a0[u] = gather(g0.j[u]);
+  auto bodies = [&](int b, auto full, const Group4& g, const T(&a)[4]) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (decltype(full)::value || kPadded || (b << 2) + u < cnt) body(g.j[u], a[u]);
+  };
+  auto even = [&](int b, auto full) {
+    g1 = decode(make_uint2(pc.z, pc.w), b + 1, full);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a1[u] = gather(g1.j[u]);
+    pn = L[(long long)min((b >> 1) + 1, np - 1) << 6];
+    asm volatile("" ::: "memory");  // keep the loads above in this step
+    bodies(b, full, g0, a0);
+  };
+  auto odd = [&](int b, auto full) {
+    g0 = decode(make_uint2(pn.x, pn.y), b + 1, full);
+#pragma unroll
+    for (int u = 0; u < 4; ++u) a0[u] = gather(g0.j[u]);
+    asm volatile("" ::: "memory");
+    bodies(b, full, g1, a1);
+    pc = pn;
+  };
+  int b = 0;
+  for (; b + 2 < nfull; b += 2) {
+    even(b, std::true_type{});
+    odd(b + 1, std::true_type{});
+  }
+  for (; b < ng; b += 2) {
+    even(b, std::false_type{});
+    if (b + 1 >= ng) break;
+    odd(b + 1, std::false_type{});
+  }
+}
+
 // Calls body(j, gather(j)) for every neighbor j of sorted particle s, in
 // ascending j. The list is used when the wave has one; otherwise (reservation
 // exhausted) the words come from the masks, or are recomputed from xn.
@@ -208,10 +265,10 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
     const int lm = nv.list_mode[wave];
     const int nfull = lm >> 1;
     if ((lm & 1) == 0) {
-      const uint2* L = reinterpret_cast<const uint2*>(nv.list + lb) + lane;
+      const uint4* L = reinterpret_cast<const uint4*>(nv.list + lb) + lane;
       const int jb = nv.list_jbase[wave];
-      list_loop<kPadded, uint2>(
-          cnt, nfull, [&](int b) { return L[(long long)b << 6]; },
+      list_loop_pairs<kPadded>(
+          cnt, nfull, L,
           [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cnt, past); }, gather,
           body);
     } else {

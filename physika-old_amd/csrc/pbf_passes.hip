@@ -194,7 +194,10 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     }
     wofs += nw;
   }
-  const int kmax = (wave_max(cnt) + 3) & ~3;
+  // entries per lane the region must hold: whole pairs of groups for 16-bit
+  // lists, whole groups for 32-bit ones
+  const int cmax = wave_max(cnt);
+  const int kmax = mode == 0 ? (cmax + 7) & ~7 : (cmax + 3) & ~3;
   const int nfull = wave_min(active ? cnt : INT_MAX) >> 2;  // groups every active lane fills
   // The wave's list lives in its own fixed region (no allocation atomic: a
   // same-address atomic per wave serializes in L2); a wave that needs more
