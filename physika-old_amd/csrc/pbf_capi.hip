@@ -43,6 +43,7 @@ struct pbf_ctx {
   int *sid = nullptr, *ssid = nullptr, *bpart = nullptr;
   int *nbr_cnt = nullptr, *cnt_orig = nullptr;
   float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr, *vv = nullptr;
+  float* xc = nullptr;  // sorted x^n as three coordinate planes of cap + kXnPad floats
   float* dpos = nullptr;
   // neighbor pass output: per-wave metadata, bit masks (scratch), lists
   long long* mask_base = nullptr;
@@ -397,7 +398,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   const int cap = std::max(((n + 1 + 255) / 256) * 256, 256);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
-  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
+  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->wave_stat);
   c->cap_n = 0;
@@ -413,6 +414,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   PBF_HIP(c, hipMalloc(&c->nbr_cnt, ni));
   PBF_HIP(c, hipMalloc(&c->cnt_orig, ni));
   PBF_HIP(c, hipMalloc(&c->xn, nf4 + sizeof(float4) * kXnPad));
+  PBF_HIP(c, hipMalloc(&c->xc, 3 * sizeof(float) * ((size_t)cap + kXnPad)));
   PBF_HIP(c, hipMalloc(&c->x0, nf4));
   PBF_HIP(c, hipMalloc(&c->xa, nf4));
   PBF_HIP(c, hipMalloc(&c->xb, nf4));
@@ -461,6 +463,9 @@ NbrView view_of(const pbf_ctx* c, int n) {
   nv.ssid = c->ssid;
   nv.cell_start = c->cell_start;
   nv.xn = c->xn;
+  nv.cx = c->xc;
+  nv.cy = c->xc + (c->cap_n + kXnPad);
+  nv.cz = c->xc + 2 * (size_t)(c->cap_n + kXnPad);
   nv.nbr_cnt = c->nbr_cnt;
   nv.list_base = c->list_base;
   nv.list_jbase = c->list_jbase;
@@ -509,9 +514,10 @@ SortBufs sort_of(pbf_ctx* c, const int* scene_off, int nscenes) {
   sb.slot = c->slot;
   sb.tmp = c->tmp;
   sb.perm = c->perm;
-  sb.inv = c->inv;
   sb.scan_part = c->scan_part;
   sb.bpart = c->bpart;
+  sb.xc = c->xc;
+  sb.xc_stride = c->cap_n + kXnPad;
   return sb;
 }
 
@@ -603,6 +609,7 @@ int export_csr(pbf_ctx* c, int n, int* d_offsets, int* d_elems, long long cap, l
     *total = 0;
     return PBF_OK;
   }
+  launch_invert(c->perm, c->inv, n, st);
   launch_csr_count(c->nbr_cnt, c->inv, n, c->cnt_orig, st);
   scan_exclusive(c->cnt_orig, d_offsets, n, c->scan_scratch, st);
   PBF_LAUNCHED(c);
@@ -903,7 +910,7 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->cell_cnt); dfree(c->cell_start); dfree(c->oog_flag);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
-  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv);
+  dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->wave_stat);
   dfree(c->mask); dfree(c->list); dfree(c->scan_scratch); dfree(c->bf.d);

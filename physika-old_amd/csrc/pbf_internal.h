@@ -70,7 +70,10 @@ struct NbrView {
   const SceneWin* win;         // per scene
   const int* ssid;             // scene of each sorted particle
   const int* cell_start;       // cell_start[c]..cell_start[c+1]: sorted range of compact cell c
-  const float4* xn;            // sorted positions the neighbor sets are defined on
+  const float4* xn;
+  const float* cx;             // sorted x^n coordinate planes (the neighbor pass reads
+  const float* cy;             // candidates from these: one 16-byte load brings the
+  const float* cz;             // same coordinate of four consecutive candidates)            // sorted positions the neighbor sets are defined on
   const int* nbr_cnt;          // neighbors per sorted particle
   const long long* list_base;  // per wave: first 16-bit slot of its list, -1 = no list
   const int* list_jbase;       // per wave: index base of 16-bit entries
@@ -123,9 +126,10 @@ struct SortBufs {
   int* slot;
   int* tmp;
   int* perm;
-  int* inv;
   int* scan_part;         // partial sums of the device-length scan
   int* bpart;             // per block of the bounds pass: scene and cell box (bounds_part_ints)
+  float* xc;              // sorted x^n as three planes (x, y, z), plane stride xc_stride
+  int xc_stride;
 };
 
 constexpr int kBlock = 256;
@@ -164,6 +168,7 @@ void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, c
                               float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr);
 void launch_write_back(const float4* xs, const float4* vs, const int* perm, int n, float* x_out,
                        float* v_out, hipStream_t st);
+void launch_invert(const int* perm, int* inv, int n, hipStream_t st);
 void launch_csr_count(const int* nbr_cnt, const int* inv, int n, int* cnt_orig, hipStream_t st);
 void launch_csr_fill(const NbrView& nv, const int* perm, const int* offsets, int* elems,
                      long long cap, hipStream_t st);

@@ -77,6 +77,28 @@ __device__ __forceinline__ uint32_t test_word(const float4* p, float4 xi, int cn
   return (bits << (32 - k0)) & (~0u << (32 - cnt));
 }
 
+// test_word over the coordinate planes: two 16-byte loads per coordinate bring
+// eight consecutive candidates (the loads need only 4-byte alignment), six
+// loads where the float4 array needs eight. The planes are padded like xn.
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+__device__ __forceinline__ uint32_t test_word_planes(const float* cx, const float* cy, const float* cz, int j0,
+                                                     float4 xi, int cnt, float thr2) {
+  uint32_t bits = 0;
+  int k0 = 0;
+  for (; k0 < cnt; k0 += 8) {
+    const int j = j0 + k0;
+    const f4u x0 = *reinterpret_cast<const f4u*>(cx + j), x1 = *reinterpret_cast<const f4u*>(cx + j + 4);
+    const f4u y0 = *reinterpret_cast<const f4u*>(cy + j), y1 = *reinterpret_cast<const f4u*>(cy + j + 4);
+    const f4u z0 = *reinterpret_cast<const f4u*>(cz + j), z1 = *reinterpret_cast<const f4u*>(cz + j + 4);
+    const float px[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+    const float py[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
+    const float pz[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
+#pragma unroll
+    for (int u = 0; u < 8; ++u) bits = push_bit(bits, sq_dist(xi, make_float4(px[u], py[u], pz[u], 0.0f)), thr2);
+  }
+  return (bits << (32 - k0)) & (~0u << (32 - cnt));
+}
+
 // Pops the first candidate (most significant set bit) of a non-zero word.
 __device__ __forceinline__ int pop_first(uint32_t& bits) {
   const int c = __builtin_clz(bits);

@@ -185,7 +185,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     const int nw = (rl[r] + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
       const int jb = ra[r] + (w << 5);
-      const uint32_t bits = test_word(nv.xn + jb, xi, min(32, rl[r] - (w << 5)), nv.thr2);
+      const uint32_t bits = test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[r] - (w << 5)), nv.thr2);
       if (in_lds)
         lcol[(wofs + w) << 6] = make_uint2(bits, (uint32_t)(jb - jbase));
       else if (mb >= 0)

@@ -389,7 +389,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int* __restrict__ key,
 
 // Final position inside the cell = rank of the particle id among the cell's
 // ids (deterministic order, independent of atomic arrival). Cells holding more
-// than kRankMax particles keep arrival order (still a valid cell sort).
+// than kRankMax particles keep arrival order (still a valid cell sort). One
+// thread per slot of the scattered order: slot t's particle lands inside its
+// own cell's range, so the sorted arrays are written nearly in order and only
+// the inputs are read scattered.
 // The integrator (ParticleIntegrator.cu:60-70,103-112, force zeroed by begin)
 // is fused here: x^n goes to xn (the neighbor sets are built on it,
 // PositionBasedFluidModel.cpp:93-94), the integrated state to x0 / vs.
@@ -398,31 +401,30 @@ constexpr int kRankMax = 1024;
 __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key, const int* __restrict__ cell_start,
                                                     const int* __restrict__ tmp, const int* __restrict__ sid,
                                                     const float* __restrict__ x, const float* __restrict__ v, int n,
-                                                    int* __restrict__ perm, int* __restrict__ inv,
-                                                    int* __restrict__ ssid, float4* __restrict__ xn,
+                                                    int* __restrict__ perm, int* __restrict__ ssid,
+                                                    float4* __restrict__ xn,
                                                     float4* __restrict__ x0, float4* __restrict__ vs, PhysConst pc,
-                                                    int integrate) {
+                                                    int integrate, float* __restrict__ xc, int xc_stride) {
 #pragma clang fp contract(off)
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) x0[n] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);  // the padding particle
-  if (i >= n) return;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t == 0) x0[n] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);  // the padding particle
+  if (t >= n) return;
+  const int i = tmp[t];
   const int k = key[i];
   const int a = cell_start[k], b = cell_start[k + 1];
-  int s;
+  int s = t;
   if (b - a <= kRankMax) {
     int rank = 0;
-    for (int t = a; t < b; ++t) rank += tmp[t] < i;
+    for (int u = a; u < b; ++u) rank += tmp[u] < i;
     s = a + rank;
-  } else {
-    s = a;
-    for (int t = a; t < b; ++t)
-      if (tmp[t] == i) s = t;
   }
   perm[s] = i;
-  inv[i] = s;
   ssid[s] = sid[i];
   const float px = x[3 * (size_t)i], py = x[3 * (size_t)i + 1], pz = x[3 * (size_t)i + 2];
   xn[s] = make_float4(px, py, pz, 0.0f);
+  xc[s] = px;
+  xc[xc_stride + s] = py;
+  xc[2 * (size_t)xc_stride + s] = pz;
   if (integrate) {
     const float vx = v[3 * (size_t)i] + pc.gdt[0];
     const float vy = v[3 * (size_t)i + 1] + pc.gdt[1];
@@ -453,8 +455,8 @@ void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const flo
   k_scatter<<<nb, kBlock, 0, st>>>(sb.key, sb.slot, sb.cell_start, sb.tmp, n);
   PhysConst pc{};
   if (integrate) pc = *integrate;
-  k_reorder<<<nb, kBlock, 0, st>>>(sb.key, sb.cell_start, sb.tmp, sb.sid, x, v, n, sb.perm, sb.inv, sb.ssid, xn, x0,
-                                   vs, pc, integrate ? 1 : 0);
+  k_reorder<<<nb, kBlock, 0, st>>>(sb.key, sb.cell_start, sb.tmp, sb.sid, x, v, n, sb.perm, sb.ssid, xn, x0,
+                                   vs, pc, integrate ? 1 : 0, sb.xc, sb.xc_stride);
 }
 
 // Copy of cell_start[0..total] (the device-side cell count of the last sort).
@@ -496,6 +498,16 @@ __global__ __launch_bounds__(kBlock) void k_write_back(const float4* __restrict_
 void launch_write_back(const float4* xs, const float4* vs, const int* perm, int n, float* x_out, float* v_out,
                        hipStream_t st) {
   k_write_back<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xs, vs, perm, n, x_out, v_out);
+}
+
+// inv[perm[s]] = s (the CSR export needs the sorted slot of every particle).
+__global__ __launch_bounds__(kBlock) void k_invert(const int* __restrict__ perm, int* __restrict__ inv, int n) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) inv[perm[s]] = s;
+}
+
+void launch_invert(const int* perm, int* inv, int n, hipStream_t st) {
+  if (n > 0) k_invert<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(perm, inv, n);
 }
 
 __global__ __launch_bounds__(kBlock) void k_csr_count(const int* __restrict__ nbr_cnt, const int* __restrict__ inv,
