@@ -70,23 +70,113 @@ def progress(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(x, steps, threads):
-    """The oracle (C++ restatement of the reference path, OpenMP over
-    particles) on this host: one warm-up step, then `steps` timed steps."""
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def oracle_rate(x, steps, threads, warm=1):
+    """Particle-steps/s of the oracle on `threads` host threads: `warm` untimed
+    steps, then `steps` timed steps. Returns (rate, threads used, seconds)."""
     import __graft_entry__ as G
     O = G.load_oracle()
     used = O.set_threads(threads)
     p = O.make_params()
     v = np.zeros_like(x)
-    xw, vw, _, _ = O.step(p, x, v, 1)
-    progress("cpu baseline: warm-up step done")
+    if warm:
+        x, v, _, _ = O.step(p, x, v, warm)
     t0 = time.perf_counter()
-    O.step(p, xw, vw, steps)
+    O.step(p, x, v, steps)
     dt = time.perf_counter() - t0
-    progress(f"cpu baseline: {steps} steps in {dt:.1f} s")
-    return {"value": len(x) * steps / dt, "unit": "particle-steps/s", "cores": used, "kind": "port",
+    return len(x) * steps / dt, used, dt
+
+
+def cpu_baseline(x, steps, threads):
+    """The oracle (C++ restatement of the reference path, OpenMP over
+    particles) on this host: one warm-up step, then `steps` timed steps on all
+    the threads given, and one step on one thread."""
+    rate, used, dt = oracle_rate(x, steps, threads)
+    progress(f"cpu baseline: {steps} steps on {used} threads in {dt:.1f} s")
+    rate1, _, dt1 = oracle_rate(x, 1, 1, warm=0)
+    progress(f"cpu baseline: 1 step on 1 thread in {dt1:.1f} s")
+    return {"value": rate, "unit": "particle-steps/s", "cores": used, "kind": "port",
             "sample": f"{len(x)}-particle lattice (the benchmark workload), {steps} steps after 1 warm-up step, "
-                      f"{dt:.1f} s"}
+                      f"{dt:.1f} s", "one_thread": {"value": rate1, "sample": f"1 step, {dt1:.1f} s"},
+            "cpu": cpu_model()}
+
+
+def gpu_rate(torch, fluid, x_np, steps, warmup, runs=3):
+    """Median over `runs` runs, each from the same initial state: `warmup`
+    untimed steps and one more step, then `steps` steps in one call timed with
+    HIP events on the stream. Returns (median ms per step, all ms per step,
+    context, x, v)."""
+    dev = torch.device("cuda:0")
+    ctx = fluid.Context(dev)
+    ctx.reserve(len(x_np))
+    stream = torch.cuda.current_stream(dev)
+    times = []
+    for _ in range(runs):
+        x = torch.from_numpy(x_np).to(dev)
+        v = torch.zeros_like(x)
+        ctx.step(x, v, None, max(warmup, 1))
+        torch.cuda.synchronize()
+        ctx.step(x, v, None, 1)  # reservations grown after the warm-up are in place before the timer starts
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        e0.record(stream)
+        ctx.step(x, v, None, steps)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        times.append(e0.elapsed_time(e1) / steps)
+    return float(np.median(times)), times, ctx, x, v
+
+
+def config1(torch, fluid, scene, threads):
+    """The 32,768-particle dam-break lattice, 100 steps: GPU (median of 3) and
+    the oracle on the host (100 steps on all threads, 10 on one)."""
+    x_np = scene.dam_break_32k()
+    ms, runs, _, _, _ = gpu_rate(torch, fluid, x_np, 100, 1)
+    rate, used, dt = oracle_rate(x_np, 100, threads)
+    rate1, _, dt1 = oracle_rate(x_np, 10, 1)
+    progress(f"config 1: GPU {ms:.4f} ms/step; CPU {dt:.1f} s for 100 steps on {used} threads")
+    n = len(x_np)
+    return {"workload": "32,768-particle dam-break lattice at (0.10, 0.50, 0.10), seed 1; 100 steps",
+            "gpu": {"value": round(n / (ms / 1e3), 1), "unit": "particle-steps/s", "ms_per_step": round(ms, 4),
+                    "runs_ms_per_step": [round(t, 4) for t in runs]},
+            "cpu": {"value": round(rate, 1), "cores": used, "seconds": round(dt, 2),
+                    "one_thread": round(rate1, 1)}}
+
+
+def config2b(torch, fluid, scene, steps=20):
+    """8 Gaussian blobs x 131,072 particles (sigma 0.05, seed 3): GPU rate over
+    `steps` steps (median of 3), neighbor-count mean and maximum from the
+    library, and the count histogram of a 20,000-particle sample of the input
+    (k-d tree, exact distances)."""
+    from scipy.spatial import cKDTree
+    x_np = scene.blobs_1m()
+    ms, runs, ctx, _, _ = gpu_rate(torch, fluid, x_np, steps, 2)
+    st = ctx.stats()
+    n = len(x_np)
+    tree = cKDTree(x_np.astype(np.float64))
+    pick = np.random.default_rng(3).choice(n, 20000, replace=False)
+    counts = np.array([len(r) for r in tree.query_ball_point(x_np[pick].astype(np.float64), 0.0125)])
+    edges = [1, 10, 30, 60, 100, 200, 300, 400, 500, 600, 800, 1200]
+    hist, _ = np.histogram(counts, bins=edges + [10 ** 9])
+    progress(f"config 2b: {ms:.3f} ms/step, mean {st['neighbor_pairs'] / n:.1f} neighbors")
+    return {"workload": "8 Gaussian blobs x 131,072 = 1,048,576 particles, sigma 0.05, seed 3; "
+                        f"{steps} steps after 2", "particles": n,
+            "value": round(n / (ms / 1e3), 1), "unit": "particle-steps/s", "ms_per_step": round(ms, 4),
+            "runs_ms_per_step": [round(t, 4) for t in runs],
+            "neighbors_mean_last_step": round(st["neighbor_pairs"] / n, 1),
+            "neighbors_max_last_step_slots": int(st["list_slots_needed"] // max(64 * ((n + 63) // 64), 1)),
+            "input_sample_histogram": {"edges": edges, "counts": hist.tolist(),
+                                       "mean": round(float(counts.mean()), 1), "max": int(counts.max())}}
 
 
 def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
@@ -261,7 +351,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--runs", type=int, default=3, help="timed runs from the same initial state (median)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the 32k (config 1) and blob (config 2b) lines")
     ap.add_argument("--workload", default="lattice_1m", choices=["lattice_1m", "blobs_1m"])
     ap.add_argument("--batched-steps", type=int, default=20, help="steps of the 256-scene batched comparison")
     ap.add_argument("--serial-steps", type=int, default=5, help="steps of its serial per-scene loop")
@@ -291,34 +383,11 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    x = torch.from_numpy(x_np).to(dev)
-    v = torch.zeros_like(x)
+    params = fluid.Context(dev).params()
+    ms_step, runs, ctx, x, v = gpu_rate(torch, fluid, x_np, args.steps, args.warmup, args.runs)
     rho = torch.empty(n, dtype=torch.float32, device=dev)
-    ctx = fluid.Context(dev)
-    params = ctx.params()
-    ctx.reserve(n)
-
-    # warm-up (also grows the neighbor-mask reservation to what the scene needs)
-    ctx.step(x, v, rho, max(args.warmup, 1))
-    torch.cuda.synchronize()
-    ctx.step(x, v, rho, 1)
-    torch.cuda.synchronize()
-    progress("warm-up done")
-
-    stream = torch.cuda.current_stream(dev)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    e0.record(stream)
-    t0 = time.perf_counter()
-    ctx.step(x, v, rho, args.steps)
-    e1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms_total = e0.elapsed_time(e1)
-    ms_step = ms_total / args.steps
-    progress(f"{args.workload}: {args.steps} steps, {ms_step:.4f} ms/step")
-    value = n * args.steps / (ms_total / 1e3)
+    progress(f"{args.workload}: {args.steps} steps x {args.runs} runs, median {ms_step:.4f} ms/step")
+    value = n / (ms_step / 1e3)
 
     stats = ctx.stats()
     pairs = stats["neighbor_pairs"]
@@ -374,7 +443,7 @@ def main():
         "roofline": roofline,
         "passes_ms": {k: round(v[0], 4) for k, v in pass_ms.items()},
         "step_roofline": whole_step,
-        "wall_s": round(wall, 3),
+        "runs_ms_per_step": [round(t, 4) for t in runs],
     }
     if not args.no_batched:
         del x, v, rho, ctx
@@ -382,8 +451,11 @@ def main():
     if not args.no_diff:
         out["differentiable"] = differentiable(torch, fluid, pkg.scene)
         out["batched_inverse"] = batched_inverse(torch, fluid, pkg.scene)
+    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    if not args.no_configs:
+        out["config1"] = config1(torch, fluid, pkg.scene, threads)
+        out["config2b"] = config2b(torch, fluid, pkg.scene)
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
         out["cpu_baseline"] = cpu_baseline(x_np, args.cpu_steps, threads)
     print(json.dumps(out), flush=True)
 
