@@ -108,7 +108,8 @@ typedef struct pbf_stats {
                                     fullest wavefront */
   long long list_slots_reserved; /* neighbor-list slots reserved */
   long long cells_used;          /* compact cells of the last search: the scenes'
-                                    occupied-cell windows plus one out-of-grid
+                                    occupied-cell windows (sort cells: half a
+                                    grid cell along x) plus one out-of-grid
                                     bucket per scene */
   long long candidates;          /* candidate pairs the last search tested (every
                                     particle of the 27 stencil cells) */

@@ -19,7 +19,7 @@ struct pbf_ctx {
   pbf_params p{};
   GridDev g{};
   PhysConst pc{};
-  long long full_cells = 0;  // nx*ny*nz of one scene's grid
+  long long full_cells = 0;  // sort-grid cells of one scene: 2nx*ny*nz (half cells along x)
   float m_factor = 0.f, mass = 0.f;
   int device = 0;
 
@@ -247,12 +247,17 @@ int derive(pbf_ctx* c, const pbf_params* p) {
     if (!(nf < 2.0e6f)) return fail(c, PBF_ERR_UNSUPPORTED, "grid too large along one axis");
     n[a] = (long long)nf;
   }
-  const long long ncell = n[0] * n[1] * n[2];
-  if (ncell > (1ll << 30)) return fail(c, PBF_ERR_UNSUPPORTED, "grid has more than 2^30 cells");
+  const long long ncell = 2 * n[0] * n[1] * n[2];  // half cells along x
+  if (ncell > (1ll << 30)) return fail(c, PBF_ERR_UNSUPPORTED, "grid has more than 2^29 cells");
   g.ds = ds;
   g.nx = (int)n[0];
   g.ny = (int)n[1];
   g.nz = (int)n[2];
+  // Half-cell distance of a neighbor pair beyond 2 that rounding can produce:
+  // s < thr2 bounds the true distance by h (1 + 2.1e-7); each q = (p - lo)/ds
+  // carries two roundings, at most 1.2e-7 q_max (q_max = nx + 2 after the
+  // clamp); in half cells 2 (2.1e-7 + 2 * 1.2e-7 q_max); four times that.
+  g.ext_margin = 4.0f * (4.2e-7f + 4.8e-7f * (float)(n[0] + 2));
 
   PhysConst pc{};
   const float h = p->smoothing_length;

@@ -20,14 +20,34 @@ __device__ __forceinline__ int cell_coord(float p, float lo, float ds, int n) {
   return (int)f;
 }
 
-__device__ __forceinline__ void cell_of(const GridDev& g, float x, float y, float z, int& ci, int& cj, int& ck) {
-  ci = cell_coord(x, g.lo[0], g.ds, g.nx);
+// Half-cell index along x: floor(2q) with q = (p - lo)/ds as above (doubling
+// is exact, so floor(2q) >> 1 is the reference cell floor(q)), clamped to
+// [-4, 2n+3] (the reference clamp in half cells); frac = 2q - floor(2q).
+__device__ __forceinline__ int half_coord(float p, float lo, float ds, int n, float& frac) {
+#pragma clang fp contract(off)
+  const float q2 = 2.0f * ((p - lo) / ds);
+  const float f = floorf(q2);
+  frac = q2 - f;
+  if (!(f >= -4.0f)) return -4;
+  if (f > (float)(2 * n + 3)) return 2 * n + 3;
+  return (int)f;
+}
+
+// Sort-grid cell of a point: half cells along x, reference cells along y and z.
+__device__ __forceinline__ void cell_of(const GridDev& g, float x, float y, float z, int& fi, int& cj, int& ck,
+                                        float& frac) {
+  fi = half_coord(x, g.lo[0], g.ds, g.nx, frac);
   cj = cell_coord(y, g.lo[1], g.ds, g.ny);
   ck = cell_coord(z, g.lo[2], g.ds, g.nz);
 }
+__device__ __forceinline__ void cell_of(const GridDev& g, float x, float y, float z, int& fi, int& cj, int& ck) {
+  float frac;
+  cell_of(g, x, y, z, fi, cj, ck, frac);
+}
 
-__device__ __forceinline__ bool in_grid(const GridDev& g, int ci, int cj, int ck) {
-  return ci >= 0 && ci < g.nx && cj >= 0 && cj < g.ny && ck >= 0 && ck < g.nz;
+// The reference's in-grid test (GridHash.h:55-62) on a sort-grid cell.
+__device__ __forceinline__ bool in_grid(const GridDev& g, int fi, int cj, int ck) {
+  return fi >= 0 && fi < 2 * g.nx && cj >= 0 && cj < g.ny && ck >= 0 && ck < g.nz;
 }
 
 // |x_i - x_j| < h exactly as NeighborQuery.cu:266-267 evaluates it: the squared

@@ -10,16 +10,22 @@ namespace pbf {
 // cell = h, origin = scene_lo - 2 cells, n = ceil((hi - lo)/h) + 1 + 4 per
 // axis. Every scene of a batch uses this geometry, so a particle's cell
 // coordinates are exactly those of a one-scene run.
+// The reference grid (GridHash::setSpace): nx x ny x nz cells of size ds from
+// lo. The sort grid refines x into half cells (index fi = floor(2q); the
+// reference cell is fi >> 1); ext_margin (in half cells) bounds how far float
+// rounding can move a neighbor pair's half-cell distance past 2.
 struct GridDev {
   float lo[3];
   float ds;
   int nx, ny, nz;
+  float ext_margin;
 };
 
-// A scene's cells are stored in a window: the bounding box of the cells of its
-// in-grid particles (cells outside it are empty), indexed
-// cell_base + (i - lo0) + dim0*((j - lo1) + dim1*(k - lo2)). The window is
-// followed by the scene's bucket of out-of-grid particles at cell_base + ncell.
+// A scene's cells are stored in a window: the bounding box of the sort-grid
+// cells (half cells along x) of its in-grid particles (cells outside it are
+// empty), indexed cell_base + (i - lo0) + dim0*((j - lo1) + dim1*(k - lo2)).
+// The window is followed by the scene's bucket of out-of-grid particles at
+// cell_base + ncell.
 struct SceneWin {
   int lo[3];
   int dim[3];

@@ -14,17 +14,35 @@
 
 namespace pbf {
 
-// Run r (0..8) of cell (ci,cj,ck): cells (ci-1..ci+1, cj+dj, ck+dk) with
-// dk = r/3-1, dj = r%3-1 (so runs come in ascending sorted index), clipped to
-// the scene's window (cells outside it hold no particle, and the window lies
-// inside the grid, so this is the reference's in-grid stencil).
-__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, int ci, int cj,
-                                          int ck, int r, int& a, int& len) {
+// The half cells along x a particle in half cell fi visits: [fi + xlo, fi + xhi].
+// The reference visits its cell and both x neighbors (GridHash cells,
+// NeighborQuery.cu:14-41), i.e. half cells [2c-2, 2c+3] with c = fi >> 1; a
+// neighbor (closer than h) lies at most 2 half cells away unless rounding
+// stretched the pair, which can only add the 3rd half cell on the side that
+// stays inside the reference cells (right when fi is even, left when odd), and
+// only for a particle within ext_margin of the face on that side.
+__device__ __forceinline__ void stencil_x(const GridDev& g, int fi, float frac, int& xlo, int& xhi) {
+  xlo = -2;
+  xhi = 2;
+  if (fi & 1) {
+    if (frac < g.ext_margin) xlo = -3;
+  } else if (frac > 1.0f - g.ext_margin) {
+    xhi = 3;
+  }
+}
+
+// Run r (0..8) of sort cell (fi,cj,ck): half cells fi+xlo..fi+xhi of row
+// (cj+dj, ck+dk) with dk = r/3-1, dj = r%3-1 (so runs come in ascending
+// sorted index), clipped to the scene's window (cells outside it hold no
+// particle, and the window lies inside the grid, so this is the reference's
+// in-grid stencil).
+__device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, int fi, int cj,
+                                          int ck, int xlo, int xhi, int r, int& a, int& len) {
   const int lj = cj + (r % 3) - 1 - w.lo[1], lk = ck + (r / 3) - 1 - w.lo[2];
   a = 0;
   len = 0;
   if (lj < 0 || lj >= w.dim[1] || lk < 0 || lk >= w.dim[2]) return;
-  const int i0 = max(ci - 1, w.lo[0]), i1 = min(ci + 1, w.lo[0] + w.dim[0] - 1);
+  const int i0 = max(fi + xlo, w.lo[0]), i1 = min(fi + xhi, w.lo[0] + w.dim[0] - 1);
   if (i0 > i1) return;
   const int row = w.cell_base + w.dim[0] * (lj + w.dim[1] * lk) - w.lo[0];
   a = cell_start[row + i0];
@@ -302,14 +320,16 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
     return;
   }
   const float4 xni = nv.xn[s];
-  int ci, cj, ck;
-  cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck);
+  int ci, cj, ck, xlo, xhi;
+  float frac;
+  cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck, frac);
+  stencil_x(nv.g, ci, frac, xlo, xhi);
   const SceneWin win = nv.win[nv.ssid[s]];
   const long long mb = nv.mask_base[wave];
   int wofs = 0;
   for (int r = 0; r < 9; ++r) {
     int ra, rl;
-    run_range(nv.cell_start, win, ci, cj, ck, r, ra, rl);
+    run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, r, ra, rl);
     const int nw = (rl + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
       const int jb = ra + (w << 5);

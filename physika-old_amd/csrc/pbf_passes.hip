@@ -129,12 +129,14 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   int words = 0, cand = 0, jlo = INT_MAX, jhi = -1;
   if (active) {
     xi = nv.xn[s];
-    int ci, cj, ck;
-    cell_of(nv.g, xi.x, xi.y, xi.z, ci, cj, ck);
+    int ci, cj, ck, xlo, xhi;
+    float frac;
+    cell_of(nv.g, xi.x, xi.y, xi.z, ci, cj, ck, frac);
+    stencil_x(nv.g, ci, frac, xlo, xhi);
     const SceneWin win = nv.win[nv.ssid[s]];
 #pragma unroll
     for (int r = 0; r < 9; ++r) {
-      run_range(nv.cell_start, win, ci, cj, ck, r, ra[r], rl[r]);
+      run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, r, ra[r], rl[r]);
       words += (rl[r] + 31) >> 5;
       cand += rl[r];
       if (rl[r] > 0) {
@@ -146,10 +148,11 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     // so each in-grid j it lists receives -dp_kj from it (DensityPBD.cu:162)
     // without listing it back: flag the cells of its stencil for pass B.
     if (!in_window(win, ci, cj, ck)) {  // the window holds every in-grid cell of the scene
+      const int h0 = 2 * ((ci >> 1) - 1);  // the half cells of its reference cells
       for (int dk = -1; dk <= 1; ++dk)
         for (int dj = -1; dj <= 1; ++dj)
-          for (int di = -1; di <= 1; ++di)
-            if (in_window(win, ci + di, cj + dj, ck + dk)) out.oog_flag[window_cell(win, ci + di, cj + dj, ck + dk)] = 1;
+          for (int fx = h0; fx < h0 + 6; ++fx)
+            if (in_window(win, fx, cj + dj, ck + dk)) out.oog_flag[window_cell(win, fx, cj + dj, ck + dk)] = 1;
     }
   } else {
 #pragma unroll
@@ -365,7 +368,8 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
           const float4 xnk = nv.xn[k];
           int ki, kj, kk;
           cell_of(nv.g, xnk.x, xnk.y, xnk.z, ki, kj, kk);
-          if (abs(ki - ci) <= 1 && abs(kj - cj) <= 1 && abs(kk - ck) <= 1 && within(xnk, xni, nv.thr2))
+          if (abs((ki >> 1) - (ci >> 1)) <= 1 && abs(kj - cj) <= 1 && abs(kk - ck) <= 1 &&
+              within(xnk, xni, nv.thr2))
             pair(xs[k], bx, by, bz);
         }
       }

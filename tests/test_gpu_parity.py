@@ -70,6 +70,33 @@ def test_neighbor_sets_crowded_cell(fluid, oracle, scene):
     assert_same_sets(nl, off, el)
 
 
+def half_cell_pair_cloud():
+    """36 copies (0.05 apart in y and z) of a pair closer than h whose half-cell
+    indices along x differ by 3: x_i = 0.106249996 (q = 10.499999, half cell 20,
+    reference cell 10) and x_j = 0.11874999 (q = 11.5, half cell 23, reference
+    cell 11). Found by a search over float32 positions near half-cell faces."""
+    xi, xj = np.float32(0.106249996), np.float32(0.11874999)
+    pts = []
+    for k in range(6):
+        for l in range(6):
+            y, z = np.float32(0.3 + 0.05 * k), np.float32(0.3 + 0.05 * l)
+            pts += [(xi, y, z), (xj, y, z)]
+    return np.array(pts, dtype=np.float32)
+
+
+def test_neighbor_sets_half_cell_rounding_pair(fluid, oracle):
+    """The sort grid halves the cells along x and visits 5 half cells; a pair
+    that rounding puts 3 half cells apart is found by the stencil's rounding
+    extension, as the reference's 3-cell stencil finds it."""
+    say("neighbor sets, pairs 3 half cells apart")
+    x = half_cell_pair_cloud()
+    ctx = fluid.Context("cuda:0")
+    nl = ctx.neighbor_query(dev(x))
+    off, el = oracle.neighbor_query(oracle.make_params(), x)
+    assert np.all(np.diff(off) == 2)  # self and the partner
+    assert_same_sets(nl, off, el)
+
+
 def test_neighbor_query_capacity_error(fluid, capi, scene):
     import ctypes as C
     say("neighbor query capacity error")

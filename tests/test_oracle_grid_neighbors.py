@@ -80,6 +80,22 @@ def test_neighbor_sets_near_grid_faces(oracle, scene):
         np.testing.assert_array_equal(got[i], want[i])
 
 
+def test_pair_three_half_cells_apart(oracle):
+    """x_i = 0.106249996 and x_j = 0.11874999 are closer than h (the strict test
+    passes in float32) and lie in adjacent reference cells 10 and 11, although
+    2q puts them 3 half cells apart (20.999996 and 23.0): the reference lists
+    the pair, as the brute force does."""
+    p = oracle.make_params()
+    x = np.array([[0.106249996, 0.5, 0.5], [0.11874999, 0.5, 0.5]], dtype=np.float32)
+    lo, ds = np.float32(-0.025), np.float32(0.0125)
+    q = ((x[:, 0] - lo).astype(np.float32) / ds).astype(np.float32)
+    assert np.floor(2 * q).tolist() == [20.0, 23.0] and np.floor(q).tolist() == [10.0, 11.0]
+    off, el = oracle.neighbor_query(p, x)
+    got, want = csr_rows(off, el), brute_sets(x, 0.0125)
+    for g, w in zip(got, want):
+        assert g.tolist() == w.tolist() == [0, 1]
+
+
 def test_symmetry_and_self(oracle, scene):
     p = oracle.make_params()
     x = scene.jittered_lattice((10, 9, 8), (0.3, 0.3, 0.3), 0.005, 0.1, seed=5)
