@@ -631,6 +631,91 @@ class BatchedPositionBasedFluidModel(PositionBasedFluidModel):
         self.ctx.batch_step(x, self.m_velocity.getValue(), self.m_sceneOffsets.getValue(), rho, nsteps)
 
 
+class ParticleFluid:
+    """ParticleFluid node (ParticleFluid.cpp:14-129): a PositionBasedFluidModel
+    whose particles may come from particle emitters (emitters.py).
+
+    currentPosition / currentVelocity / currentForce are the model's
+    m_position / m_velocity / m_forceDensity. advance(dt) follows
+    ParticleFluid::advance: with emitters, the fluid's state is copied back
+    into each emitter's segment (emitters own consecutive segments, in the
+    order added), every emitter appends its new particles (advance2), the
+    fluid's arrays become the emitters' arrays concatenated, and the model
+    steps when there is at least one particle and self_update is set. Without
+    emitters the fluid steps the particles it holds. The particle count can
+    change every frame; the library grows its buffers as needed."""
+
+    def __init__(self, name: str = "fluid", model: Optional[PositionBasedFluidModel] = None, **model_args):
+        self.name = name
+        self.model = model if model is not None else PositionBasedFluidModel(**model_args)
+        self.currentPosition = self.model.m_position
+        self.currentVelocity = self.model.m_velocity
+        self.currentForce = self.model.m_forceDensity
+        self.self_update = True
+        self.dt = self.model.ctx.params()["dt"]
+        self._emitters = []
+
+    def addParticleEmitter(self, emitter) -> None:
+        self._emitters.append(emitter)
+
+    def getParticleEmitters(self):
+        return list(self._emitters)
+
+    def setDt(self, dt: float) -> None:
+        self.dt = float(dt)
+
+    def getDt(self) -> float:
+        return self.dt
+
+    def resetStatus(self) -> bool:
+        """With emitters the fluid starts empty (ParticleFluid.cpp:132-145)."""
+        if self._emitters:
+            dev = self.model.ctx.device
+            for f in (self.currentPosition, self.currentVelocity, self.currentForce):
+                f.setValue(torch.empty((0, 3), dtype=torch.float32, device=dev))
+        return True
+
+    def _copy_out(self) -> None:
+        x, v, f = self.currentPosition.getValue(), self.currentVelocity.getValue(), self.currentForce.getValue()
+        start = 0
+        for e in self._emitters:
+            num = e.currentPosition.getElementCount()
+            if num == 0:
+                continue
+            if start + num > x.shape[0]:
+                raise ValueError(f"emitter {e.name!r} owns {num} particles past the fluid's {x.shape[0]}")
+            e.currentPosition.setValue(x[start:start + num].clone())
+            e.currentVelocity.setValue(v[start:start + num].clone())
+            e.currentForce.setValue(f[start:start + num].clone() if f is not None and f.shape[0] >= start + num
+                                    else torch.zeros_like(x[start:start + num]))
+            start += num
+
+    def advance(self, dt: Optional[float] = None) -> int:
+        """One frame; returns the particle count stepped (0 when nothing ran)."""
+        if dt is not None:
+            self.dt = float(dt)
+        if self._emitters:
+            if self.currentPosition.getElementCount() > 0:
+                self._copy_out()
+            for e in self._emitters:
+                e.advance2(self.dt)
+            parts = [e for e in self._emitters if e.currentPosition.getElementCount() > 0]
+            total = sum(e.currentPosition.getElementCount() for e in parts)
+            if total > 0:
+                self.currentPosition.setValue(torch.cat([e.currentPosition.getValue() for e in parts]).contiguous())
+                self.currentVelocity.setValue(torch.cat([e.currentVelocity.getValue() for e in parts]).contiguous())
+                self.currentForce.setValue(torch.cat([e.currentForce.getValue() for e in parts]).contiguous())
+        else:
+            total = self.currentPosition.getElementCount()
+        if total > 0 and self.self_update:
+            f = self.currentForce.getValue()
+            if f is not None:
+                f.zero_()  # ParticleIntegrator::begin resets the force density
+            self.model.step(self.dt)
+            return total
+        return 0
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(fctx, x0, v0, ctx, nsteps, checkpoint_every, scene_offsets):
