@@ -32,7 +32,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 2
+#define PBF_ABI_VERSION 3
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -85,6 +85,11 @@ typedef struct pbf_params {
   float normal_friction;   /* BoundaryConstraint m_normal_friction (0.95) */
   float tangent_friction;  /* BoundaryConstraint m_tangent_friction (0) */
   unsigned flags;          /* PBF_FLAG_* (0: the plain PBD step) */
+  int stage_min_particles; /* searches over at least this many particles let
+                              each wavefront's PBD passes read the neighbors in
+                              its own z-plane rows from a copy in LDS (staged
+                              lists); 0 (default) = 262144, < 0 = never. The
+                              results are identical either way */
 } pbf_params;
 
 /* pbf_params.flags */
@@ -113,6 +118,9 @@ typedef struct pbf_stats {
                                     bucket per scene */
   long long candidates;          /* candidate pairs the last search tested (every
                                     particle of the 27 stencil cells) */
+  long long staged_waves;        /* wavefronts of the last search whose PBD passes read
+                                    the neighbors of their own z-plane rows from a
+                                    copy in LDS */
   int grid_dims[3];              /* cells per axis (GridHash::setSpace) */
   float grid_lo[3];              /* grid origin = scene_lo - 2 cells */
   float cell_size;

@@ -47,6 +47,7 @@ class pbf_params(C.Structure):
         ("normal_friction", C.c_float),
         ("tangent_friction", C.c_float),
         ("flags", C.c_uint),
+        ("stage_min_particles", C.c_int),
     ]
 
 
@@ -62,6 +63,7 @@ class pbf_stats(C.Structure):
         ("list_slots_reserved", C.c_longlong),
         ("cells_used", C.c_longlong),
         ("candidates", C.c_longlong),
+        ("staged_waves", C.c_longlong),
         ("grid_dims", C.c_int * 3),
         ("grid_lo", C.c_float * 3),
         ("cell_size", C.c_float),

@@ -35,7 +35,6 @@ struct pbf_ctx {
   long long cell_cap = 0;
   int* cell_cnt = nullptr;    // cell_cap + 1
   int* cell_start = nullptr;  // cell_cap + 2
-  uint8_t* oog_flag = nullptr;
 
   // particle arrays (capacity cap_n)
   int cap_n = 0;
@@ -50,6 +49,10 @@ struct pbf_ctx {
   long long* list_base = nullptr;
   int* list_jbase = nullptr;
   int* list_mode = nullptr;
+  int* stage = nullptr;         // per wave: kStageInts ints (staged lists)
+  uint16_t* cnt_a = nullptr;    // per sorted particle: neighbors in the stage part (staged lists)
+  int* oog_rcnt = nullptr;      // per sorted particle: out-of-grid particles that list it
+  int* oog_rev = nullptr;       // kOogRev per sorted particle: those particles
   uint32_t* mask = nullptr;
   long long mask_cap = 0;  // words
   uint16_t* list = nullptr;
@@ -94,8 +97,8 @@ struct pbf_ctx {
     int *perm = nullptr, *ssid = nullptr, *nbr_cnt = nullptr;
     float4 *xn = nullptr, *x0 = nullptr;
     long long *list_base = nullptr, *mask_base = nullptr;
-    int *list_jbase = nullptr, *list_mode = nullptr;
-    uint16_t* list = nullptr;
+    int *list_jbase = nullptr, *list_mode = nullptr, *stage = nullptr;
+    uint16_t *cnt_a = nullptr, *list = nullptr;
     uint32_t* mask = nullptr;
     int* cell_start = nullptr;
     SceneWin* win = nullptr;
@@ -338,11 +341,9 @@ int ensure_scenes(pbf_ctx* c, int nscenes) {
   if (need_cells > c->cell_cap) {
     dfree(c->cell_cnt);
     dfree(c->cell_start);
-    dfree(c->oog_flag);
     c->cell_cap = 0;
     PBF_HIP(c, hipMalloc(&c->cell_cnt, sizeof(int) * ((size_t)need_cells + 1)));
     PBF_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * ((size_t)need_cells + 2)));
-    PBF_HIP(c, hipMalloc(&c->oog_flag, (size_t)need_cells + 1));
     c->cell_cap = need_cells;
     c->last_n = -1;
   }
@@ -405,7 +406,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
-  dfree(c->wave_stat);
+  dfree(c->stage); dfree(c->cnt_a); dfree(c->wave_stat); dfree(c->oog_rcnt); dfree(c->oog_rev);
   c->cap_n = 0;
   const size_t ni = sizeof(int) * (size_t)cap, nf4 = sizeof(float4) * (size_t)cap;
   PBF_HIP(c, hipMalloc(&c->key, ni));
@@ -432,6 +433,10 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   PBF_HIP(c, hipMalloc(&c->list_base, sizeof(long long) * nwv));
   PBF_HIP(c, hipMalloc(&c->list_jbase, sizeof(int) * nwv));
   PBF_HIP(c, hipMalloc(&c->list_mode, sizeof(int) * nwv));
+  PBF_HIP(c, hipMalloc(&c->stage, sizeof(int) * kStageInts * nwv));
+  PBF_HIP(c, hipMalloc(&c->cnt_a, sizeof(uint16_t) * (size_t)cap));
+  PBF_HIP(c, hipMalloc(&c->oog_rcnt, ni));
+  PBF_HIP(c, hipMalloc(&c->oog_rev, ni * kOogRev));
   PBF_HIP(c, hipMalloc(&c->wave_stat, sizeof(int4) * nwv));
   c->cap_n = cap;
   st = ensure_mask(c, mask_words_for(c, cap));
@@ -475,10 +480,13 @@ NbrView view_of(const pbf_ctx* c, int n) {
   nv.list_base = c->list_base;
   nv.list_jbase = c->list_jbase;
   nv.list_mode = c->list_mode;
+  nv.stage = c->stage;
+  nv.cnt_a = c->cnt_a;
+  nv.oog_rcnt = c->oog_rcnt;
+  nv.oog_rev = c->oog_rev;
   nv.list = c->list;
   nv.mask_base = c->mask_base;
   nv.mask = c->mask;
-  nv.oog_flag = c->oog_flag;
   nv.counters = c->counters;
   nv.thr2 = c->pc.thr2;
   nv.n = n;
@@ -496,8 +504,12 @@ NbrOut out_of(pbf_ctx* c) {
   o.list_base = c->list_base;
   o.list_jbase = c->list_jbase;
   o.list_mode = c->list_mode;
+  o.stage = c->stage;
+  o.stage_lists = 0;
+  o.oog_rcnt = c->oog_rcnt;
+  o.oog_rev = c->oog_rev;
+  o.cnt_a = c->cnt_a;
   o.nbr_cnt = c->nbr_cnt;
-  o.oog_flag = c->oog_flag;
   o.wave_stat = c->wave_stat;
   return o;
 }
@@ -514,7 +526,6 @@ SortBufs sort_of(pbf_ctx* c, const int* scene_off, int nscenes) {
   sb.cell_cap = c->cell_cap;
   sb.cell_cnt = c->cell_cnt;
   sb.cell_start = c->cell_start;
-  sb.oog_flag = c->oog_flag;
   sb.key = c->key;
   sb.slot = c->slot;
   sb.tmp = c->tmp;
@@ -522,6 +533,7 @@ SortBufs sort_of(pbf_ctx* c, const int* scene_off, int nscenes) {
   sb.scan_part = c->scan_part;
   sb.bpart = c->bpart;
   sb.xc = c->xc;
+  sb.oog_rcnt = c->oog_rcnt;
   sb.xc_stride = c->cap_n + kXnPad;
   return sb;
 }
@@ -589,7 +601,10 @@ int search(pbf_ctx* c, const float* x, const float* v, int n, const int* scene_o
   }
   {
     PassTimer t(c, PBF_PASS_NEIGHBORS, st);
-    launch_neighbors(view_of(c, n), out_of(c), c->counters, st);
+    NbrOut o = out_of(c);
+    const int smin = c->p.stage_min_particles == 0 ? kStageMinParticles : c->p.stage_min_particles;
+    o.stage_lists = (smin > 0 && n >= smin) ? 1 : 0;
+    launch_neighbors(view_of(c, n), o, c->counters, st);
   }
   c->stats_waves = (n + 63) / 64;
   PBF_LAUNCHED(c);
@@ -724,7 +739,7 @@ template <class T> int ensure_buf(pbf_ctx* c, T*& p, size_t& cap, size_t need) {
 
 void free_slot(pbf_ctx::TapeSlot& t) {
   dfree(t.perm); dfree(t.ssid); dfree(t.nbr_cnt); dfree(t.xn); dfree(t.x0);
-  dfree(t.list_base); dfree(t.mask_base); dfree(t.list_jbase); dfree(t.list_mode);
+  dfree(t.list_base); dfree(t.mask_base); dfree(t.list_jbase); dfree(t.list_mode); dfree(t.stage); dfree(t.cnt_a);
   dfree(t.list); dfree(t.mask); dfree(t.cell_start); dfree(t.win); dfree(t.counters); dfree(t.it);
 }
 
@@ -762,6 +777,8 @@ int ensure_tape(pbf_ctx* c, int m) {
     PBF_HIP(c, hipMalloc(&t.mask_base, sizeof(long long) * nwv));
     PBF_HIP(c, hipMalloc(&t.list_jbase, sizeof(int) * nwv));
     PBF_HIP(c, hipMalloc(&t.list_mode, sizeof(int) * nwv));
+    PBF_HIP(c, hipMalloc(&t.stage, sizeof(int) * kStageInts * nwv));
+    PBF_HIP(c, hipMalloc(&t.cnt_a, sizeof(uint16_t) * cap));
     PBF_HIP(c, hipMalloc(&t.list, sizeof(uint16_t) * std::max<size_t>(c->slot_list, 1)));
     PBF_HIP(c, hipMalloc(&t.mask, sizeof(uint32_t) * std::max<size_t>(c->slot_mask, 1)));
     PBF_HIP(c, hipMalloc(&t.cell_start, sizeof(int) * ((size_t)c->cell_cap + 2)));
@@ -793,6 +810,8 @@ void swap_slot(pbf_ctx* c, pbf_ctx::TapeSlot& t) {
   std::swap(c->mask_base, t.mask_base);
   std::swap(c->list_jbase, t.list_jbase);
   std::swap(c->list_mode, t.list_mode);
+  std::swap(c->stage, t.stage);
+  std::swap(c->cnt_a, t.cnt_a);
   std::swap(c->list, t.list);
   std::swap(c->mask, t.mask);
 }
@@ -807,6 +826,8 @@ NbrView slot_view(const pbf_ctx* c, const pbf_ctx::TapeSlot& t, int n) {
   nv.list_base = t.list_base;
   nv.list_jbase = t.list_jbase;
   nv.list_mode = t.list_mode;
+  nv.stage = t.stage;
+  nv.cnt_a = t.cnt_a;
   nv.list = t.list;
   nv.mask_base = t.mask_base;
   nv.mask = t.mask;
@@ -865,6 +886,7 @@ void pbf_params_default(pbf_params* p) {
   p->normal_friction = 0.95f;
   p->tangent_friction = 0.0f;
   p->flags = 0;
+  p->stage_min_particles = 0;
 }
 
 int pbf_create(const pbf_params* p, pbf_ctx** out) {
@@ -912,12 +934,12 @@ int pbf_destroy(pbf_ctx* c) {
   if (!c) return PBF_ERR_INVALID;
   (void)hipDeviceSynchronize();
   dfree(c->off1); dfree(c->scene_minmax); dfree(c->win); dfree(c->total_cells); dfree(c->scan_part);
-  dfree(c->cell_cnt); dfree(c->cell_start); dfree(c->oog_flag);
+  dfree(c->cell_cnt); dfree(c->cell_start);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
-  dfree(c->wave_stat);
+  dfree(c->stage); dfree(c->cnt_a); dfree(c->wave_stat); dfree(c->oog_rcnt); dfree(c->oog_rev);
   dfree(c->mask); dfree(c->list); dfree(c->scan_scratch); dfree(c->bf.d);
   free_slots(c);
   dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.adj);
@@ -1291,6 +1313,7 @@ int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
   out->list_slots_reserved = c->list_cap;
   out->cells_used = (long long)c->h_counters[C_CELLS];
   out->candidates = (long long)c->h_counters[C_CANDIDATES];
+  out->staged_waves = (long long)c->h_counters[C_STAGED];
   out->grid_dims[0] = c->g.nx;
   out->grid_dims[1] = c->g.ny;
   out->grid_dims[2] = c->g.nz;

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <climits>
 #include <cstdint>
 
 #include "pbf_internal.h"
@@ -83,6 +84,31 @@ __device__ __forceinline__ int wave_sum(int v) {
   for (int d = 32; d > 0; d >>= 1) v += __shfl_xor(v, d);
   return v;
 }
+// Wave-wide minimum / maximum with every lane active, in scalar registers:
+// four DPP row shifts (1, 2, 4, 8; lanes shifted in from outside the row keep
+// the identity) leave each 16-lane row's result in its lane 15, and four lane
+// reads combine the rows. Cheaper than the six exchanges of wave_min/wave_max.
+template <int kCtrl>
+__device__ __forceinline__ int dpp_row_shr(int identity, int v) {
+  return __builtin_amdgcn_update_dpp(identity, v, kCtrl, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int wave_min_full(int v) {
+  v = min(v, dpp_row_shr<0x111>(INT_MAX, v));
+  v = min(v, dpp_row_shr<0x112>(INT_MAX, v));
+  v = min(v, dpp_row_shr<0x114>(INT_MAX, v));
+  v = min(v, dpp_row_shr<0x118>(INT_MAX, v));
+  return min(min(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+             min(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
+}
+__device__ __forceinline__ int wave_max_full(int v) {
+  v = max(v, dpp_row_shr<0x111>(INT_MIN, v));
+  v = max(v, dpp_row_shr<0x112>(INT_MIN, v));
+  v = max(v, dpp_row_shr<0x114>(INT_MIN, v));
+  v = max(v, dpp_row_shr<0x118>(INT_MIN, v));
+  return max(max(__builtin_amdgcn_readlane(v, 15), __builtin_amdgcn_readlane(v, 31)),
+             max(__builtin_amdgcn_readlane(v, 47), __builtin_amdgcn_readlane(v, 63)));
+}
+
 __device__ __forceinline__ int wave_inclusive_scan(int v) {
   const int lane = threadIdx.x & 63;
 #pragma unroll

@@ -64,31 +64,65 @@ struct VisConst {
 
 enum Counter {
   C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_CANDIDATES, C_LIST_NEED,
-  C_COUNT
+  C_STAGED, C_COUNT
 };
 
+// Runs are visited in the order q = 0..8 of run_of(q): first the three rows of
+// the particle's own z-plane (dk = 0), which hold most of its neighbors, then
+// the other six.
+__host__ __device__ constexpr int run_of(int q) { return q < 3 ? q + 3 : (q < 6 ? q - 3 : q); }
+
+// List encodings (list_mode bits 0-1). Particles are processed in wavefronts
+// of 64 consecutive sorted indices; a wave's neighbor list holds, in groups of
+// four per lane:
+//   kListDelta16: 16-bit j - list_jbase;
+//   kListIndex32: 32-bit j (the wave's candidates span more than 65536 indices);
+//   kListStaged:  the neighbors in the first three runs (q < 3) as 16-bit slots
+//                 of the wave's stage, padded with kStagePad to kA entries on
+//                 every lane, then the others as 16-bit j - list_jbase. The
+//                 stage concatenates the union over the wave's lanes of run q,
+//                 q = 0..2: slot P_q + (j - A_q) for j in interval q.
+// A staged wave's PBD passes copy the stage (at most kStageCap records) into
+// LDS once and read those neighbors from there.
+constexpr int kListDelta16 = 0, kListIndex32 = 1, kListStaged = 2;
+constexpr int kStageCap = 344;
+constexpr uint32_t kStagePad = kStageCap;  // the stage slot of the padding particle
+// Per wave, stage[kStageInts * wave + ...]: A_0, A_1, A_2, P_0, P_1, P_2, P_3 (the
+// stage's length), kA (a multiple of 8).
+constexpr int kStageInts = 8;
+// Default of pbf_params.stage_min_particles: searches over fewer particles
+// keep unstaged lists (their passes run about one round of waves,
+// latency-bound, and the copy into LDS lengthens each wave).
+constexpr int kStageMinParticles = 1 << 18;
+
+// Out-of-grid particles that list one hashed particle, kept per particle for
+// the displacement pass (a particle listed by more takes a scan of the scene's
+// out-of-grid particles instead).
+constexpr int kOogRev = 8;
+
 // Cell-sorted view used by the neighbor pass and every pass over its lists.
-// Particles are processed in wavefronts of 64 consecutive sorted indices; a
-// wave's neighbor list holds 16-bit j - list_jbase (or 32-bit j when the
-// wave's candidates span more than 65536 indices), in groups of four per lane.
 struct NbrView {
   GridDev g;
   const SceneWin* win;         // per scene
   const int* ssid;             // scene of each sorted particle
   const int* cell_start;       // cell_start[c]..cell_start[c+1]: sorted range of compact cell c
-  const float4* xn;
+  const float4* xn;            // sorted positions the neighbor sets are defined on
   const float* cx;             // sorted x^n coordinate planes (the neighbor pass reads
   const float* cy;             // candidates from these: one 16-byte load brings the
-  const float* cz;             // same coordinate of four consecutive candidates)            // sorted positions the neighbor sets are defined on
+  const float* cz;             // same coordinate of four consecutive candidates)
   const int* nbr_cnt;          // neighbors per sorted particle
+  const uint16_t* cnt_a;       // staged waves: neighbors per particle in the first three runs
   const long long* list_base;  // per wave: first 16-bit slot of its list, -1 = no list
-  const int* list_jbase;       // per wave: index base of 16-bit entries
-  const int* list_mode;        // per wave: bit 0: 0 = 16-bit deltas, 1 = 32-bit indices;
-                               // bits 1..: groups of four that every lane of the wave fills
+  const int* list_jbase;       // per wave: index base of 16-bit offsets
+  const int* list_mode;        // per wave: bits 0-1: the encoding (kList*); bits 2..: groups of
+                               // four that every lane of the wave fills (kListStaged: of the
+                               // part after the stage slots)
+  const int* stage;            // per wave: the stage (kStageInts ints; kListStaged only)
   const uint16_t* list;
   const long long* mask_base;  // per wave: first word of its global bit masks, -1 = none
   const uint32_t* mask;        // wave-interleaved bit masks over the cell runs
-  const uint8_t* oog_flag;     // per compact cell: an out-of-grid particle lists it
+  const int* oog_rcnt;         // per sorted particle: out-of-grid particles that list it
+  const int* oog_rev;          // their sorted indices, kOogRev slots per particle (in no order)
   const unsigned long long* counters;
   float thr2;
   int n;
@@ -110,9 +144,13 @@ struct NbrOut {
   long long* list_base;
   int* list_jbase;
   int* list_mode;
+  int* stage;
+  int stage_lists;      // 1: waves that allow it write kListStaged lists
   int* nbr_cnt;
-  uint8_t* oog_flag;
-  int4* wave_stat;      // per wave: neighbor pairs, candidates, list slots per lane it needs
+  uint16_t* cnt_a;
+  int* oog_rcnt;        // zeroed by the sort; out-of-grid lanes count themselves into their neighbors'
+  int* oog_rev;
+  int4* wave_stat;      // per wave: neighbor pairs, candidates, list slots per lane it needs, staged (0/1)
 };
 
 // Sorting state of a step (device arrays).
@@ -127,7 +165,6 @@ struct SortBufs {
   long long cell_cap;     // allocated compact cells
   int* cell_cnt;          // cell_cap + 1
   int* cell_start;        // cell_cap + 2
-  uint8_t* oog_flag;      // cell_cap + 1
   int* key;
   int* slot;
   int* tmp;
@@ -136,6 +173,7 @@ struct SortBufs {
   int* bpart;             // per block of the bounds pass: scene and cell box (bounds_part_ints)
   float* xc;              // sorted x^n as three planes (x, y, z), plane stride xc_stride
   int xc_stride;
+  int* oog_rcnt;          // per sorted particle, zeroed by the reorder
 };
 
 constexpr int kBlock = 256;

@@ -4,9 +4,10 @@
 // A particle's 27 stencil cells form 9 runs of 3 x-consecutive cells; in cell
 // order each run is one contiguous index range [a, a+len). The neighbor pass
 // keeps every particle's neighbors as a wave-interleaved list (16-bit offsets
-// from a per-wave base, or 32-bit indices), with bit masks over the runs or a
-// re-test of the candidates as fallbacks when the reservations are full.
-// Neighbor order is ascending sorted index on every path.
+// from a per-wave base, stage slots, or 32-bit indices), with bit masks over
+// the runs or a re-test of the candidates as fallbacks when the reservations
+// are full. Neighbor order on every path: runs in visit order (run_of), then
+// ascending sorted index.
 #pragma once
 #include <type_traits>
 
@@ -31,13 +32,14 @@ __device__ __forceinline__ void stencil_x(const GridDev& g, int fi, float frac, 
   }
 }
 
-// Run r (0..8) of sort cell (fi,cj,ck): half cells fi+xlo..fi+xhi of row
-// (cj+dj, ck+dk) with dk = r/3-1, dj = r%3-1 (so runs come in ascending
-// sorted index), clipped to the scene's window (cells outside it hold no
+// The q-th run visited (q = 0..8) of sort cell (fi,cj,ck): half cells
+// fi+xlo..fi+xhi of row (cj+dj, ck+dk) with dk = r/3-1, dj = r%3-1 for
+// r = run_of(q), clipped to the scene's window (cells outside it hold no
 // particle, and the window lies inside the grid, so this is the reference's
 // in-grid stencil).
 __device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, int fi, int cj,
-                                          int ck, int xlo, int xhi, int r, int& a, int& len) {
+                                          int ck, int xlo, int xhi, int q, int& a, int& len) {
+  const int r = run_of(q);
   const int lj = cj + (r % 3) - 1 - w.lo[1], lk = ck + (r / 3) - 1 - w.lo[2];
   a = 0;
   len = 0;
@@ -169,6 +171,42 @@ __device__ __forceinline__ Group4 decode16(uint2 e, int jb, int b, int cnt, int 
   }
   return g;
 }
+// The stage of a staged wave (kListStaged): slot t of interval q holds sorted
+// index t + D_q, D_q = A_q - P_q; interval q covers slots [P_q, P_{q+1}).
+struct StageMap {
+  int d0, p1, p2, inc1, inc2;  // inc_q = D_q - D_{q-1}
+  __device__ __forceinline__ void load(const int* __restrict__ st) {
+    const int d1 = st[1] - st[4], d2 = st[2] - st[5];
+    d0 = st[0] - st[3];
+    p1 = st[4];
+    p2 = st[5];
+    inc1 = d1 - d0;
+    inc2 = d2 - d1;
+  }
+  // the sorted index of slot t (t below the stage's length): D of the last
+  // interval starting at or before t (an empty interval starts where the next
+  // one does)
+  __device__ __forceinline__ int index_of(int t) const {
+    return t + d0 + (t >= p1 ? inc1 : 0) + (t >= p2 ? inc2 : 0);
+  }
+};
+
+// decode16 of stage slots into sorted indices.
+template <bool kFull>
+__device__ __forceinline__ Group4 decode16_staged(uint2 e, const StageMap& m, int b, int cnt, int past) {
+  const int k = b << 2;
+  Group4 g;
+  g.j[0] = m.index_of((int)(e.x & 0xffffu));
+  g.j[1] = m.index_of((int)(e.x >> 16));
+  g.j[2] = m.index_of((int)(e.y & 0xffffu));
+  g.j[3] = m.index_of((int)(e.y >> 16));
+  if (!kFull) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g.j[u] = k + u < cnt ? g.j[u] : past;
+  }
+  return g;
+}
+
 template <bool kFull>
 __device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int past) {
   const int k = b << 2;
@@ -303,13 +341,27 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
     const int cnt = nv.nbr_cnt[s];
     const int past = kPadded ? nv.pad : s;
     const int lm = nv.list_mode[wave];
-    const int nfull = lm >> 1;
-    if ((lm & 1) == 0) {
+    const int nfull = lm >> 2;
+    if ((lm & 3) == kListDelta16) {
       const uint4* L = reinterpret_cast<const uint4*>(nv.list + lb) + lane;
       const int jb = nv.list_jbase[wave];
       list_loop_pairs<kPadded>(
           cnt, nfull, L,
           [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cnt, past); }, gather,
+          body);
+    } else if ((lm & 3) == kListStaged) {
+      // stage slots of the first three runs (then pads up to kA), then offsets
+      const uint4* L = reinterpret_cast<const uint4*>(nv.list + lb) + lane;
+      const int* st = nv.stage + kStageInts * wave;
+      StageMap m;
+      m.load(st);
+      const int ka = st[7], ca = nv.cnt_a[s], cb = cnt - ca;
+      list_loop_pairs<kPadded>(
+          ca, 0, L, [&](uint2 e, int b, auto) { return decode16_staged<false>(e, m, b, ca, past); }, gather, body);
+      const int jb = nv.list_jbase[wave];
+      list_loop_pairs<kPadded>(
+          cb, nfull, L + ((long long)(ka >> 3) << 6),
+          [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cb, past); }, gather,
           body);
     } else {
       const int4* L = reinterpret_cast<const int4*>(nv.list + lb) + lane;
@@ -327,9 +379,9 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
   const SceneWin win = nv.win[nv.ssid[s]];
   const long long mb = nv.mask_base[wave];
   int wofs = 0;
-  for (int r = 0; r < 9; ++r) {
+  for (int q = 0; q < 9; ++q) {
     int ra, rl;
-    run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, r, ra, rl);
+    run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, q, ra, rl);
     const int nw = (rl + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
       const int jb = ra + (w << 5);
@@ -349,6 +401,55 @@ template <bool kW, bool kPadded = false, class Body>
 __device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const float4* __restrict__ xs,
                                                     Body&& body) {
   for_each_neighbor<kPadded>(nv, s, [&](int j) { return load_pos<kW>(xs, j); }, body);
+}
+
+// LDS records a wave needs for for_each_neighbor_staged.
+constexpr int kStageRecords = kStageCap + 1;
+
+// for_each_neighbor_x<kW, true> for the PBD passes: a staged wave copies its
+// stage from xs into its LDS region buf (kStageRecords records; coalesced
+// loads) and reads the neighbors of its first three runs from there (the
+// padding particle sits at slot kStagePad), so only the others are gathered
+// through the vector-memory path. Other waves take for_each_neighbor_x. The
+// body receives stage slots or sorted indices as j (the PBD bodies ignore it).
+// The sums run in the same neighbor order as on every other path.
+template <bool kW, class Body>
+__device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int s, const float4* __restrict__ xs,
+                                                         float4* buf, Body&& body) {
+  const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
+  const long long lb = nv.list_base[wave];
+  const int lm = lb >= 0 ? nv.list_mode[wave] : 0;
+  if (lb < 0 || (lm & 3) != kListStaged) {
+    for_each_neighbor_x<kW, true>(nv, s, xs, body);
+    return;
+  }
+  const int* st = nv.stage + kStageInts * wave;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int a = st[q], p0 = st[3 + q], len = st[4 + q] - p0;
+    for (int t = lane; t < len; t += 64) buf[p0 + t] = xs[a + t];
+  }
+  if (lane == 0) buf[kStagePad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
+  // LDS operations of one wave complete in order: keep the reads below after
+  // the writes above
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+  const int ka = st[7];
+  const uint4* L = reinterpret_cast<const uint4*>(nv.list + lb) + lane;
+  // stage part: ka entries on every lane (pads read the padding particle)
+  list_loop_pairs<true>(
+      ka, ka >> 2, L, [&](uint2 e, int b, auto) { return decode16<true>(e, 0, b, ka, 0); },
+      [&](int t) {
+        const float4 v = buf[t];
+        return kW ? v : make_float4(v.x, v.y, v.z, 0.0f);
+      },
+      body);
+  const int cb = nv.nbr_cnt[s] - nv.cnt_a[s];
+  const int jb = nv.list_jbase[wave];
+  list_loop_pairs<true>(
+      cb, lm >> 2, L + ((long long)(ka >> 3) << 6),
+      [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cb, nv.pad); },
+      [&](int j) { return load_pos<kW>(xs, j); }, body);
 }
 
 }  // namespace pbf

@@ -76,41 +76,57 @@ struct ListWriter {
 };
 
 // Phase 2 over LDS words: each word sits with the list value of its first
-// candidate ({bits, base}); one flat loop over the lane's cnt entries pops the
-// next set bit and moves to the next word when one runs empty, so the wave
-// runs as many iterations as its largest count.
+// candidate ({bits, base}); a flat loop over the lane's entries pops the next
+// set bit and moves to the next word when one runs empty, so the wave runs as
+// many iterations as its largest count. Part 1 takes n1 entries from the words
+// from 0 on, then pads are pushed up to pad_to entries, then part 2 takes n2
+// entries from the words from w2 on. The base of a word: 32-bit j (kMode 1),
+// or two 16-bit values: j - list_jbase (low half) and the stage slot of j (high
+// half, the first three runs of a staged wave); part 1 reads the high half
+// when slot1 is set.
 template <int kMode>
-__device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int lane, const uint2* lw, int cnt) {
+__device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int lane, const uint2* lw, int n1,
+                                               bool slot1, int pad_to, int w2, int n2) {
   ListWriter<kMode> wr{list, lb, lane};
-  uint32_t bits = 0u;
-  int jb = 0, wi = 0;
-  for (int k = 0; k < cnt; ++k) {
-    while (bits == 0u) {  // the remaining words hold cnt - k > 0 set bits
-      const uint2 e = lw[wi << 6];
-      bits = e.x;
-      jb = (int)e.y;
-      ++wi;
+  auto part = [&](int wi, int n, bool hi) {
+    uint32_t bits = 0u, jb = 0u;
+    for (int k = 0; k < n; ++k) {
+      while (bits == 0u) {  // the remaining words hold n - k > 0 set bits
+        const uint2 e = lw[wi << 6];
+        bits = e.x;
+        jb = kMode == 1 ? e.y : (hi ? e.y >> 16 : e.y & 0xffffu);
+        ++wi;
+      }
+      wr.push(jb + (uint32_t)pop_first(bits));
     }
-    wr.push((uint32_t)(jb + pop_first(bits)));
-  }
+  };
+  part(0, n1, slot1);
+  for (int k = n1; k < pad_to; ++k) wr.push(kStagePad);
+  part(w2, n2, false);
   wr.finish();
 }
 
-// Phase 2 over words in the global mask reservation, run by run.
+// Phase 2 over words in the global mask reservation, run by run: candidate j
+// of run q is written as j - sdel[q] (the stage slot) for q < 3 of a staged
+// wave, followed by pads up to kA entries, else as j - jbase.
 template <int kMode>
 __device__ __forceinline__ void write_list_global(uint16_t* list, long long lb, int lane, const int (&ra)[9],
-                                                  const int (&rl)[9], const uint32_t* gcol, int jbase) {
+                                                  const int (&rl)[9], const uint32_t* gcol, int jbase, bool staged,
+                                                  const int (&sdel)[3], int kA) {
   ListWriter<kMode> wr{list, lb, lane};
   int wofs = 0;
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int nw = (rl[r] + 31) >> 5;
+  for (int q = 0; q < 9; ++q) {
+    const int nw = (rl[q] + 31) >> 5;
+    const int base = (staged && q < 3) ? sdel[q < 3 ? q : 0] : jbase;
     for (int w = 0; w < nw; ++w) {
       uint32_t bits = gcol[(long long)(wofs + w) << 6];
-      const int jb = ra[r] + (w << 5) - jbase;
+      const int jb = ra[q] + (w << 5) - base;
       while (bits) wr.push((uint32_t)(jb + pop_first(bits)));
     }
     wofs += nw;
+    if (staged && q == 2)
+      while (wr.k < kA) wr.push(kStagePad);
   }
   wr.finish();
 }
@@ -118,6 +134,12 @@ __device__ __forceinline__ void write_list_global(uint16_t* list, long long lb, 
 // One wave per block: phase 1 tests the candidates of the 9 runs (a lane's
 // words in LDS, or in the global mask reservation when they do not fit),
 // phase 2 writes the list into the wave's fixed region.
+//
+// Stage: the union over the lanes of each of the first three runs (the rows of
+// the particle's own z-plane) is one index interval [A_q, E_q); a full wave
+// whose three intervals hold at most kStageCap particles together (and contain
+// every lane's runs, checked) writes those neighbors as stage slots, so that
+// the PBD passes read them from LDS.
 __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsigned long long* __restrict__ counters) {
   __shared__ uint2 lword[kNbLdsWords * 64];
   const int wave = xcd_block(blockIdx.x, gridDim.x);
@@ -127,6 +149,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   float4 xi = make_float4(0.f, 0.f, 0.f, 0.f);
   int ra[9], rl[9];
   int words = 0, cand = 0, jlo = INT_MAX, jhi = -1;
+  bool oog_lane = false;
   if (active) {
     xi = nv.xn[s];
     int ci, cj, ck, xlo, xhi;
@@ -135,36 +158,48 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     stencil_x(nv.g, ci, frac, xlo, xhi);
     const SceneWin win = nv.win[nv.ssid[s]];
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, r, ra[r], rl[r]);
-      words += (rl[r] + 31) >> 5;
-      cand += rl[r];
-      if (rl[r] > 0) {
-        jlo = min(jlo, ra[r]);
-        jhi = max(jhi, ra[r] + rl[r] - 1);
+    for (int q = 0; q < 9; ++q) {
+      run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, q, ra[q], rl[q]);
+      words += (rl[q] + 31) >> 5;
+      cand += rl[q];
+      if (rl[q] > 0) {
+        jlo = min(jlo, ra[q]);
+        jhi = max(jhi, ra[q] + rl[q] - 1);
       }
     }
     // An out-of-grid particle is not hashed (GridHash.cu:85,194) yet queries,
     // so each in-grid j it lists receives -dp_kj from it (DensityPBD.cu:162)
-    // without listing it back: flag the cells of its stencil for pass B.
-    if (!in_window(win, ci, cj, ck)) {  // the window holds every in-grid cell of the scene
-      const int h0 = 2 * ((ci >> 1) - 1);  // the half cells of its reference cells
-      for (int dk = -1; dk <= 1; ++dk)
-        for (int dj = -1; dj <= 1; ++dj)
-          for (int fx = h0; fx < h0 + 6; ++fx)
-            if (in_window(win, fx, cj + dj, ck + dk)) out.oog_flag[window_cell(win, fx, cj + dj, ck + dk)] = 1;
-    }
+    // without listing it back (registered below for pass B). The window holds
+    // every in-grid cell of the scene.
+    oog_lane = !in_window(win, ci, cj, ck);
   } else {
 #pragma unroll
-    for (int r = 0; r < 9; ++r) {
-      ra[r] = 0;
-      rl[r] = 0;
+    for (int q = 0; q < 9; ++q) {
+      ra[q] = 0;
+      rl[q] = 0;
     }
   }
   const int wmax = wave_max(words);
   const int wjlo = wave_min(jlo), wjhi = wave_max(jhi);
-  const int mode = (wjhi >= wjlo && wjhi - wjlo > 65535) ? 1 : 0;
-  const int jbase = mode == 0 ? wjlo : 0;  // list values are j - jbase
+  const int mode = (wjhi >= wjlo && wjhi - wjlo > 65535) ? kListIndex32 : kListDelta16;
+  const int jbase = mode == kListDelta16 ? wjlo : 0;  // list offsets are j - jbase
+  // the stage: intervals [sa[q], sa[q] + slen[q]) at slots sp[q]..; slot = j - sdel[q]
+  int sa[3], sp[4], sdel[3];
+  bool fits = true, inside = true;
+  sp[0] = 0;
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int a = wave_min_full(rl[q] > 0 ? ra[q] : INT_MAX);
+    const int e = wave_max_full(rl[q] > 0 ? ra[q] + rl[q] : INT_MIN);
+    const bool used = e > a;  // every lane's run q empty: a = INT_MAX, e = INT_MIN
+    inside = inside && (rl[q] == 0 || (ra[q] >= a && ra[q] + rl[q] <= e));
+    fits = fits && (!used || (a >= 0 && e - a <= kStageCap));
+    sa[q] = used ? a : 0;
+    sp[q + 1] = sp[q] + ((used && fits) ? e - a : 0);
+    sdel[q] = sa[q] - sp[q];
+  }
+  const bool stage_ok =
+      out.stage_lists && mode == kListDelta16 && fits && sp[3] <= kStageCap && __all(inside ? 1 : 0) && __all(active ? 1 : 0);
   // word storage: this wave's LDS block, else the global mask reservation
   const bool in_lds = wmax <= kNbLdsWords;
   long long mb = -1;
@@ -181,54 +216,108 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   uint32_t* gcol = out.mask + (mb >= 0 ? mb + lane : 0);
   const bool keep = in_lds || mb >= 0;
 
-  // phase 1: test every candidate, keep the words, count
-  int cnt = 0, wofs = 0;
+  // phase 1: test every candidate, keep the words, count (cnt_a: the first
+  // three runs, whose words end at word wofs_a)
+  int cnt = 0, cnt_a = 0, wofs = 0, wofs_a = 0;
 #pragma unroll
-  for (int r = 0; r < 9; ++r) {
-    const int nw = (rl[r] + 31) >> 5;
+  for (int q = 0; q < 9; ++q) {
+    const int nw = (rl[q] + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
-      const int jb = ra[r] + (w << 5);
-      const uint32_t bits = test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[r] - (w << 5)), nv.thr2);
-      if (in_lds)
-        lcol[(wofs + w) << 6] = make_uint2(bits, (uint32_t)(jb - jbase));
-      else if (mb >= 0)
+      const int jb = ra[q] + (w << 5);
+      const uint32_t bits = test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
+      if (in_lds) {
+        const uint32_t base = mode == kListIndex32
+                                  ? (uint32_t)jb
+                                  : (((uint32_t)(jb - jbase) & 0xffffu) |
+                                     (q < 3 ? ((uint32_t)(jb - sdel[q < 3 ? q : 0]) & 0xffffu) << 16 : 0u));
+        lcol[(wofs + w) << 6] = make_uint2(bits, base);
+      } else if (mb >= 0) {
         gcol[(long long)(wofs + w) << 6] = bits;
+      }
       cnt += __popc(bits);
     }
     wofs += nw;
+    if (q == 2) {
+      cnt_a = cnt;
+      wofs_a = wofs;
+    }
+  }
+  // An out-of-grid lane (the scenes' out-of-grid buckets only) registers
+  // itself with each particle it lists, for pass B.
+  if (oog_lane) {
+    int wo = 0;
+    for (int q = 0; q < 9; ++q) {
+      const int nw = (rl[q] + 31) >> 5;
+      for (int w = 0; w < nw; ++w) {
+        const int jb = ra[q] + (w << 5);
+        uint32_t bits = in_lds    ? lcol[(wo + w) << 6].x
+                        : mb >= 0 ? gcol[(long long)(wo + w) << 6]
+                                  : test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
+        while (bits) {
+          const int j = jb + pop_first(bits);
+          const int slot = atomicAdd(&out.oog_rcnt[j], 1);
+          if (slot < kOogRev) out.oog_rev[(size_t)j * kOogRev + slot] = s;
+        }
+      }
+      wo += nw;
+    }
   }
   // entries per lane the region must hold: whole pairs of groups for 16-bit
-  // lists, whole groups for 32-bit ones
+  // lists, whole groups for 32-bit ones; a staged list pads its stage part to
+  // kA entries on every lane
   const int cmax = wave_max(cnt);
-  const int kmax = mode == 0 ? (cmax + 7) & ~7 : (cmax + 3) & ~3;
-  const int nfull = wave_min(active ? cnt : INT_MAX) >> 2;  // groups every active lane fills
+  const int need_plain = mode == kListDelta16 ? (cmax + 7) & ~7 : ((cmax + 3) & ~3) << 1;  // 16-bit slots per lane
+  int kA = 0, need_staged = 0, nfull_b = 0;
+  if (stage_ok) {
+    kA = (wave_max_full(cnt_a) + 7) & ~7;
+    need_staged = kA + ((wave_max_full(cnt - cnt_a) + 7) & ~7);
+    nfull_b = wave_min_full(cnt - cnt_a) >> 2;
+  }
   // The wave's list lives in its own fixed region (no allocation atomic: a
   // same-address atomic per wave serializes in L2); a wave that needs more
-  // than the region takes the mask or re-test path and reports its need.
-  const int need = kmax << mode;  // 16-bit slots per lane
-  const long long lb = (keep && need <= out.list_per) ? (long long)wave * 64 * out.list_per : -1;
+  // than the region takes the mask or re-test path (or the unstaged list) and
+  // reports its need.
+  const bool staged = stage_ok && keep && need_staged <= out.list_per;
+  const int need = stage_ok ? need_staged : need_plain;
+  const int use = staged ? need_staged : need_plain;
+  const long long lb = (keep && use <= out.list_per) ? (long long)wave * 64 * out.list_per : -1;
+  const int nfull = staged ? nfull_b : wave_min(active ? cnt : INT_MAX) >> 2;  // groups every lane fills
   const int tot = wave_sum(cnt), tcand = wave_sum(cand);
   if (lane == 0) {
     out.mask_base[wave] = mb;
     out.list_base[wave] = lb;
     out.list_jbase[wave] = wjlo;
-    out.list_mode[wave] = mode | (nfull << 1);
-    out.wave_stat[wave] = make_int4(tot, tcand, need, 0);
+    out.list_mode[wave] = (staged ? kListStaged : mode) | (nfull << 2);
+    out.wave_stat[wave] = make_int4(tot, tcand, need, staged ? 1 : 0);
+    if (staged) {
+      int* st = out.stage + kStageInts * wave;
+      st[0] = sa[0];
+      st[1] = sa[1];
+      st[2] = sa[2];
+      st[3] = sp[0];
+      st[4] = sp[1];
+      st[5] = sp[2];
+      st[6] = sp[3];
+      st[7] = kA;
+    }
   }
   if (active) out.nbr_cnt[s] = cnt;
+  if (staged) out.cnt_a[s] = (uint16_t)cnt_a;
 
   // phase 2: decode this lane's words into the list, four entries per store
   if (active && lb >= 0) {
     if (in_lds) {
-      if (mode == 0)
-        write_list_lds<0>(out.list, lb, lane, lcol, cnt);
+      if (mode == kListIndex32)
+        write_list_lds<1>(out.list, lb, lane, lcol, cnt, false, 0, 0, 0);
+      else if (staged)
+        write_list_lds<0>(out.list, lb, lane, lcol, cnt_a, true, kA, wofs_a, cnt - cnt_a);
       else
-        write_list_lds<1>(out.list, lb, lane, lcol, cnt);
+        write_list_lds<0>(out.list, lb, lane, lcol, cnt, false, 0, 0, 0);
     } else {
-      if (mode == 0)
-        write_list_global<0>(out.list, lb, lane, ra, rl, gcol, jbase);
+      if (mode == kListIndex32)
+        write_list_global<1>(out.list, lb, lane, ra, rl, gcol, jbase, false, sdel, 0);
       else
-        write_list_global<1>(out.list, lb, lane, ra, rl, gcol, jbase);
+        write_list_global<0>(out.list, lb, lane, ra, rl, gcol, jbase, staged, sdel, kA);
     }
   }
 }
@@ -241,26 +330,29 @@ void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* 
 __global__ __launch_bounds__(1024) void k_wave_stats(const int4* __restrict__ ws, int nwaves,
                                                      unsigned long long* __restrict__ counters) {
   __shared__ unsigned long long sp[16], sc[16];
-  __shared__ int sn[16];
+  __shared__ int sn[16], ss[16];
   unsigned long long pairs = 0, cand = 0;
-  int need = 0;
+  int need = 0, staged = 0;
   for (int w = threadIdx.x; w < nwaves; w += blockDim.x) {
     const int4 v = ws[w];
     pairs += (unsigned long long)v.x;
     cand += (unsigned long long)v.y;
     need = max(need, v.z);
+    staged += v.w;
   }
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
     pairs += __shfl_xor(pairs, d);
     cand += __shfl_xor(cand, d);
     need = max(need, __shfl_xor(need, d));
+    staged += __shfl_xor(staged, d);
   }
   const int wv = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
     sp[wv] = pairs;
     sc[wv] = cand;
     sn[wv] = need;
+    ss[wv] = staged;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -268,10 +360,12 @@ __global__ __launch_bounds__(1024) void k_wave_stats(const int4* __restrict__ ws
       pairs += sp[k];
       cand += sc[k];
       need = max(need, sn[k]);
+      staged += ss[k];
     }
     counters[C_PAIRS] = pairs;
     counters[C_CANDIDATES] = cand;
     counters[C_LIST_NEED] = (unsigned long long)need;
+    counters[C_STAGED] = (unsigned long long)staged;
     counters[C_LIST_SLOTS] = 64ull * (unsigned long long)nwaves * (unsigned long long)need;
   }
 }
@@ -293,9 +387,10 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* _
                                                            float4* __restrict__ aux) {
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= nv.n) return;
+  __shared__ float4 stage_buf[kBlock / 64][kStageRecords];
   const float4 xi = xs[s];
   float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
-  for_each_neighbor_x<false, true>(nv, s, xs, [&](int, float4 xj) {
+  for_each_neighbor_staged<false>(nv, s, xs, stage_buf[threadIdx.x >> 6], [&](int, float4 xj) {
 #pragma clang fp contract(off)
     const PairGeom p = pair_geom(xi, xj, pc.inv_h);
     const float t2 = p.t * p.t;
@@ -348,8 +443,10 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
     cy = fmaf(c, p.dy, cy);
     cz = fmaf(c, p.dz, cz);
   };
+  __shared__ float4 stage_buf[kBlock / 64][kStageRecords];
   float ax = 0.f, ay = 0.f, az = 0.f;
-  for_each_neighbor_x<true, true>(nv, s, xs, [&](int, float4 xj) { pair(xj, ax, ay, az); });
+  for_each_neighbor_staged<true>(nv, s, xs, stage_buf[threadIdx.x >> 6],
+                                 [&](int, float4 xj) { pair(xj, ax, ay, az); });
   float fa = 2.0f;
   float bx = 0.f, by = 0.f, bz = 0.f;
   if (nv.counters[C_OOG] != 0) {  // some scene has out-of-grid particles
@@ -359,11 +456,25 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
     if (s >= o0) {
       fa = 1.0f;  // out-of-grid particle: its own pairs once
     } else if (o1 > o0) {
-      const float4 xni = nv.xn[s];
-      int ci, cj, ck;
-      cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck);
-      if (nv.oog_flag[window_cell(win, ci, cj, ck)]) {
-        // out-of-grid particles k that list s add -dp_ks = dp_sk to s once
+      // out-of-grid particles k that list s add -dp_ks = dp_sk to s once, in
+      // ascending k
+      const int rc = nv.oog_rcnt[s];
+      if (rc <= kOogRev) {
+        const int* rv = nv.oog_rev + (size_t)s * kOogRev;
+        int prev = -1;
+        for (int u = 0; u < rc; ++u) {
+          int k = INT_MAX;
+          for (int w = 0; w < rc; ++w) {
+            const int c = rv[w];
+            k = (c > prev && c < k) ? c : k;
+          }
+          pair(xs[k], bx, by, bz);
+          prev = k;
+        }
+      } else {  // listed by more than kOogRev: scan the scene's out-of-grid particles
+        const float4 xni = nv.xn[s];
+        int ci, cj, ck;
+        cell_of(nv.g, xni.x, xni.y, xni.z, ci, cj, ck);
         for (int k = o0; k < o1; ++k) {
           const float4 xnk = nv.xn[k];
           int ki, kj, kk;

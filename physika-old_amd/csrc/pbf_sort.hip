@@ -351,14 +351,10 @@ __global__ __launch_bounds__(1024) void k_scene_windows(const int* __restrict__ 
   }
 }
 
-__global__ void k_clear_cells(int* __restrict__ cell_cnt, uint8_t* __restrict__ oog_flag,
-                              const int* __restrict__ total_cells) {
+__global__ void k_clear_cells(int* __restrict__ cell_cnt, const int* __restrict__ total_cells) {
   const int total = *total_cells;
   const int stride = gridDim.x * blockDim.x;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += stride) {
-    cell_cnt[c] = 0;
-    oog_flag[c] = 0;
-  }
+  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += stride) cell_cnt[c] = 0;
 }
 
 // K_CalculateParticleNumber (GridHash.cu:78-87): key = compact cell or the
@@ -404,7 +400,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
                                                     int* __restrict__ perm, int* __restrict__ ssid,
                                                     float4* __restrict__ xn,
                                                     float4* __restrict__ x0, float4* __restrict__ vs, PhysConst pc,
-                                                    int integrate, float* __restrict__ xc, int xc_stride) {
+                                                    int integrate, float* __restrict__ xc, int xc_stride,
+                                                    int* __restrict__ oog_rcnt) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) x0[n] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);  // the padding particle
@@ -420,6 +417,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
   }
   perm[s] = i;
   ssid[s] = sid[i];
+  oog_rcnt[s] = 0;
   const float px = x[3 * (size_t)i], py = x[3 * (size_t)i + 1], pz = x[3 * (size_t)i + 2];
   xn[s] = make_float4(px, py, pz, 0.0f);
   xc[s] = px;
@@ -447,7 +445,7 @@ void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const flo
   k_scene_reduce<<<sb.nscenes, 256, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
   k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, counters);
   const int clear_blocks = (int)std::min<long long>((sb.cell_cap + kBlock - 1) / kBlock, 2048);
-  k_clear_cells<<<clear_blocks, kBlock, 0, st>>>(sb.cell_cnt, sb.oog_flag, sb.total_cells);
+  k_clear_cells<<<clear_blocks, kBlock, 0, st>>>(sb.cell_cnt, sb.total_cells);
   k_bin<<<nb, kBlock, 0, st>>>(x, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
   k_scan_dev_reduce<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.total_cells, sb.scan_part);
   k_scan_dev_mid<<<1, 1024, 0, st>>>(sb.scan_part, kScanDevBlocks);
@@ -456,7 +454,7 @@ void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const flo
   PhysConst pc{};
   if (integrate) pc = *integrate;
   k_reorder<<<nb, kBlock, 0, st>>>(sb.key, sb.cell_start, sb.tmp, sb.sid, x, v, n, sb.perm, sb.ssid, xn, x0,
-                                   vs, pc, integrate ? 1 : 0, sb.xc, sb.xc_stride);
+                                   vs, pc, integrate ? 1 : 0, sb.xc, sb.xc_stride, sb.oog_rcnt);
 }
 
 // Copy of cell_start[0..total] (the device-side cell count of the last sort).

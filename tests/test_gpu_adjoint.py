@@ -124,6 +124,29 @@ def test_rollout_adjoint_storage_paths_identical(fluid, scene, reserve):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+def test_rollout_adjoint_staged_lists_identical(fluid, scene):
+    """The taped forward reads staged lists through LDS and the adjoint passes
+    map their stage slots back to indices: the gradients equal those of
+    unstaged lists bit for bit."""
+    say("rollout adjoint, staged vs unstaged lists")
+    # long rows at density 1.28 x rest (lambda active), so that waves stage
+    x0 = scene.jittered_lattice((30, 8, 8), (0.3, 0.5, 0.3), 0.0046, 0.1, seed=23)
+    n = len(x0)
+    c = scene.uniform01(24, 3 * n).reshape(n, 3) - 0.5
+    outs = []
+    for smin in (1, -1):
+        ctx = fluid.Context("cuda:0", list_slots=512, mask_words=64, stage_min_particles=smin)
+        x, v = dev(x0), dev(np.zeros_like(x0))
+        ctx.rollout_forward(x, v, None, 4, 2)
+        if smin == 1:
+            assert ctx.stats()["staged_waves"] > 0
+        xb, vb = dev(c), dev(np.zeros_like(c))
+        ctx.rollout_backward(xb, vb)
+        outs.append((x.cpu().numpy(), xb.cpu().numpy(), vb.cpu().numpy()))
+    for u, w in zip(*outs):
+        np.testing.assert_array_equal(u, w)
+
+
 def test_batched_rollout_gradients_equal_per_scene(fluid, scene):
     say("batched rollout adjoint vs per-scene rollouts, 3 scenes, 4 steps")
     scenes = [compressed_block(scene, (5, 6, 7), 30), compressed_block(scene, (4, 4, 4), 31),

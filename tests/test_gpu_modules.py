@@ -73,6 +73,24 @@ def test_fused_step_with_modules(fluid, oracle, scene, flags):
     assert_step_parity(gpu, *oracle_both(oracle, x, v, 1, flags=flags))
 
 
+def test_fused_step_with_modules_staged_identical(fluid, scene):
+    """Viscosity reads staged lists by mapping their stage slots back to
+    indices: with the box and viscosity, staged and unstaged lists give the
+    same bits."""
+    say("fused step with box and viscosity, staged vs unstaged lists, 2 steps")
+    x = box_scene(scene)
+    v = np.zeros_like(x)
+    v[:, 0] = -0.3
+    outs = []
+    for smin in (1, -1):
+        model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=True, box=((0, 0, 0), (1, 1, 1), 0.005),
+                                              list_slots=320, stage_min_particles=smin)
+        outs.append(model.step_arrays(x, v, 2))
+        assert (model.ctx.stats()["staged_waves"] > 0) == (smin == 1)
+    for u, w in zip(*outs):
+        np.testing.assert_array_equal(u, w)
+
+
 def test_module_path_with_modules(fluid, oracle, scene):
     say("module path with box and viscosity vs oracle, 1 step")
     x = box_scene(scene)
