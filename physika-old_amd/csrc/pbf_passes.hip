@@ -54,6 +54,15 @@ struct ListWriter {
     ++k;
     if ((k & 3) == 0) store((k >> 2) - 1);
   }
+  // Pads (kStagePad) up to entry `target` (a multiple of 8, 16-bit lists):
+  // the open group is completed by pushes, the rest stored as whole groups.
+  __device__ __forceinline__ void pad_to(int target) {
+    while ((k & 3) != 0 && k < target) push(kStagePad);
+    const uint32_t pp = kStagePad | (kStagePad << 16);
+    for (int g = k >> 2; g < (target >> 2); ++g)
+      *reinterpret_cast<uint2*>(list + list_slot(lb, g << 2, lane, 0)) = make_uint2(pp, pp);
+    k = max(k, target);
+  }
   // the last, partial group: entries moved to its start (the slots past the
   // count hold stale offsets of this wave, which no pass reads)
   __device__ __forceinline__ void finish() {
@@ -101,7 +110,7 @@ __device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int
     }
   };
   part(0, n1, slot1);
-  for (int k = n1; k < pad_to; ++k) wr.push(kStagePad);
+  if (kMode == 0) wr.pad_to(pad_to);
   part(w2, n2, false);
   wr.finish();
 }
@@ -125,8 +134,7 @@ __device__ __forceinline__ void write_list_global(uint16_t* list, long long lb, 
       while (bits) wr.push((uint32_t)(jb + pop_first(bits)));
     }
     wofs += nw;
-    if (staged && q == 2)
-      while (wr.k < kA) wr.push(kStagePad);
+    if (kMode == 0 && staged && q == 2) wr.pad_to(kA);
   }
   wr.finish();
 }
