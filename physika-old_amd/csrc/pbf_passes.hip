@@ -87,55 +87,67 @@ struct ListWriter {
 // Phase 2 over LDS words: each word sits with the list value of its first
 // candidate ({bits, base}); a flat loop over the lane's entries pops the next
 // set bit and moves to the next word when one runs empty, so the wave runs as
-// many iterations as its largest count. Part 1 takes n1 entries from the words
-// from 0 on, then pads are pushed up to pad_to entries, then part 2 takes n2
-// entries from the words from w2 on. The base of a word: 32-bit j (kMode 1),
-// or two 16-bit values: j - list_jbase (low half) and the stage slot of j (high
-// half, the first three runs of a staged wave); part 1 reads the high half
-// when slot1 is set.
+// many iterations as its largest count. The base of a word: 32-bit j (kMode
+// 1), or two 16-bit values: j - list_jbase (low half) and the stage slot of j
+// (high half, the first three runs of a staged wave). The first n_slot entries
+// are written as stage slots, followed by pads up to pad_to entries; the rest
+// as offsets.
 template <int kMode>
-__device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int lane, const uint2* lw, int n1,
-                                               bool slot1, int pad_to, int w2, int n2) {
+__device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int lane, const uint2* lw, int cnt,
+                                               int n_slot, int pad_to) {
   ListWriter<kMode> wr{list, lb, lane};
-  auto part = [&](int wi, int n, bool hi) {
-    uint32_t bits = 0u, jb = 0u;
-    for (int k = 0; k < n; ++k) {
-      while (bits == 0u) {  // the remaining words hold n - k > 0 set bits
+  uint32_t bits = 0u, ey = 0u;
+  int wi = 0;
+  auto run = [&](int k0, int k1, bool hi) {
+    for (int k = k0; k < k1; ++k) {
+      while (bits == 0u) {  // the remaining words hold cnt - k > 0 set bits
         const uint2 e = lw[wi << 6];
         bits = e.x;
-        jb = kMode == 1 ? e.y : (hi ? e.y >> 16 : e.y & 0xffffu);
+        ey = e.y;
         ++wi;
       }
+      const uint32_t jb = kMode == 1 ? ey : (hi ? ey >> 16 : ey & 0xffffu);
       wr.push(jb + (uint32_t)pop_first(bits));
     }
   };
-  part(0, n1, slot1);
+  run(0, n_slot, true);
   if (kMode == 0) wr.pad_to(pad_to);
-  part(w2, n2, false);
+  run(n_slot, cnt, false);
   wr.finish();
 }
 
-// Phase 2 over words in the global mask reservation, run by run: candidate j
-// of run q is written as j - sdel[q] (the stage slot) for q < 3 of a staged
-// wave, followed by pads up to kA entries, else as j - jbase.
+// Phase 2 over words in the global mask reservation, run by run: the first
+// n_slot entries (all in runs q < 3) are written as stage slots j - sdel[q],
+// followed by pads up to pad_to entries; the rest as j - jbase.
 template <int kMode>
 __device__ __forceinline__ void write_list_global(uint16_t* list, long long lb, int lane, const int (&ra)[9],
-                                                  const int (&rl)[9], const uint32_t* gcol, int jbase, bool staged,
-                                                  const int (&sdel)[3], int kA) {
+                                                  const int (&rl)[9], const uint32_t* gcol, int jbase,
+                                                  const int (&sdel)[3], int n_slot, int pad_to) {
   ListWriter<kMode> wr{list, lb, lane};
   int wofs = 0;
+  bool padded = pad_to == 0;
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
     const int nw = (rl[q] + 31) >> 5;
-    const int base = (staged && q < 3) ? sdel[q < 3 ? q : 0] : jbase;
     for (int w = 0; w < nw; ++w) {
       uint32_t bits = gcol[(long long)(wofs + w) << 6];
-      const int jb = ra[q] + (w << 5) - base;
-      while (bits) wr.push((uint32_t)(jb + pop_first(bits)));
+      const int j0 = ra[q] + (w << 5);
+      while (bits) {
+        const int j = j0 + pop_first(bits);
+        if (wr.k < n_slot) {
+          wr.push((uint32_t)(j - sdel[q < 3 ? q : 0]));
+        } else {
+          if (!padded) {
+            if (kMode == 0) wr.pad_to(pad_to);
+            padded = true;
+          }
+          wr.push((uint32_t)(j - jbase));
+        }
+      }
     }
     wofs += nw;
-    if (kMode == 0 && staged && q == 2) wr.pad_to(kA);
   }
+  if (!padded && kMode == 0) wr.pad_to(pad_to);
   wr.finish();
 }
 
@@ -225,8 +237,8 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   const bool keep = in_lds || mb >= 0;
 
   // phase 1: test every candidate, keep the words, count (cnt_a: the first
-  // three runs, whose words end at word wofs_a)
-  int cnt = 0, cnt_a = 0, wofs = 0, wofs_a = 0;
+  // three runs)
+  int cnt = 0, cnt_a = 0, wofs = 0;
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
     const int nw = (rl[q] + 31) >> 5;
@@ -247,7 +259,6 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     wofs += nw;
     if (q == 2) {
       cnt_a = cnt;
-      wofs_a = wofs;
     }
   }
   // An out-of-grid lane (the scenes' out-of-grid buckets only) registers
@@ -271,15 +282,18 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     }
   }
   // entries per lane the region must hold: whole pairs of groups for 16-bit
-  // lists, whole groups for 32-bit ones; a staged list pads its stage part to
-  // kA entries on every lane
+  // lists, whole groups for 32-bit ones. A staged list gives every lane a stage
+  // part of kA entries (the wave's mean stage count rounded up to 8): a lane
+  // with fewer pads it, a lane with more lists the excess as offsets after it
+  // (the wave then runs about as many entries as its longest list).
   const int cmax = wave_max(cnt);
   const int need_plain = mode == kListDelta16 ? (cmax + 7) & ~7 : ((cmax + 3) & ~3) << 1;  // 16-bit slots per lane
-  int kA = 0, need_staged = 0, nfull_b = 0;
+  int kA = 0, need_staged = 0, nfull_b = 0, n_slot = 0;
   if (stage_ok) {
-    kA = (wave_max_full(cnt_a) + 7) & ~7;
-    need_staged = kA + ((wave_max_full(cnt - cnt_a) + 7) & ~7);
-    nfull_b = wave_min_full(cnt - cnt_a) >> 2;
+    kA = ((wave_sum(cnt_a) >> 6) + 7) & ~7;
+    n_slot = min(cnt_a, kA);
+    need_staged = kA + ((wave_max_full(cnt - n_slot) + 7) & ~7);
+    nfull_b = wave_min_full(cnt - n_slot) >> 2;
   }
   // The wave's list lives in its own fixed region (no allocation atomic: a
   // same-address atomic per wave serializes in L2); a wave that needs more
@@ -310,22 +324,20 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     }
   }
   if (active) out.nbr_cnt[s] = cnt;
-  if (staged) out.cnt_a[s] = (uint16_t)cnt_a;
+  if (staged) out.cnt_a[s] = (uint16_t)n_slot;
 
   // phase 2: decode this lane's words into the list, four entries per store
   if (active && lb >= 0) {
     if (in_lds) {
       if (mode == kListIndex32)
-        write_list_lds<1>(out.list, lb, lane, lcol, cnt, false, 0, 0, 0);
-      else if (staged)
-        write_list_lds<0>(out.list, lb, lane, lcol, cnt_a, true, kA, wofs_a, cnt - cnt_a);
+        write_list_lds<1>(out.list, lb, lane, lcol, cnt, 0, 0);
       else
-        write_list_lds<0>(out.list, lb, lane, lcol, cnt, false, 0, 0, 0);
+        write_list_lds<0>(out.list, lb, lane, lcol, cnt, staged ? n_slot : 0, staged ? kA : 0);
     } else {
       if (mode == kListIndex32)
-        write_list_global<1>(out.list, lb, lane, ra, rl, gcol, jbase, false, sdel, 0);
+        write_list_global<1>(out.list, lb, lane, ra, rl, gcol, jbase, sdel, 0, 0);
       else
-        write_list_global<0>(out.list, lb, lane, ra, rl, gcol, jbase, staged, sdel, kA);
+        write_list_global<0>(out.list, lb, lane, ra, rl, gcol, jbase, sdel, staged ? n_slot : 0, staged ? kA : 0);
     }
   }
 }
