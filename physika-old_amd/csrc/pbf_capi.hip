@@ -336,6 +336,9 @@ int ensure_scenes(pbf_ctx* c, int nscenes) {
     c->scene_cap = 0;
     PBF_HIP(c, hipMalloc(&c->scene_minmax, sizeof(int) * 6 * (size_t)nscenes));
     PBF_HIP(c, hipMalloc(&c->win, sizeof(SceneWin) * (size_t)nscenes));
+    launch_scene_reset(c->scene_minmax, nscenes, nullptr);  // the sort keeps them empty between calls
+    PBF_HIP(c, hipGetLastError());
+    PBF_HIP(c, hipDeviceSynchronize());
     c->scene_cap = nscenes;
   }
   if (need_cells > c->cell_cap) {
@@ -344,6 +347,8 @@ int ensure_scenes(pbf_ctx* c, int nscenes) {
     c->cell_cap = 0;
     PBF_HIP(c, hipMalloc(&c->cell_cnt, sizeof(int) * ((size_t)need_cells + 1)));
     PBF_HIP(c, hipMalloc(&c->cell_start, sizeof(int) * ((size_t)need_cells + 2)));
+    PBF_HIP(c, hipMemset(c->cell_cnt, 0, sizeof(int) * ((size_t)need_cells + 1)));  // the scan re-zeroes what it reads
+    PBF_HIP(c, hipDeviceSynchronize());
     c->cell_cap = need_cells;
     c->last_n = -1;
   }

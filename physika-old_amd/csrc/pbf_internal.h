@@ -196,6 +196,7 @@ void scan_exclusive(const int* in, int* out, int n, int* scratch, hipStream_t st
 void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const float* v, int n, float4* xn,
                  float4* x0, float4* vs, const PhysConst* integrate, unsigned long long* counters,
                  hipStream_t st);
+void launch_scene_reset(int* minmax, int nscenes, hipStream_t st);
 void launch_set_offsets(int* off, int n, hipStream_t st);
 // cell_start[0..*total_cells] from src to dst (cap: allocated cells).
 void launch_copy_cells(const int* src, int* dst, const int* total_cells, long long cap, hipStream_t st);

@@ -124,21 +124,27 @@ __global__ __launch_bounds__(1024) void k_scan_dev_reduce(const int* in, const i
   }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_dev_mid(int* part, int nb) {
-  // nb <= 1024 block sums, exclusive in place; part[nb] = total
-  const int i = threadIdx.x;
-  const int v = i < nb ? part[i] : 0;
-  int total;
-  const int e = block_scan4(v, 0, 0, 0, total);
-  if (i < nb) part[i] = e;
-  if (i == 0) part[nb] = total;
-}
-
-__global__ __launch_bounds__(1024) void k_scan_dev_down(const int* in, int* out, const int* n_ptr, const int* part) {
+// Each block first sums the block totals before its own (gridDim.x <= 1024)
+// and the last block writes the grand total. The counts are zeroed once read,
+// so the next sort starts from zero counts without a clearing pass (the array
+// is zeroed when allocated).
+__global__ __launch_bounds__(1024) void k_scan_dev_down(int* in, int* out, const int* n_ptr, const int* part) {
   const int n = *n_ptr;
   int lo, hi;
   dev_chunk(n, lo, hi);
-  int running = part[blockIdx.x];
+  int before;
+  {
+    const int i = threadIdx.x;
+    const int v = i < (int)gridDim.x ? part[i] : 0;
+    int total;
+    const int e = block_scan4(v, 0, 0, 0, total);
+    __shared__ int off;
+    if (i == (int)blockIdx.x) off = e;
+    if (blockIdx.x == gridDim.x - 1 && i == 0) out[n] = total;
+    __syncthreads();
+    before = off;
+  }
+  int running = before;
   for (int t0 = lo; t0 < hi; t0 += kScanPerBlock) {
     const int base = t0 + threadIdx.x * 4;
     const int a0 = base < hi ? in[base] : 0;
@@ -148,18 +154,25 @@ __global__ __launch_bounds__(1024) void k_scan_dev_down(const int* in, int* out,
     int total;
     const int e = block_scan4(a0, a1, a2, a3, total);
     store4(out, base, hi, running + e, a0, a1, a2);
+#pragma unroll
+    for (int t = 0; t < 4; ++t)
+      if (base + t < hi) in[base + t] = 0;
     running += total;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = part[gridDim.x];
 }
 
 // ---------------------------------------------------------------------------
 // Scene windows
 
-__global__ void k_scene_reset(int* minmax, int nscenes, unsigned long long* counters) {
+// Empty boxes (min INT_MAX, max INT_MIN). Run once when the boxes are
+// allocated; k_scene_windows empties each box again once it has read it.
+__global__ void k_scene_reset(int* minmax, int nscenes) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < 6 * nscenes) minmax[t] = (t % 6) < 3 ? INT_MAX : INT_MIN;
-  if (t < C_COUNT) counters[t] = 0;
+}
+
+void launch_scene_reset(int* minmax, int nscenes, hipStream_t st) {
+  if (nscenes > 0) k_scene_reset<<<(6 * nscenes + kBlock - 1) / kBlock, kBlock, 0, st>>>(minmax, nscenes);
 }
 
 __device__ __forceinline__ int scene_of(const int* __restrict__ off, int nscenes, int i) {
@@ -184,7 +197,8 @@ __device__ __forceinline__ bool grid_cell(const GridDev& g, const float* __restr
 
 // Scene of every particle and the cell bounding box of each scene's in-grid
 // particles. A block whose particles all belong to one scene writes its
-// box to bpart (8 ints: scene, min i j k, max i j k, -); blocks that straddle
+// box to bpart (8 ints: scene, min i j k, max i j k, and its out-of-grid and
+// non-finite counts packed as o | f << 16); blocks that straddle
 // scenes mark bpart -1 and update the scenes' boxes with atomics (per wave,
 // per lane where a wave straddles). Same-address atomics from every block
 // would serialize in L2, so the single-scene blocks are reduced per scene by
@@ -196,8 +210,7 @@ size_t bounds_part_ints(long long n) { return 8 * (size_t)((n + kBoundsBlock - 1
 __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const float* __restrict__ x, int n, GridDev g,
                                                                const int* __restrict__ off, int nscenes,
                                                                int* __restrict__ sid, int* __restrict__ minmax,
-                                                               int* __restrict__ bpart,
-                                                               unsigned long long* __restrict__ counters) {
+                                                               int* __restrict__ bpart) {
   __shared__ int red[kBoundsBlock / 64][6];
   __shared__ unsigned cnt[kBoundsBlock / 64][2];
   __shared__ int bscene[2];
@@ -239,8 +252,6 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const float* __re
       o += cnt[w][0];
       f += cnt[w][1];
     }
-    if (o) atomicAdd(&counters[C_OOG], (unsigned long long)o);
-    if (f) atomicAdd(&counters[C_NONFINITE], (unsigned long long)f);
     int r[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
     for (int w = 0; w < kBoundsBlock / 64; ++w) {
       for (int a = 0; a < 3; ++a) r[a] = min(r[a], red[w][a]);
@@ -249,6 +260,7 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const float* __re
     int* bp = bpart + 8 * (size_t)blockIdx.x;
     bp[0] = one_scene ? bscene[0] : -1;
     for (int a = 0; a < 6; ++a) bp[1 + a] = r[a];
+    bp[7] = (int)(o | (f << 16));
   }
   if (one_scene) return;  // block-uniform
   const int s0 = __shfl(s, 0);  // lane 0 is valid whenever any lane is
@@ -311,17 +323,37 @@ __global__ __launch_bounds__(256) void k_scene_reduce(const int* __restrict__ of
   }
 }
 
-// Windows and compact cell bases of all scenes (one block).
-__global__ __launch_bounds__(1024) void k_scene_windows(const int* __restrict__ minmax, int nscenes,
+// Windows and compact cell bases of all scenes (one block); each scene's box
+// is emptied again for the next sort. The block also sets the search's
+// counters: the out-of-grid and non-finite totals of the bounds pass, the cell
+// total, and zero for the counters the neighbor pass accumulates.
+__global__ __launch_bounds__(1024) void k_scene_windows(int* __restrict__ minmax, int nscenes,
                                                         SceneWin* __restrict__ win, int* __restrict__ total_cells,
+                                                        const int* __restrict__ bpart, int nbounds,
                                                         unsigned long long* __restrict__ counters) {
+  __shared__ unsigned long long red[2][16];
+  unsigned long long oog = 0, nonfinite = 0;
+  for (int b = threadIdx.x; b < nbounds; b += blockDim.x) {
+    const unsigned v = (unsigned)bpart[8 * (size_t)b + 7];
+    oog += v & 0xffffu;
+    nonfinite += v >> 16;
+  }
+#pragma unroll
+  for (int d = 32; d > 0; d >>= 1) {
+    oog += __shfl_xor(oog, d);
+    nonfinite += __shfl_xor(nonfinite, d);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = oog;
+    red[1][threadIdx.x >> 6] = nonfinite;
+  }
   int running = 0;
   for (int s0 = 0; s0 < nscenes; s0 += 1024) {
     const int s = s0 + threadIdx.x;
     SceneWin w{};
     int cells = 0;
     if (s < nscenes) {
-      const int* m = minmax + 6 * s;
+      int* m = minmax + 6 * s;
       if (m[3] >= m[0]) {
         for (int a = 0; a < 3; ++a) {
           w.lo[a] = m[a];
@@ -336,6 +368,7 @@ __global__ __launch_bounds__(1024) void k_scene_windows(const int* __restrict__ 
         w.ncell = 0;
       }
       cells = w.ncell + 1;  // + out-of-grid bucket
+      for (int a = 0; a < 6; ++a) m[a] = a < 3 ? INT_MAX : INT_MIN;
     }
     int total;
     const int e = block_scan4(cells, 0, 0, 0, total);
@@ -347,14 +380,17 @@ __global__ __launch_bounds__(1024) void k_scene_windows(const int* __restrict__ 
   }
   if (threadIdx.x == 0) {
     *total_cells = running;
+    unsigned long long o = 0, f = 0;
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) {
+      o += red[0][w];
+      f += red[1][w];
+    }
     counters[C_CELLS] = (unsigned long long)running;
+    counters[C_OOG] = o;
+    counters[C_NONFINITE] = f;
+    counters[C_MASK_WORDS] = 0;
+    counters[C_FALLBACK] = 0;
   }
-}
-
-__global__ void k_clear_cells(int* __restrict__ cell_cnt, const int* __restrict__ total_cells) {
-  const int total = *total_cells;
-  const int stride = gridDim.x * blockDim.x;
-  for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < total; c += stride) cell_cnt[c] = 0;
 }
 
 // K_CalculateParticleNumber (GridHash.cu:78-87): key = compact cell or the
@@ -437,18 +473,14 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
 void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const float* v, int n, float4* xn, float4* x0,
                  float4* vs, const PhysConst* integrate, unsigned long long* counters, hipStream_t st) {
   const int nb = (n + kBlock - 1) / kBlock;
-  k_scene_reset<<<(6 * sb.nscenes + C_COUNT + kBlock - 1) / kBlock, kBlock, 0, st>>>(sb.scene_minmax, sb.nscenes,
-                                                                                     counters);
-  k_scene_bounds<<<(n + kBoundsBlock - 1) / kBoundsBlock, kBoundsBlock, 0, st>>>(x, n, g, sb.scene_off, sb.nscenes,
-                                                                                 sb.sid, sb.scene_minmax, sb.bpart,
-                                                                                 counters);
+  const int nbounds = (n + kBoundsBlock - 1) / kBoundsBlock;
+  k_scene_bounds<<<nbounds, kBoundsBlock, 0, st>>>(x, n, g, sb.scene_off, sb.nscenes, sb.sid, sb.scene_minmax,
+                                                   sb.bpart);
   k_scene_reduce<<<sb.nscenes, 256, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
-  k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, counters);
-  const int clear_blocks = (int)std::min<long long>((sb.cell_cap + kBlock - 1) / kBlock, 2048);
-  k_clear_cells<<<clear_blocks, kBlock, 0, st>>>(sb.cell_cnt, sb.total_cells);
+  k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, sb.bpart, nbounds,
+                                      counters);
   k_bin<<<nb, kBlock, 0, st>>>(x, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
   k_scan_dev_reduce<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.total_cells, sb.scan_part);
-  k_scan_dev_mid<<<1, 1024, 0, st>>>(sb.scan_part, kScanDevBlocks);
   k_scan_dev_down<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.cell_start, sb.total_cells, sb.scan_part);
   k_scatter<<<nb, kBlock, 0, st>>>(sb.key, sb.slot, sb.cell_start, sb.tmp, n);
   PhysConst pc{};
