@@ -62,12 +62,12 @@ __device__ __forceinline__ int window_cell(const SceneWin& w, int ci, int cj, in
 
 // bits * 2 + (s < thr2): the squared distance of one candidate shifted into a
 // word (v_cmp into VCC, then an add with carry-in; NaN compares false as in the
-// reference's test).
+// reference's test). thr2 is uniform and read from a scalar register.
 __device__ __forceinline__ uint32_t push_bit(uint32_t bits, float s, float thr2) {
   uint32_t out;
   asm("v_cmp_lt_f32_e64 vcc, %2, %3\n\tv_addc_co_u32_e32 %0, vcc, %1, %1, vcc"
       : "=v"(out)
-      : "v"(bits), "v"(s), "v"(thr2)
+      : "v"(bits), "v"(s), "s"(thr2)
       : "vcc");
   return out;
 }
@@ -105,11 +105,15 @@ __device__ __forceinline__ uint32_t test_word_planes(const float* cx, const floa
                                                      float4 xi, int cnt, float thr2) {
   uint32_t bits = 0;
   int k0 = 0;
+  // one 32-bit byte offset for the three planes (scalar bases): the loads take
+  // the scalar-base form and the loop advances a single register
+  const char *bx = reinterpret_cast<const char*>(cx), *by = reinterpret_cast<const char*>(cy),
+             *bz = reinterpret_cast<const char*>(cz);
   for (; k0 < cnt; k0 += 8) {
-    const int j = j0 + k0;
-    const f4u x0 = *reinterpret_cast<const f4u*>(cx + j), x1 = *reinterpret_cast<const f4u*>(cx + j + 4);
-    const f4u y0 = *reinterpret_cast<const f4u*>(cy + j), y1 = *reinterpret_cast<const f4u*>(cy + j + 4);
-    const f4u z0 = *reinterpret_cast<const f4u*>(cz + j), z1 = *reinterpret_cast<const f4u*>(cz + j + 4);
+    const uint32_t o = (uint32_t)(j0 + k0) << 2;
+    const f4u x0 = *reinterpret_cast<const f4u*>(bx + o), x1 = *reinterpret_cast<const f4u*>(bx + o + 16);
+    const f4u y0 = *reinterpret_cast<const f4u*>(by + o), y1 = *reinterpret_cast<const f4u*>(by + o + 16);
+    const f4u z0 = *reinterpret_cast<const f4u*>(bz + o), z1 = *reinterpret_cast<const f4u*>(bz + o + 16);
     const float px[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
     const float py[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
     const float pz[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
