@@ -50,7 +50,7 @@ struct pbf_ctx {
   int* list_jbase = nullptr;
   int* list_mode = nullptr;
   int* stage = nullptr;         // per wave: kStageInts ints (staged lists)
-  uint16_t* cnt_a = nullptr;    // per sorted particle: neighbors in the stage part (staged lists)
+  uint16_t* cnt_a = nullptr;    // per sorted particle: stage-slot entries of its list (staged lists)
   int* oog_rcnt = nullptr;      // per sorted particle: out-of-grid particles that list it
   int* oog_rev = nullptr;       // kOogRev per sorted particle: those particles
   uint32_t* mask = nullptr;

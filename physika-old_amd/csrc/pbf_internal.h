@@ -77,11 +77,13 @@ __host__ __device__ constexpr int run_of(int q) { return q < 3 ? q + 3 : (q < 6 
 // four per lane:
 //   kListDelta16: 16-bit j - list_jbase;
 //   kListIndex32: 32-bit j (the wave's candidates span more than 65536 indices);
-//   kListStaged:  the neighbors in the first three runs (q < 3) as 16-bit slots
-//                 of the wave's stage, padded with kStagePad to kA entries on
-//                 every lane, then the others as 16-bit j - list_jbase. The
-//                 stage concatenates the union over the wave's lanes of run q,
-//                 q = 0..2: slot P_q + (j - A_q) for j in interval q.
+//   kListStaged:  a lane's first neighbors in the first three runs (q < 3), at
+//                 most kA of them, as 16-bit slots of the wave's stage, padded
+//                 with kStagePad to kA entries, then its other neighbors as
+//                 16-bit j - list_jbase; kA is the wave's mean count in those
+//                 runs rounded up to 8. The stage concatenates the union over
+//                 the wave's lanes of run q, q = 0..2: slot P_q + (j - A_q) for
+//                 j in interval q.
 // A staged wave's PBD passes copy the stage (at most kStageCap records) into
 // LDS once and read those neighbors from there.
 constexpr int kListDelta16 = 0, kListIndex32 = 1, kListStaged = 2;
@@ -111,7 +113,7 @@ struct NbrView {
   const float* cy;             // candidates from these: one 16-byte load brings the
   const float* cz;             // same coordinate of four consecutive candidates)
   const int* nbr_cnt;          // neighbors per sorted particle
-  const uint16_t* cnt_a;       // staged waves: neighbors per particle in the first three runs
+  const uint16_t* cnt_a;       // staged waves: stage-slot entries per particle (at most kA)
   const long long* list_base;  // per wave: first 16-bit slot of its list, -1 = no list
   const int* list_jbase;       // per wave: index base of 16-bit offsets
   const int* list_mode;        // per wave: bits 0-1: the encoding (kList*); bits 2..: groups of

@@ -1,15 +1,16 @@
 // Neighbor lists and the PBD passes over them (NeighborQuery::compute,
 // SummationDensity, DensityPBD::takeOneIteration).
 //
-// A particle's 27 stencil cells form 9 runs of 3 x-consecutive cells; in cell
-// order each run is one contiguous index range [a, a+len). The neighbor pass
-// tests every candidate of the runs (the reference's set: every j in a valid
-// stencil cell with sqrtf(|x_i - x_j|^2) < h, NeighborQuery.cu:245-276), keeps
-// the result as bit masks (scratch) and decodes them into the wave's neighbor
-// list (16-bit offsets from a per-wave base, wave-interleaved: entry k of lane
-// l at base + 64*k + l). The ten PBD passes of a step then read one coalesced
-// list entry and gather one position per neighbor. Neighbor order is run
-// order, ascending index inside a run, on every path.
+// A particle's 27 stencil cells form 9 runs of x-consecutive sort cells; in
+// cell order each run is one contiguous index range [a, a+len). The neighbor
+// pass tests every candidate of the runs (the reference's set: every j in a
+// valid stencil cell with sqrtf(|x_i - x_j|^2) < h, NeighborQuery.cu:245-276),
+// keeps the result as bit masks (scratch) and decodes them into the wave's
+// neighbor list (the encodings of pbf_internal.h, wave-interleaved in groups of
+// four). The ten PBD passes of a step then read the list with coalesced loads
+// and take each neighbor's position from LDS (staged lists) or a gather.
+// Neighbor order is the run visit order, ascending index inside a run, on
+// every path.
 #include <climits>
 
 #include "pbf_nbr.h"
@@ -203,7 +204,8 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   const int wjlo = wave_min(jlo), wjhi = wave_max(jhi);
   const int mode = (wjhi >= wjlo && wjhi - wjlo > 65535) ? kListIndex32 : kListDelta16;
   const int jbase = mode == kListDelta16 ? wjlo : 0;  // list offsets are j - jbase
-  // the stage: intervals [sa[q], sa[q] + slen[q]) at slots sp[q]..; slot = j - sdel[q]
+  // the stage: interval q starts at index sa[q] and at slot sp[q] (its length
+  // is sp[q + 1] - sp[q]); candidate j of run q has slot j - sdel[q]
   int sa[3], sp[4], sdel[3];
   bool fits = true, inside = true;
   sp[0] = 0;
@@ -218,8 +220,8 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     sp[q + 1] = sp[q] + ((used && fits) ? e - a : 0);
     sdel[q] = sa[q] - sp[q];
   }
-  const bool stage_ok =
-      out.stage_lists && mode == kListDelta16 && fits && sp[3] <= kStageCap && __all(inside ? 1 : 0) && __all(active ? 1 : 0);
+  const bool stage_ok = out.stage_lists && mode == kListDelta16 && fits && sp[3] <= kStageCap &&
+                        __all(inside ? 1 : 0) && __all(active ? 1 : 0);
   // word storage: this wave's LDS block, else the global mask reservation
   const bool in_lds = wmax <= kNbLdsWords;
   long long mb = -1;
