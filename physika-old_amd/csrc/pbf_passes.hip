@@ -99,13 +99,15 @@ __device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int
   ListWriter<kMode> wr{list, lb, lane};
   uint32_t bits = 0u, ey = 0u;
   int wi = 0;
+  // the next word is read one word ahead, so that a refill does not wait on LDS
+  uint2 nxt = lw[0];
   auto run = [&](int k0, int k1, bool hi) {
     for (int k = k0; k < k1; ++k) {
       while (bits == 0u) {  // the remaining words hold cnt - k > 0 set bits
-        const uint2 e = lw[wi << 6];
-        bits = e.x;
-        ey = e.y;
+        bits = nxt.x;
+        ey = nxt.y;
         ++wi;
+        nxt = lw[min(wi, kNbLdsWords - 1) << 6];
       }
       const uint32_t jb = kMode == 1 ? ey : (hi ? ey >> 16 : ey & 0xffffu);
       wr.push(jb + (uint32_t)pop_first(bits));
