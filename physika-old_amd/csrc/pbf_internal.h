@@ -100,7 +100,7 @@ constexpr int kStageMinParticles = 1 << 18;
 // Out-of-grid particles that list one hashed particle, kept per particle for
 // the displacement pass (a particle listed by more takes a scan of the scene's
 // out-of-grid particles instead).
-constexpr int kOogRev = 8;
+constexpr int kOogRev = 16;
 
 // Cell-sorted view used by the neighbor pass and every pass over its lists.
 struct NbrView {

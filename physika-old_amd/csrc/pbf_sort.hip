@@ -408,7 +408,24 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ x, int
   int k = w.cell_base + w.ncell;
   if (inside) k = w.cell_base + (ci - w.lo[0]) + w.dim[0] * ((cj - w.lo[1]) + w.dim[1] * (ck - w.lo[2]));
   key[i] = k;
-  slot[i] = atomicAdd(&cell_cnt[k], 1);
+  // Out-of-grid particles of one scene all count into its bucket: one atomic
+  // per bucket and wave (same-address atomics serialize in L2); the others
+  // count into their cells directly.
+  unsigned long long pending = __ballot(!inside);
+  const int lane = threadIdx.x & 63;
+  int sl = 0;
+  while (pending) {  // uniform: every active lane runs the same iterations
+    const int leader = __ffsll((long long)pending) - 1;
+    const int kb = __shfl(k, leader);
+    const unsigned long long peers = __ballot(!inside && k == kb) & pending;
+    int base = 0;
+    if (lane == leader) base = atomicAdd(&cell_cnt[kb], __popcll(peers));
+    base = __shfl(base, leader);
+    if ((peers >> lane) & 1ull) sl = base + __popcll(peers & ((1ull << lane) - 1ull));
+    pending &= ~peers;
+  }
+  if (inside) sl = atomicAdd(&cell_cnt[k], 1);
+  slot[i] = sl;
 }
 
 // K_ConstructHashTable (GridHash.cu:187-198): ids into their cell's slots.
