@@ -21,9 +21,10 @@ namespace pbf {
 // Neighbor pass: K_CalNeighborSize + K_GetNeighborElements
 // (NeighborQuery.cu:245-276,337-367) in one traversal of the candidates.
 
-// Mask words per lane a wave keeps in LDS during the neighbor pass; waves that
+// Mask words per lane a wave keeps in LDS during the neighbor pass (10 KB per
+// wave: 16 waves per CU; 24 words, 13 waves per CU, ran 15% slower); waves that
 // need more keep them in the global mask reservation.
-constexpr int kNbLdsWords = 24;
+constexpr int kNbLdsWords = 20;
 
 // Appends neighbor j to a lane's group of four and stores complete groups:
 // 16-bit offsets from the wave base (mode 0: a 64-bit shift register, entry 0
