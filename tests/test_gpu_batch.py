@@ -114,6 +114,18 @@ def test_batch_equals_serial_generator(fluid, scene):
     assert 48 < st["cells_used"] < 48 * 2 * 85 ** 3 // 20  # sort cells: half a grid cell along x
 
 
+def test_batch_staged_equals_serial(fluid, scene):
+    """Staged lists in a batch (waves may straddle scenes, and the serial runs
+    of small scenes keep unstaged lists) give the serial bits."""
+    say("batched with staged lists vs serial, 12 ragged scenes, 3 steps")
+    scenes = mixed_scenes(scene)
+    vels = [velocities(scene, len(s), 80 + k) for k, s in enumerate(scenes)]
+    ctx = fluid.Context("cuda:0", stage_min_particles=1, list_slots=256)
+    got, ctx = batched(fluid, scenes, vels, 3, ctx)
+    assert ctx.stats()["staged_waves"] > 0
+    assert_bit_identical(got, serial(fluid, scenes, vels, 3))
+
+
 def test_batch_more_than_1024_scenes(fluid, scene):
     say("batched vs serial, 1500 scenes of 8..120 particles, 1 step")
     u = scene.uniform01(71, 1500)
