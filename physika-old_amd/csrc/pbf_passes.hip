@@ -86,10 +86,11 @@ struct ListWriter {
   }
 };
 
-// Phase 2 over LDS words: each word sits with the list value of its first
-// candidate ({bits, base}); a flat loop over the lane's entries pops the next
-// set bit and moves to the next word when one runs empty, so the wave runs as
-// many iterations as its largest count. The base of a word: 32-bit j (kMode
+// Phase 2 over LDS words: each word holding a neighbor sits with the list
+// value of its first candidate ({bits, base}; words without one are not
+// stored); a flat loop over the lane's entries pops the next set bit and moves
+// to the next word when one runs empty, so the wave runs as many iterations as
+// its largest count. The base of a word: 32-bit j (kMode
 // 1), or two 16-bit values: j - list_jbase (low half) and the stage slot of j
 // (high half, the first three runs of a staged wave). The first n_slot entries
 // are written as stage slots, followed by pads up to pad_to entries; the rest
@@ -104,7 +105,7 @@ __device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int
   uint2 nxt = lw[0];
   auto run = [&](int k0, int k1, bool hi) {
     for (int k = k0; k < k1; ++k) {
-      while (bits == 0u) {  // the remaining words hold cnt - k > 0 set bits
+      if (bits == 0u) {  // every stored word holds a set bit
         bits = nxt.x;
         ey = nxt.y;
         ++wi;
@@ -241,9 +242,9 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   uint32_t* gcol = out.mask + (mb >= 0 ? mb + lane : 0);
   const bool keep = in_lds || mb >= 0;
 
-  // phase 1: test every candidate, keep the words, count (cnt_a: the first
-  // three runs)
-  int cnt = 0, cnt_a = 0, wofs = 0;
+  // phase 1: test every candidate, keep the words (in LDS only those holding
+  // a neighbor, nlw of them), count (cnt_a: the first three runs)
+  int cnt = 0, cnt_a = 0, wofs = 0, nlw = 0;
 #pragma unroll
   for (int q = 0; q < 9; ++q) {
     const int nw = (rl[q] + 31) >> 5;
@@ -255,7 +256,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
                                   ? (uint32_t)jb
                                   : (((uint32_t)(jb - jbase) & 0xffffu) |
                                      (q < 3 ? ((uint32_t)(jb - sdel[q < 3 ? q : 0]) & 0xffffu) << 16 : 0u));
-        lcol[(wofs + w) << 6] = make_uint2(bits, base);
+        if (bits) lcol[(nlw++) << 6] = make_uint2(bits, base);
       } else if (mb >= 0) {
         gcol[(long long)(wofs + w) << 6] = bits;
       }
@@ -274,9 +275,9 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
       const int nw = (rl[q] + 31) >> 5;
       for (int w = 0; w < nw; ++w) {
         const int jb = ra[q] + (w << 5);
-        uint32_t bits = in_lds    ? lcol[(wo + w) << 6].x
-                        : mb >= 0 ? gcol[(long long)(wo + w) << 6]
-                                  : test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
+        uint32_t bits = (!in_lds && mb >= 0)
+                            ? gcol[(long long)(wo + w) << 6]
+                            : test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
         while (bits) {
           const int j = jb + pop_first(bits);
           const int slot = atomicAdd(&out.oog_rcnt[j], 1);
