@@ -21,10 +21,13 @@ namespace pbf {
 // Neighbor pass: K_CalNeighborSize + K_GetNeighborElements
 // (NeighborQuery.cu:245-276,337-367) in one traversal of the candidates.
 
-// Mask words per lane a wave keeps in LDS during the neighbor pass (10 KB per
-// wave: 16 waves per CU; 24 words, 13 waves per CU, ran 15% slower); waves that
-// need more keep them in the global mask reservation.
-constexpr int kNbLdsWords = 20;
+// Mask words per lane a wave keeps in LDS during the neighbor pass, only those
+// holding a neighbor (8 KB per wave: 20 waves per CU; 24 words, 13 waves per
+// CU, ran 15% slower). A lane with more decodes its list from the candidates
+// tested again. Waves with more than kNbMaxWords words per lane keep them all
+// in the global mask reservation.
+constexpr int kNbLdsWords = 16;
+constexpr int kNbMaxWords = 28;
 
 // Appends neighbor j to a lane's group of four and stores complete groups:
 // 16-bit offsets from the wave base (mode 0: a 64-bit shift register, entry 0
@@ -121,13 +124,14 @@ __device__ __forceinline__ void write_list_lds(uint16_t* list, long long lb, int
   wr.finish();
 }
 
-// Phase 2 over words in the global mask reservation, run by run: the first
-// n_slot entries (all in runs q < 3) are written as stage slots j - sdel[q],
-// followed by pads up to pad_to entries; the rest as j - jbase.
-template <int kMode>
-__device__ __forceinline__ void write_list_global(uint16_t* list, long long lb, int lane, const int (&ra)[9],
-                                                  const int (&rl)[9], const uint32_t* gcol, int jbase,
-                                                  const int (&sdel)[3], int n_slot, int pad_to) {
+// Phase 2 run by run over words from word_of(q, w, wofs + w) (the global mask
+// reservation, or the candidates tested again): the first n_slot entries (all
+// in runs q < 3) are written as stage slots j - sdel[q], followed by pads up to
+// pad_to entries; the rest as j - jbase.
+template <int kMode, class WordFn>
+__device__ __forceinline__ void write_list_runs(uint16_t* list, long long lb, int lane, const int (&ra)[9],
+                                                const int (&rl)[9], WordFn&& word_of, int jbase,
+                                                const int (&sdel)[3], int n_slot, int pad_to) {
   ListWriter<kMode> wr{list, lb, lane};
   int wofs = 0;
   bool padded = pad_to == 0;
@@ -135,7 +139,7 @@ __device__ __forceinline__ void write_list_global(uint16_t* list, long long lb, 
   for (int q = 0; q < 9; ++q) {
     const int nw = (rl[q] + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
-      uint32_t bits = gcol[(long long)(wofs + w) << 6];
+      uint32_t bits = word_of(q, w, wofs + w);
       const int j0 = ra[q] + (w << 5);
       while (bits) {
         const int j = j0 + pop_first(bits);
@@ -227,7 +231,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   const bool stage_ok = out.stage_lists && mode == kListDelta16 && fits && sp[3] <= kStageCap &&
                         __all(inside ? 1 : 0) && __all(active ? 1 : 0);
   // word storage: this wave's LDS block, else the global mask reservation
-  const bool in_lds = wmax <= kNbLdsWords;
+  const bool in_lds = wmax <= kNbMaxWords;
   long long mb = -1;
   if (!in_lds && lane == 0) {
     const unsigned long long need = 64ull * (unsigned long long)wmax;
@@ -256,7 +260,10 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
                                   ? (uint32_t)jb
                                   : (((uint32_t)(jb - jbase) & 0xffffu) |
                                      (q < 3 ? ((uint32_t)(jb - sdel[q < 3 ? q : 0]) & 0xffffu) << 16 : 0u));
-        if (bits) lcol[(nlw++) << 6] = make_uint2(bits, base);
+        if (bits) {
+          if (nlw < kNbLdsWords) lcol[nlw << 6] = make_uint2(bits, base);
+          ++nlw;
+        }
       } else if (mb >= 0) {
         gcol[(long long)(wofs + w) << 6] = bits;
       }
@@ -334,16 +341,26 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
 
   // phase 2: decode this lane's words into the list, four entries per store
   if (active && lb >= 0) {
-    if (in_lds) {
+    const int n_sl = staged ? n_slot : 0, pad_to = staged ? kA : 0;
+    if (in_lds && nlw <= kNbLdsWords) {
       if (mode == kListIndex32)
         write_list_lds<1>(out.list, lb, lane, lcol, cnt, 0, 0);
       else
-        write_list_lds<0>(out.list, lb, lane, lcol, cnt, staged ? n_slot : 0, staged ? kA : 0);
-    } else {
+        write_list_lds<0>(out.list, lb, lane, lcol, cnt, n_sl, pad_to);
+    } else if (in_lds) {  // this lane's words did not fit: test its candidates again
+      auto retest = [&](int q, int w, int) {
+        return test_word_planes(nv.cx, nv.cy, nv.cz, ra[q] + (w << 5), xi, min(32, rl[q] - (w << 5)), nv.thr2);
+      };
       if (mode == kListIndex32)
-        write_list_global<1>(out.list, lb, lane, ra, rl, gcol, jbase, sdel, 0, 0);
+        write_list_runs<1>(out.list, lb, lane, ra, rl, retest, jbase, sdel, 0, 0);
       else
-        write_list_global<0>(out.list, lb, lane, ra, rl, gcol, jbase, sdel, staged ? n_slot : 0, staged ? kA : 0);
+        write_list_runs<0>(out.list, lb, lane, ra, rl, retest, jbase, sdel, n_sl, pad_to);
+    } else {
+      auto global = [&](int, int, int wo) { return gcol[(long long)wo << 6]; };
+      if (mode == kListIndex32)
+        write_list_runs<1>(out.list, lb, lane, ra, rl, global, jbase, sdel, 0, 0);
+      else
+        write_list_runs<0>(out.list, lb, lane, ra, rl, global, jbase, sdel, n_sl, pad_to);
     }
   }
 }
