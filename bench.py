@@ -153,13 +153,15 @@ def config1(torch, fluid, scene, threads):
                     "one_thread": round(rate1, 1)}}
 
 
-def config2b(torch, fluid, scene, steps=20):
+def config2b(torch, fluid, scene, steps=200, dense_steps=20):
     """8 Gaussian blobs x 131,072 particles (sigma 0.05, seed 3): GPU rate over
-    `steps` steps (median of 3), neighbor-count mean and maximum from the
-    library, and the count histogram of a 20,000-particle sample of the input
-    (k-d tree, exact distances)."""
+    `steps` steps after 2 (median of 3), the same over the first `dense_steps`
+    of them (the dense phase before the blobs fly apart), neighbor-count mean
+    and maximum from the library, and the count histogram of a 20,000-particle
+    sample of the input (k-d tree, exact distances)."""
     from scipy.spatial import cKDTree
     x_np = scene.blobs_1m()
+    ms_d, runs_d, _, _, _ = gpu_rate(torch, fluid, x_np, dense_steps, 2)
     ms, runs, ctx, _, _ = gpu_rate(torch, fluid, x_np, steps, 2)
     st = ctx.stats()
     n = len(x_np)
@@ -168,11 +170,15 @@ def config2b(torch, fluid, scene, steps=20):
     counts = np.array([len(r) for r in tree.query_ball_point(x_np[pick].astype(np.float64), 0.0125)])
     edges = [1, 10, 30, 60, 100, 200, 300, 400, 500, 600, 800, 1200]
     hist, _ = np.histogram(counts, bins=edges + [10 ** 9])
-    progress(f"config 2b: {ms:.3f} ms/step, mean {st['neighbor_pairs'] / n:.1f} neighbors")
+    progress(f"config 2b: {ms:.3f} ms/step over {steps}, {ms_d:.3f} over the first {dense_steps}; "
+             f"mean {st['neighbor_pairs'] / n:.1f} neighbors")
     return {"workload": "8 Gaussian blobs x 131,072 = 1,048,576 particles, sigma 0.05, seed 3; "
                         f"{steps} steps after 2", "particles": n,
             "value": round(n / (ms / 1e3), 1), "unit": "particle-steps/s", "ms_per_step": round(ms, 4),
             "runs_ms_per_step": [round(t, 4) for t in runs],
+            "out_of_grid_last_step": int(st["out_of_grid"]),
+            "dense_phase": {"steps": dense_steps, "value": round(n / (ms_d / 1e3), 1), "ms_per_step": round(ms_d, 4),
+                            "runs_ms_per_step": [round(t, 4) for t in runs_d]},
             "neighbors_mean_last_step": round(st["neighbor_pairs"] / n, 1),
             "neighbors_max_last_step_slots": int(st["list_slots_needed"] // max(64 * ((n + 63) // 64), 1)),
             "input_sample_histogram": {"edges": edges, "counts": hist.tolist(),
@@ -355,8 +361,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the 32k (config 1) and blob (config 2b) lines")
     ap.add_argument("--workload", default="lattice_1m", choices=["lattice_1m", "blobs_1m"])
-    ap.add_argument("--batched-steps", type=int, default=20, help="steps of the 256-scene batched comparison")
-    ap.add_argument("--serial-steps", type=int, default=5, help="steps of its serial per-scene loop")
+    ap.add_argument("--batched-steps", type=int, default=100, help="steps of the 256-scene batched comparison")
+    ap.add_argument("--serial-steps", type=int, default=10, help="steps of its serial per-scene loop")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable and batched-inverse workloads")
     ap.add_argument("--copy-calibration", action="store_true",
