@@ -40,6 +40,7 @@ struct pbf_ctx {
   int cap_n = 0;
   int *key = nullptr, *slot = nullptr, *tmp = nullptr, *perm = nullptr, *inv = nullptr;
   int *sid = nullptr, *ssid = nullptr, *bpart = nullptr;
+  int *tid = nullptr, *pid = nullptr;  // original indices: of tmp's entries, of the state kept in cell order
   int *nbr_cnt = nullptr, *cnt_orig = nullptr;
   float4 *xn = nullptr, *x0 = nullptr, *xa = nullptr, *xb = nullptr, *vs = nullptr, *vv = nullptr;
   float* xc = nullptr;  // sorted x^n as three coordinate planes of cap + kXnPad floats
@@ -409,6 +410,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   const int cap = std::max(((n + 1 + 255) / 256) * 256, 256);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->tid); dfree(c->pid);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->stage); dfree(c->cnt_a); dfree(c->wave_stat); dfree(c->oog_rcnt); dfree(c->oog_rev);
@@ -421,6 +423,8 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   PBF_HIP(c, hipMalloc(&c->inv, ni));
   PBF_HIP(c, hipMalloc(&c->sid, ni));
   PBF_HIP(c, hipMalloc(&c->ssid, ni));
+  PBF_HIP(c, hipMalloc(&c->tid, ni));
+  PBF_HIP(c, hipMalloc(&c->pid, ni));
   PBF_HIP(c, hipMalloc(&c->bpart, sizeof(int) * bounds_part_ints(cap)));
   PBF_HIP(c, hipMalloc(&c->nbr_cnt, ni));
   PBF_HIP(c, hipMalloc(&c->cnt_orig, ni));
@@ -539,6 +543,7 @@ SortBufs sort_of(pbf_ctx* c, const int* scene_off, int nscenes) {
   sb.bpart = c->bpart;
   sb.xc = c->xc;
   sb.oog_rcnt = c->oog_rcnt;
+  sb.tid = c->tid;
   sb.xc_stride = c->cap_n + kXnPad;
   return sb;
 }
@@ -597,11 +602,10 @@ int drain_marks(pbf_ctx* c) {
 // (GridHash::construct + queryNeighborDynamic), leaving the cell-sorted state
 // and the neighbor lists in the context; with `integrate` the integrator is
 // fused into the reorder.
-int search(pbf_ctx* c, const float* x, const float* v, int n, const int* scene_off, int nscenes, bool integrate,
-           hipStream_t st) {
+int search(pbf_ctx* c, const SortIn& in, int n, const int* scene_off, int nscenes, bool integrate, hipStream_t st) {
   {
     PassTimer t(c, PBF_PASS_SORT, st);
-    launch_sort(sort_of(c, scene_off, nscenes), c->g, x, v, n, c->xn, c->x0, c->vs, integrate ? &c->pc : nullptr,
+    launch_sort(sort_of(c, scene_off, nscenes), c->g, in, n, c->xn, c->x0, c->vs, integrate ? &c->pc : nullptr,
                 c->counters, st);
   }
   {
@@ -615,6 +619,11 @@ int search(pbf_ctx* c, const float* x, const float* v, int n, const int* scene_o
   PBF_LAUNCHED(c);
   c->last_n = n;
   return PBF_OK;
+}
+
+int search(pbf_ctx* c, const float* x, const float* v, int n, const int* scene_off, int nscenes, bool integrate,
+           hipStream_t st) {
+  return search(c, SortIn{x, v, nullptr, nullptr, nullptr}, n, scene_off, nscenes, integrate, st);
 }
 
 int record_counters(pbf_ctx* c, hipStream_t st) {
@@ -673,9 +682,18 @@ int ensure_box(pbf_ctx* c, hipStream_t st) {
 // a tape (P: K iterate arrays, P[0] = the integrated positions; aux: K arrays)
 // every PBD iterate, its lambda and the adjoint's per-iteration sums are kept.
 // The box boundary runs first and the viscosity last when their flags are set
-// (never with a tape).
+// (never with a tape). A plain step (no box, no viscosity, K >= 1) may take its
+// state from `from` and leave it in `to` instead of the caller's arrays (the
+// cell order of the step, between the steps of one call).
+struct CellState {
+  const float4* x;  // an iterate buffer
+  const float4* v;  // vv
+  const int* id;    // pid
+};
+
 int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, float* v_out, float* d_rho, int n,
-              const int* scene_off, int nscenes, float4* const* P, float4* const* aux, hipStream_t st) {
+              const int* scene_off, int nscenes, float4* const* P, float4* const* aux, hipStream_t st,
+              const CellState* from = nullptr, CellState* to = nullptr) {
   const int K = c->p.iterations;
   const bool box = (c->p.flags & PBF_FLAG_BOX) != 0;
   const bool visc = (c->p.flags & PBF_FLAG_VISCOSITY) != 0 && c->p.viscosity_iterations > 0;
@@ -686,7 +704,13 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
     x_in = x_out;
     v_in = v_out;
   }
-  s = search(c, x_in, v_in, n, scene_off, nscenes, true, st);
+  SortIn in{x_in, v_in, nullptr, nullptr, nullptr};
+  if (from) {
+    in.x4 = from->x;
+    in.v4 = from->v;
+    in.id = from->id;
+  }
+  s = search(c, in, n, scene_off, nscenes, true, st);
   if (s) return s;
   NbrView nv = view_of(c, n);
   float4* bufs[2] = {c->xa, c->xb};
@@ -701,8 +725,14 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
     }
     {
       PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
-      launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st,
-                               (last && visc) ? c->vv : nullptr);
+      if (last && to) {
+        launch_pass_displacement(nv, src, dst, c->pc, true, c->x0, c->vs, c->perm, nullptr, nullptr, st, c->vv,
+                                 c->pid);
+        *to = CellState{dst, c->vv, c->pid};
+      } else {
+        launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st,
+                                 (last && visc) ? c->vv : nullptr);
+      }
     }
   }
   if (visc) {
@@ -723,11 +753,16 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
 }
 
 // nsteps fused steps in place; d_rho receives the last step's density.
+// Plain steps keep the state in cell order between the steps (the next sort
+// reads nearly in order); the last step writes the caller's arrays.
 int run_steps(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, const int* scene_off, int nscenes, int nsteps,
               hipStream_t st) {
+  const bool keep = nsteps > 1 && c->p.iterations > 0 && (c->p.flags & (PBF_FLAG_BOX | PBF_FLAG_VISCOSITY)) == 0;
+  CellState cs{};
   for (int step = 0; step < nsteps; ++step) {
-    int s = step_once(c, d_x, d_v, d_x, d_v, step + 1 == nsteps ? d_rho : nullptr, n, scene_off, nscenes, nullptr,
-                      nullptr, st);
+    const bool last = step + 1 == nsteps;
+    int s = step_once(c, d_x, d_v, d_x, d_v, last ? d_rho : nullptr, n, scene_off, nscenes, nullptr, nullptr, st,
+                      keep && step > 0 ? &cs : nullptr, keep && !last ? &cs : nullptr);
     if (s) return s;
   }
   return record_counters(c, st);
@@ -942,6 +977,7 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->cell_cnt); dfree(c->cell_start);
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
+  dfree(c->tid); dfree(c->pid);
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->stage); dfree(c->cnt_a); dfree(c->wave_stat); dfree(c->oog_rcnt); dfree(c->oog_rev);

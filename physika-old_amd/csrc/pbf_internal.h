@@ -176,6 +176,19 @@ struct SortBufs {
   float* xc;              // sorted x^n as three planes (x, y, z), plane stride xc_stride
   int xc_stride;
   int* oog_rcnt;          // per sorted particle, zeroed by the reorder
+  int* tid;               // original index of each particle of tmp (input in cell order only)
+};
+
+// Input of a sort: the caller's state in original order (x, v: xyz float
+// triples), or, with x4 set, the state the previous step of the same call
+// left in its cell order (x4, v4: one float4 per particle; id: the original
+// index of each).
+struct SortIn {
+  const float* x;
+  const float* v;
+  const float4* x4;
+  const float4* v4;
+  const int* id;
 };
 
 constexpr int kBlock = 256;
@@ -195,9 +208,8 @@ void scan_exclusive(const int* in, int* out, int n, int* scratch, hipStream_t st
 
 // Counting sort of a batch by (scene, window cell); integrate != nullptr fuses
 // the integrator into the reorder.
-void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const float* v, int n, float4* xn,
-                 float4* x0, float4* vs, const PhysConst* integrate, unsigned long long* counters,
-                 hipStream_t st);
+void launch_sort(const SortBufs& sb, const GridDev& g, const SortIn& in, int n, float4* xn, float4* x0,
+                 float4* vs, const PhysConst* integrate, unsigned long long* counters, hipStream_t st);
 void launch_scene_reset(int* minmax, int nscenes, hipStream_t st);
 void launch_set_offsets(int* off, int n, hipStream_t st);
 // cell_start[0..*total_cells] from src to dst (cap: allocated cells).
@@ -212,7 +224,8 @@ void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& 
 // the velocities to v_sorted (sorted order, for the viscosity passes).
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc,
                               bool last, const float4* x0, const float4* vs, const int* perm,
-                              float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr);
+                              float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr,
+                              int* id_out = nullptr);
 void launch_write_back(const float4* xs, const float4* vs, const int* perm, int n, float* x_out,
                        float* v_out, hipStream_t st);
 void launch_invert(const int* perm, int* inv, int n, hipStream_t st);

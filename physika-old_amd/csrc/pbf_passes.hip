@@ -465,13 +465,16 @@ void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& 
 // receives 2*sum_j dp_ij; an out-of-grid particle k is listed by nobody and
 // receives sum_j dp_kj once, while each hashed j it lists receives dp_jk once
 // more. The last iteration fuses DP_UpdateVelocity (DensityPBD.cu:368-378) and
-// the scatter back to original order.
+// the scatter back to original order, or keeps x and v in cell order (with
+// each particle's original index in id_out) for the viscosity passes or the
+// next step of the call.
 __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float4* __restrict__ xs,
                                                          float4* __restrict__ xd, PhysConst pc, int last,
                                                          const float4* __restrict__ x0,
                                                          const float4* __restrict__ vs,
                                                          const int* __restrict__ perm, float* __restrict__ x_out,
-                                                         float* __restrict__ v_out, float4* __restrict__ v_sorted) {
+                                                         float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                         int* __restrict__ id_out) {
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s == 0 && (!last || v_sorted)) xd[nv.pad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
   if (s >= nv.n) return;
@@ -539,10 +542,11 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
   }
   const float4 xo = x0[s];
   const float4 vv = vs[s];
-  if (v_sorted) {  // the viscosity passes follow
+  if (v_sorted) {  // the viscosity passes or the next step follow
     xd[s] = make_float4(nx, ny, nz, 0.0f);
     v_sorted[s] = make_float4(vv.x + (nx - xo.x) / pc.dt, vv.y + (ny - xo.y) / pc.dt, vv.z + (nz - xo.z) / pc.dt,
                               0.0f);
+    if (id_out) id_out[s] = perm[s];
     return;
   }
   const size_t i = (size_t)perm[s];
@@ -556,9 +560,9 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
 
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                               const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out,
-                              hipStream_t st, float4* v_sorted) {
+                              hipStream_t st, float4* v_sorted, int* id_out) {
   k_displacement<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm,
-                                                                 x_out, v_out, v_sorted);
+                                                                 x_out, v_out, v_sorted, id_out);
 }
 
 // ---------------------------------------------------------------------------

@@ -8,6 +8,10 @@
 // (scene, cell) order (w of the position buffers carries lambda between
 // passes). Cells are compact: each scene owns the window of cells spanned by
 // its in-grid particles plus one bucket for its out-of-grid particles.
+// Between the steps of one call the state stays in the previous step's cell
+// order (float4 x and v plus each particle's original index): the sort then
+// reads nearly in order, and the order within a cell is still by original
+// index, so the result is the same as from the caller's order.
 #include <climits>
 
 #include "pbf_device.h"
@@ -187,11 +191,27 @@ __device__ __forceinline__ int scene_of(const int* __restrict__ off, int nscenes
   return lo;
 }
 
-__device__ __forceinline__ bool grid_cell(const GridDev& g, const float* __restrict__ x, int i, int& ci, int& cj,
-                                          int& ck, bool& nonfinite) {
-  const float px = x[3 * (size_t)i], py = x[3 * (size_t)i + 1], pz = x[3 * (size_t)i + 2];
-  nonfinite = !(isfinite(px) && isfinite(py) && isfinite(pz));
-  cell_of(g, px, py, pz, ci, cj, ck);
+// Position and velocity of input particle i (caller order or cell order).
+template <bool kCellOrder>
+__device__ __forceinline__ float3 in_pos(const SortIn& in, int i) {
+  if (kCellOrder) {
+    const float4 p = in.x4[i];
+    return make_float3(p.x, p.y, p.z);
+  }
+  return make_float3(in.x[3 * (size_t)i], in.x[3 * (size_t)i + 1], in.x[3 * (size_t)i + 2]);
+}
+template <bool kCellOrder>
+__device__ __forceinline__ float3 in_vel(const SortIn& in, int i) {
+  if (kCellOrder) {
+    const float4 p = in.v4[i];
+    return make_float3(p.x, p.y, p.z);
+  }
+  return make_float3(in.v[3 * (size_t)i], in.v[3 * (size_t)i + 1], in.v[3 * (size_t)i + 2]);
+}
+
+__device__ __forceinline__ bool grid_cell(const GridDev& g, float3 p, int& ci, int& cj, int& ck, bool& nonfinite) {
+  nonfinite = !(isfinite(p.x) && isfinite(p.y) && isfinite(p.z));
+  cell_of(g, p.x, p.y, p.z, ci, cj, ck);
   return !nonfinite && in_grid(g, ci, cj, ck);
 }
 
@@ -203,11 +223,12 @@ __device__ __forceinline__ bool grid_cell(const GridDev& g, const float* __restr
 // per lane where a wave straddles). Same-address atomics from every block
 // would serialize in L2, so the single-scene blocks are reduced per scene by
 // k_scene_reduce.
-constexpr int kBoundsBlock = 1024;
+constexpr int kBoundsBlock = 256;
 
 size_t bounds_part_ints(long long n) { return 8 * (size_t)((n + kBoundsBlock - 1) / kBoundsBlock + 1); }
 
-__global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const float* __restrict__ x, int n, GridDev g,
+template <bool kCellOrder>
+__global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(SortIn in, int n, GridDev g,
                                                                const int* __restrict__ off, int nscenes,
                                                                int* __restrict__ sid, int* __restrict__ minmax,
                                                                int* __restrict__ bpart) {
@@ -222,7 +243,7 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const float* __re
   if (valid) {
     s = scene_of(off, nscenes, i);
     sid[i] = s;
-    inside = grid_cell(g, x, i, ci, cj, ck, nonfinite);
+    inside = grid_cell(g, in_pos<kCellOrder>(in, i), ci, cj, ck, nonfinite);
   }
   const int last = min(n, (int)(blockIdx.x * blockDim.x + blockDim.x)) - 1 - (int)(blockIdx.x * blockDim.x);
   if (threadIdx.x == 0) bscene[0] = s;
@@ -232,10 +253,11 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(const float* __re
     cnt[wave][0] = (unsigned)__popcll(b_oog);
     cnt[wave][1] = (unsigned)__popcll(b_nf);
   }
-  const int mi = wave_min(inside ? ci : INT_MAX), mj = wave_min(inside ? cj : INT_MAX);
-  const int mk = wave_min(inside ? ck : INT_MAX);
-  const int xi = wave_max(inside ? ci : INT_MIN), xj = wave_max(inside ? cj : INT_MIN);
-  const int xk = wave_max(inside ? ck : INT_MIN);
+  // every lane of the block is active (no early exit)
+  const int mi = wave_min_full(inside ? ci : INT_MAX), mj = wave_min_full(inside ? cj : INT_MAX);
+  const int mk = wave_min_full(inside ? ck : INT_MAX);
+  const int xi = wave_max_full(inside ? ci : INT_MIN), xj = wave_max_full(inside ? cj : INT_MIN);
+  const int xk = wave_max_full(inside ? ck : INT_MIN);
   if (lane == 0) {
     red[wave][0] = mi;
     red[wave][1] = mj;
@@ -395,7 +417,8 @@ __global__ __launch_bounds__(1024) void k_scene_windows(int* __restrict__ minmax
 
 // K_CalculateParticleNumber (GridHash.cu:78-87): key = compact cell or the
 // scene's out-of-grid bucket; slot = arrival order inside the cell.
-__global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ x, int n, GridDev g,
+template <bool kCellOrder>
+__global__ __launch_bounds__(kBlock) void k_bin(SortIn in, int n, GridDev g,
                                                 const int* __restrict__ sid, const SceneWin* __restrict__ win,
                                                 int* __restrict__ key, int* __restrict__ slot,
                                                 int* __restrict__ cell_cnt) {
@@ -403,17 +426,36 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ x, int
   if (i >= n) return;
   int ci, cj, ck;
   bool nonfinite;
-  const bool inside = grid_cell(g, x, i, ci, cj, ck, nonfinite);
+  const bool inside = grid_cell(g, in_pos<kCellOrder>(in, i), ci, cj, ck, nonfinite);
   const SceneWin w = win[sid[i]];
   int k = w.cell_base + w.ncell;
   if (inside) k = w.cell_base + (ci - w.lo[0]) + w.dim[0] * ((cj - w.lo[1]) + w.dim[1] * (ck - w.lo[2]));
   key[i] = k;
-  // Out-of-grid particles of one scene all count into its bucket: one atomic
-  // per bucket and wave (same-address atomics serialize in L2); the others
-  // count into their cells directly.
-  unsigned long long pending = __ballot(!inside);
   const int lane = threadIdx.x & 63;
   int sl = 0;
+  if (kCellOrder) {
+    // Input in the previous step's cell order: in-grid lanes of one cell are
+    // mostly adjacent. One atomic per run of equal keys, at the run's first
+    // lane (a key split over several runs of the wave counts once per run);
+    // out-of-grid lanes end runs and count below. Active lanes are the low
+    // lanes of the wave.
+    const int prev = __shfl_up(k, 1);
+    const bool cut = !inside || lane == 0 || prev != k;
+    const unsigned long long cuts = __ballot(cut), act = __ballot(true);
+    const unsigned long long upto = (2ull << lane) - 1ull;  // lanes 0..lane
+    const int hl = 63 - __clzll(cuts & upto);               // first lane of this run
+    const unsigned long long after = cuts & ~upto;
+    const int end = after ? __ffsll((long long)after) - 1 : 64 - __clzll(act);
+    int base = 0;
+    if (inside && cut) base = atomicAdd(&cell_cnt[k], end - lane);
+    const int b = __shfl(base, hl);
+    if (inside) sl = b + (lane - hl);
+  } else if (inside) {
+    sl = atomicAdd(&cell_cnt[k], 1);
+  }
+  // Out-of-grid particles of one scene all count into its bucket: one atomic
+  // per bucket and wave (same-address atomics serialize in L2).
+  unsigned long long pending = __ballot(!inside);
   while (pending) {  // uniform: every active lane runs the same iterations
     const int leader = __ffsll((long long)pending) - 1;
     const int kb = __shfl(k, leader);
@@ -424,16 +466,20 @@ __global__ __launch_bounds__(kBlock) void k_bin(const float* __restrict__ x, int
     if ((peers >> lane) & 1ull) sl = base + __popcll(peers & ((1ull << lane) - 1ull));
     pending &= ~peers;
   }
-  if (inside) sl = atomicAdd(&cell_cnt[k], 1);
   slot[i] = sl;
 }
 
 // K_ConstructHashTable (GridHash.cu:187-198): ids into their cell's slots.
+// Input in cell order: tid receives the original index beside each input index.
+template <bool kCellOrder>
 __global__ __launch_bounds__(kBlock) void k_scatter(const int* __restrict__ key, const int* __restrict__ slot,
                                                     const int* __restrict__ cell_start, int* __restrict__ tmp,
-                                                    int n) {
+                                                    const int* __restrict__ id, int* __restrict__ tid, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) tmp[cell_start[key[i]] + slot[i]] = i;
+  if (i >= n) return;
+  const int p = cell_start[key[i]] + slot[i];
+  tmp[p] = i;
+  if (kCellOrder) tid[p] = id[i];
 }
 
 // Final position inside the cell = rank of the particle id among the cell's
@@ -447,9 +493,10 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int* __restrict__ key,
 // PositionBasedFluidModel.cpp:93-94), the integrated state to x0 / vs.
 constexpr int kRankMax = 1024;
 
+template <bool kCellOrder>
 __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key, const int* __restrict__ cell_start,
-                                                    const int* __restrict__ tmp, const int* __restrict__ sid,
-                                                    const float* __restrict__ x, const float* __restrict__ v, int n,
+                                                    const int* __restrict__ tmp, const int* __restrict__ tid,
+                                                    const int* __restrict__ sid, SortIn in, int n,
                                                     int* __restrict__ perm, int* __restrict__ ssid,
                                                     float4* __restrict__ xn,
                                                     float4* __restrict__ x0, float4* __restrict__ vs, PhysConst pc,
@@ -459,27 +506,31 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) x0[n] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);  // the padding particle
   if (t >= n) return;
-  const int i = tmp[t];
+  const int i = tmp[t];  // input index
+  const int id = kCellOrder ? tid[t] : i;  // original index
   const int k = key[i];
   const int a = cell_start[k], b = cell_start[k + 1];
   int s = t;
   if (b - a <= kRankMax) {
     int rank = 0;
-    for (int u = a; u < b; ++u) rank += tmp[u] < i;
+    const int* ids = kCellOrder ? tid : tmp;
+    for (int u = a; u < b; ++u) rank += ids[u] < id;
     s = a + rank;
   }
-  perm[s] = i;
+  perm[s] = id;
   ssid[s] = sid[i];
   oog_rcnt[s] = 0;
-  const float px = x[3 * (size_t)i], py = x[3 * (size_t)i + 1], pz = x[3 * (size_t)i + 2];
+  const float3 p = in_pos<kCellOrder>(in, i);
+  const float px = p.x, py = p.y, pz = p.z;
   xn[s] = make_float4(px, py, pz, 0.0f);
   xc[s] = px;
   xc[xc_stride + s] = py;
   xc[2 * (size_t)xc_stride + s] = pz;
   if (integrate) {
-    const float vx = v[3 * (size_t)i] + pc.gdt[0];
-    const float vy = v[3 * (size_t)i + 1] + pc.gdt[1];
-    const float vz = v[3 * (size_t)i + 2] + pc.gdt[2];
+    const float3 v = in_vel<kCellOrder>(in, i);
+    const float vx = v.x + pc.gdt[0];
+    const float vy = v.y + pc.gdt[1];
+    const float vz = v.z + pc.gdt[2];
     x0[s] = make_float4(px + vx * pc.dt, py + vy * pc.dt, pz + vz * pc.dt, 0.0f);
     vs[s] = make_float4(vx, vy, vz, 0.0f);
   } else {
@@ -487,23 +538,33 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
   }
 }
 
-void launch_sort(const SortBufs& sb, const GridDev& g, const float* x, const float* v, int n, float4* xn, float4* x0,
-                 float4* vs, const PhysConst* integrate, unsigned long long* counters, hipStream_t st) {
+template <bool kCellOrder>
+void sort_in(const SortBufs& sb, const GridDev& g, const SortIn& in, int n, float4* xn, float4* x0, float4* vs,
+             const PhysConst* integrate, unsigned long long* counters, hipStream_t st) {
   const int nb = (n + kBlock - 1) / kBlock;
   const int nbounds = (n + kBoundsBlock - 1) / kBoundsBlock;
-  k_scene_bounds<<<nbounds, kBoundsBlock, 0, st>>>(x, n, g, sb.scene_off, sb.nscenes, sb.sid, sb.scene_minmax,
-                                                   sb.bpart);
+  k_scene_bounds<kCellOrder><<<nbounds, kBoundsBlock, 0, st>>>(in, n, g, sb.scene_off, sb.nscenes, sb.sid,
+                                                               sb.scene_minmax, sb.bpart);
   k_scene_reduce<<<sb.nscenes, 256, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
   k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, sb.bpart, nbounds,
                                       counters);
-  k_bin<<<nb, kBlock, 0, st>>>(x, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
+  k_bin<kCellOrder><<<nb, kBlock, 0, st>>>(in, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
   k_scan_dev_reduce<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.total_cells, sb.scan_part);
   k_scan_dev_down<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.cell_start, sb.total_cells, sb.scan_part);
-  k_scatter<<<nb, kBlock, 0, st>>>(sb.key, sb.slot, sb.cell_start, sb.tmp, n);
+  k_scatter<kCellOrder><<<nb, kBlock, 0, st>>>(sb.key, sb.slot, sb.cell_start, sb.tmp, in.id, sb.tid, n);
   PhysConst pc{};
   if (integrate) pc = *integrate;
-  k_reorder<<<nb, kBlock, 0, st>>>(sb.key, sb.cell_start, sb.tmp, sb.sid, x, v, n, sb.perm, sb.ssid, xn, x0,
-                                   vs, pc, integrate ? 1 : 0, sb.xc, sb.xc_stride, sb.oog_rcnt);
+  k_reorder<kCellOrder><<<nb, kBlock, 0, st>>>(sb.key, sb.cell_start, sb.tmp, sb.tid, sb.sid, in, n, sb.perm,
+                                               sb.ssid, xn, x0, vs, pc, integrate ? 1 : 0, sb.xc, sb.xc_stride,
+                                               sb.oog_rcnt);
+}
+
+void launch_sort(const SortBufs& sb, const GridDev& g, const SortIn& in, int n, float4* xn, float4* x0, float4* vs,
+                 const PhysConst* integrate, unsigned long long* counters, hipStream_t st) {
+  if (in.x4)
+    sort_in<true>(sb, g, in, n, xn, x0, vs, integrate, counters, st);
+  else
+    sort_in<false>(sb, g, in, n, xn, x0, vs, integrate, counters, st);
 }
 
 // Copy of cell_start[0..total] (the device-side cell count of the last sort).
