@@ -308,28 +308,45 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(SortIn in, int n,
 }
 
 // One block per scene: reduce the boxes of the single-scene blocks of the
-// scene's particle range and merge them into the scene's box.
-__global__ __launch_bounds__(256) void k_scene_reduce(const int* __restrict__ off, int nscenes,
-                                                      const int* __restrict__ bpart, int* __restrict__ minmax) {
-  __shared__ int red[4][6];
+// scene's particle range and merge them into the scene's box. Each entry is
+// read as two 16-byte loads, four entries per thread in flight.
+constexpr int kReduceBlock = 1024;
+
+__device__ __forceinline__ void merge_box(int r[6], const int* __restrict__ bpart, int b, int s) {
+  const int4 u = reinterpret_cast<const int4*>(bpart)[2 * (size_t)b];
+  const int4 w = reinterpret_cast<const int4*>(bpart)[2 * (size_t)b + 1];
+  if (u.x != s) return;
+  r[0] = min(r[0], u.y);
+  r[1] = min(r[1], u.z);
+  r[2] = min(r[2], u.w);
+  r[3] = max(r[3], w.x);
+  r[4] = max(r[4], w.y);
+  r[5] = max(r[5], w.z);
+}
+
+__global__ __launch_bounds__(kReduceBlock) void k_scene_reduce(const int* __restrict__ off, int nscenes,
+                                                               const int* __restrict__ bpart,
+                                                               int* __restrict__ minmax) {
+  __shared__ int red[kReduceBlock / 64][6];
   const int s = blockIdx.x;
   const int b0 = off[s] / kBoundsBlock;
   const int b1 = off[s + 1] > off[s] ? (off[s + 1] - 1) / kBoundsBlock + 1 : b0;
   int r[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
-  for (int b = b0 + (int)threadIdx.x; b < b1; b += blockDim.x) {
-    const int* bp = bpart + 8 * (size_t)b;
-    if (bp[0] != s) continue;
-    for (int a = 0; a < 3; ++a) r[a] = min(r[a], bp[1 + a]);
-    for (int a = 3; a < 6; ++a) r[a] = max(r[a], bp[1 + a]);
+  int b = b0 + (int)threadIdx.x;
+  for (; b + 3 * kReduceBlock < b1; b += 4 * kReduceBlock) {
+#pragma unroll
+    for (int q = 0; q < 4; ++q) merge_box(r, bpart, b + q * kReduceBlock, s);
   }
-  for (int a = 0; a < 3; ++a) r[a] = wave_min(r[a]);
-  for (int a = 3; a < 6; ++a) r[a] = wave_max(r[a]);
+  for (; b < b1; b += kReduceBlock) merge_box(r, bpart, b, s);
+  // every lane of the block is active
+  for (int a = 0; a < 3; ++a) r[a] = wave_min_full(r[a]);
+  for (int a = 3; a < 6; ++a) r[a] = wave_max_full(r[a]);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   if (lane == 0)
     for (int a = 0; a < 6; ++a) red[wave][a] = r[a];
   __syncthreads();
   if (threadIdx.x == 0) {
-    for (int w = 1; w < 4; ++w) {
+    for (int w = 1; w < kReduceBlock / 64; ++w) {
       for (int a = 0; a < 3; ++a) r[a] = min(r[a], red[w][a]);
       for (int a = 3; a < 6; ++a) r[a] = max(r[a], red[w][a]);
     }
@@ -355,6 +372,7 @@ __global__ __launch_bounds__(1024) void k_scene_windows(int* __restrict__ minmax
                                                         unsigned long long* __restrict__ counters) {
   __shared__ unsigned long long red[2][16];
   unsigned long long oog = 0, nonfinite = 0;
+#pragma unroll 4
   for (int b = threadIdx.x; b < nbounds; b += blockDim.x) {
     const unsigned v = (unsigned)bpart[8 * (size_t)b + 7];
     oog += v & 0xffffu;
@@ -514,7 +532,12 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
   if (b - a <= kRankMax) {
     int rank = 0;
     const int* ids = kCellOrder ? tid : tmp;
-    for (int u = a; u < b; ++u) rank += ids[u] < id;
+    int u = a;
+    for (; u + 4 <= b; u += 4) {  // four loads in flight
+      const int c0 = ids[u], c1 = ids[u + 1], c2 = ids[u + 2], c3 = ids[u + 3];
+      rank += (c0 < id) + (c1 < id) + (c2 < id) + (c3 < id);
+    }
+    for (; u < b; ++u) rank += ids[u] < id;
     s = a + rank;
   }
   perm[s] = id;
@@ -545,7 +568,7 @@ void sort_in(const SortBufs& sb, const GridDev& g, const SortIn& in, int n, floa
   const int nbounds = (n + kBoundsBlock - 1) / kBoundsBlock;
   k_scene_bounds<kCellOrder><<<nbounds, kBoundsBlock, 0, st>>>(in, n, g, sb.scene_off, sb.nscenes, sb.sid,
                                                                sb.scene_minmax, sb.bpart);
-  k_scene_reduce<<<sb.nscenes, 256, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
+  k_scene_reduce<<<sb.nscenes, kReduceBlock, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
   k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, sb.bpart, nbounds,
                                       counters);
   k_bin<kCellOrder><<<nb, kBlock, 0, st>>>(in, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
