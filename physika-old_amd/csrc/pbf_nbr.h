@@ -136,11 +136,15 @@ __device__ __forceinline__ int pop_first(uint32_t& bits) {
 // The byte offset is 32-bit (fewer than kMaxParticles entries), so the load
 // takes the scalar-base + vector-offset form.
 template <bool kW>
-__device__ __forceinline__ float4 load_pos(const float4* __restrict__ xs, int j) {
-  const char* b = reinterpret_cast<const char*>(xs) + ((uint32_t)j << 4);
+__device__ __forceinline__ float4 load_pos_bytes(const float4* __restrict__ xs, uint32_t o) {
+  const char* b = reinterpret_cast<const char*>(xs) + o;
   if (kW) return *reinterpret_cast<const float4*>(b);
   const float* q = reinterpret_cast<const float*>(b);
   return make_float4(q[0], q[1], q[2], 0.0f);
+}
+template <bool kW>
+__device__ __forceinline__ float4 load_pos(const float4* __restrict__ xs, int j) {
+  return load_pos_bytes<kW>(xs, (uint32_t)j << 4);
 }
 
 // List layout: a lane's entries come in groups of four. 32-bit indices: a
@@ -194,6 +198,36 @@ struct StageMap {
     return t + d0 + (t >= p1 ? inc1 : 0) + (t >= p2 ? inc2 : 0);
   }
 };
+
+// Byte offset e * 16 + base of the 16-bit list entry e held in the low
+// (kHigh false) or high half of w, in one v_mad_u32_u16 (op_sel selects the
+// half of the first operand); base is wave-uniform.
+template <bool kHigh>
+__device__ __forceinline__ int entry_bytes(uint32_t w, uint32_t base) {
+  uint32_t r;
+  if (kHigh)
+    asm("v_mad_u32_u16 %0, %1, 16, %2 op_sel:[1,0,0,0]" : "=v"(r) : "v"(w), "s"(base));
+  else
+    asm("v_mad_u32_u16 %0, %1, 16, %2" : "=v"(r) : "v"(w), "s"(base));
+  return (int)r;
+}
+
+// decode16 into byte offsets: entry e becomes e * 16 + base, entries past cnt
+// become past (a byte offset too).
+template <bool kFull>
+__device__ __forceinline__ Group4 decode16_bytes(uint2 e, uint32_t base, int b, int cnt, int past) {
+  const int k = b << 2;
+  Group4 g;
+  g.j[0] = entry_bytes<false>(e.x, base);
+  g.j[1] = entry_bytes<true>(e.x, base);
+  g.j[2] = entry_bytes<false>(e.y, base);
+  g.j[3] = entry_bytes<true>(e.y, base);
+  if (!kFull) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g.j[u] = k + u < cnt ? g.j[u] : past;
+  }
+  return g;
+}
 
 // decode16 of stage slots into sorted indices.
 template <bool kFull>
@@ -410,13 +444,30 @@ __device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, co
 // LDS records a wave needs for for_each_neighbor_staged.
 constexpr int kStageRecords = kStageCap + 1;
 
+// LDS byte address of a pointer into a __shared__ array, and a load from one.
+__device__ __forceinline__ uint32_t lds_address(const float4* p) {
+  return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) float4*)p;
+}
+typedef float lds_record4 __attribute__((ext_vector_type(4)));  // 16-byte aligned
+typedef float lds_record3 __attribute__((ext_vector_type(3)));  // 16-byte aligned, 12 bytes read
+template <bool kW>
+__device__ __forceinline__ float4 lds_load_pos(uint32_t a) {
+  if (kW) {
+    const lds_record4 v = *(const __attribute__((address_space(3))) lds_record4*)(uintptr_t)a;
+    return make_float4(v.x, v.y, v.z, v.w);
+  }
+  const lds_record3 v = *(const __attribute__((address_space(3))) lds_record3*)(uintptr_t)a;
+  return make_float4(v.x, v.y, v.z, 0.0f);
+}
+
 // for_each_neighbor_x<kW, true> for the PBD passes: a staged wave copies its
 // stage from xs into its LDS region buf (kStageRecords records; coalesced
 // loads) and reads the neighbors of its first three runs from there (the
 // padding particle sits at slot kStagePad), so only the others are gathered
 // through the vector-memory path. Other waves take for_each_neighbor_x. The
-// body receives stage slots or sorted indices as j (the PBD bodies ignore it).
-// The sums run in the same neighbor order as on every other path.
+// entries decode straight into byte addresses (LDS addresses of stage
+// records, byte offsets into xs), which the body receives as j (the PBD bodies
+// ignore it). The sums run in the same neighbor order as on every other path.
 template <bool kW, class Body>
 __device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int s, const float4* __restrict__ xs,
                                                          float4* buf, Body&& body) {
@@ -441,19 +492,17 @@ __device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int 
   const int ka = st[7];
   const uint4* L = reinterpret_cast<const uint4*>(nv.list + lb) + lane;
   // stage part: ka entries on every lane (pads read the padding particle)
+  const uint32_t lds = __builtin_amdgcn_readfirstlane(lds_address(buf));
   list_loop_pairs<true>(
-      ka, ka >> 2, L, [&](uint2 e, int b, auto) { return decode16<true>(e, 0, b, ka, 0); },
-      [&](int t) {
-        const float4 v = buf[t];
-        return kW ? v : make_float4(v.x, v.y, v.z, 0.0f);
-      },
-      body);
+      ka, ka >> 2, L, [&](uint2 e, int b, auto) { return decode16_bytes<true>(e, lds, b, ka, 0); },
+      [&](int a) { return lds_load_pos<kW>((uint32_t)a); }, body);
   const int cb = nv.nbr_cnt[s] - nv.cnt_a[s];
-  const int jb = nv.list_jbase[wave];
+  const uint32_t jb16 = (uint32_t)nv.list_jbase[wave] << 4;
+  const int past = (int)((uint32_t)nv.pad << 4);
   list_loop_pairs<true>(
       cb, lm >> 2, L + ((long long)(ka >> 3) << 6),
-      [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cb, nv.pad); },
-      [&](int j) { return load_pos<kW>(xs, j); }, body);
+      [&](uint2 e, int b, auto full) { return decode16_bytes<decltype(full)::value>(e, jb16, b, cb, past); },
+      [&](int o) { return load_pos_bytes<kW>(xs, (uint32_t)o); }, body);
 }
 
 }  // namespace pbf

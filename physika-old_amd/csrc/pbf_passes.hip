@@ -438,12 +438,13 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* _
     const PairGeom p = pair_geom(xi, xj, pc.inv_h);
     const float t2 = p.t * p.t;
     rho = fmaf(t2, p.t, rho);
-    const float gq = p.r > pc.pair_eps ? t2 * p.rinv : 0.0f;
+    // |g| / |cG| = t^2 (|x_i - x_j| / r = 1); no gradient closer than pair_eps
+    const float tg = p.r > pc.pair_eps ? t2 : 0.0f;
+    const float gq = tg * p.rinv;
     sgx = fmaf(gq, p.dx, sgx);
     sgy = fmaf(gq, p.dy, sgy);
     sgz = fmaf(gq, p.dz, sgz);
-    const float gr = gq * p.r;  // |g| / |cG| = t^2
-    sgg = fmaf(gr, gr, sgg);
+    sgg = fmaf(tg, tg, sgg);
   });
   rho *= pc.cW;
   const float S = (pc.cG * pc.cG) * (sgg + (sgx * sgx + sgy * sgy + sgz * sgz));
