@@ -216,94 +216,115 @@ __device__ __forceinline__ bool grid_cell(const GridDev& g, float3 p, int& ci, i
 }
 
 // Scene of every particle and the cell bounding box of each scene's in-grid
-// particles. A block whose particles all belong to one scene writes its
+// particles. Each block covers kBoundsSpan particles (kBoundsPer per thread,
+// coalesced). A block whose particles all belong to one scene writes its
 // box to bpart (8 ints: scene, min i j k, max i j k, and its out-of-grid and
 // non-finite counts packed as o | f << 16); blocks that straddle
 // scenes mark bpart -1 and update the scenes' boxes with atomics (per wave,
 // per lane where a wave straddles). Same-address atomics from every block
 // would serialize in L2, so the single-scene blocks are reduced per scene by
-// k_scene_reduce.
-constexpr int kBoundsBlock = 256;
+// k_scene_reduce (by k_scene_windows for one scene).
+constexpr int kBoundsBlock = 256, kBoundsPer = 4, kBoundsSpan = kBoundsBlock * kBoundsPer;
 
-size_t bounds_part_ints(long long n) { return 8 * (size_t)((n + kBoundsBlock - 1) / kBoundsBlock + 1); }
+size_t bounds_part_ints(long long n) { return 8 * (size_t)((n + kBoundsSpan - 1) / kBoundsSpan + 1); }
+
+// Wave-wide sum with every lane active (DPP row shifts, then lane reads).
+__device__ __forceinline__ int wave_sum_full(int v) {
+  v += dpp_row_shr<0x111>(0, v);
+  v += dpp_row_shr<0x112>(0, v);
+  v += dpp_row_shr<0x114>(0, v);
+  v += dpp_row_shr<0x118>(0, v);
+  return __builtin_amdgcn_readlane(v, 15) + __builtin_amdgcn_readlane(v, 31) + __builtin_amdgcn_readlane(v, 47) +
+         __builtin_amdgcn_readlane(v, 63);
+}
 
 template <bool kCellOrder>
 __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(SortIn in, int n, GridDev g,
                                                                const int* __restrict__ off, int nscenes,
                                                                int* __restrict__ sid, int* __restrict__ minmax,
                                                                int* __restrict__ bpart) {
-  __shared__ int red[kBoundsBlock / 64][6];
-  __shared__ unsigned cnt[kBoundsBlock / 64][2];
-  __shared__ int bscene[2];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ int red[kBoundsBlock / 64][7];
+  const int base = blockIdx.x * kBoundsSpan;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const bool valid = i < n;
-  int s = -1, ci = 0, cj = 0, ck = 0;
-  bool nonfinite = false, inside = false;
-  if (valid) {
-    s = scene_of(off, nscenes, i);
-    sid[i] = s;
-    inside = grid_cell(g, in_pos<kCellOrder>(in, i), ci, cj, ck, nonfinite);
-  }
-  const int last = min(n, (int)(blockIdx.x * blockDim.x + blockDim.x)) - 1 - (int)(blockIdx.x * blockDim.x);
-  if (threadIdx.x == 0) bscene[0] = s;
-  if ((int)threadIdx.x == last) bscene[1] = s;
-  const unsigned long long b_oog = __ballot(valid && !inside), b_nf = __ballot(valid && nonfinite);
-  if (lane == 0) {
-    cnt[wave][0] = (unsigned)__popcll(b_oog);
-    cnt[wave][1] = (unsigned)__popcll(b_nf);
+  int sq[kBoundsPer], cq[kBoundsPer][3];
+  bool inq[kBoundsPer];
+  int lo[3] = {INT_MAX, INT_MAX, INT_MAX}, hi[3] = {INT_MIN, INT_MIN, INT_MIN};
+  int counts = 0;  // out-of-grid | non-finite << 16
+#pragma unroll
+  for (int q = 0; q < kBoundsPer; ++q) {
+    const int i = base + q * kBoundsBlock + (int)threadIdx.x;
+    int sc = -1, c[3] = {0, 0, 0};
+    bool nonfinite = false, inside = false;
+    if (i < n) {
+      sc = scene_of(off, nscenes, i);
+      sid[i] = sc;
+      inside = grid_cell(g, in_pos<kCellOrder>(in, i), c[0], c[1], c[2], nonfinite);
+      counts += (inside ? 0 : 1) + (nonfinite ? 1 << 16 : 0);
+    }
+    sq[q] = sc;
+    inq[q] = inside;
+    for (int a = 0; a < 3; ++a) {
+      cq[q][a] = c[a];
+      if (inside) {
+        lo[a] = min(lo[a], c[a]);
+        hi[a] = max(hi[a], c[a]);
+      }
+    }
   }
   // every lane of the block is active (no early exit)
-  const int mi = wave_min_full(inside ? ci : INT_MAX), mj = wave_min_full(inside ? cj : INT_MAX);
-  const int mk = wave_min_full(inside ? ck : INT_MAX);
-  const int xi = wave_max_full(inside ? ci : INT_MIN), xj = wave_max_full(inside ? cj : INT_MIN);
-  const int xk = wave_max_full(inside ? ck : INT_MIN);
+  for (int a = 0; a < 3; ++a) {
+    lo[a] = wave_min_full(lo[a]);
+    hi[a] = wave_max_full(hi[a]);
+  }
+  counts = wave_sum_full(counts);
   if (lane == 0) {
-    red[wave][0] = mi;
-    red[wave][1] = mj;
-    red[wave][2] = mk;
-    red[wave][3] = xi;
-    red[wave][4] = xj;
-    red[wave][5] = xk;
+    for (int a = 0; a < 3; ++a) {
+      red[wave][a] = lo[a];
+      red[wave][3 + a] = hi[a];
+    }
+    red[wave][6] = counts;
   }
   __syncthreads();
-  const bool one_scene = bscene[0] == bscene[1];
+  // one scene when the block's first and last particles share it (uniform)
+  const int last = min(n, base + kBoundsSpan) - 1;
+  const bool one_scene = scene_of(off, nscenes, base) == scene_of(off, nscenes, last);
   if (threadIdx.x == 0) {
-    unsigned o = 0, f = 0;
-    for (int w = 0; w < kBoundsBlock / 64; ++w) {
-      o += cnt[w][0];
-      f += cnt[w][1];
-    }
     int r[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    int tot = 0;
     for (int w = 0; w < kBoundsBlock / 64; ++w) {
       for (int a = 0; a < 3; ++a) r[a] = min(r[a], red[w][a]);
       for (int a = 3; a < 6; ++a) r[a] = max(r[a], red[w][a]);
+      tot += red[w][6];
     }
     int* bp = bpart + 8 * (size_t)blockIdx.x;
-    bp[0] = one_scene ? bscene[0] : -1;
+    bp[0] = one_scene ? scene_of(off, nscenes, base) : -1;
     for (int a = 0; a < 6; ++a) bp[1 + a] = r[a];
-    bp[7] = (int)(o | (f << 16));
+    bp[7] = tot;
   }
   if (one_scene) return;  // block-uniform
-  const int s0 = __shfl(s, 0);  // lane 0 is valid whenever any lane is
-  if (__all(!valid || s == s0)) {
-    if (lane == 0 && s0 >= 0 && xi >= mi) {
-      int* m = minmax + 6 * s0;
-      atomicMin(m + 0, mi);
-      atomicMin(m + 1, mj);
-      atomicMin(m + 2, mk);
-      atomicMax(m + 3, xi);
-      atomicMax(m + 4, xj);
-      atomicMax(m + 5, xk);
+  // a block that straddles scenes: per particle slot q, per wave where the
+  // wave's particles share a scene, else per lane
+#pragma unroll
+  for (int q = 0; q < kBoundsPer; ++q) {
+    const int i = base + q * kBoundsBlock + (int)threadIdx.x;
+    const bool valid = i < n;
+    const int s0 = __shfl(sq[q], 0);  // lane 0 is valid whenever any lane is
+    if (__all(!valid || sq[q] == s0)) {
+      int m[6];
+      for (int a = 0; a < 3; ++a) {
+        m[a] = wave_min_full(inq[q] ? cq[q][a] : INT_MAX);
+        m[3 + a] = wave_max_full(inq[q] ? cq[q][a] : INT_MIN);
+      }
+      if (lane == 0 && s0 >= 0 && m[3] >= m[0]) {
+        int* box = minmax + 6 * s0;
+        for (int a = 0; a < 3; ++a) atomicMin(box + a, m[a]);
+        for (int a = 3; a < 6; ++a) atomicMax(box + a, m[a]);
+      }
+    } else if (inq[q]) {
+      int* box = minmax + 6 * sq[q];
+      for (int a = 0; a < 3; ++a) atomicMin(box + a, cq[q][a]);
+      for (int a = 0; a < 3; ++a) atomicMax(box + 3 + a, cq[q][a]);
     }
-  } else if (inside) {
-    int* m = minmax + 6 * s;
-    atomicMin(m + 0, ci);
-    atomicMin(m + 1, cj);
-    atomicMin(m + 2, ck);
-    atomicMax(m + 3, ci);
-    atomicMax(m + 4, cj);
-    atomicMax(m + 5, ck);
   }
 }
 
@@ -329,8 +350,8 @@ __global__ __launch_bounds__(kReduceBlock) void k_scene_reduce(const int* __rest
                                                                int* __restrict__ minmax) {
   __shared__ int red[kReduceBlock / 64][6];
   const int s = blockIdx.x;
-  const int b0 = off[s] / kBoundsBlock;
-  const int b1 = off[s + 1] > off[s] ? (off[s + 1] - 1) / kBoundsBlock + 1 : b0;
+  const int b0 = off[s] / kBoundsSpan;
+  const int b1 = off[s + 1] > off[s] ? (off[s + 1] - 1) / kBoundsSpan + 1 : b0;
   int r[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
   int b = b0 + (int)threadIdx.x;
   for (; b + 3 * kReduceBlock < b1; b += 4 * kReduceBlock) {
@@ -363,20 +384,41 @@ __global__ __launch_bounds__(kReduceBlock) void k_scene_reduce(const int* __rest
 }
 
 // Windows and compact cell bases of all scenes (one block); each scene's box
-// is emptied again for the next sort. The block also sets the search's
-// counters: the out-of-grid and non-finite totals of the bounds pass, the cell
-// total, and zero for the counters the neighbor pass accumulates.
+// is emptied again for the next sort. With one scene the block also reduces
+// the bounds blocks' boxes (k_scene_reduce is not launched). The block also
+// sets the search's counters: the out-of-grid and non-finite totals of the
+// bounds pass, the cell total, and zero for the counters the neighbor pass
+// accumulates.
 __global__ __launch_bounds__(1024) void k_scene_windows(int* __restrict__ minmax, int nscenes,
                                                         SceneWin* __restrict__ win, int* __restrict__ total_cells,
                                                         const int* __restrict__ bpart, int nbounds,
                                                         unsigned long long* __restrict__ counters) {
   __shared__ unsigned long long red[2][16];
+  __shared__ int box[16][6];
   unsigned long long oog = 0, nonfinite = 0;
+  const bool one = nscenes == 1;
+  int r[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
 #pragma unroll 4
   for (int b = threadIdx.x; b < nbounds; b += blockDim.x) {
     const unsigned v = (unsigned)bpart[8 * (size_t)b + 7];
     oog += v & 0xffffu;
     nonfinite += v >> 16;
+    if (one) merge_box(r, bpart, b, 0);
+  }
+  if (one) {  // every lane of the block is active
+    for (int a = 0; a < 3; ++a) r[a] = wave_min_full(r[a]);
+    for (int a = 3; a < 6; ++a) r[a] = wave_max_full(r[a]);
+    if ((threadIdx.x & 63) == 0)
+      for (int a = 0; a < 6; ++a) box[threadIdx.x >> 6][a] = r[a];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int w = 1; w < (int)(blockDim.x >> 6); ++w) {
+        for (int a = 0; a < 3; ++a) r[a] = min(r[a], box[w][a]);
+        for (int a = 3; a < 6; ++a) r[a] = max(r[a], box[w][a]);
+      }
+      for (int a = 0; a < 3; ++a) minmax[a] = min(minmax[a], r[a]);
+      for (int a = 3; a < 6; ++a) minmax[a] = max(minmax[a], r[a]);
+    }
   }
 #pragma unroll
   for (int d = 32; d > 0; d >>= 1) {
@@ -565,10 +607,11 @@ template <bool kCellOrder>
 void sort_in(const SortBufs& sb, const GridDev& g, const SortIn& in, int n, float4* xn, float4* x0, float4* vs,
              const PhysConst* integrate, unsigned long long* counters, hipStream_t st) {
   const int nb = (n + kBlock - 1) / kBlock;
-  const int nbounds = (n + kBoundsBlock - 1) / kBoundsBlock;
+  const int nbounds = (n + kBoundsSpan - 1) / kBoundsSpan;
   k_scene_bounds<kCellOrder><<<nbounds, kBoundsBlock, 0, st>>>(in, n, g, sb.scene_off, sb.nscenes, sb.sid,
                                                                sb.scene_minmax, sb.bpart);
-  k_scene_reduce<<<sb.nscenes, kReduceBlock, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
+  if (sb.nscenes > 1)
+    k_scene_reduce<<<sb.nscenes, kReduceBlock, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
   k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, sb.bpart, nbounds,
                                       counters);
   k_bin<kCellOrder><<<nb, kBlock, 0, st>>>(in, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
