@@ -159,7 +159,7 @@ struct NbrOut {
 struct SortBufs {
   const int* scene_off;   // nscenes + 1 particle offsets
   int nscenes;
-  int* sid;               // scene of each particle, original order
+  int* sid;               // scene of each input particle (in the input's order)
   int* ssid;              // scene of each sorted particle
   int* scene_minmax;      // 6 per scene: min i,j,k then max i,j,k of in-grid cells
   SceneWin* win;          // per scene
@@ -167,10 +167,10 @@ struct SortBufs {
   long long cell_cap;     // allocated compact cells
   int* cell_cnt;          // cell_cap + 1
   int* cell_start;        // cell_cap + 2
-  int* key;
-  int* slot;
-  int* tmp;
-  int* perm;
+  int* key;               // compact cell of each input particle
+  int* slot;              // its arrival slot in the cell
+  int* tmp;               // input indices scattered into cell order
+  int* perm;              // original index of each sorted particle
   int* scan_part;         // partial sums of the device-length scan
   int* bpart;             // per block of the bounds pass: scene and cell box (bounds_part_ints)
   float* xc;              // sorted x^n as three planes (x, y, z), plane stride xc_stride
@@ -221,7 +221,8 @@ void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* co
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
                                 float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr);
 // last: write x/v to original order, or with v_sorted the positions to xd and
-// the velocities to v_sorted (sorted order, for the viscosity passes).
+// the velocities to v_sorted (sorted order, for the viscosity passes or the
+// next step of the call; id_out then receives each particle's original index).
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc,
                               bool last, const float4* x0, const float4* vs, const int* perm,
                               float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr,
