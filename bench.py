@@ -61,6 +61,47 @@ def profiled_traffic(kernel):
     return None
 
 
+def achievable_peaks(torch, pkg_dir, reps=10):
+    """Achievable rates on this GPU from the probes of libpbf_peaks.so: a float4
+    copy of 1 GiB (read + write bytes per second) and chains of FP32 fused
+    multiply-adds (non-packed, as the library is built), each timed with HIP
+    events on the stream it runs on."""
+    import ctypes as C
+    lib = C.CDLL(str(Path(pkg_dir) / "_build" / "libpbf_peaks.so"))
+    lib.pbf_peak_copy.argtypes = [C.c_void_p, C.c_void_p, C.c_longlong, C.c_void_p]
+    lib.pbf_peak_fma.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p]
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    sh = C.c_void_p(stream.cuda_stream)
+    n4 = 1 << 26  # 1 GiB of float4
+    a = torch.ones(4 * n4, dtype=torch.float32, device=dev)
+    b = torch.empty_like(a)
+    blocks, iters = 256 * 32, 2048
+    f = torch.empty(blocks * 256, dtype=torch.float32, device=dev)
+
+    def timed(launch):
+        for _ in range(2):
+            if launch() != 0:
+                raise RuntimeError("peak probe launch failed")
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(reps):
+            launch()
+        e1.record(stream)
+        torch.cuda.synchronize(dev)
+        return e0.elapsed_time(e1) / 1e3 / reps
+
+    t_copy = timed(lambda: lib.pbf_peak_copy(C.c_void_p(a.data_ptr()), C.c_void_p(b.data_ptr()), n4, sh))
+    t_fma = timed(lambda: lib.pbf_peak_fma(C.c_void_p(f.data_ptr()), blocks, iters, sh))
+    if not torch.equal(a, b):
+        raise RuntimeError("peak copy probe gave a wrong copy")
+    del a, b, f
+    return {"hbm_copy_gbs": round(2 * 16 * n4 / t_copy / 1e9, 1),
+            "fp32_fma_tflops": round(16.0 * iters * blocks * 256 / t_fma / 1e12, 2),
+            "probe": "float4 copy of 1 GiB, four per thread; 8 independent FMA chains per thread, "
+                     "2,048 rounds, 2M threads"}
+
+
 def algorithmic_flops(n, pairs, cand, iters):
     return 9 * cand + 57 * iters * pairs + 30 * n
 
@@ -365,6 +406,7 @@ def main():
     ap.add_argument("--serial-steps", type=int, default=10, help="steps of its serial per-scene loop")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable and batched-inverse workloads")
+    ap.add_argument("--no-peaks", action="store_true", help="skip the achievable copy / FMA rate probes")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
@@ -432,6 +474,12 @@ def main():
                  "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)},
         "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2),
     }
+    if not args.no_peaks:
+        peaks = achievable_peaks(torch, Path(pkg.__file__).parent)
+        peaks["hbm_frac"] = round(achieved_gbs / peaks["hbm_copy_gbs"], 4)
+        peaks["valu_frac"] = round(achieved_tflops / peaks["fp32_fma_tflops"], 4)
+        roofline["achievable"] = peaks
+        progress(f"achievable: copy {peaks['hbm_copy_gbs']} GB/s, FP32 FMA {peaks['fp32_fma_tflops']} TFLOP/s")
     b_alg = algorithmic_bytes(n, ncell, pairs, iters)
     f_alg = algorithmic_flops(n, pairs, cand, iters)
     step_s = ms_step / 1e3
