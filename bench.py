@@ -474,6 +474,8 @@ def main():
                  "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)},
         "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2),
     }
+    if roofline["traffic"]:  # measured HBM bytes per launch over the launch time
+        roofline["traffic_gbs"] = round(roofline["traffic"] / t_dom / 1e9, 1)
     if not args.no_peaks:
         peaks = achievable_peaks(torch, Path(pkg.__file__).parent)
         peaks["hbm_frac"] = round(achieved_gbs / peaks["hbm_copy_gbs"], 4)
