@@ -226,6 +226,22 @@ def config2b(torch, fluid, scene, steps=200, dense_steps=20):
                                        "mean": round(float(counts.mean()), 1), "max": int(counts.max())}}
 
 
+def shard_share(torch, fluid, scene, steps=200, warmup=5):
+    """One GPU's share of the 1M lattice split into 8 slabs along x: 16 of the
+    128 columns (131,072 owned particles) plus 3 ghost columns on each side
+    (h = 2.5 spacings), 180,224 particles, stepped at that shape. The halo
+    exchange itself (ghost x before each density pass, ghost lambda before each
+    displacement pass) is not run."""
+    x = scene.jittered_lattice((22, 64, 128), (0.18 + 53 * 0.005, 0.40, 0.18), 0.005, 0.1, seed=2)
+    ms, runs, ctx, xd, vd = gpu_rate(torch, fluid, x, steps, warmup)
+    owned = 16 * 64 * 128
+    progress(f"shard share: {len(x)} particles, {ms:.4f} ms/step")
+    return {"workload": "one of 8 x-slabs of the 1M lattice: 16 owned + 2 x 3 ghost columns (x 64 x 128)",
+            "particles": len(x), "owned": owned, "steps": steps, "ms_per_step": round(ms, 4),
+            "runs_ms_per_step": [round(t, 4) for t in runs],
+            "owned_particle_steps_per_s": round(owned / (ms / 1e3), 1)}
+
+
 def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
     """256 independent lattice scenes, n_s log-uniform in [2048, 65536] (seed
     4), each in its own [0,1]^3: all scenes in one pbf_batch_step call per
@@ -501,6 +517,8 @@ def main():
         "step_roofline": whole_step,
         "runs_ms_per_step": [round(t, 4) for t in runs],
     }
+    if not args.no_configs:
+        out["shard_share"] = shard_share(torch, fluid, pkg.scene)
     if not args.no_batched:
         del x, v, rho, ctx
         out["batched_vs_serial"] = batched_vs_serial(torch, fluid, pkg.scene, args.batched_steps, args.serial_steps)
