@@ -26,27 +26,54 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "particle-steps/s at 1M PBD fluid; batched vs serial; bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
-FP32_PEAK_TFLOPS = 157.3
+FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed issue)
+FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32, which this build disables
 
-# Algorithmic traffic per particle and per neighbor pair, and flops per pair
-# (fp32, the pass structure of the reference):
-#   sort/bin 60 B, integrate 48 B, per iteration density+lambda 32 B and
-#   displacement 40 B, velocity update 48 B per particle; grid arrays 12 B per
-#   cell; each pass reads the neighbor list, 2 B per pair (16-bit entries; the
-#   reference's int32 list is 4 B), and the neighbor pass writes it.
-#   Flops: 9 per candidate pair in the neighbor test (every particle of the
-#   27 stencil cells), 33 per neighbor pair in density+lambda, 24 in
-#   displacement; 30 per particle.
+# Algorithmic traffic and flops (fp32, the reference's pass structure; neighbor
+# data count as on-chip, so no list bytes):
+#   bytes per particle: sort/bin 60, integrate 48, per iteration
+#   density+lambda 32 and displacement 40, velocity update 48; 12 per grid cell
+#   (the reference grid GridHash::setSpace builds, nx*ny*nz cells).
+#   flops per neighbor pair (P = sum of neighbor-set sizes): 9 for the distance
+#   test of the neighbor pass, 33 in density+lambda, 24 in displacement; 30 per
+#   particle for the per-particle terms.
 BYTES_PASS_A = 32
 BYTES_PASS_B = 40
-BYTES_LIST = 2
+BYTES_NEIGHBORS = 16  # reads x^n once
 FLOPS_PAIR_A = 33
 FLOPS_PAIR_B = 24
+FLOPS_PAIR_NBR = 9
 TRAFFIC_PROFILE = ROOT / "profiles" / "pbf_step_lattice_1m_hbm_traffic.md"
 
 
-def algorithmic_bytes(n, ncell, pairs, iters):
-    return (60 + 48 + 72 * iters + 48) * n + 12 * ncell + BYTES_LIST * pairs * (2 * iters + 1)
+def algorithmic_bytes(n, ncell, iters):
+    """B_alg = (60 + 48 + 72 K + 48) N + 12 C bytes per step."""
+    return (60 + 48 + (BYTES_PASS_A + BYTES_PASS_B) * iters + 48) * n + 12 * ncell
+
+
+def algorithmic_flops(n, pairs, iters):
+    """F_alg = (9 + 57 K) P + 30 N flops per step (true neighbor pairs only)."""
+    return (FLOPS_PAIR_NBR + (FLOPS_PAIR_A + FLOPS_PAIR_B) * iters) * pairs + 30 * n
+
+
+def kernel_counts(kernel, n, pairs):
+    """(algorithmic bytes, flops) of one launch of `kernel` over n particles
+    with `pairs` neighbor pairs."""
+    return {"k_density_lambda": (BYTES_PASS_A * n, FLOPS_PAIR_A * pairs),
+            "k_displacement": (BYTES_PASS_B * n, FLOPS_PAIR_B * pairs),
+            "k_neighbors": (BYTES_NEIGHBORS * n, FLOPS_PAIR_NBR * pairs)}[kernel]
+
+
+def roofline_of(bytes_, flops, seconds):
+    """Position of `bytes_` and `flops` done in `seconds` against the HBM and
+    FP32 peaks: the bound is the larger fraction."""
+    hbm = bytes_ / seconds / (HBM_PEAK_GBS * 1e9)
+    valu = flops / seconds / (FP32_PEAK_TFLOPS * 1e12)
+    if valu >= hbm:
+        return {"bound": "valu", "achieved": round(flops / seconds / 1e12, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(valu, 4), "hbm_frac": round(hbm, 4), "valu_frac": round(valu, 4)}
+    return {"bound": "hbm", "achieved": round(bytes_ / seconds / 1e9, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(hbm, 4), "hbm_frac": round(hbm, 4), "valu_frac": round(valu, 4)}
 
 
 def profiled_traffic(kernel):
@@ -102,10 +129,6 @@ def achievable_peaks(torch, pkg_dir, reps=10):
                      "2,048 rounds, 2M threads"}
 
 
-def algorithmic_flops(n, pairs, cand, iters):
-    return 9 * cand + 57 * iters * pairs + 30 * n
-
-
 def progress(msg):
     """One line per finished phase on stderr (the JSON result stays alone on stdout)."""
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
@@ -137,18 +160,25 @@ def oracle_rate(x, steps, threads, warm=1):
     return len(x) * steps / dt, used, dt
 
 
-def cpu_baseline(x, steps, threads):
+def cpu_baseline(x, steps, threads, runs=3):
     """The oracle (C++ restatement of the reference path, OpenMP over
-    particles) on this host: one warm-up step, then `steps` timed steps on all
-    the threads given, and one step on one thread."""
-    rate, used, dt = oracle_rate(x, steps, threads)
-    progress(f"cpu baseline: {steps} steps on {used} threads in {dt:.1f} s")
+    particles) on this host: `runs` runs, each one warm-up step and then
+    `steps` timed steps on `threads` threads (median), and one step on one
+    thread."""
+    rates, secs = [], []
+    for _ in range(runs):
+        rate, used, dt = oracle_rate(x, steps, threads)
+        rates.append(rate)
+        secs.append(dt)
+    progress(f"cpu baseline: {runs} x {steps} steps on {used} threads, {sum(secs):.1f} s")
     rate1, _, dt1 = oracle_rate(x, 1, 1, warm=0)
     progress(f"cpu baseline: 1 step on 1 thread in {dt1:.1f} s")
-    return {"value": rate, "unit": "particle-steps/s", "cores": used, "kind": "port",
-            "sample": f"{len(x)}-particle lattice (the benchmark workload), {steps} steps after 1 warm-up step, "
-                      f"{dt:.1f} s", "one_thread": {"value": rate1, "sample": f"1 step, {dt1:.1f} s"},
-            "cpu": cpu_model()}
+    return {"value": float(np.median(rates)), "unit": "particle-steps/s", "cores": used, "kind": "port",
+            "sample": f"{len(x)}-particle lattice (the benchmark workload): median of {runs} runs of {steps} steps, "
+                      f"each after 1 warm-up step ({', '.join(f'{t:.2f}' for t in secs)} s)",
+            "runs": [round(r, 1) for r in rates],
+            "one_thread": {"value": rate1, "sample": f"1 step, {dt1:.1f} s"}, "host": host_cpu(),
+            "threads_rule": "OMP_NUM_THREADS when set (the process's CPU share), else all CPUs of its affinity"}
 
 
 def gpu_rate(torch, fluid, x_np, steps, warmup, runs=3):
@@ -176,6 +206,64 @@ def gpu_rate(torch, fluid, x_np, steps, warmup, runs=3):
         torch.cuda.synchronize()
         times.append(e0.elapsed_time(e1) / steps)
     return float(np.median(times)), times, ctx, x, v
+
+
+def profiled_run(torch, fluid, ctx, x_np, steps, warmup):
+    """The timed run again from the same initial state (`warmup` + 1 untimed
+    steps, then `steps` steps in one call) with HIP events around every pass;
+    then the same trajectory as single-step calls for the neighbor-pair count
+    of each timed step. Returns ({pass: (ms per launch, launches)}, pairs per
+    timed step)."""
+    dev = torch.device("cuda:0")
+    x = torch.from_numpy(x_np).to(dev)
+    v = torch.zeros_like(x)
+    ctx.step(x, v, None, max(warmup, 1))
+    ctx.step(x, v, None, 1)
+    torch.cuda.synchronize()
+    ctx.set_profiling(True)
+    ctx.step(x, v, None, steps)
+    prof = ctx.profile()
+    ctx.set_profiling(False)
+    pass_ms = {k: (ms / max(cnt, 1), cnt) for k, (ms, cnt) in prof.items()}
+    x = torch.from_numpy(x_np).to(dev)
+    v = torch.zeros_like(x)
+    ctx.step(x, v, None, max(warmup, 1))
+    ctx.step(x, v, None, 1)
+    pairs = []
+    for _ in range(steps):
+        ctx.step(x, v, None, 1)
+        pairs.append(int(ctx.stats()["neighbor_pairs"]))
+    return pass_ms, pairs
+
+
+def host_cpu():
+    """Model, sockets, physical cores and the CPUs this process may run on
+    (lscpu where present, /proc/cpuinfo otherwise)."""
+    info = {"model": cpu_model(), "affinity_cpus": len(os.sched_getaffinity(0))}
+    try:
+        import subprocess
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        kv = {k.strip(): v.strip() for k, v in (ln.split(":", 1) for ln in out.splitlines() if ":" in ln)}
+        info["model"] = kv.get("Model name", info["model"])
+        for key, name in (("Socket(s)", "sockets"), ("Core(s) per socket", "cores_per_socket"),
+                          ("Thread(s) per core", "threads_per_core"), ("CPU(s)", "cpus")):
+            if key in kv and kv[key].isdigit():
+                info[name] = int(kv[key])
+        if "sockets" in info and "cores_per_socket" in info:
+            info["physical_cores"] = info["sockets"] * info["cores_per_socket"]
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def baseline_threads():
+    """Host threads for the CPU baseline: OMP_NUM_THREADS when the environment
+    sets it (the process's share of the host), else every CPU this process may
+    run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return min(int(env), len(os.sched_getaffinity(0)))
+    return len(os.sched_getaffinity(0))
 
 
 def config1(torch, fluid, scene, threads):
@@ -454,56 +542,47 @@ def main():
     value = n / (ms_step / 1e3)
 
     stats = ctx.stats()
-    pairs = stats["neighbor_pairs"]
     cand = stats["candidates"]
-    ncell = int(stats["cells_used"])  # compact cells the sort touches (occupied window + bucket)
+    ncell_compact = int(stats["cells_used"])  # compact cells the sort touches (occupied window + bucket)
+    ncell = int(np.prod(stats["grid_dims"]))   # the reference grid (GridHash::setSpace)
     iters = params["iterations"]
 
-    # per-pass timing: up to 50 more steps with HIP events around every pass
-    # (recorded on the kernels' stream)
-    ctx.set_profiling(True)
-    ctx.step(x, v, rho, min(args.steps, 50))
-    prof = ctx.profile()
-    ctx.set_profiling(False)
-    progress("per-pass profile done")
-    pass_ms = {k: (ms / max(cnt, 1), cnt) for k, (ms, cnt) in prof.items()}
-    dom = max(("density_lambda", "displacement", "neighbors", "sort"), key=lambda k: pass_ms[k][0] * pass_ms[k][1])
+    # Per-pass timing of the timed region: one more run from the same initial
+    # state with the same warm-up and steps, HIP events around every pass on
+    # the kernels' stream; the neighbor-pair count of each of its steps from
+    # single-step calls along the same trajectory (bit-identical to one call).
+    pass_ms, pairs_per_step = profiled_run(torch, fluid, ctx, x_np, args.steps, args.warmup)
+    pairs = float(np.mean(pairs_per_step))
+    progress(f"per-pass profile done: mean P {pairs:.4g} over the timed steps")
+    dom = max(("density_lambda", "displacement", "neighbors"), key=lambda k: pass_ms[k][0] * pass_ms[k][1])
     t_dom = pass_ms[dom][0] / 1e3
-    per_launch_bytes = {"density_lambda": BYTES_PASS_A * n + BYTES_LIST * pairs,
-                        "displacement": BYTES_PASS_B * n + BYTES_LIST * pairs,
-                        "neighbors": 28 * n + BYTES_LIST * pairs, "sort": 60 * n + 12 * ncell}[dom]
-    per_launch_flops = {"density_lambda": FLOPS_PAIR_A * pairs, "displacement": FLOPS_PAIR_B * pairs,
-                        "neighbors": 9 * cand, "sort": 0}[dom]
-    kname = {"density_lambda": "k_density_lambda", "displacement": "k_displacement",
-             "neighbors": "k_neighbors", "sort": "sort passes"}[dom]
-    achieved_tflops = per_launch_flops / t_dom / 1e12
-    achieved_gbs = per_launch_bytes / t_dom / 1e9
-    roofline = {
-        "bound": "hbm",
+    kname = {"density_lambda": "k_density_lambda", "displacement": "k_displacement", "neighbors": "k_neighbors"}[dom]
+    kb, kf = kernel_counts(kname, n, pairs)
+    roofline = roofline_of(kb, kf, t_dom)
+    roofline.update({
         "kernel": kname,
-        "achieved": round(achieved_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved_gbs / HBM_PEAK_GBS, 4),
         "traffic": profiled_traffic(kname),
         "traffic_source": TRAFFIC_PROFILE.name,
-        "algorithmic_bytes": per_launch_bytes,
-        "valu": {"achieved": round(achieved_tflops, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                 "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4)},
-        "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2),
-    }
+        "algorithmic_bytes": int(kb), "algorithmic_flops": int(kf),
+        "valu_frac_nonpacked": round(kf / t_dom / (FP32_NONPACKED_TFLOPS * 1e12), 4),
+        "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2), "launches": pass_ms[dom][1],
+        "neighbor_pairs_mean": int(round(pairs)),
+        "counting": "bytes 32 N (density+lambda) / 40 N (displacement) / 16 N (neighbors), no list bytes; "
+                    "flops 33 P / 24 P / 9 P; P = mean neighbor pairs over the timed steps",
+    })
     if roofline["traffic"]:  # measured HBM bytes per launch over the launch time
         roofline["traffic_gbs"] = round(roofline["traffic"] / t_dom / 1e9, 1)
+        roofline["traffic_over_algorithmic"] = round(roofline["traffic"] / kb, 2)
     if not args.no_peaks:
         peaks = achievable_peaks(torch, Path(pkg.__file__).parent)
-        peaks["hbm_frac"] = round(achieved_gbs / peaks["hbm_copy_gbs"], 4)
-        peaks["valu_frac"] = round(achieved_tflops / peaks["fp32_fma_tflops"], 4)
+        peaks["hbm_frac"] = round(kb / t_dom / 1e9 / peaks["hbm_copy_gbs"], 4)
+        peaks["valu_frac"] = round(kf / t_dom / 1e12 / peaks["fp32_fma_tflops"], 4)
         roofline["achievable"] = peaks
         progress(f"achievable: copy {peaks['hbm_copy_gbs']} GB/s, FP32 FMA {peaks['fp32_fma_tflops']} TFLOP/s")
-    b_alg = algorithmic_bytes(n, ncell, pairs, iters)
-    f_alg = algorithmic_flops(n, pairs, cand, iters)
+    b_alg = algorithmic_bytes(n, ncell, iters)
+    f_alg = algorithmic_flops(n, pairs, iters)
     step_s = ms_step / 1e3
-    whole_step = {"bytes_alg": b_alg, "flops_alg": f_alg,
-                  "hbm_fraction": round(b_alg / step_s / (HBM_PEAK_GBS * 1e9), 4),
-                  "valu_fraction": round(f_alg / step_s / (FP32_PEAK_TFLOPS * 1e12), 4)}
+    whole_step = {"bytes_alg": int(b_alg), "flops_alg": int(f_alg), **roofline_of(b_alg, f_alg, step_s)}
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "particle-steps/s", "n_gpus": 1,
@@ -511,9 +590,11 @@ def main():
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": args.workload, "particles": n, "smoothing_length": params["smoothing_length"],
                    "spacing": params["sampling_distance"], "iterations": iters, "dt": params["dt"],
-                   "grid_cells": ncell, "neighbor_pairs": pairs, "candidate_pairs": cand},
+                   "grid_cells": ncell, "compact_cells": ncell_compact, "neighbor_pairs_mean": int(round(pairs)),
+                   "neighbor_pairs_last": int(pairs_per_step[-1]), "candidate_pairs_last": cand},
         "roofline": roofline,
         "passes_ms": {k: round(v[0], 4) for k, v in pass_ms.items()},
+        "passes_launches": {k: v[1] for k, v in pass_ms.items()},
         "step_roofline": whole_step,
         "runs_ms_per_step": [round(t, 4) for t in runs],
     }
@@ -525,7 +606,7 @@ def main():
     if not args.no_diff:
         out["differentiable"] = differentiable(torch, fluid, pkg.scene)
         out["batched_inverse"] = batched_inverse(torch, fluid, pkg.scene)
-    threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+    threads = args.cpu_threads or baseline_threads()
     if not args.no_configs:
         out["config1"] = config1(torch, fluid, pkg.scene, threads)
         out["config2b"] = config2b(torch, fluid, pkg.scene)

@@ -9,8 +9,11 @@
  *   SummationDensity::compute         (Dynamics/ParticleSystem/SummationDensity.cu:59-110)
  * with field semantics kept: positions and velocities are packed xyz float
  * triples (the reference's Vector<float,3>, Core/Vector/vector_3d.h:31), the
- * neighborhood is a CSR list (Framework/Topology/NeighborList.h:27-66), the
- * density is one float per particle.
+ * neighborhood is the reference's NeighborList layout
+ * (Framework/Topology/NeighborList.h:27-66): `index`, n exclusive row starts
+ * (NeighborList::m_index, n entries), and `elements`, nelems ints
+ * (m_elements); row i ends at index[i+1], the last row at nelems
+ * (NeighborList::getNeighborSize). The density is one float per particle.
  *
  * Conventions
  *  - Every function returns an int status: PBF_OK (0) or a negative
@@ -32,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 3
+#define PBF_ABI_VERSION 4
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -43,8 +46,8 @@ typedef enum pbf_status {
   PBF_ERR_UNSUPPORTED = -5  /* size or configuration outside what the library handles */
 } pbf_status;
 
-/* Parameters of the step. pbf_params_default() gives the reference's values
- * with the fluid workload's smoothing length. */
+/* Parameters of the step. pbf_params_default() gives the benchmark workload's
+ * values (see there); the reference's own defaults differ in h and K. */
 typedef struct pbf_params {
   float smoothing_length;  /* h: NeighborQuery.inRadius (search radius and grid
                               cell, NeighborQuery.cu:142) and DensityPBD /
@@ -134,11 +137,18 @@ typedef struct pbf_ctx pbf_ctx;
 int pbf_abi_version(void);
 const char* pbf_status_string(int status);
 
-/* Reference defaults: h 0.0125, d 0.005, rho0 1000, dt 0.001, g (0,-9.8,0),
- * 5 iterations, lambda constants 1000 / 0.1, correction 10, pair eps 1e-6,
- * scene [0,1]^3; viscosity 1 with 5 iterations, box [0,1]^3 sampled at 0.005
- * with normal friction 0.95 and tangential friction 0; flags 0 (the
- * viscosity and the box are off until their flag is set). */
+/* Workload defaults (the fluid benchmark: lattice spacing 0.005 with
+ * h = 2.5 spacings and 5 PBD iterations): h 0.0125, d 0.005, rho0 1000,
+ * dt 0.001, g (0,-9.8,0), 5 iterations, lambda constants 1000 / 0.1,
+ * correction 10, pair eps 1e-6, scene [0,1]^3; viscosity 1 with 5
+ * iterations, box [0,1]^3 sampled at 0.005 with normal friction 0.95 and
+ * tangential friction 0; flags 0 (the viscosity and the box are off until
+ * their flag is set).
+ * The reference's own defaults differ in two fields: the model's smoothing
+ * length is 0.006 (PositionBasedFluidModel.cpp:25) and DensityPBD runs 3
+ * iterations (DensityPBD.cu:242); set smoothing_length and iterations to
+ * those values to run the reference's default model. Every other field
+ * equals the reference's value or literal. */
 void pbf_params_default(pbf_params* p);
 
 /* Creates a context on the current HIP device. Replaces module construction
@@ -159,14 +169,14 @@ int pbf_reserve(pbf_ctx* ctx, int n);
 /* Same for a batch of up to nscenes scenes holding n particles in total. */
 int pbf_reserve_batch(pbf_ctx* ctx, int n, int nscenes);
 
-/* NeighborQuery::compute (NeighborQuery.cu:152-199): CSR neighbor lists of n
+/* NeighborQuery::compute (NeighborQuery.cu:152-199): neighbor lists of n
  * particles at d_x (original order, original ids, self included, strict
- * |x_i - x_j| < h). d_offsets receives n+1 exclusive offsets (the reference's
- * m_index plus the total). Rows are written when the total fits in `cap`
- * elements (d_elems may be NULL to size only); *total receives the total.
- * Returns PBF_ERR_CAPACITY when cap < total. Blocking: the total is read back,
- * as the reference does (NeighborQuery.cu:463). */
-int pbf_neighbor_query(pbf_ctx* ctx, const float* d_x, int n, int* d_offsets, int* d_elems,
+ * |x_i - x_j| < h) in the NeighborList layout: d_index receives the n row
+ * starts (m_index), *total the element count (m_elements.size()). Rows are
+ * written when the total fits in `cap` elements (d_elems may be NULL to size
+ * only). Returns PBF_ERR_CAPACITY when cap < total. Blocking: the total is
+ * read back, as the reference does (NeighborQuery.cu:463). */
+int pbf_neighbor_query(pbf_ctx* ctx, const float* d_x, int n, int* d_index, int* d_elems,
                        long long cap, long long* total, void* stream);
 
 /* NeighborQuery::compute with a neighbor limit (NeighborList::isLimited;
@@ -182,24 +192,26 @@ int pbf_neighbor_query_fixed(pbf_ctx* ctx, const float* d_x, int n, int limit, i
  * v += dt*(f + g), x += dt*v. d_f may be NULL (the zeroed force of begin()). */
 int pbf_integrate(pbf_ctx* ctx, float* d_x, float* d_v, const float* d_f, int n, void* stream);
 
-/* SummationDensity::compute (SummationDensity.cu:59-110) over a CSR list. */
-int pbf_density(pbf_ctx* ctx, const float* d_x, const int* d_offsets, const int* d_elems,
+/* SummationDensity::compute (SummationDensity.cu:59-110) over a caller
+ * NeighborList (d_index: n row starts, d_elems: nelems ids). */
+int pbf_density(pbf_ctx* ctx, const float* d_x, const int* d_index, const int* d_elems, long long nelems,
                 float* d_rho, int n, void* stream);
 
-/* DensityPBD::constrain (DensityPBD.cu:271-302) over a caller CSR list (any
- * list, symmetric or not): `iterations` Jacobi rounds of density, lambda and
- * pairwise displacement (float atomics, as the reference), then
- * v += (x - x_old)/dt. x and v are updated in place; d_rho (may be NULL)
- * receives the last iteration's density. */
-int pbf_pbd_constrain(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_offsets,
-                      const int* d_elems, float* d_rho, int n, void* stream);
+/* DensityPBD::constrain (DensityPBD.cu:271-302) over a caller NeighborList
+ * (d_index: n row starts, d_elems: nelems ids; any list, symmetric or not):
+ * `iterations` Jacobi rounds of density, lambda and pairwise displacement
+ * (float atomics, as the reference), then v += (x - x_old)/dt. x and v are
+ * updated in place; d_rho (may be NULL) receives the last iteration's
+ * density. */
+int pbf_pbd_constrain(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_index,
+                      const int* d_elems, long long nelems, float* d_rho, int n, void* stream);
 
 /* DensityPBD::constrain with DensityPBD::m_massInv set (the massInv variants
  * of K_ComputeLambdas and K_ComputeDisplacement, DensityPBD.cu:93-131,182-224):
  * d_mass_inv holds one inverse mass per particle. Otherwise as
  * pbf_pbd_constrain. */
-int pbf_pbd_constrain_mass(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_offsets, const int* d_elems,
-                           const float* d_mass_inv, float* d_rho, int n, void* stream);
+int pbf_pbd_constrain_mass(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_index, const int* d_elems,
+                           long long nelems, const float* d_mass_inv, float* d_rho, int n, void* stream);
 
 /* nsteps of PositionBasedFluidModel::step without viscosity
  * (PositionBasedFluidModel.cpp:83-101): begin -> NeighborQuery on x^n ->
@@ -242,10 +254,11 @@ int pbf_rollout_forward(pbf_ctx* ctx, float* d_x, float* d_v, float* d_rho, cons
                         int nscenes, int n, int nsteps, int checkpoint_every, void* stream);
 int pbf_rollout_backward(pbf_ctx* ctx, float* d_xbar, float* d_vbar, void* stream);
 
-/* CSR export of the neighbor lists built by the last pbf_step (its last step,
- * on the positions before that step's integration), original ids; same
- * contract as pbf_neighbor_query. Blocking. */
-int pbf_step_neighbors(pbf_ctx* ctx, int* d_offsets, int* d_elems, long long cap, long long* total,
+/* Export of the neighbor lists built by the last pbf_step (its last step, on
+ * the positions before that step's integration), original ids, NeighborList
+ * layout; same contract as pbf_neighbor_query (the outNeighborhood field of
+ * the fused step). Blocking. */
+int pbf_step_neighbors(pbf_ctx* ctx, int* d_index, int* d_elems, long long cap, long long* total,
                        void* stream);
 
 /* Diagnostics of the last neighbor search. Blocking (synchronises the stream
@@ -253,10 +266,10 @@ int pbf_step_neighbors(pbf_ctx* ctx, int* d_offsets, int* d_elems, long long cap
 int pbf_get_stats(pbf_ctx* ctx, pbf_stats* out);
 
 /* ImplicitViscosity::constrain (ImplicitViscosity.cu:99-128) over a caller
- * CSR list: viscosity_iterations Jacobi rounds on v (in place) with weights
- * from the positions d_x. */
-int pbf_viscosity(pbf_ctx* ctx, const float* d_x, float* d_v, const int* d_offsets, const int* d_elems, int n,
-                  void* stream);
+ * NeighborList: viscosity_iterations Jacobi rounds on v (in place) with
+ * weights from the positions d_x. */
+int pbf_viscosity(pbf_ctx* ctx, const float* d_x, float* d_v, const int* d_index, const int* d_elems,
+                  long long nelems, int n, void* stream);
 
 /* BoundaryConstraint::constrain (BoundaryConstraint.cu:30-84) with the box of
  * box_lo/box_hi/box_spacing: particles at or outside the walls are projected

@@ -96,15 +96,15 @@ SIGNATURES = {
     "pbf_neighbor_query": (C.c_int, [_CTX, _VP, C.c_int, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
     "pbf_neighbor_query_fixed": (C.c_int, [_CTX, _VP, C.c_int, C.c_int, _VP, _VP, _VP]),
     "pbf_integrate": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, _VP]),
-    "pbf_density": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, _VP]),
-    "pbf_pbd_constrain": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
-    "pbf_pbd_constrain_mass": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+    "pbf_density": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_longlong, _VP, C.c_int, _VP]),
+    "pbf_pbd_constrain": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_longlong, _VP, C.c_int, _VP]),
+    "pbf_pbd_constrain_mass": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_longlong, _VP, _VP, C.c_int, _VP]),
     "pbf_step": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, C.c_int, _VP]),
     "pbf_batch_step": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP]),
     "pbf_rollout_forward": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
     "pbf_rollout_backward": (C.c_int, [_CTX, _VP, _VP, _VP]),
     "pbf_step_neighbors": (C.c_int, [_CTX, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
-    "pbf_viscosity": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, _VP]),
+    "pbf_viscosity": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_longlong, C.c_int, _VP]),
     "pbf_box_constrain": (C.c_int, [_CTX, _VP, _VP, C.c_int, _VP]),
     "pbf_get_stats": (C.c_int, [_CTX, C.POINTER(pbf_stats)]),
     "pbf_set_profiling": (C.c_int, [_CTX, C.c_int]),

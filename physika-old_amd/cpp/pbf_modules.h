@@ -114,8 +114,9 @@ template <class T> class DeviceArrayField {
 };
 
 // NeighborList<int> (Framework/Topology/NeighborList.h:9-125): dynamic rows as
-// CSR (index: n+1 offsets here, the reference's n plus the total) or limited
-// rows of getNeighborLimit() slots (index: row sizes).
+// CSR (index: the n row starts, as the reference's m_index; the last row ends
+// at getElements().size()) or limited rows of getNeighborLimit() slots
+// (index: row sizes).
 class NeighborList {
  public:
   DeviceArray<int>& getIndex() { return index_; }
@@ -123,7 +124,7 @@ class NeighborList {
   bool isLimited() const { return limit_ > 0; }
   int getNeighborLimit() const { return limit_; }
   void setNeighborLimit(int n) { limit_ = n; }
-  int size() const { return limit_ ? index_.size() : (index_.size() > 0 ? index_.size() - 1 : 0); }
+  int size() const { return index_.size(); }
 
  private:
   DeviceArray<int> index_, elements_;
@@ -222,7 +223,7 @@ class NeighborQuery {
       return;
     }
     nl.setNeighborLimit(0);
-    nl.getIndex().resize(n + 1);
+    nl.getIndex().resize(n);
     long long total = 0;
     Context::check(pbf_neighbor_query(ctx_->get(), (const float*)x.getDataPtr(), n, nl.getIndex().getDataPtr(),
                                       nullptr, 0, &total, nullptr),
@@ -296,12 +297,13 @@ class DensityPBD {
     float* v = (float*)m_velocity.getValue().getDataPtr();
     if (m_massInv.isEmpty())
       Context::check(pbf_pbd_constrain(ctx_->get(), x, v, nl.getIndex().getDataPtr(), nl.getElements().getDataPtr(),
-                                       m_density.getValue().getDataPtr(), n, nullptr),
+                                       nl.getElements().size(), m_density.getValue().getDataPtr(), n, nullptr),
                      ctx_->get());
     else
       Context::check(pbf_pbd_constrain_mass(ctx_->get(), x, v, nl.getIndex().getDataPtr(),
-                                            nl.getElements().getDataPtr(), m_massInv.getValue().getDataPtr(),
-                                            m_density.getValue().getDataPtr(), n, nullptr),
+                                            nl.getElements().getDataPtr(), nl.getElements().size(),
+                                            m_massInv.getValue().getDataPtr(), m_density.getValue().getDataPtr(), n,
+                                            nullptr),
                      ctx_->get());
     return true;
   }
@@ -332,7 +334,7 @@ class ImplicitViscosity {
     NeighborList& nl = m_neighborhood.getValue();
     Context::check(pbf_viscosity(ctx_->get(), (const float*)m_position.getValue().getDataPtr(),
                                  (float*)m_velocity.getValue().getDataPtr(), nl.getIndex().getDataPtr(),
-                                 nl.getElements().getDataPtr(), n, nullptr),
+                                 nl.getElements().getDataPtr(), nl.getElements().size(), n, nullptr),
                    ctx_->get());
     return true;
   }

@@ -289,18 +289,17 @@ void launch_viscosity(const NbrView& nv, const float4* xs, const float4* v_old, 
 // Module path: one iteration over a caller CSR list in original order
 // (positions xa as float4; v_old, v_in float4; the last iteration writes the
 // caller's packed velocities).
-__global__ __launch_bounds__(kBlock) void k_csr_viscosity(const float4* __restrict__ xa, const int* __restrict__ off,
-                                                          const int* __restrict__ el,
+__global__ __launch_bounds__(kBlock) void k_csr_viscosity(const float4* __restrict__ xa, Csr nb,
                                                           const float4* __restrict__ v_old,
                                                           const float4* __restrict__ v_in,
                                                           float4* __restrict__ v_next, float* __restrict__ v_out,
-                                                          int n, VisConst vc) {
+                                                          VisConst vc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= nb.n) return;
   const float4 xi = xa[i];
   float tw = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  for (int e = off[i]; e < off[i + 1]; ++e) {
-    const int j = el[e];
+  for (int e = nb.begin(i), end = nb.end(i); e < end; ++e) {
+    const int j = nb.elems[e];
     const PairGeom g = pair_geom(xi, xa[j], vc.inv_h);
     if (g.r > vc.eps) {
       const float w = vc.cw * g.t;
@@ -327,9 +326,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_viscosity(const float4* __restri
   }
 }
 
-void launch_csr_viscosity(const float4* xa, const int* off, const int* el, const float4* v_old, const float4* v_in,
-                          float4* v_next, float* v_out, int n, const VisConst& vc, hipStream_t st) {
-  k_csr_viscosity<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, off, el, v_old, v_in, v_next, v_out, n, vc);
+void launch_csr_viscosity(const float4* xa, const Csr& nb, const float4* v_old, const float4* v_in, float4* v_next,
+                          float* v_out, const VisConst& vc, hipStream_t st) {
+  k_csr_viscosity<<<(nb.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, nb, v_old, v_in, v_next, v_out, vc);
 }
 
 }  // namespace pbf

@@ -636,27 +636,38 @@ int record_counters(pbf_ctx* c, hipStream_t st) {
   return PBF_OK;
 }
 
-int export_csr(pbf_ctx* c, int n, int* d_offsets, int* d_elems, long long cap, long long* total, hipStream_t st) {
+// CSR of the last search in original ids, in the reference's NeighborList
+// layout: d_index[0..n-1] exclusive row starts, *total elements.
+int export_csr(pbf_ctx* c, int n, int* d_index, int* d_elems, long long cap, long long* total, hipStream_t st) {
   if (n == 0) {
-    PBF_HIP(c, hipMemsetAsync(d_offsets, 0, sizeof(int), st));
-    PBF_HIP(c, hipStreamSynchronize(st));
     *total = 0;
     return PBF_OK;
   }
   launch_invert(c->perm, c->inv, n, st);
   launch_csr_count(c->nbr_cnt, c->inv, n, c->cnt_orig, st);
-  scan_exclusive(c->cnt_orig, d_offsets, n, c->scan_scratch, st);
+  // cnt_orig holds cap_n > n entries: the scan writes n + 1 in place
+  scan_exclusive(c->cnt_orig, c->cnt_orig, n, c->scan_scratch, st);
   PBF_LAUNCHED(c);
+  PBF_HIP(c, hipMemcpyAsync(d_index, c->cnt_orig, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, st));
   int tot = 0;
-  PBF_HIP(c, hipMemcpyAsync(&tot, d_offsets + n, sizeof(int), hipMemcpyDeviceToHost, st));
+  PBF_HIP(c, hipMemcpyAsync(&tot, c->cnt_orig + n, sizeof(int), hipMemcpyDeviceToHost, st));
   PBF_HIP(c, hipStreamSynchronize(st));
   *total = tot;
   if (d_elems == nullptr) return PBF_OK;
   if (cap < (long long)tot)
     return fail(c, PBF_ERR_CAPACITY, "neighbor element buffer holds " + std::to_string(cap) + ", need " +
                                          std::to_string(tot));
-  launch_csr_fill(view_of(c, n), c->perm, d_offsets, d_elems, cap, st);
+  launch_csr_fill(view_of(c, n), c->perm, c->cnt_orig, d_elems, cap, st);
   PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+// A caller's list (index of n row starts, nelems elements) as the kernels
+// read it.
+int caller_csr(pbf_ctx* c, const int* d_index, const int* d_elems, long long nelems, int n, Csr* out) {
+  if (nelems < 0 || nelems > INT32_MAX) return fail(c, PBF_ERR_INVALID, "nelems must be in [0, 2^31 - 1]");
+  if (nelems > 0 && !d_elems) return fail(c, PBF_ERR_INVALID, "null neighbor elements");
+  *out = Csr{d_index, d_elems, n, (int)nelems};
   return PBF_OK;
 }
 
@@ -1036,16 +1047,16 @@ int pbf_reserve_batch(pbf_ctx* c, int n, int nscenes) {
   return ensure_particles(c, n, nscenes);
 }
 
-int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_offsets, int* d_elems, long long cap,
+int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_index, int* d_elems, long long cap,
                        long long* total, void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (bad_n(n) || !total || !d_offsets || (n > 0 && !d_x) || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (bad_n(n) || !total || (n > 0 && (!d_x || !d_index)) || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
   hipStream_t st = (hipStream_t)stream;
   int s = ensure_particles(c, n, 1);
   if (s) return s;
   if (n == 0) {
     c->last_n = 0;
-    return export_csr(c, 0, d_offsets, d_elems, cap, total, st);
+    return export_csr(c, 0, d_index, d_elems, cap, total, st);
   }
   s = adapt_reservations(c);
   if (s) return s;
@@ -1054,14 +1065,15 @@ int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_offsets, int*
   if (s) return s;
   s = record_counters(c, st);
   if (s) return s;
-  return export_csr(c, n, d_offsets, d_elems, cap, total, st);
+  return export_csr(c, n, d_index, d_elems, cap, total, st);
 }
 
-int pbf_step_neighbors(pbf_ctx* c, int* d_offsets, int* d_elems, long long cap, long long* total, void* stream) {
+int pbf_step_neighbors(pbf_ctx* c, int* d_index, int* d_elems, long long cap, long long* total, void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (!d_offsets || !total || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (!total || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
   if (c->last_n < 0) return fail(c, PBF_ERR_INVALID, "no neighbor search has run on this context");
-  return export_csr(c, c->last_n, d_offsets, d_elems, cap, total, (hipStream_t)stream);
+  if (c->last_n > 0 && !d_index) return fail(c, PBF_ERR_INVALID, "null neighbor index");
+  return export_csr(c, c->last_n, d_index, d_elems, cap, total, (hipStream_t)stream);
 }
 
 int pbf_integrate(pbf_ctx* c, float* d_x, float* d_v, const float* d_f, int n, void* stream) {
@@ -1073,31 +1085,35 @@ int pbf_integrate(pbf_ctx* c, float* d_x, float* d_v, const float* d_f, int n, v
   return PBF_OK;
 }
 
-int pbf_density(pbf_ctx* c, const float* d_x, const int* d_offsets, const int* d_elems, float* d_rho, int n,
-                void* stream) {
+int pbf_density(pbf_ctx* c, const float* d_x, const int* d_index, const int* d_elems, long long nelems, float* d_rho,
+                int n, void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (bad_n(n) || (n > 0 && (!d_x || !d_offsets || !d_elems || !d_rho)))
-    return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (bad_n(n) || (n > 0 && (!d_x || !d_index || !d_rho))) return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0) return PBF_OK;
-  launch_csr_density(d_x, d_offsets, d_elems, d_rho, n, c->pc, (hipStream_t)stream);
+  Csr nb;
+  int s = caller_csr(c, d_index, d_elems, nelems, n, &nb);
+  if (s) return s;
+  launch_csr_density(d_x, nb, d_rho, c->pc, (hipStream_t)stream);
   PBF_LAUNCHED(c);
   return PBF_OK;
 }
 
-int pbf_pbd_constrain(pbf_ctx* c, float* d_x, float* d_v, const int* d_offsets, const int* d_elems, float* d_rho,
-                      int n, void* stream) {
+int pbf_pbd_constrain(pbf_ctx* c, float* d_x, float* d_v, const int* d_index, const int* d_elems, long long nelems,
+                      float* d_rho, int n, void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems)))
-    return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_index))) return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0) return PBF_OK;
-  int s = ensure_particles(c, n, 1);
+  Csr nb;
+  int s = caller_csr(c, d_index, d_elems, nelems, n, &nb);
+  if (s) return s;
+  s = ensure_particles(c, n, 1);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   launch_aos_to_f4(d_x, c->xa, n, st);
   for (int it = 0; it < c->p.iterations; ++it) {
     const bool last = it + 1 == c->p.iterations;
-    launch_csr_lambda(c->xa, d_offsets, d_elems, last ? d_rho : nullptr, n, c->pc, st);
-    launch_csr_dx_atomic(c->xa, d_offsets, d_elems, c->dpos, n, c->pc, st);
+    launch_csr_lambda(c->xa, nb, last ? d_rho : nullptr, c->pc, st);
+    launch_csr_dx_atomic(c->xa, nb, c->dpos, c->pc, st);
     launch_csr_apply(c->xa, c->dpos, n, st);
   }
   launch_csr_finish(c->xa, d_x, d_v, n, c->pc.dt, st);
@@ -1281,20 +1297,23 @@ int pbf_neighbor_query_fixed(pbf_ctx* c, const float* d_x, int n, int limit, int
   return record_counters(c, st);
 }
 
-int pbf_pbd_constrain_mass(pbf_ctx* c, float* d_x, float* d_v, const int* d_offsets, const int* d_elems,
-                           const float* d_mass_inv, float* d_rho, int n, void* stream) {
+int pbf_pbd_constrain_mass(pbf_ctx* c, float* d_x, float* d_v, const int* d_index, const int* d_elems,
+                           long long nelems, const float* d_mass_inv, float* d_rho, int n, void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems || !d_mass_inv)))
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_index || !d_mass_inv)))
     return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0) return PBF_OK;
-  int s = ensure_particles(c, n, 1);
+  Csr nb;
+  int s = caller_csr(c, d_index, d_elems, nelems, n, &nb);
+  if (s) return s;
+  s = ensure_particles(c, n, 1);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   launch_aos_to_f4(d_x, c->xa, n, st);
   for (int it = 0; it < c->p.iterations; ++it) {
     const bool last = it + 1 == c->p.iterations;
-    launch_csr_lambda(c->xa, d_offsets, d_elems, last ? d_rho : nullptr, n, c->pc, st, d_mass_inv);
-    launch_csr_dx_atomic(c->xa, d_offsets, d_elems, c->dpos, n, c->pc, st, d_mass_inv);
+    launch_csr_lambda(c->xa, nb, last ? d_rho : nullptr, c->pc, st, d_mass_inv);
+    launch_csr_dx_atomic(c->xa, nb, c->dpos, c->pc, st, d_mass_inv);
     launch_csr_apply(c->xa, c->dpos, n, st);
   }
   launch_csr_finish(c->xa, d_x, d_v, n, c->pc.dt, st);
@@ -1302,13 +1321,16 @@ int pbf_pbd_constrain_mass(pbf_ctx* c, float* d_x, float* d_v, const int* d_offs
   return PBF_OK;
 }
 
-int pbf_viscosity(pbf_ctx* c, const float* d_x, float* d_v, const int* d_offsets, const int* d_elems, int n,
-                  void* stream) {
+int pbf_viscosity(pbf_ctx* c, const float* d_x, float* d_v, const int* d_index, const int* d_elems,
+                  long long nelems, int n, void* stream) {
   if (!c) return PBF_ERR_INVALID;
-  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_offsets || !d_elems))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_index))) return fail(c, PBF_ERR_INVALID, "bad argument");
   const int T = c->p.viscosity_iterations;
   if (n == 0 || T == 0) return PBF_OK;
-  int s = ensure_particles(c, n, 1);
+  Csr nb;
+  int s = caller_csr(c, d_index, d_elems, nelems, n, &nb);
+  if (s) return s;
+  s = ensure_particles(c, n, 1);
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   launch_aos_to_f4(d_x, c->xa, n, st);
@@ -1317,8 +1339,7 @@ int pbf_viscosity(pbf_ctx* c, const float* d_x, float* d_v, const int* d_offsets
   for (int t = 0; t < T; ++t) {
     const bool last = t + 1 == T;
     const float4* vin = t == 0 ? c->vv : V[(t - 1) & 1];
-    launch_csr_viscosity(c->xa, d_offsets, d_elems, c->vv, vin, last ? nullptr : V[t & 1], last ? d_v : nullptr, n,
-                         c->vc, st);
+    launch_csr_viscosity(c->xa, nb, c->vv, vin, last ? nullptr : V[t & 1], last ? d_v : nullptr, c->vc, st);
   }
   PBF_LAUNCHED(c);
   return PBF_OK;

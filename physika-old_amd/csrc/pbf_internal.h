@@ -62,6 +62,19 @@ struct VisConst {
   float inv_h, cw, b, eps;
 };
 
+// A caller's neighbor list in the layout of the reference's NeighborList
+// (Framework/Topology/NeighborList.h:27-66): index[0..n-1] exclusive row
+// starts and elems[0..nelems-1]; row i ends at index[i+1], the last row at
+// nelems (NeighborList::getNeighborSize).
+struct Csr {
+  const int* index;
+  const int* elems;
+  int n;
+  int nelems;
+  __device__ __forceinline__ int begin(int i) const { return index[i]; }
+  __device__ __forceinline__ int end(int i) const { return i + 1 < n ? index[i + 1] : nelems; }
+};
+
 enum Counter {
   C_PAIRS = 0, C_OOG, C_NONFINITE, C_MASK_WORDS, C_FALLBACK, C_LIST_SLOTS, C_CELLS, C_CANDIDATES, C_LIST_NEED,
   C_STAGED, C_COUNT
@@ -256,18 +269,17 @@ void launch_box_constrain(const BoxField& f, const float* x_in, const float* v_i
                           const int* scene_off, int nscenes, float keep_normal, float damp, hipStream_t st);
 void launch_viscosity(const NbrView& nv, const float4* xs, const float4* v_old, const float4* v_in, float4* v_next,
                       const VisConst& vc, const int* perm, float* x_out, float* v_out, hipStream_t st);
-void launch_csr_viscosity(const float4* xa, const int* off, const int* el, const float4* v_old, const float4* v_in,
-                          float4* v_next, float* v_out, int n, const VisConst& vc, hipStream_t st);
+void launch_csr_viscosity(const float4* xa, const Csr& nb, const float4* v_old, const float4* v_in, float4* v_next,
+                          float* v_out, const VisConst& vc, hipStream_t st);
 
 // Module-granular kernels, original particle order, caller CSR.
 void launch_integrate(float* x, float* v, const float* f, int n, const PhysConst& pc, hipStream_t st);
-void launch_csr_density(const float* x, const int* off, const int* el, float* rho, int n,
-                        const PhysConst& pc, hipStream_t st);
+void launch_csr_density(const float* x, const Csr& nb, float* rho, const PhysConst& pc, hipStream_t st);
 void launch_aos_to_f4(const float* x, float4* xa, int n, hipStream_t st);
-void launch_csr_lambda(float4* xa, const int* off, const int* el, float* rho, int n,
-                       const PhysConst& pc, hipStream_t st, const float* minv = nullptr);
-void launch_csr_dx_atomic(const float4* xa, const int* off, const int* el, float* dpos, int n,
-                          const PhysConst& pc, hipStream_t st, const float* minv = nullptr);
+void launch_csr_lambda(float4* xa, const Csr& nb, float* rho, const PhysConst& pc, hipStream_t st,
+                       const float* minv = nullptr);
+void launch_csr_dx_atomic(const float4* xa, const Csr& nb, float* dpos, const PhysConst& pc, hipStream_t st,
+                          const float* minv = nullptr);
 // Fixed-capacity rows (the `limit` nearest of each neighbor set, by distance
 // then id) in original ids; limit <= kFixedMax.
 constexpr int kFixedMax = 32;

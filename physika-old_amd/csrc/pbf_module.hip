@@ -37,23 +37,21 @@ void launch_integrate(float* x, float* v, const float* f, int n, const PhysConst
 }
 
 // K_ComputeDensity (SummationDensity.cu:13-36) over a CSR list.
-__global__ __launch_bounds__(kBlock) void k_csr_density(const float* __restrict__ x, const int* __restrict__ off,
-                                                        const int* __restrict__ el, float* __restrict__ rho, int n,
-                                                        PhysConst pc) {
+__global__ __launch_bounds__(kBlock) void k_csr_density(const float* __restrict__ x, Csr nb,
+                                                        float* __restrict__ rho, PhysConst pc) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= nb.n) return;
   const float4 xi = load3(x, i);
   float acc = 0.f;
-  for (int k = off[i]; k < off[i + 1]; ++k) {
-    const PairGeom p = pair_geom(xi, load3(x, el[k]), pc.inv_h);
+  for (int k = nb.begin(i), e = nb.end(i); k < e; ++k) {
+    const PairGeom p = pair_geom(xi, load3(x, nb.elems[k]), pc.inv_h);
     acc += p.t * p.t * p.t;
   }
   rho[i] = acc * pc.cW;
 }
 
-void launch_csr_density(const float* x, const int* off, const int* el, float* rho, int n, const PhysConst& pc,
-                        hipStream_t st) {
-  k_csr_density<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(x, off, el, rho, n, pc);
+void launch_csr_density(const float* x, const Csr& nb, float* rho, const PhysConst& pc, hipStream_t st) {
+  k_csr_density<<<(nb.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(x, nb, rho, pc);
 }
 
 __global__ __launch_bounds__(kBlock) void k_aos_to_f4(const float* __restrict__ x, float4* __restrict__ xa, int n) {
@@ -68,15 +66,14 @@ void launch_aos_to_f4(const float* x, float4* xa, int n, hipStream_t st) {
 // Density and K_ComputeLambdas (DensityPBD.cu:46-90) over a CSR list; lambda
 // goes to xa[i].w. With minv (inverse masses) the massInv variant
 // (DensityPBD.cu:93-131): S = sum_j |g|^2 m_j^-1 + |sum g|^2 m_i^-1.
-__global__ __launch_bounds__(kBlock) void k_csr_lambda(float4* __restrict__ xa, const int* __restrict__ off,
-                                                       const int* __restrict__ el, float* __restrict__ rho, int n,
+__global__ __launch_bounds__(kBlock) void k_csr_lambda(float4* __restrict__ xa, Csr nb, float* __restrict__ rho,
                                                        PhysConst pc, const float* __restrict__ minv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= nb.n) return;
   const float4 xi = xa[i];
   float acc = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
-  for (int k = off[i]; k < off[i + 1]; ++k) {
-    const int j = el[k];
+  for (int k = nb.begin(i), e = nb.end(i); k < e; ++k) {
+    const int j = nb.elems[k];
     const PairGeom p = pair_geom(xi, xa[j], pc.inv_h);
     const float t2 = p.t * p.t;
     acc += t2 * p.t;
@@ -97,26 +94,26 @@ __global__ __launch_bounds__(kBlock) void k_csr_lambda(float4* __restrict__ xa, 
   if (rho) rho[i] = acc;
 }
 
-void launch_csr_lambda(float4* xa, const int* off, const int* el, float* rho, int n, const PhysConst& pc,
-                       hipStream_t st, const float* minv) {
-  k_csr_lambda<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, off, el, rho, n, pc, minv);
+void launch_csr_lambda(float4* xa, const Csr& nb, float* rho, const PhysConst& pc, hipStream_t st,
+                       const float* minv) {
+  k_csr_lambda<<<(nb.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, nb, rho, pc, minv);
 }
 
 // K_ComputeDisplacement (DensityPBD.cu:135-179): pairwise scatter with float
 // atomics, valid for any list (symmetric or not), order nondeterministic as in
 // the reference. With minv the massInv variant (DensityPBD.cu:182-224):
 // dp_ij m_i^-1 to i, -dp_ij m_j^-1 to j.
-__global__ __launch_bounds__(kBlock) void k_csr_dx_atomic(const float4* __restrict__ xa, const int* __restrict__ off,
-                                                          const int* __restrict__ el, float* __restrict__ dpos, int n,
-                                                          PhysConst pc, const float* __restrict__ minv) {
+__global__ __launch_bounds__(kBlock) void k_csr_dx_atomic(const float4* __restrict__ xa, Csr nb,
+                                                          float* __restrict__ dpos, PhysConst pc,
+                                                          const float* __restrict__ minv) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= nb.n) return;
   const float4 xi = xa[i];
   const float scale = pc.corr * pc.cG;
   const float mi = minv ? minv[i] : 1.0f;
   float ax = 0.f, ay = 0.f, az = 0.f;
-  for (int k = off[i]; k < off[i + 1]; ++k) {
-    const int j = el[k];
+  for (int k = nb.begin(i), e = nb.end(i); k < e; ++k) {
+    const int j = nb.elems[k];
     const float4 xj = xa[j];
     const PairGeom p = pair_geom(xi, xj, pc.inv_h);
     if (!(p.r > pc.pair_eps)) continue;
@@ -135,9 +132,9 @@ __global__ __launch_bounds__(kBlock) void k_csr_dx_atomic(const float4* __restri
   atomicAdd(&dpos[3 * (size_t)i + 2], az);
 }
 
-void launch_csr_dx_atomic(const float4* xa, const int* off, const int* el, float* dpos, int n, const PhysConst& pc,
-                          hipStream_t st, const float* minv) {
-  k_csr_dx_atomic<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, off, el, dpos, n, pc, minv);
+void launch_csr_dx_atomic(const float4* xa, const Csr& nb, float* dpos, const PhysConst& pc, hipStream_t st,
+                          const float* minv) {
+  k_csr_dx_atomic<<<(nb.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(xa, nb, dpos, pc, minv);
 }
 
 // K_UpdatePosition (DensityPBD.cu:227-237); clears dpos for the next round

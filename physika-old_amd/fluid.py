@@ -42,14 +42,63 @@ def _stream(device) -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
-def _check_vec3(t: torch.Tensor, name: str) -> torch.Tensor:
-    if t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 3 or not t.is_cuda:
-        raise ValueError(f"{name}: expected a (n, 3) float32 device tensor, got {tuple(t.shape)} {t.dtype} {t.device}")
-    return t.contiguous()
+def _check_vec3(t: torch.Tensor, name: str, n: Optional[int] = None, device=None) -> torch.Tensor:
+    """A (n, 3) float32 contiguous device tensor, checked, not copied: the
+    in-place entry points write into it, so a copy would leave the caller's
+    tensor unchanged."""
+    if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 3 \
+            or not t.is_cuda:
+        shape = tuple(t.shape) if isinstance(t, torch.Tensor) else type(t).__name__
+        raise ValueError(f"{name}: expected a (n, 3) float32 device tensor, got {shape}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: expected a contiguous tensor (the call writes it in place)")
+    if n is not None and t.shape[0] != n:
+        raise ValueError(f"{name}: expected {n} rows, got {t.shape[0]}")
+    if device is not None and t.device != device:
+        raise ValueError(f"{name}: expected a tensor on {device}, got {t.device}")
+    return t
+
+
+def _check_scalar(t: Optional[torch.Tensor], name: str, n: int, device, dtype=torch.float32):
+    """None, or a contiguous 1-D tensor of `dtype` on `device` with at least n
+    entries."""
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor) or t.dtype != dtype or t.dim() != 1 or t.device != device \
+            or not t.is_contiguous() or t.shape[0] < n:
+        raise ValueError(f"{name}: expected a contiguous 1-D {dtype} tensor on {device} with at least {n} entries")
+    return t
+
+
+def _check_scene_offsets(off: torch.Tensor, n: int, device, validate: bool = True) -> torch.Tensor:
+    """nscenes + 1 int32 offsets on `device`: first 0, last n, non-decreasing
+    (checked on the host when `validate`; one small device-to-host copy)."""
+    if not isinstance(off, torch.Tensor) or off.dtype != torch.int32 or off.dim() != 1 or off.device != device \
+            or off.shape[0] < 2 or not off.is_contiguous():
+        raise ValueError("scene_offsets: expected a contiguous int32 device tensor of nscenes+1 offsets")
+    if validate:
+        h = off.cpu()
+        if int(h[0]) != 0 or int(h[-1]) != n or bool((h[1:] < h[:-1]).any()):
+            raise ValueError(f"scene_offsets: must start at 0, end at {n} and never decrease")
+    return off
+
+
+def _check_nbr(nbr: "NeighborList", n: int, device) -> None:
+    if nbr.isLimited():
+        raise ValueError("the module passes take a dynamic (CSR) neighbor list, not a limited one")
+    if nbr.index.dtype != torch.int32 or nbr.index.device != device or nbr.index.shape != (n,) \
+            or not nbr.index.is_contiguous():
+        raise ValueError(f"neighbor index: expected {n} int32 row starts on {device}")
+    if nbr.elements.dtype != torch.int32 or nbr.elements.device != device or not nbr.elements.is_contiguous():
+        raise ValueError(f"neighbor elements: expected a contiguous int32 tensor on {device}")
 
 
 class Context:
     """A pbf_ctx (include/pbf.h) bound to one HIP device."""
+
+    # scene offsets are checked on the host before each batched call (one small
+    # copy); a caller that builds them once may turn this off
+    validate_offsets = True
 
     def __init__(self, device="cuda:0", **params):
         self.device = torch.device(device)
@@ -118,7 +167,7 @@ class Context:
     def neighbor_query(self, x: torch.Tensor) -> "NeighborList":
         x = _check_vec3(x, "position")
         n = x.shape[0]
-        off = torch.empty(n + 1, dtype=torch.int32, device=x.device)
+        off = torch.empty(max(n, 1), dtype=torch.int32, device=x.device)[:n]
         total = C.c_longlong()
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_neighbor_query(self._ctx, _ptr(x), n, _ptr(off), None, 0, C.byref(total),
@@ -129,7 +178,7 @@ class Context:
         return NeighborList(off, el[: total.value])
 
     def step_neighbors(self, n: int) -> "NeighborList":
-        off = torch.empty(n + 1, dtype=torch.int32, device=self.device)
+        off = torch.empty(max(n, 1), dtype=torch.int32, device=self.device)[:n]
         total = C.c_longlong()
         with torch.cuda.device(self.device):
             _capi.check(self._lib.pbf_step_neighbors(self._ctx, _ptr(off), None, 0, C.byref(total),
@@ -153,42 +202,55 @@ class Context:
 
     def integrate(self, x, v, f=None) -> None:
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
+        v = _check_vec3(v, "velocity", x.shape[0], x.device)
         if f is not None:
-            f = _check_vec3(f, "force")
+            f = _check_vec3(f, "force", x.shape[0], x.device)
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_integrate(self._ctx, _ptr(x), _ptr(v), _ptr(f), x.shape[0], _stream(x.device)),
                         self._ctx)
 
     def density(self, x, nbr: "NeighborList", rho: torch.Tensor) -> None:
         x = _check_vec3(x, "position")
+        n = x.shape[0]
+        _check_nbr(nbr, n, x.device)
+        if rho is None:
+            raise ValueError("rho: expected an output tensor")
+        rho = _check_scalar(rho, "rho", n, x.device)
         with torch.cuda.device(x.device):
-            _capi.check(self._lib.pbf_density(self._ctx, _ptr(x), _ptr(nbr.index), _ptr(nbr.elements), _ptr(rho),
-                                              x.shape[0], _stream(x.device)), self._ctx)
+            _capi.check(self._lib.pbf_density(self._ctx, _ptr(x), _ptr(nbr.index), _ptr(nbr.elements),
+                                              nbr.elements.numel(), _ptr(rho), n, _stream(x.device)), self._ctx)
 
     def pbd_constrain_mass(self, x, v, nbr: "NeighborList", mass_inv: torch.Tensor,
                            rho: Optional[torch.Tensor]) -> None:
         """pbf_pbd_constrain_mass: DensityPBD::constrain with inverse masses."""
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
-        if mass_inv.dtype != torch.float32 or mass_inv.shape != (x.shape[0],) or mass_inv.device != x.device:
+        n = x.shape[0]
+        v = _check_vec3(v, "velocity", n, x.device)
+        _check_nbr(nbr, n, x.device)
+        rho = _check_scalar(rho, "rho", n, x.device)
+        if mass_inv.dtype != torch.float32 or mass_inv.shape != (n,) or mass_inv.device != x.device:
             raise ValueError("mass_inv: expected an (n,) float32 tensor on the positions' device")
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_pbd_constrain_mass(self._ctx, _ptr(x), _ptr(v), _ptr(nbr.index),
-                                                         _ptr(nbr.elements), _ptr(mass_inv.contiguous()), _ptr(rho),
-                                                         x.shape[0], _stream(x.device)), self._ctx)
+                                                         _ptr(nbr.elements), nbr.elements.numel(),
+                                                         _ptr(mass_inv.contiguous()), _ptr(rho), n,
+                                                         _stream(x.device)), self._ctx)
 
     def pbd_constrain(self, x, v, nbr: "NeighborList", rho: Optional[torch.Tensor]) -> None:
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
+        n = x.shape[0]
+        v = _check_vec3(v, "velocity", n, x.device)
+        _check_nbr(nbr, n, x.device)
+        rho = _check_scalar(rho, "rho", n, x.device)
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_pbd_constrain(self._ctx, _ptr(x), _ptr(v), _ptr(nbr.index),
-                                                    _ptr(nbr.elements), _ptr(rho), x.shape[0], _stream(x.device)),
-                        self._ctx)
+                                                    _ptr(nbr.elements), nbr.elements.numel(), _ptr(rho), n,
+                                                    _stream(x.device)), self._ctx)
 
     def step(self, x, v, rho: Optional[torch.Tensor] = None, nsteps: int = 1) -> None:
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
+        v = _check_vec3(v, "velocity", x.shape[0], x.device)
+        rho = _check_scalar(rho, "rho", x.shape[0], x.device)
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_step(self._ctx, _ptr(x), _ptr(v), _ptr(rho), x.shape[0], int(nsteps),
                                            _stream(x.device)), self._ctx)
@@ -198,11 +260,9 @@ class Context:
         """nsteps of every scene of a batch; scene s holds rows
         scene_offsets[s]:scene_offsets[s+1] (int32 device tensor, nscenes+1)."""
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
-        off = scene_offsets
-        if off.dtype != torch.int32 or off.dim() != 1 or off.device != x.device or off.shape[0] < 2:
-            raise ValueError("scene_offsets: expected an int32 device tensor of nscenes+1 offsets")
-        off = off.contiguous()
+        v = _check_vec3(v, "velocity", x.shape[0], x.device)
+        rho = _check_scalar(rho, "rho", x.shape[0], x.device)
+        off = _check_scene_offsets(scene_offsets, x.shape[0], x.device, self.validate_offsets)
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_batch_step(self._ctx, _ptr(x), _ptr(v), _ptr(rho), _ptr(off),
                                                  off.shape[0] - 1, x.shape[0], int(nsteps), _stream(x.device)),
@@ -212,15 +272,17 @@ class Context:
     def viscosity(self, x, v, nbr: "NeighborList") -> None:
         """pbf_viscosity: ImplicitViscosity::constrain on v (in place)."""
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
+        n = x.shape[0]
+        v = _check_vec3(v, "velocity", n, x.device)
+        _check_nbr(nbr, n, x.device)
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_viscosity(self._ctx, _ptr(x), _ptr(v), _ptr(nbr.index), _ptr(nbr.elements),
-                                                x.shape[0], _stream(x.device)), self._ctx)
+                                                nbr.elements.numel(), n, _stream(x.device)), self._ctx)
 
     def box_constrain(self, x, v) -> None:
         """pbf_box_constrain: the box boundary on x, v (in place)."""
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
+        v = _check_vec3(v, "velocity", x.shape[0], x.device)
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_box_constrain(self._ctx, _ptr(x), _ptr(v), x.shape[0], _stream(x.device)),
                         self._ctx)
@@ -231,12 +293,11 @@ class Context:
         """pbf_rollout_forward: nsteps in place, checkpoints kept in the
         context. Returns a token that rollout_backward checks."""
         x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity")
+        v = _check_vec3(v, "velocity", x.shape[0], x.device)
+        rho = _check_scalar(rho, "rho", x.shape[0], x.device)
         off, nscenes = None, 1
         if scene_offsets is not None:
-            if scene_offsets.dtype != torch.int32 or scene_offsets.device != x.device:
-                raise ValueError("scene_offsets: expected an int32 device tensor of nscenes+1 offsets")
-            off = scene_offsets.contiguous()
+            off = _check_scene_offsets(scene_offsets, x.shape[0], x.device, self.validate_offsets)
             nscenes = off.shape[0] - 1
         with torch.cuda.device(x.device):
             _capi.check(self._lib.pbf_rollout_forward(self._ctx, _ptr(x), _ptr(v), _ptr(rho), _ptr(off), nscenes,
@@ -250,7 +311,7 @@ class Context:
         if token is not None and token != getattr(self, "_rollout_token", None):
             raise RuntimeError("the context ran another rollout since this one; its checkpoints are gone")
         xbar = _check_vec3(xbar, "xbar")
-        vbar = _check_vec3(vbar, "vbar")
+        vbar = _check_vec3(vbar, "vbar", xbar.shape[0], xbar.device)
         with torch.cuda.device(xbar.device):
             _capi.check(self._lib.pbf_rollout_backward(self._ctx, _ptr(xbar), _ptr(vbar), _stream(xbar.device)),
                         self._ctx)
@@ -258,9 +319,10 @@ class Context:
 
 class NeighborList:
     """Neighbor lists (NeighborList.h:9-125). Dynamic (CSR): ``index`` holds
-    n+1 exclusive offsets (the reference's m_index plus the total) and
-    ``elements`` the neighbor ids. Limited (``limit`` > 0, isLimited()):
-    ``index`` holds the n row sizes and row i sits at elements[limit*i:]."""
+    the n exclusive row starts (the reference's m_index) and ``elements`` the
+    neighbor ids; row i ends at index[i+1], the last row at elements.numel()
+    (getNeighborSize). Limited (``limit`` > 0, isLimited()): ``index`` holds
+    the n row sizes and row i sits at elements[limit*i:]."""
 
     def __init__(self, index: torch.Tensor, elements: torch.Tensor, limit: int = 0):
         self.index = index
@@ -274,12 +336,18 @@ class NeighborList:
         return self.limit
 
     def size(self) -> int:
-        return self.index.numel() if self.limit else self.index.numel() - 1
+        return self.index.numel()
 
     def getNeighborSize(self, i: int) -> int:
         if self.limit:
             return int(self.index[i])
-        return int(self.index[i + 1] - self.index[i])
+        end = self.elements.numel() if i + 1 == self.index.numel() else int(self.index[i + 1])
+        return end - int(self.index[i])
+
+    def offsets(self) -> torch.Tensor:
+        """The n + 1 row offsets (row starts plus the element count)."""
+        tail = torch.tensor([self.elements.numel()], dtype=torch.int32, device=self.index.device)
+        return torch.cat([self.index, tail])
 
     def getElement(self, i: int, j: int) -> int:
         if self.limit:
@@ -292,7 +360,8 @@ class NeighborList:
         el = self.elements.cpu().numpy()
         if self.limit:
             return [el[self.limit * i:self.limit * i + idx[i]] for i in range(len(idx))]
-        return [el[idx[i]:idx[i + 1]] for i in range(len(idx) - 1)]
+        ends = np.append(idx[1:], len(el))
+        return [el[idx[i]:ends[i]] for i in range(len(idx))]
 
 
 class VarField:

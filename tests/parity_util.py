@@ -43,7 +43,10 @@ def row_checksums(off, el):
 
 
 def assert_same_sets(gpu_nl, off, el):
-    go = gpu_nl.index.cpu().numpy()
+    """gpu_nl: a NeighborList (n row starts + elements); off: the oracle's
+    n + 1 offsets."""
+    assert gpu_nl.index.numel() == len(off) - 1 and gpu_nl.elements.numel() == off[-1]
+    go = gpu_nl.offsets().cpu().numpy()
     ge = gpu_nl.elements.cpu().numpy()
     np.testing.assert_array_equal(go, off)
     np.testing.assert_array_equal(row_keys(go, ge), row_keys(off, el))

@@ -22,11 +22,46 @@ def test_more_than_one_gpu_exits_with_message():
 
 
 def test_algorithmic_bytes_of_the_step():
-    # per particle: sort 60, integrate 48, K x (32 + 40), velocity 48; 12 per cell;
-    # the 2-byte list is written once and read by 2K passes
-    n, c, p, k = 1000, 77, 5000, 5
-    assert bench.algorithmic_bytes(n, c, p, k) == (60 + 48 + 72 * k + 48) * n + 12 * c + 2 * p * (2 * k + 1)
-    assert bench.algorithmic_bytes(n, c, p, 5) == 516 * n + 12 * c + 22 * p
+    # per particle: sort 60, integrate 48, K x (32 + 40), velocity 48; 12 per
+    # grid cell; no neighbor-list bytes (neighbor data count as on-chip)
+    n, c, k = 1000, 77, 5
+    assert bench.algorithmic_bytes(n, c, k) == (60 + 48 + 72 * k + 48) * n + 12 * c
+    assert bench.algorithmic_bytes(n, c, 5) == 516 * n + 12 * c
+    # the 1M lattice on the 85^3 reference grid: 548.4 MB
+    assert abs(bench.algorithmic_bytes(1048576, 85 ** 3, 5) - 548.4e6) < 0.1e6
+
+
+def test_algorithmic_flops_of_the_step():
+    # 9 per pair (distance test) + 57 per pair and iteration + 30 per particle
+    n, p = 1000, 70000
+    assert bench.algorithmic_flops(n, p, 5) == (9 + 57 * 5) * p + 30 * n
+    assert bench.algorithmic_flops(n, p, 0) == 9 * p + 30 * n
+
+
+def test_kernel_counts_and_bound():
+    n, p = 1 << 20, 71_000_000
+    assert bench.kernel_counts("k_density_lambda", n, p) == (32 * n, 33 * p)
+    assert bench.kernel_counts("k_displacement", n, p) == (40 * n, 24 * p)
+    assert bench.kernel_counts("k_neighbors", n, p) == (16 * n, 9 * p)
+    # 41.9 MB and 1.70 GFLOP in 72.8 us: VALU-bound at 0.149 of 157.3 TFLOP/s
+    b, f = bench.kernel_counts("k_displacement", n, p)
+    r = bench.roofline_of(b, f, 72.8e-6)
+    assert r["bound"] == "valu" and r["unit"] == "TFLOP/s" and r["peak"] == 157.3
+    assert abs(r["frac"] - f / 72.8e-6 / 157.3e12) < 1e-4 and 0.14 < r["frac"] < 0.16
+    assert abs(r["hbm_frac"] - b / 72.8e-6 / 8e12) < 1e-4
+    # a copy-like kernel is HBM-bound
+    r = bench.roofline_of(1e9, 1e6, 1e9 / 4e12)
+    assert r["bound"] == "hbm" and r["unit"] == "GB/s" and abs(r["frac"] - 0.5) < 1e-4
+
+
+def test_baseline_threads_follow_the_environment(monkeypatch):
+    aff = len(__import__("os").sched_getaffinity(0))
+    monkeypatch.setenv("OMP_NUM_THREADS", "1")
+    assert bench.baseline_threads() == 1
+    monkeypatch.delenv("OMP_NUM_THREADS")
+    assert bench.baseline_threads() == aff
+    info = bench.host_cpu()
+    assert info["affinity_cpus"] == aff and isinstance(info["model"], str)
 
 
 def test_committed_traffic_summary_is_read():

@@ -31,10 +31,53 @@ def test_binding_table_matches_header(capi):
     assert sorted(capi.SIGNATURES) == declared_functions()
 
 
+def declared_signatures():
+    """{name: [parameter type strings]} of every function include/pbf.h
+    declares (comments removed, one declaration per match)."""
+    text = (ROOT / "include" / "pbf.h").read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    out = {}
+    for m in re.finditer(r"\b(pbf_[a-z_]+)\s*\(([^;{]*?)\)\s*;", text):
+        args = [a.strip() for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
+        out[m.group(1)] = args
+    return out
+
+
+def _kind(decl):
+    """Coarse C type class of one parameter declaration."""
+    d = re.sub(r"\s+", " ", decl)
+    if "*" in d:
+        return "ptr"
+    if "long long" in d:
+        return "i64"
+    if re.search(r"\b(int|unsigned)\b", d):
+        return "i32"
+    return "other"
+
+
+def test_binding_argument_lists_match_header(capi):
+    """Every ctypes signature has the header's parameter count and kinds
+    (pointer, 32-bit or 64-bit integer): a call with a stale table would hand
+    the library shifted arguments."""
+    sigs = declared_signatures()
+    assert set(sigs) == set(capi.SIGNATURES)
+    for name, args in sigs.items():
+        _, argtypes = capi.SIGNATURES[name]
+        assert len(argtypes) == len(args), (name, len(argtypes), args)
+        for decl, t in zip(args, argtypes):
+            k = _kind(decl)
+            if k == "ptr":
+                assert t in (ctypes.c_void_p, ctypes.c_char_p) or issubclass(t, ctypes._Pointer), (name, decl, t)
+            elif k == "i64":
+                assert t is ctypes.c_longlong, (name, decl, t)
+            elif k == "i32":
+                assert t in (ctypes.c_int, ctypes.c_uint), (name, decl, t)
+
+
 def test_host_only_calls(capi):
     """Calls that touch no device: version, status strings, defaults."""
     L = capi.lib()
-    assert L.pbf_abi_version() == 3
+    assert L.pbf_abi_version() == 4
     assert L.pbf_status_string(0) == b"ok"
     assert L.pbf_status_string(-3) == b"buffer capacity exceeded"
     p = capi.default_params()

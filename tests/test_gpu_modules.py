@@ -186,7 +186,7 @@ def test_pbd_constrain_mass_vs_oracle(fluid, oracle, scene):
     off, el = oracle.neighbor_query(P, x)
     ctx = fluid.Context("cuda:0")
     xd, vd = dev(x), dev(v)
-    nl = fluid.NeighborList(torch.from_numpy(off.astype(np.int32)).to("cuda:0"),
+    nl = fluid.NeighborList(torch.from_numpy(off[:-1].astype(np.int32)).to("cuda:0"),
                             torch.from_numpy(el.astype(np.int32)).to("cuda:0"))
     rho = torch.zeros(n, dtype=torch.float32, device="cuda:0")
     ctx.pbd_constrain_mass(xd, vd, nl, dev(minv).reshape(-1), rho)

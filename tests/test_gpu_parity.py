@@ -196,7 +196,7 @@ def test_empty_input(fluid):
     x = torch.empty((0, 3), dtype=torch.float32, device="cuda:0")
     ctx.step(x, x.clone(), None, 3)
     nl = ctx.neighbor_query(x)
-    assert nl.index.numel() == 1 and int(nl.index[0]) == 0
+    assert nl.index.numel() == 0 and nl.elements.numel() == 0 and nl.size() == 0
 
 
 @pytest.mark.parametrize("scene_name,reserve", [
@@ -260,7 +260,7 @@ def test_staged_lists_are_identical(fluid, scene, scene_name):
         outs.append(m.step_arrays(x, v, 3))
         stats.append(m.ctx.stats())
         nl = m.ctx.step_neighbors(len(x))
-        csr.append((nl.index.cpu().numpy(), nl.elements.cpu().numpy()))
+        csr.append((nl.offsets().cpu().numpy(), nl.elements.cpu().numpy()))
     nw = (len(x) + 63) // 64
     say(f"staged waves {stats[0]['staged_waves']} of {nw}")
     if scene_name == "lattice":
@@ -319,7 +319,7 @@ def test_density_on_caller_list(fluid, oracle, scene):
     p = oracle.make_params()
     off, el = oracle.neighbor_query(p, x)
     ctx = fluid.Context("cuda:0")
-    nl = fluid.NeighborList(torch.from_numpy(off).to("cuda:0"), torch.from_numpy(el).to("cuda:0"))
+    nl = fluid.NeighborList(torch.from_numpy(off[:-1].copy()).to("cuda:0"), torch.from_numpy(el).to("cuda:0"))
     rho = torch.empty(len(x), dtype=torch.float32, device="cuda:0")
     ctx.density(dev(x), nl, rho)
     ro = oracle.density(p, x, off, el)
@@ -343,7 +343,7 @@ def test_pbd_constrain_on_asymmetric_list(fluid, oracle, scene):
     off2 = np.concatenate([[0], np.cumsum(cnt)]).astype(np.int32)
     el2 = el[keep].astype(np.int32)
     ctx = fluid.Context("cuda:0")
-    nl = fluid.NeighborList(torch.from_numpy(off2).to("cuda:0"), torch.from_numpy(el2).to("cuda:0"))
+    nl = fluid.NeighborList(torch.from_numpy(off2[:-1].copy()).to("cuda:0"), torch.from_numpy(el2).to("cuda:0"))
     xd, vd = dev(x), dev(v)
     rho = torch.empty(len(x), dtype=torch.float32, device="cuda:0")
     ctx.pbd_constrain(xd, vd, nl, rho)
