@@ -12,6 +12,10 @@ from pathlib import Path
 _PKG = Path(__file__).resolve().parent
 _LIB = _PKG / "_build" / "libpbf.so"
 
+# PBF_ABI_VERSION of the include/pbf.h this table mirrors; lib() refuses a
+# library built from another version (its entry points may take other arguments)
+ABI_VERSION = 4
+
 PBF_OK = 0
 PBF_ERR_INVALID = -1
 PBF_ERR_HIP = -2
@@ -133,6 +137,11 @@ def lib():
     global _lib
     if _lib is None:
         L = C.CDLL(str(library_path(True)))
+        L.pbf_abi_version.restype = C.c_int
+        L.pbf_abi_version.argtypes = []
+        if L.pbf_abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{_LIB} has ABI version {L.pbf_abi_version()}, this binding expects {ABI_VERSION}; "
+                               "rebuild it (`python __graft_entry__.py build`)")
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)
             fn.restype = res

@@ -77,7 +77,7 @@ def test_binding_argument_lists_match_header(capi):
 def test_host_only_calls(capi):
     """Calls that touch no device: version, status strings, defaults."""
     L = capi.lib()
-    assert L.pbf_abi_version() == 4
+    assert L.pbf_abi_version() == capi.ABI_VERSION == 4
     assert L.pbf_status_string(0) == b"ok"
     assert L.pbf_status_string(-3) == b"buffer capacity exceeded"
     p = capi.default_params()
@@ -115,3 +115,8 @@ def test_null_context_rejected(capi):
     assert L.pbf_destroy(None) == capi.PBF_ERR_INVALID
     assert L.pbf_step(None, None, None, None, 0, 1, None) == capi.PBF_ERR_INVALID
     assert L.pbf_last_error(None) == b"null context"
+
+
+def test_header_and_binding_agree_on_the_abi_version(capi):
+    m = re.search(r"#define PBF_ABI_VERSION (\d+)", (ROOT / "include" / "pbf.h").read_text())
+    assert int(m.group(1)) == capi.ABI_VERSION
