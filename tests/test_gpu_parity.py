@@ -370,7 +370,7 @@ def test_million_particle_neighbor_checksums(fluid, oracle, scene):
     ctx = fluid.Context("cuda:0")
     nl = ctx.neighbor_query(dev(x))
     off, el = oracle.neighbor_query(oracle.make_params(), x)
-    go = nl.index.cpu().numpy()
+    go = nl.offsets().cpu().numpy()
     ge = nl.elements.cpu().numpy()
     np.testing.assert_array_equal(go, off)
     for a, b in zip(row_checksums(go, ge), row_checksums(off, el)):
