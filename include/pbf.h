@@ -254,6 +254,22 @@ int pbf_rollout_forward(pbf_ctx* ctx, float* d_x, float* d_v, float* d_rho, cons
                         int nscenes, int n, int nsteps, int checkpoint_every, void* stream);
 int pbf_rollout_backward(pbf_ctx* ctx, float* d_xbar, float* d_vbar, void* stream);
 
+/* Reverse mode in double precision: pbf_rollout_forward / _backward with the
+ * state (x, v, rho and their adjoints: packed xyz doubles, original order) and
+ * every quantity of the integrator and the PBD iterations in double, the
+ * constants computed from the parameters promoted to double. The neighbor
+ * sets come from the single-precision search on the positions rounded to
+ * float (they differ from a double-precision search only for pairs within a
+ * float ulp of h, where the kernel and its derivative vanish). For gradients
+ * that must match FP64 finite differences over horizons where the fp32
+ * trajectory departs from the FP64 one (lambda = min(0, .) switches on
+ * rounding). pbf_rollout_forward_f64 is blocking and returns
+ * PBF_ERR_UNSUPPORTED when a step had particles outside the grid; otherwise
+ * as pbf_rollout_forward / pbf_rollout_backward. */
+int pbf_rollout_forward_f64(pbf_ctx* ctx, double* d_x, double* d_v, double* d_rho, const int* d_scene_offsets,
+                            int nscenes, int n, int nsteps, int checkpoint_every, void* stream);
+int pbf_rollout_backward_f64(pbf_ctx* ctx, double* d_xbar, double* d_vbar, void* stream);
+
 /* Export of the neighbor lists built by the last pbf_step (its last step, on
  * the positions before that step's integration), original ids, NeighborList
  * layout; same contract as pbf_neighbor_query (the outNeighborhood field of

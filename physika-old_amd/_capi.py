@@ -107,6 +107,8 @@ SIGNATURES = {
     "pbf_batch_step": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP]),
     "pbf_rollout_forward": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
     "pbf_rollout_backward": (C.c_int, [_CTX, _VP, _VP, _VP]),
+    "pbf_rollout_forward_f64": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),
+    "pbf_rollout_backward_f64": (C.c_int, [_CTX, _VP, _VP, _VP]),
     "pbf_step_neighbors": (C.c_int, [_CTX, _VP, _VP, C.c_longlong, C.POINTER(C.c_longlong), _VP]),
     "pbf_viscosity": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_longlong, C.c_int, _VP]),
     "pbf_box_constrain": (C.c_int, [_CTX, _VP, _VP, C.c_int, _VP]),

@@ -105,7 +105,35 @@ struct pbf_ctx {
     SceneWin* win = nullptr;
     unsigned long long* counters = nullptr;
     float4* it = nullptr;  // iterates 1..K-1, then the K aux arrays (cap_n each)
+    double4* it64 = nullptr;  // double-precision rollout: P_0..P_{K-1}, then the K aux arrays (cap_n each)
+    size_t it64_n = 0;        // double4 entries allocated
   };
+  // rollout in double precision: checkpoints and segment states (x then v,
+  // 3n doubles each), the rounded positions of the search, the iterates of an
+  // untaped step (P_0, two ping-pong iterates, the integrated velocities) and
+  // the adjoint arrays (e, e2, f, c, q1, q2, v*_bar)
+  struct Rollout64 {
+    bool valid = false;
+    int n = 0, nscenes = 0, nsteps = 0, every = 0, iters = 0;
+    pbf_params p{};
+    int* off = nullptr;
+    int off_cap = 0;
+    double* ckpt = nullptr;
+    size_t ckpt_cap = 0;
+    double* seg = nullptr;
+    size_t seg_cap = 0;
+    float* xf = nullptr;
+    size_t xf_cap = 0;
+    double4* w4 = nullptr;  // 4 x cap_n
+    size_t w4_cap = 0;
+    double4* adj = nullptr;  // 7 x cap_n
+    size_t adj_cap = 0;
+    double* out = nullptr;  // 6 x cap_n: x, v written by a taped step (discarded)
+    size_t out_cap = 0;
+    int* flag = nullptr;
+  } roll64;
+  PhysConst64 pc64{};
+
   std::vector<TapeSlot> slots;
   size_t slot_n = 0, slot_list = 0, slot_mask = 0;
   long long slot_cells = 0;
@@ -236,6 +264,50 @@ int validate(const pbf_params* p, std::string& why) {
   return PBF_OK;
 }
 
+// SpikyKernel::Weight and SummationDensity::calculateScalingFactor as above,
+// in double (the parameters promoted to double).
+double spiky_w_host64(double r, double h) {
+#pragma clang fp contract(off)
+  const double q = r / h;
+  if (q > 1.0) return 0.0;
+  const double d = 1.0 - q;
+  const double hh = h * h;
+  return 15.0 / (kPi * hh * h) * d * d * d * 1.0;
+}
+
+double scaling_factor_host64(double d, double H) {
+#pragma clang fp contract(off)
+  const double V = d * d * d;
+  double total = 0.0;
+  const int half_res = (int)(H / d + 1.0);
+  for (int i = -half_res; i <= half_res; i++)
+    for (int j = -half_res; j <= half_res; j++)
+      for (int k = -half_res; k <= half_res; k++) {
+        const double x = (double)i * d, y = (double)j * d, z = (double)k * d;
+        const double r = std::sqrt(x * x + y * y + z * z);
+        total += V * spiky_w_host64(r, H);
+      }
+  return 1.0 / total;
+}
+
+PhysConst64 derive64(const pbf_params* p) {
+#pragma clang fp contract(off)
+  PhysConst64 pc{};
+  const double h = p->smoothing_length, d = p->sampling_distance;
+  pc.h = h;
+  pc.inv_h = 1.0 / h;
+  const double mass = scaling_factor_host64(d, h) * ((double)p->rest_density * (d * d * d));
+  pc.cW = mass * (15.0 / (kPi * (h * h) * h));
+  pc.cG = -45.0 / (kPi * (h * h) * h);
+  pc.lam_rest = p->lambda_rest;
+  pc.lam_eps = p->lambda_eps;
+  pc.corr = p->corr_scale;
+  pc.pair_eps = p->pair_eps;
+  pc.dt = p->dt;
+  for (int a = 0; a < 3; ++a) pc.g[a] = p->gravity[a];
+  return pc;
+}
+
 // GridHash::setSpace (Framework/Topology/GridHash.cu:47-75) and the constants
 // of the passes.
 int derive(pbf_ctx* c, const pbf_params* p) {
@@ -319,6 +391,7 @@ int derive(pbf_ctx* c, const pbf_params* p) {
   c->g = g;
   c->full_cells = ncell;
   c->pc = pc;
+  c->pc64 = derive64(p);
   c->m_factor = m_factor;
   c->mass = mass;
   c->p = *p;
@@ -789,6 +862,8 @@ template <class T> int ensure_buf(pbf_ctx* c, T*& p, size_t& cap, size_t need) {
 }
 
 void free_slot(pbf_ctx::TapeSlot& t) {
+  dfree(t.it64);
+  t.it64_n = 0;
   dfree(t.perm); dfree(t.ssid); dfree(t.nbr_cnt); dfree(t.xn); dfree(t.x0);
   dfree(t.list_base); dfree(t.mask_base); dfree(t.list_jbase); dfree(t.list_mode); dfree(t.stage); dfree(t.cnt_a);
   dfree(t.list); dfree(t.mask); dfree(t.cell_start); dfree(t.win); dfree(t.counters); dfree(t.it);
@@ -887,6 +962,57 @@ NbrView slot_view(const pbf_ctx* c, const pbf_ctx::TapeSlot& t, int n) {
 }
 
 bool bad_n(int n) { return n < 0; }
+
+// Buffers of the double-precision rollout for the current capacity.
+int ensure_roll64(pbf_ctx* c, int nscenes) {
+  auto& r = c->roll64;
+  const size_t cap = (size_t)c->cap_n;
+  int s;
+  if ((s = ensure_buf(c, r.xf, r.xf_cap, 3 * cap))) return s;
+  if ((s = ensure_buf(c, r.w4, r.w4_cap, 4 * cap))) return s;
+  if (nscenes + 1 > r.off_cap) {
+    dfree(r.off);
+    r.off_cap = 0;
+    PBF_HIP(c, hipMalloc(&r.off, sizeof(int) * (size_t)(nscenes + 1)));
+    r.off_cap = nscenes + 1;
+  }
+  if (!r.flag) PBF_HIP(c, hipMalloc(&r.flag, sizeof(int)));
+  return PBF_OK;
+}
+
+// One fused step in double (x_in, v_in -> x_out, v_out; may alias): the
+// neighbor search on the positions rounded to float, then the integrator and
+// the PBD iterations in double. With a tape, P and aux hold the K iterates
+// and the K per-iteration sums.
+int step64_once(pbf_ctx* c, const double* x_in, const double* v_in, double* x_out, double* v_out, double* rho,
+                int n, const int* off, int nscenes, double4* const* P, double4* const* aux, hipStream_t st) {
+  auto& r = c->roll64;
+  const int K = c->p.iterations;
+  launch64_round(x_in, r.xf, n, st);
+  int s = search(c, SortIn{r.xf, nullptr, nullptr, nullptr, nullptr}, n, off, nscenes, false, st);
+  if (s) return s;
+  const size_t cap = (size_t)c->cap_n;
+  double4* vs = r.w4 + 3 * cap;
+  std::vector<double4*> Pu;
+  if (!P) {  // untaped: P_0, then two ping-pong iterates
+    Pu.resize(std::max(K, 1));
+    Pu[0] = r.w4;
+    for (int k = 1; k < K; ++k) Pu[k] = r.w4 + (size_t)(1 + ((k - 1) & 1)) * cap;
+    P = Pu.data();
+  }
+  launch64_step(view_of(c, n), c->perm, x_in, v_in, x_out, v_out, rho, P, aux, K, vs, c->pc64, c->counters, r.flag,
+                st);
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+int read_flag64(pbf_ctx* c, hipStream_t st, const char* what) {
+  int flag = 0;
+  PBF_HIP(c, hipMemcpyAsync(&flag, c->roll64.flag, sizeof(int), hipMemcpyDeviceToHost, st));
+  PBF_HIP(c, hipStreamSynchronize(st));
+  if (flag) return fail(c, PBF_ERR_UNSUPPORTED, std::string(what) + ": a step had particles outside the grid");
+  return PBF_OK;
+}
 
 }  // namespace
 
@@ -996,6 +1122,8 @@ int pbf_destroy(pbf_ctx* c) {
   free_slots(c);
   dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.adj);
   dfree(c->roll.out); dfree(c->roll.flag);
+  dfree(c->roll64.off); dfree(c->roll64.ckpt); dfree(c->roll64.seg); dfree(c->roll64.xf); dfree(c->roll64.w4);
+  dfree(c->roll64.adj); dfree(c->roll64.out); dfree(c->roll64.flag);
   dfree(c->counters);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->counters_ev) (void)hipEventDestroy(c->counters_ev);
@@ -1275,6 +1403,135 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
     return fail(c, PBF_ERR_UNSUPPORTED,
                 "a step of the rollout had particles outside the grid; its adjoint is not implemented");
   return PBF_OK;
+}
+
+int pbf_rollout_forward_f64(pbf_ctx* c, double* d_x, double* d_v, double* d_rho, const int* d_scene_offsets,
+                            int nscenes, int n, int nsteps, int checkpoint_every, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || n == 0 || nsteps < 1 || checkpoint_every < 1 || nscenes < 1 || !d_x || !d_v ||
+      (!d_scene_offsets && nscenes != 1))
+    return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (c->p.flags != 0)
+    return fail(c, PBF_ERR_UNSUPPORTED, "reverse mode covers the plain PBD step (no viscosity, no box)");
+  hipStream_t st = (hipStream_t)stream;
+  auto& r = c->roll64;
+  r.valid = false;
+  int s = ensure_particles(c, n, nscenes);
+  if (s) return s;
+  if ((s = adapt_reservations(c))) return s;
+  if ((s = ensure_roll64(c, nscenes))) return s;
+  const size_t state = 6 * (size_t)n;
+  const int nck = (nsteps + checkpoint_every - 1) / checkpoint_every;
+  if ((s = ensure_buf(c, r.ckpt, r.ckpt_cap, (size_t)nck * state))) return s;
+  if (d_scene_offsets)
+    PBF_HIP(c, hipMemcpyAsync(r.off, d_scene_offsets, sizeof(int) * (size_t)(nscenes + 1), hipMemcpyDeviceToDevice,
+                              st));
+  else
+    launch_set_offsets(r.off, n, st);
+  PBF_HIP(c, hipMemsetAsync(r.flag, 0, sizeof(int), st));
+  for (int k = 0; k < nsteps; ++k) {
+    if (k % checkpoint_every == 0) {
+      double* ck = r.ckpt + (size_t)(k / checkpoint_every) * state;
+      PBF_HIP(c, hipMemcpyAsync(ck, d_x, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice, st));
+      PBF_HIP(c, hipMemcpyAsync(ck + 3 * (size_t)n, d_v, sizeof(double) * 3 * (size_t)n, hipMemcpyDeviceToDevice,
+                                st));
+    }
+    s = step64_once(c, d_x, d_v, d_x, d_v, k + 1 == nsteps ? d_rho : nullptr, n, r.off, nscenes, nullptr, nullptr,
+                    st);
+    if (s) return s;
+  }
+  if ((s = record_counters(c, st))) return s;
+  if ((s = read_flag64(c, st, "pbf_rollout_forward_f64"))) return s;
+  r.n = n;
+  r.nscenes = nscenes;
+  r.nsteps = nsteps;
+  r.every = checkpoint_every;
+  r.iters = c->p.iterations;
+  r.p = c->p;
+  r.valid = true;
+  return PBF_OK;
+}
+
+int pbf_rollout_backward_f64(pbf_ctx* c, double* d_xbar, double* d_vbar, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (!d_xbar || !d_vbar) return fail(c, PBF_ERR_INVALID, "bad argument");
+  auto& r = c->roll64;
+  if (!r.valid) return fail(c, PBF_ERR_INVALID, "no double-precision rollout has run on this context");
+  if (std::memcmp(&r.p, &c->p, sizeof(pbf_params)) != 0)
+    return fail(c, PBF_ERR_INVALID, "parameters changed since the rollout");
+  hipStream_t st = (hipStream_t)stream;
+  const int n = r.n, K = r.iters;
+  int s = ensure_particles(c, n, r.nscenes);
+  if (s) return s;
+  if ((s = adapt_reservations(c))) return s;
+  if ((s = ensure_roll64(c, r.nscenes))) return s;
+  const int m_max = std::min(r.every, r.nsteps);
+  if ((s = ensure_tape(c, m_max))) return s;
+  const size_t cap = (size_t)c->cap_n, state = 6 * (size_t)n;
+  if ((s = ensure_buf(c, r.seg, r.seg_cap, 2 * state))) return s;
+  if ((s = ensure_buf(c, r.adj, r.adj_cap, 7 * cap))) return s;
+  if ((s = ensure_buf(c, r.out, r.out_cap, 6 * cap))) return s;
+  const size_t per_slot = 2 * (size_t)std::max(K, 1) * cap;
+  for (int q = 0; q < m_max; ++q) {
+    auto& t = c->slots[q];
+    if (t.it64_n < per_slot) {
+      dfree(t.it64);
+      t.it64_n = 0;
+      PBF_HIP(c, hipMalloc(&t.it64, sizeof(double4) * per_slot));
+      t.it64_n = per_slot;
+    }
+  }
+  AdjBufs64 ab;
+  ab.e = r.adj;
+  ab.e2 = r.adj + cap;
+  ab.f = r.adj + 2 * cap;
+  ab.c = r.adj + 3 * cap;
+  ab.q1 = r.adj + 4 * cap;
+  ab.q2 = r.adj + 5 * cap;
+  ab.vsb = r.adj + 6 * cap;
+  std::vector<double4*> P(std::max(K, 1)), aux(std::max(K, 1));
+  auto slot_arrays = [&](pbf_ctx::TapeSlot& t) {
+    for (int k = 0; k < K; ++k) {
+      P[k] = t.it64 + (size_t)k * cap;
+      aux[k] = t.it64 + (size_t)(K + k) * cap;
+    }
+  };
+  double* sa = r.seg;
+  double* sb = r.seg + state;
+  const int nck = (r.nsteps + r.every - 1) / r.every;
+  PBF_HIP(c, hipMemsetAsync(r.flag, 0, sizeof(int), st));
+  for (int ck = nck - 1; ck >= 0; --ck) {
+    const int s0 = ck * r.every, m = std::min(s0 + r.every, r.nsteps) - s0;
+    PBF_HIP(c, hipMemcpyAsync(sa, r.ckpt + (size_t)ck * state, sizeof(double) * state, hipMemcpyDeviceToDevice, st));
+    for (int q = 0; q < m; ++q) {
+      const double* in = (q & 1) ? sb : sa;
+      double* xout = q + 1 == m ? r.out : ((q & 1) ? sa : sb);
+      double* vout = q + 1 == m ? r.out + 3 * cap : xout + 3 * (size_t)n;
+      auto& t = c->slots[q];
+      swap_slot(c, t);
+      slot_arrays(t);
+      s = step64_once(c, in, in + 3 * (size_t)n, xout, vout, nullptr, n, r.off, r.nscenes, P.data(), aux.data(), st);
+      if (!s) {
+        launch_copy_cells(c->cell_start, t.cell_start, c->total_cells, c->cell_cap, st);
+        if (hipMemcpyAsync(t.win, c->win, sizeof(SceneWin) * (size_t)r.nscenes, hipMemcpyDeviceToDevice, st) !=
+                hipSuccess ||
+            hipMemcpyAsync(t.counters, c->counters, sizeof(unsigned long long) * C_COUNT, hipMemcpyDeviceToDevice,
+                           st) != hipSuccess)
+          s = fail(c, PBF_ERR_HIP, "tape copy failed");
+      }
+      swap_slot(c, t);
+      if (s) return s;
+    }
+    for (int q = m - 1; q >= 0; --q) {
+      auto& t = c->slots[q];
+      slot_arrays(t);
+      launch64_adjoint_step(slot_view(c, t, n), P.data(), aux.data(), K, c->pc64, t.perm, d_xbar, d_vbar, ab, st);
+      PBF_LAUNCHED(c);
+    }
+  }
+  c->last_n = -1;
+  if ((s = record_counters(c, st))) return s;
+  return read_flag64(c, st, "pbf_rollout_backward_f64");
 }
 
 int pbf_neighbor_query_fixed(pbf_ctx* c, const float* d_x, int n, int limit, int* d_counts, int* d_elems,

@@ -263,6 +263,29 @@ struct AdjBufs {
 void launch_adjoint_step(const NbrView& nv, float4* const* P, float4* const* aux, int K, const PhysConst& pc,
                          const int* perm, float* xbar, float* vbar, const AdjBufs& ab, hipStream_t st);
 
+// Double-precision step and adjoint (reverse mode against FP64 references):
+// the constants of PhysConst in double, computed from the parameters
+// promoted to double.
+struct PhysConst64 {
+  double h, inv_h, cW, cG, lam_rest, lam_eps, corr, pair_eps, dt;
+  double g[3];
+};
+struct AdjBufs64 {
+  double4 *e, *e2, *f, *c, *q1, *q2, *vsb;
+};
+// xf[0..3n) = (float) x[0..3n)
+void launch64_round(const double* x, float* xf, int n, hipStream_t st);
+// One step on the sorted view of the search just run on the rounded
+// positions: P[0] receives the integrated positions (w: lambda of iteration
+// 0), P[k] iterate k (w: its lambda), aux[k] (taped steps, else nullptr)
+// (G, 1/(S + eps)) of iteration k; vs the integrated velocities. Sets *flag
+// when the search had particles outside the grid.
+void launch64_step(const NbrView& nv, const int* perm, const double* x_in, const double* v_in, double* x_out,
+                   double* v_out, double* rho, double4* const* P, double4* const* aux, int K, double4* vs,
+                   const PhysConst64& pc, const unsigned long long* counters, int* flag, hipStream_t st);
+void launch64_adjoint_step(const NbrView& nv, double4* const* P, double4* const* aux, int K, const PhysConst64& pc,
+                           const int* perm, double* xbar, double* vbar, const AdjBufs64& ab, hipStream_t st);
+
 // Box boundary and viscosity.
 void launch_box_sdf(const BoxField& f, const float lo[3], const float hi[3], hipStream_t st);
 void launch_box_constrain(const BoxField& f, const float* x_in, const float* v_in, float* x_out, float* v_out, int n,

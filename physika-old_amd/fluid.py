@@ -42,14 +42,16 @@ def _stream(device) -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
-def _check_vec3(t: torch.Tensor, name: str, n: Optional[int] = None, device=None) -> torch.Tensor:
-    """A (n, 3) float32 contiguous device tensor, checked, not copied: the
+def _check_vec3(t: torch.Tensor, name: str, n: Optional[int] = None, device=None,
+                dtype=torch.float32) -> torch.Tensor:
+    """A (n, 3) contiguous device tensor of `dtype`, checked, not copied: the
     in-place entry points write into it, so a copy would leave the caller's
     tensor unchanged."""
-    if not isinstance(t, torch.Tensor) or t.dtype != torch.float32 or t.dim() != 2 or t.shape[1] != 3 \
+    if not isinstance(t, torch.Tensor) or t.dtype != dtype or t.dim() != 2 or t.shape[1] != 3 \
             or not t.is_cuda:
         shape = tuple(t.shape) if isinstance(t, torch.Tensor) else type(t).__name__
-        raise ValueError(f"{name}: expected a (n, 3) float32 device tensor, got {shape}")
+        dt = t.dtype if isinstance(t, torch.Tensor) else ""
+        raise ValueError(f"{name}: expected a (n, 3) {dtype} device tensor, got {shape} {dt}")
     if not t.is_contiguous():
         raise ValueError(f"{name}: expected a contiguous tensor (the call writes it in place)")
     if n is not None and t.shape[0] != n:
@@ -290,31 +292,40 @@ class Context:
     # -- reverse mode -----------------------------------------------------------
     def rollout_forward(self, x, v, rho: Optional[torch.Tensor] = None, nsteps: int = 1,
                         checkpoint_every: int = 10, scene_offsets: Optional[torch.Tensor] = None) -> int:
-        """pbf_rollout_forward: nsteps in place, checkpoints kept in the
-        context. Returns a token that rollout_backward checks."""
-        x = _check_vec3(x, "position")
-        v = _check_vec3(v, "velocity", x.shape[0], x.device)
-        rho = _check_scalar(rho, "rho", x.shape[0], x.device)
+        """pbf_rollout_forward (float32 state) or pbf_rollout_forward_f64
+        (float64 state): nsteps in place, checkpoints kept in the context.
+        Returns a token that rollout_backward checks."""
+        f64 = isinstance(x, torch.Tensor) and x.dtype == torch.float64
+        dt = torch.float64 if f64 else torch.float32
+        x = _check_vec3(x, "position", dtype=dt)
+        v = _check_vec3(v, "velocity", x.shape[0], x.device, dtype=dt)
+        rho = _check_scalar(rho, "rho", x.shape[0], x.device, dtype=dt)
         off, nscenes = None, 1
         if scene_offsets is not None:
             off = _check_scene_offsets(scene_offsets, x.shape[0], x.device, self.validate_offsets)
             nscenes = off.shape[0] - 1
+        fn = self._lib.pbf_rollout_forward_f64 if f64 else self._lib.pbf_rollout_forward
         with torch.cuda.device(x.device):
-            _capi.check(self._lib.pbf_rollout_forward(self._ctx, _ptr(x), _ptr(v), _ptr(rho), _ptr(off), nscenes,
-                                                      x.shape[0], int(nsteps), int(checkpoint_every),
-                                                      _stream(x.device)), self._ctx)
-        self._rollout_token = getattr(self, "_rollout_token", 0) + 1
-        return self._rollout_token
+            _capi.check(fn(self._ctx, _ptr(x), _ptr(v), _ptr(rho), _ptr(off), nscenes, x.shape[0], int(nsteps),
+                           int(checkpoint_every), _stream(x.device)), self._ctx)
+        key = "_rollout_token64" if f64 else "_rollout_token"
+        setattr(self, key, getattr(self, key, 0) + 1)
+        return (f64, getattr(self, key))
 
     def rollout_backward(self, xbar, vbar, token: Optional[int] = None) -> None:
-        """pbf_rollout_backward: (dL/dx_N, dL/dv_N) -> (dL/dx_0, dL/dv_0) in place."""
-        if token is not None and token != getattr(self, "_rollout_token", None):
-            raise RuntimeError("the context ran another rollout since this one; its checkpoints are gone")
-        xbar = _check_vec3(xbar, "xbar")
-        vbar = _check_vec3(vbar, "vbar", xbar.shape[0], xbar.device)
+        """pbf_rollout_backward (float32) or pbf_rollout_backward_f64 (float64):
+        (dL/dx_N, dL/dv_N) -> (dL/dx_0, dL/dv_0) in place."""
+        f64 = isinstance(xbar, torch.Tensor) and xbar.dtype == torch.float64
+        key = "_rollout_token64" if f64 else "_rollout_token"
+        if token is not None and token != (f64, getattr(self, key, None)):
+            raise RuntimeError("the context ran another rollout of this precision since this one; its checkpoints "
+                               "are gone")
+        dt = torch.float64 if f64 else torch.float32
+        xbar = _check_vec3(xbar, "xbar", dtype=dt)
+        vbar = _check_vec3(vbar, "vbar", xbar.shape[0], xbar.device, dtype=dt)
+        fn = self._lib.pbf_rollout_backward_f64 if f64 else self._lib.pbf_rollout_backward
         with torch.cuda.device(xbar.device):
-            _capi.check(self._lib.pbf_rollout_backward(self._ctx, _ptr(xbar), _ptr(vbar), _stream(xbar.device)),
-                        self._ctx)
+            _capi.check(fn(self._ctx, _ptr(xbar), _ptr(vbar), _stream(xbar.device)), self._ctx)
 
 
 class NeighborList:
@@ -790,7 +801,7 @@ class _Rollout(torch.autograd.Function):
     def forward(fctx, x0, v0, ctx, nsteps, checkpoint_every, scene_offsets):
         x = x0.detach().clone().contiguous()
         v = v0.detach().clone().contiguous()
-        rho = torch.empty(x.shape[0], dtype=torch.float32, device=x.device)
+        rho = torch.empty(x.shape[0], dtype=x.dtype, device=x.device)
         fctx.pbf = ctx
         fctx.token = ctx.rollout_forward(x, v, rho, nsteps, checkpoint_every, scene_offsets)
         fctx.mark_non_differentiable(rho)
@@ -808,6 +819,8 @@ def rollout(ctx: Context, x0: torch.Tensor, v0: torch.Tensor, nsteps: int, check
             scene_offsets: Optional[torch.Tensor] = None):
     """Differentiable nsteps of the fluid step (one scene, or a batch delimited
     by scene_offsets): returns (x_N, v_N, rho_N); gradients flow to x0 and v0
-    through the library's adjoint kernels (rho_N carries none). The context
-    keeps one rollout: run backward before the next rollout on it."""
+    through the library's adjoint kernels (rho_N carries none). float32 x0, v0
+    run the single-precision kernels, float64 the double-precision path
+    (pbf_rollout_forward_f64). The context keeps one rollout per precision:
+    run backward before the next rollout of that precision on it."""
     return _Rollout.apply(x0, v0, ctx, int(nsteps), int(checkpoint_every), scene_offsets)

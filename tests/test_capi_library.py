@@ -10,7 +10,7 @@ ROOT = Path(__file__).resolve().parents[1]
 def declared_functions():
     text = (ROOT / "include" / "pbf.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(pbf_[a-z_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(pbf_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_header_declares_the_entry_points():
@@ -37,7 +37,7 @@ def declared_signatures():
     text = (ROOT / "include" / "pbf.h").read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     out = {}
-    for m in re.finditer(r"\b(pbf_[a-z_]+)\s*\(([^;{]*?)\)\s*;", text):
+    for m in re.finditer(r"\b(pbf_[a-z0-9_]+)\s*\(([^;{]*?)\)\s*;", text):
         args = [a.strip() for a in m.group(2).split(",") if a.strip() and a.strip() != "void"]
         out[m.group(1)] = args
     return out

@@ -247,28 +247,54 @@ def oracle_workload_grad(oracle, x0, nsteps, loss_torch, double=True):
     return np.asarray(g, np.float64)
 
 
+def chained_fp64_adjoint(fluid, oracle, x0, nsteps, xbar_N):
+    """The FP64 adjoint of the GPU's own fp32 trajectory: the states x_k, v_k
+    of single GPU steps from (x0, 0) (bit-identical to the rollout's forward),
+    and the oracle's FP64 one-step adjoint linearized at each of them, chained
+    from step nsteps - 1 down to 0. It separates the error of the adjoint
+    kernels from the divergence of an fp32 trajectory from the FP64 one."""
+    ctx = fluid.Context("cuda:0")
+    x, v = dev(x0), dev(np.zeros_like(x0))
+    states = []
+    for _ in range(nsteps):
+        states.append((x.cpu().numpy().astype(np.float64), v.cpu().numpy().astype(np.float64)))
+        ctx.step(x, v, None, 1)
+    Pd = oracle.make_params(double=True)
+    xb, vb = np.asarray(xbar_N, np.float64), np.zeros_like(xbar_N, dtype=np.float64)
+    for xk, vk in reversed(states):
+        xb, vb = oracle.rollout_adjoint(Pd, xk, vk, 1, xb, vb)
+    return x.cpu().numpy(), vb
+
+
 def test_differentiable_workload_gradient(fluid, oracle, scene):
     """The differentiable workload (4096 particles, 50 steps, checkpoint every
-    10), gradient w.r.t. v0 = 0. The oracle's FP64 adjoint is checked against
-    FP64 central differences on 24 seeded components. Over 50 steps the
-    gradient of this workload depends on the rounding of the forward
-    trajectory: lambda = min(0, ...) has a kink at the rest density, the
-    interior of the block sits on it, and an fp32 trajectory (the oracle's
-    own fp32 instantiation as much as the GPU's) switches some lambdas on or
-    off differently from the FP64 one. The bar for the GPU against FP64 is
-    therefore twice the distance of the oracle's fp32 gradient from its FP64
-    gradient (at least 1e-3); the well-conditioned 50-step case is
-    test_rollout_fifty_steps_vs_fp64."""
+    10), gradient w.r.t. v0 = 0, fp32 path.
+
+    The fp32 adjoint kernels are held to 1e-3 of the FP64 adjoint of the same
+    (GPU, fp32) trajectory: the oracle's FP64 one-step adjoint chained along
+    the GPU's states. Against the FP64 trajectory's gradient the fp32 path is
+    only reported: lambda = min(0, ...) has a kink at the rest density, the
+    interior of the block sits on it, and any fp32 trajectory (the oracle's
+    own fp32 instantiation as much as the GPU's) switches some lambdas
+    differently from the FP64 one; the double-precision path
+    (test_gpu_adjoint64.py) meets 1e-3 against FP64 there. The oracle's FP64
+    adjoint is checked against FP64 central differences on 24 components."""
     nsteps, every = 50, 10
-    say("differentiable workload: 4096 particles, 50 steps, GPU gradient vs FP64 adjoint and FD")
+    say("differentiable workload: 4096 particles, 50 steps, fp32 GPU gradient")
     x0, n, loss_torch, loss_np = workload_setup(fluid, oracle, scene, nsteps)
     g_gpu = gpu_workload_grad(fluid, x0, nsteps, every, loss_torch)
+    # the adjoint of the GPU's own trajectory
+    xN, _ = chained_fp64_adjoint(fluid, oracle, x0, nsteps, np.zeros((n, 3)))
+    t = torch.from_numpy(xN.astype(np.float64)).requires_grad_(True)
+    loss_torch(t).backward()
+    _, g_chain = chained_fp64_adjoint(fluid, oracle, x0, nsteps, t.grad.numpy())
+    assert_grad_close(g_gpu, g_chain, "fp32 adjoint kernels vs FP64 adjoint of the GPU trajectory")
+    # the FP64 trajectory: reported
     g64 = oracle_workload_grad(oracle, x0, nsteps, loss_torch, True)
     g32 = oracle_workload_grad(oracle, x0, nsteps, loss_torch, False)
     floor = np.linalg.norm(g32 - g64) / np.linalg.norm(g64)
     rel = np.linalg.norm(g_gpu - g64) / np.linalg.norm(g64)
-    say(f"fp32-oracle vs FP64 {floor:.3e}; GPU vs FP64 {rel:.3e}")
-    assert rel <= max(GRAD_REL, 2 * floor)
+    say(f"vs the FP64 trajectory's gradient (reported): fp32 oracle {floor:.3e}, fp32 GPU {rel:.3e}")
     Pd = oracle.make_params(double=True)
     rng = np.random.default_rng(7)
     comps = [(int(rng.integers(n)), int(rng.integers(3))) for _ in range(24)]

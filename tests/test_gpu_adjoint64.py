@@ -1,0 +1,289 @@
+"""Reverse mode in double precision (pbf_rollout_forward_f64 /
+pbf_rollout_backward_f64, and fluid.rollout on float64 tensors) against the
+oracle's FP64 instantiation and FP64 central differences, on the
+differentiable workload (4096-particle copy, 50 steps, checkpoint every 10)
+and the batched inverse workload (64 scenes of 8,192-32,768 particles, 30
+steps).
+
+The oracle runs with the library's parameters promoted to double (h =
+(double)0.0125f and so on), so both evaluate the same function: over 50
+steps lambda = min(0, .) switches wherever the density crosses the rest
+density, and a parameter differing by 1e-8 relative moves such switches as
+fp32 rounding does.
+
+Bars: the double forward within 1e-9 m of the oracle's FP64 forward;
+gradients within 1e-3 relative (norm-wise, and per component against 1e-3
+of the largest component) of the FP64 adjoint and of FP64 central
+differences; batched gradients bit-identical to per-scene ones."""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+pytestmark = pytest.mark.gpu
+
+from parity_util import say  # noqa: E402
+
+GRAD_REL = 1e-3
+X64_ABS = 1e-9
+F32 = np.float32
+
+
+@pytest.fixture(scope="module", autouse=True)
+def need_gpu():
+    if not torch.cuda.is_available():
+        pytest.skip("no HIP device")
+
+
+def oracle_params64(oracle, fluid):
+    """FP64 oracle parameters equal to the library's float parameters."""
+    p = fluid.Context("cuda:0").params()
+    return oracle.make_params(double=True, h=float(F32(p["smoothing_length"])), d=float(F32(p["sampling_distance"])),
+                              rho0=float(F32(p["rest_density"])), dt=float(F32(p["dt"])),
+                              g=tuple(float(F32(g)) for g in p["gravity"]), iters=int(p["iterations"]),
+                              lam_rest=float(F32(p["lambda_rest"])), lam_eps=float(F32(p["lambda_eps"])),
+                              corr=float(F32(p["corr_scale"])), pair_eps=float(F32(p["pair_eps"])))
+
+
+def d64(a):
+    return torch.from_numpy(np.ascontiguousarray(a, np.float64)).to("cuda:0")
+
+
+def assert_grad_close(got, want, what, bar=GRAD_REL):
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    rel = np.linalg.norm(got - want) / np.linalg.norm(want)
+    comp = np.abs(got - want).max() / np.abs(want).max()
+    say(f"{what}: norm-wise rel err {rel:.3e}, max component err / max component {comp:.3e}")
+    assert rel <= bar and comp <= bar, (what, rel, comp)
+    return rel
+
+
+def compressed_block(scene, dims=(16, 16, 16), spacing=0.0045, seed=5):
+    return scene.jittered_lattice(dims, (0.3, 0.5, 0.3), spacing, 0.1, seed=seed)
+
+
+def test_f64_forward_vs_oracle(fluid, oracle, scene):
+    say("double forward, 4096 compressed particles, 5 steps, vs oracle FP64")
+    x0 = compressed_block(scene).astype(np.float64)
+    n = len(x0)
+    v0 = 0.05 * (scene.uniform01(31, 3 * n).reshape(n, 3) - 0.5)
+    ctx = fluid.Context("cuda:0")
+    x, v = d64(x0), d64(v0)
+    rho = torch.empty(n, dtype=torch.float64, device="cuda:0")
+    ctx.rollout_forward(x, v, rho, 5, 2)
+    xo, vo, ro, _ = oracle.step(oracle_params64(oracle, fluid), x0, v0, 5)
+    ex = np.abs(x.cpu().numpy() - xo).max()
+    ev = np.abs(v.cpu().numpy() - vo).max()
+    er = (np.abs(rho.cpu().numpy() - ro) / ro).max()
+    say(f"|x| {ex:.3e} m, |v| {ev:.3e} m/s, rho rel {er:.3e}; moved {np.abs(xo - x0).max():.3e}")
+    assert ex <= X64_ABS and ev <= X64_ABS / 1e-3 and er <= 1e-9
+
+
+@pytest.mark.parametrize("nsteps,every", [(1, 1), (4, 3)])
+def test_f64_adjoint_vs_oracle(fluid, oracle, scene, nsteps, every):
+    say(f"double rollout adjoint, {nsteps} steps, checkpoint every {every}, vs oracle FP64 adjoint")
+    x0 = compressed_block(scene, (9, 9, 9), 0.0036, seed=17).astype(np.float64)
+    n = len(x0)
+    cx = scene.uniform01(18, 3 * n).reshape(n, 3) - 0.5
+    cv = 1e-3 * (scene.uniform01(19, 3 * n).reshape(n, 3) - 0.5)
+    ctx = fluid.Context("cuda:0")
+    x, v = d64(x0), d64(np.zeros((n, 3)))
+    tok = ctx.rollout_forward(x, v, None, nsteps, every)
+    xb, vb = d64(cx), d64(cv)
+    ctx.rollout_backward(xb, vb, tok)
+    xbo, vbo = oracle.rollout_adjoint(oracle_params64(oracle, fluid), x0, np.zeros((n, 3)), nsteps, cx, cv)
+    assert_grad_close(xb.cpu().numpy(), xbo, "dL/dx0", 1e-8)
+    assert_grad_close(vb.cpu().numpy(), vbo, "dL/dv0", 1e-8)
+
+
+def test_f64_backward_repeatable_and_separate_from_fp32(fluid, scene):
+    say("double rollout: repeatable backward; an fp32 rollout in between leaves it valid")
+    x0 = compressed_block(scene, (7, 7, 7), 0.0036, seed=21)
+    n = len(x0)
+    ctx = fluid.Context("cuda:0")
+    tok = ctx.rollout_forward(d64(x0), d64(np.zeros((n, 3))), None, 3, 2)
+    c = d64(scene.uniform01(22, 3 * n).reshape(n, 3) - 0.5)
+    outs = []
+    for k in range(2):
+        if k == 1:  # an fp32 rollout on the same context keeps its own checkpoints
+            xf = torch.from_numpy(x0).to("cuda:0")
+            ctx.rollout_forward(xf, torch.zeros_like(xf), None, 2, 1)
+        xb, vb = c.clone(), torch.zeros_like(c)
+        ctx.rollout_backward(xb, vb, tok)
+        outs.append((xb.cpu().numpy(), vb.cpu().numpy()))
+    np.testing.assert_array_equal(outs[0][0], outs[1][0])
+    np.testing.assert_array_equal(outs[0][1], outs[1][1])
+
+
+def test_f64_batched_equals_per_scene(fluid, scene):
+    say("double rollout: batched gradients == per-scene gradients, 5 scenes")
+    scenes = [compressed_block(scene, d, 0.0036, seed=50 + k) for k, d in
+              enumerate([(5, 5, 5), (8, 6, 7), (3, 3, 3), (9, 9, 9), (6, 6, 6)])]
+    off = np.zeros(len(scenes) + 1, np.int32)
+    off[1:] = np.cumsum([len(s) for s in scenes])
+    x0 = np.concatenate(scenes).astype(np.float64)
+    n = len(x0)
+    cx = scene.uniform01(60, 3 * n).reshape(n, 3) - 0.5
+    ctx = fluid.Context("cuda:0")
+    ctx.rollout_forward(d64(x0), d64(np.zeros((n, 3))), None, 3, 2, torch.from_numpy(off).to("cuda:0"))
+    xb, vb = d64(cx), d64(np.zeros((n, 3)))
+    ctx.rollout_backward(xb, vb)
+    gb = vb.cpu().numpy()
+    for s, (a, e) in enumerate(zip(off[:-1], off[1:])):
+        c1 = fluid.Context("cuda:0")
+        c1.rollout_forward(d64(x0[a:e]), d64(np.zeros((e - a, 3))), None, 3, 2)
+        xs, vs = d64(cx[a:e]), d64(np.zeros((e - a, 3)))
+        c1.rollout_backward(xs, vs)
+        np.testing.assert_array_equal(gb[a:e], vs.cpu().numpy(), err_msg=f"scene {s}")
+
+
+# ---------------------------------------------------------------- workload losses
+def node_density(x, cW, h, nodes=32, scene_of=None, nscenes=1):
+    """rho_g = sum_j m m_factor W(|x_g - x_j|, h) on the cell-centre nodes of
+    [0,1]^3 (one grid per scene); torch, any dtype."""
+    idx = torch.clamp(torch.floor(x * nodes), 0, nodes - 1)
+    d = x - (idx + 0.5) / nodes
+    r = torch.sqrt((d * d).sum(1) + 1e-30)
+    t = torch.clamp(1.0 - r / h, min=0.0)
+    flat = (idx[:, 0] + nodes * (idx[:, 1] + nodes * idx[:, 2])).long()
+    if scene_of is not None:
+        flat = flat + scene_of * nodes ** 3
+    return torch.zeros(nscenes * nodes ** 3, dtype=x.dtype, device=x.device).index_add(0, flat, cW * t ** 3)
+
+
+def density_constant(fluid):
+    st = fluid.Context("cuda:0").stats()
+    return float(np.float32(st["density_factor"]) * np.float32(st["particle_mass"])) * 15.0 / (np.pi * 0.0125 ** 3)
+
+
+def test_f64_differentiable_workload_gradient(fluid, oracle, scene):
+    """The differentiable workload at the FD size (16^3 = 4096 particles at
+    (0.3, 0.5, 0.3), seed 5, spacing 0.005, 50 steps, checkpoint every 10;
+    L = |mean(x_50) - c*|^2 + sum_g (rho_g - rho*_g)^2 with targets from a
+    run with seeded random v0), gradient w.r.t. v0 = 0: the double path
+    against the oracle's FP64 adjoint and against FP64 central differences of
+    the oracle's forward on 24 seeded components."""
+    nsteps, every = 50, 10
+    say("differentiable workload, double path: 4096 particles, 50 steps, vs FP64 adjoint and FD")
+    Pd = oracle_params64(oracle, fluid)
+    x0 = scene.jittered_lattice((16, 16, 16), (0.3, 0.5, 0.3), 0.005, 0.1, seed=5).astype(np.float64)
+    n = len(x0)
+    cW = density_constant(fluid)
+    vt = 0.2 * (scene.uniform01(6, 3 * n).reshape(n, 3) - 0.5)
+    xt = oracle.step(Pd, x0, vt, nsteps)[0]
+    c_star = torch.from_numpy(xt.mean(0))
+    rho_star = node_density(torch.from_numpy(xt), cW, 0.0125)
+
+    def loss(x):
+        m = x.mean(0) - c_star.to(x.device)
+        return (m * m).sum() + ((node_density(x, cW, 0.0125) - rho_star.to(x.device)) ** 2).sum()
+
+    ctx = fluid.Context("cuda:0")
+    v0 = torch.zeros((n, 3), dtype=torch.float64, device="cuda:0", requires_grad=True)
+    xN, _, _ = fluid.rollout(ctx, d64(x0), v0, nsteps, every)
+    loss(xN).backward()
+    g_gpu = v0.grad.cpu().numpy()
+    xo = oracle.step(Pd, x0, np.zeros((n, 3)), nsteps)[0]
+    t = torch.from_numpy(xo).requires_grad_(True)
+    loss(t).backward()
+    _, g64 = oracle.rollout_adjoint(Pd, x0, np.zeros((n, 3)), nsteps, t.grad.numpy(), np.zeros((n, 3)))
+    assert_grad_close(g_gpu, g64, "double path vs oracle FP64 adjoint (all components)")
+    rng = np.random.default_rng(7)
+    comps = [(int(rng.integers(n)), int(rng.integers(3))) for _ in range(24)]
+    eps = 1e-6
+    fd = []
+    for i, c in comps:
+        vp = np.zeros((n, 3))
+        vp[i, c] = eps
+        lp = float(loss(torch.from_numpy(oracle.step(Pd, x0, vp, nsteps)[0])))
+        lm = float(loss(torch.from_numpy(oracle.step(Pd, x0, -vp, nsteps)[0])))
+        fd.append((lp - lm) / (2 * eps))
+    assert_grad_close(np.array([g_gpu[i, c] for i, c in comps]), np.array(fd),
+                      "double path vs FP64 central differences (24 components)")
+
+
+def inverse_workload(scene, fluid, nsteps):
+    """The batched inverse workload: 64 scenes, n_s log-uniform in [8192,
+    32768] (seed 8); targets (centroid and 32^3 density field per scene) from
+    a run with seeded random v0 (seed 9)."""
+    b = scene.batched_lattice_scenes(nscenes=64, nmin=8192, nmax=32768, seed=8)
+    n = int(b.offsets[-1])
+    dv = torch.device("cuda:0")
+    off = torch.from_numpy(b.offsets.astype(np.int32)).to(dv)
+    sid = torch.from_numpy(np.repeat(np.arange(64), b.counts)).to(dv)
+    counts = torch.from_numpy(b.counts.astype(np.float64)).to(dv)
+    cW = density_constant(fluid)
+    vt = d64(0.2 * (scene.uniform01(9, 3 * n).reshape(n, 3) - 0.5))
+    xt = d64(b.pos)
+    ctx = fluid.Context("cuda:0")
+    ctx.rollout_forward(xt, vt, None, nsteps, nsteps, off)
+
+    def centroids(x):
+        return torch.zeros(64, 3, dtype=x.dtype, device=dv).index_add(0, sid, x) / counts[:, None].to(x.dtype)
+
+    c_star = centroids(xt)
+    rho_star = node_density(xt, cW, 0.0125, scene_of=sid, nscenes=64)
+
+    def scene_losses(x):
+        m = centroids(x) - c_star.to(x.dtype)
+        r = (node_density(x, cW, 0.0125, scene_of=sid, nscenes=64) - rho_star.to(x.dtype)) ** 2
+        return (m * m).sum(1) + r.reshape(64, -1).sum(1)
+    return b, off, scene_losses
+
+
+def test_batched_inverse_gradient_vs_fp64_fd(fluid, oracle, scene):
+    """The batched inverse workload's gradient (30-step rollout, checkpoint
+    every 10, v0 = 0) on 8 scenes x 16 seeded components: the double path
+    within 1e-3 of FP64 central differences; the fp32 path's distance from
+    them is reported beside it. The differences are taken on the double
+    forward of the library, pinned here to the oracle's FP64 forward on two
+    of the scenes; scenes are independent, so one batched run perturbs one
+    component in each of the 8 scenes and reads the 8 per-scene losses."""
+    nsteps, every = 30, 10
+    b, off, scene_losses = inverse_workload(scene, fluid, nsteps)
+    n = int(b.offsets[-1])
+    say(f"batched inverse workload: {n} particles in 64 scenes, 30 steps; gradient vs FP64 FD")
+    x0 = b.pos.astype(np.float64)
+    ctx = fluid.Context("cuda:0")
+    grads = {}
+    for name, dt in (("f64", torch.float64), ("f32", torch.float32)):
+        v0 = torch.zeros((n, 3), dtype=dt, device="cuda:0", requires_grad=True)
+        xN, _, _ = fluid.rollout(ctx, torch.from_numpy(x0).to("cuda:0", dt), v0, nsteps, every, off)
+        scene_losses(xN.double()).sum().backward()
+        grads[name] = v0.grad.double().cpu().numpy()
+    pick = [int(s) for s in np.random.default_rng(10).choice(64, 8, replace=False)]
+    rng = np.random.default_rng(11)
+    comps = {s: [(int(b.offsets[s]) + int(rng.integers(b.counts[s])), int(rng.integers(3))) for _ in range(16)]
+             for s in pick}
+    # the double forward of the library is the oracle's FP64 forward
+    Pd = oracle_params64(oracle, fluid)
+    xg, vg = d64(x0), d64(np.zeros((n, 3)))
+    ctx.rollout_forward(xg, vg, None, nsteps, nsteps, off)
+    xg = xg.cpu().numpy()
+    for s in pick[:2]:
+        a, e = int(b.offsets[s]), int(b.offsets[s + 1])
+        xo = oracle.step(Pd, x0[a:e], np.zeros((e - a, 3)), nsteps)[0]
+        err = np.abs(xg[a:e] - xo).max()
+        say(f"scene {s} ({e - a} particles): double forward vs oracle FP64, 30 steps: {err:.3e} m")
+        assert err <= X64_ABS
+    eps = 1e-6
+    fd = {s: [] for s in pick}
+    for k in range(16):
+        ls = []
+        for sign in (1.0, -1.0):
+            vp = np.zeros((n, 3))
+            for s in pick:
+                i, c = comps[s][k]
+                vp[i, c] = sign * eps
+            x, v = d64(x0), d64(vp)
+            ctx.rollout_forward(x, v, None, nsteps, nsteps, off)
+            ls.append(scene_losses(x).cpu().numpy())
+        for s in pick:
+            fd[s].append((ls[0][s] - ls[1][s]) / (2 * eps))
+    want = np.concatenate([fd[s] for s in pick])
+    got64 = np.array([grads["f64"][i, c] for s in pick for i, c in comps[s]])
+    got32 = np.array([grads["f32"][i, c] for s in pick for i, c in comps[s]])
+    rel32 = np.linalg.norm(got32 - want) / np.linalg.norm(want)
+    say(f"fp32 path vs FP64 FD (8 scenes x 16 components): norm-wise rel err {rel32:.3e} (reported, not asserted)")
+    assert_grad_close(got64, want, "double path vs FP64 FD (8 scenes x 16 components)")
