@@ -181,13 +181,13 @@ def cpu_baseline(x, steps, threads, runs=3):
             "threads_rule": "OMP_NUM_THREADS when set (the process's CPU share), else all CPUs of its affinity"}
 
 
-def gpu_rate(torch, fluid, x_np, steps, warmup, runs=3):
+def gpu_rate(torch, fluid, x_np, steps, warmup, runs=3, params=None):
     """Median over `runs` runs, each from the same initial state: `warmup`
     untimed steps and one more step, then `steps` steps in one call timed with
     HIP events on the stream. Returns (median ms per step, all ms per step,
     context, x, v)."""
     dev = torch.device("cuda:0")
-    ctx = fluid.Context(dev)
+    ctx = fluid.Context(dev, **(params or {}))
     ctx.reserve(len(x_np))
     stream = torch.cuda.current_stream(dev)
     times = []
@@ -270,14 +270,18 @@ def config1(torch, fluid, scene, threads):
     """The 32,768-particle dam-break lattice, 100 steps: GPU (median of 3) and
     the oracle on the host (100 steps on all threads, 10 on one)."""
     x_np = scene.dam_break_32k()
-    ms, runs, _, _, _ = gpu_rate(torch, fluid, x_np, 100, 1)
+    ms, runs, ctx, _, _ = gpu_rate(torch, fluid, x_np, 100, 5)
+    replays = ctx.stats()["graph_replays"]
     rate, used, dt = oracle_rate(x_np, 100, threads)
     rate1, _, dt1 = oracle_rate(x_np, 10, 1)
     progress(f"config 1: GPU {ms:.4f} ms/step; CPU {dt:.1f} s for 100 steps on {used} threads")
     n = len(x_np)
+    eager, _, _, _, _ = gpu_rate(torch, fluid, x_np, 100, 5, params={"flags": 4})
+    progress(f"config 1: eager launches {eager:.4f} ms/step")
     return {"workload": "32,768-particle dam-break lattice at (0.10, 0.50, 0.10), seed 1; 100 steps",
             "gpu": {"value": round(n / (ms / 1e3), 1), "unit": "particle-steps/s", "ms_per_step": round(ms, 4),
-                    "runs_ms_per_step": [round(t, 4) for t in runs]},
+                    "runs_ms_per_step": [round(t, 4) for t in runs], "graph_replays": replays,
+                    "eager_ms_per_step": round(eager, 4)},
             "cpu": {"value": round(rate, 1), "cores": used, "seconds": round(dt, 2),
                     "one_thread": round(rate1, 1)}}
 
@@ -355,7 +359,7 @@ def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
     rb = torch.empty(n, dtype=torch.float32, device=dev)
     cb = fluid.Context(dev)
     cb.reserve(n, 256)
-    cb.batch_step(xb, vb, off, rb, 2)
+    cb.batch_step(xb, vb, off, rb, 3)  # also captures the replayed middle step
     ms_b = timed(lambda k: cb.batch_step(xb, vb, off, rb, k), steps)
     cells = cb.stats()["cells_used"]
     progress(f"batched: {steps} steps of {n} particles in 256 scenes, {ms_b:.1f} ms")
@@ -381,7 +385,8 @@ def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
             "particles": n, "batched": {"value": round(vb_, 1), "unit": "particle-steps/s", "steps": steps,
                                         "ms_per_step": round(ms_b / steps, 4), "cells_used": cells},
             "serial": {"value": round(vs_, 1), "unit": "particle-steps/s", "steps": serial_steps,
-                       "ms_per_step": round(ms_s / serial_steps, 4)},
+                       "ms_per_step": round(ms_s / serial_steps, 4),
+                       "method": "one pbf_step call per scene and step (the plain per-scene launch loop)"},
             "speedup": round(vb_ / vs_, 3)}
 
 
@@ -421,27 +426,34 @@ def differentiable(torch, fluid, scene, nsteps=50, every=10):
     rho_star = node_density(torch, xt, cW, 0.0125)
     stream = torch.cuda.current_stream(dev)
 
-    def run():
-        v0 = torch.zeros_like(x0, requires_grad=True)
+    def run(dtype):
+        xs = x0.to(dtype)
+        v0 = torch.zeros_like(xs, requires_grad=True)
         e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
         torch.cuda.synchronize()
         e0.record(stream)
-        xN, vN, _ = fluid.rollout(ctx, x0, v0, nsteps, every)
-        m = xN.mean(0) - c_star
-        L = (m * m).sum() + ((node_density(torch, xN, cW, 0.0125) - rho_star) ** 2).sum()
+        xN, vN, _ = fluid.rollout(ctx, xs, v0, nsteps, every)
+        m = xN.mean(0) - c_star.to(dtype)
+        L = (m * m).sum() + ((node_density(torch, xN, cW, 0.0125) - rho_star.to(dtype)) ** 2).sum()
         e1.record(stream)
         L.backward()
         e2.record(stream)
         torch.cuda.synchronize()
         return e0.elapsed_time(e1), e1.elapsed_time(e2), float(L), v0.grad
 
-    run()
-    f_ms, b_ms, L, g = run()
+    run(torch.float32)
+    f_ms, b_ms, L, g = run(torch.float32)
     progress(f"differentiable: fwd {f_ms:.2f} ms, bwd {b_ms:.2f} ms")
+    run(torch.float64)
+    f64_f, f64_b, L64, g64 = run(torch.float64)
+    progress(f"differentiable, double path: fwd {f64_f:.2f} ms, bwd {f64_b:.2f} ms")
+    rel = float((g.double() - g64).norm() / g64.norm())
     return {"workload": "65,536-particle block, 50 steps, checkpoint every 10, loss centroid + 32^3 density L2",
             "particles": n, "steps": nsteps, "checkpoint_every": every, "fwd_ms": round(f_ms, 3),
             "bwd_ms": round(b_ms, 3), "bwd_fwd_ratio": round(b_ms / f_ms, 3), "loss": L,
-            "grad_norm": float(g.norm())}
+            "grad_norm": float(g.norm()),
+            "f64": {"fwd_ms": round(f64_f, 3), "bwd_ms": round(f64_b, 3), "bwd_fwd_ratio": round(f64_b / f64_f, 3),
+                    "loss": L64, "grad_norm": float(g64.norm()), "fp32_vs_f64_grad_rel": round(rel, 6)}}
 
 
 def batched_inverse(torch, fluid, scene, adam_steps=20, nsteps=30, every=10, lr=0.05):
@@ -490,9 +502,73 @@ def batched_inverse(torch, fluid, scene, adam_steps=20, nsteps=30, every=10, lr=
     e1.record(stream)
     torch.cuda.synchronize()
     ms = e0.elapsed_time(e1) / adam_steps
+    grad = inverse_gradient_check(torch, fluid, ctx, b, off, sid, counts, cW, nsteps, every, c_star, rho_star)
     return {"workload": "64 scenes, n_s log-uniform [8192, 32768], seed 8; 30-step rollout + adjoint per Adam step",
             "scenes": 64, "particles": n, "adam_steps": adam_steps, "ms_per_adam_step": round(ms, 3),
-            "loss_first": losses[0], "loss_last": losses[-1]}
+            "loss_first": losses[0], "loss_last": losses[-1], "gradient_check": grad}
+
+
+def inverse_gradient_check(torch, fluid, ctx, b, off, sid, counts, cW, nsteps, every, c_star, rho_star,
+                           nscenes_fd=8, ncomp=16, eps=1e-6):
+    """Gradient of the batched inverse loss at v0 = 0 against FP64 central
+    differences on `nscenes_fd` scenes x `ncomp` seeded components, for the
+    fp32 path and the double path. The differences run on the library's
+    double forward (one batched run perturbs one component in each chosen
+    scene; scenes are independent, so the per-scene losses separate them)."""
+    dev = torch.device("cuda:0")
+    n = int(b.offsets[-1])
+    x0 = torch.from_numpy(b.pos).to(dev)
+
+    def scene_losses(x):
+        c = torch.zeros(64, 3, dtype=x.dtype, device=dev).index_add(0, sid, x) / counts[:, None].to(x.dtype)
+        m = c - c_star.to(x.dtype)
+        r = (node_density(torch, x, cW, 0.0125, scene_of=sid, nscenes=64) - rho_star.to(x.dtype)) ** 2
+        return (m * m).sum(1) + r.reshape(64, -1).sum(1)
+
+    grads, times = {}, {}
+    stream = torch.cuda.current_stream(dev)
+    for name, dt in (("fp32", torch.float32), ("f64", torch.float64)):
+        v0 = torch.zeros((n, 3), dtype=dt, device=dev, requires_grad=True)
+        e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        torch.cuda.synchronize()
+        e0.record(stream)
+        xN, _, _ = fluid.rollout(ctx, x0.to(dt), v0, nsteps, every, off)
+        L = scene_losses(xN).sum()
+        e1.record(stream)
+        L.backward()
+        e2.record(stream)
+        torch.cuda.synchronize()
+        grads[name] = v0.grad.double()
+        times[name] = (e0.elapsed_time(e1), e1.elapsed_time(e2))
+    rng = np.random.default_rng(10)
+    pick = [int(s) for s in rng.choice(64, nscenes_fd, replace=False)]
+    comps = {s: [(int(b.offsets[s]) + int(rng.integers(b.counts[s])), int(rng.integers(3))) for _ in range(ncomp)]
+             for s in pick}
+    fd = {s: [] for s in pick}
+    for k in range(ncomp):
+        ls = []
+        for sign in (1.0, -1.0):
+            vp = torch.zeros((n, 3), dtype=torch.float64, device=dev)
+            for s in pick:
+                i, c = comps[s][k]
+                vp[i, c] = sign * eps
+            x = x0.double().clone()
+            ctx.rollout_forward(x, vp, None, nsteps, nsteps, off)
+            ls.append(scene_losses(x))
+        for s in pick:
+            fd[s].append(float((ls[0][s] - ls[1][s]) / (2 * eps)))
+    want = np.concatenate([fd[s] for s in pick])
+    out = {"sample": f"{nscenes_fd} scenes x {ncomp} components of v0, central differences (eps {eps}) of the "
+                     "double forward", "scenes": pick}
+    for name in ("fp32", "f64"):
+        g = grads[name].cpu().numpy()
+        got = np.array([g[i, c] for s in pick for i, c in comps[s]])
+        out[name] = {"rel_err": float(np.linalg.norm(got - want) / np.linalg.norm(want)),
+                     "max_comp_err": float(np.abs(got - want).max() / np.abs(want).max()),
+                     "fwd_ms": round(times[name][0], 3), "bwd_ms": round(times[name][1], 3)}
+    progress(f"batched inverse gradient vs FP64 FD: fp32 {out['fp32']['rel_err']:.3e}, "
+             f"double {out['f64']['rel_err']:.3e}")
+    return out
 
 
 def main():

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 4
+#define PBF_ABI_VERSION 5
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -101,6 +101,12 @@ typedef struct pbf_params {
 #define PBF_FLAG_BOX 2u       /* box boundary on x, v before every step (the
                                  StaticBoundary root node advances before the
                                  fluid, SceneGraph.cpp:101-114) */
+#define PBF_FLAG_EAGER 4u     /* launch every kernel of a multi-step call
+                                 directly. By default the steps between the
+                                 first and the last of a plain pbf_step /
+                                 pbf_batch_step call (no viscosity, no box)
+                                 replay one captured hipGraph of the step's
+                                 kernels; the results are identical */
 
 /* Diagnostics of the last neighbor search of a context. */
 typedef struct pbf_stats {
@@ -130,6 +136,8 @@ typedef struct pbf_stats {
   float density_factor;          /* m_factor of SummationDensity */
   float particle_mass;           /* rest_density * d^3 */
   float neighbor_threshold;      /* squared-distance bound T: sqrtf(s) < h  <=>  s < T */
+  long long graph_captures;      /* hipGraph captures of a multi-step call's middle step (context lifetime) */
+  long long graph_replays;       /* middle steps replayed from that graph (context lifetime) */
 } pbf_stats;
 
 typedef struct pbf_ctx pbf_ctx;

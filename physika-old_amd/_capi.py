@@ -14,7 +14,7 @@ _LIB = _PKG / "_build" / "libpbf.so"
 
 # PBF_ABI_VERSION of the include/pbf.h this table mirrors; lib() refuses a
 # library built from another version (its entry points may take other arguments)
-ABI_VERSION = 4
+ABI_VERSION = 5
 
 PBF_OK = 0
 PBF_ERR_INVALID = -1
@@ -25,6 +25,7 @@ PBF_ERR_UNSUPPORTED = -5
 
 PBF_FLAG_VISCOSITY = 1
 PBF_FLAG_BOX = 2
+PBF_FLAG_EAGER = 4
 
 
 class pbf_params(C.Structure):
@@ -74,6 +75,8 @@ class pbf_stats(C.Structure):
         ("density_factor", C.c_float),
         ("particle_mass", C.c_float),
         ("neighbor_threshold", C.c_float),
+        ("graph_captures", C.c_longlong),
+        ("graph_replays", C.c_longlong),
     ]
 
 

@@ -139,6 +139,21 @@ struct pbf_ctx {
   long long slot_cells = 0;
   int slot_scenes = 0, slot_K = -1;
 
+  // Captured middle step of a multi-step call (hipGraph), replayed for every
+  // step between the first and the last: valid while gen, n, the scenes and
+  // the iteration count are those of the capture. gen is bumped whenever a
+  // device buffer of the context is reallocated or swapped or a kernel
+  // constant changes (a graph holds pointers and constants by value).
+  hipStream_t cap_stream = nullptr;
+  unsigned long long gen = 0;
+  struct StepGraph {
+    hipGraphExec_t exec = nullptr;
+    unsigned long long gen = ~0ull;
+    int n = -1, nscenes = -1, iters = -1;
+    const int* off = nullptr;
+  } sg;
+  long long graph_replays = 0, graph_captures = 0;
+
   unsigned long long* counters = nullptr;     // device, C_COUNT
   unsigned long long* h_counters = nullptr;   // pinned host copy
   hipEvent_t counters_ev = nullptr;
@@ -243,7 +258,7 @@ int validate(const pbf_params* p, std::string& why) {
   if (p->iterations < 0 || p->iterations > 1000) return why = "iterations must be in [0, 1000]", PBF_ERR_INVALID;
   if (p->mask_words < 0) return why = "mask_words must be >= 0", PBF_ERR_INVALID;
   if (p->list_slots < 0) return why = "list_slots must be >= 0", PBF_ERR_INVALID;
-  if (p->flags & ~(PBF_FLAG_VISCOSITY | PBF_FLAG_BOX)) return why = "unknown flag bits", PBF_ERR_INVALID;
+  if (p->flags & ~(PBF_FLAG_VISCOSITY | PBF_FLAG_BOX | PBF_FLAG_EAGER)) return why = "unknown flag bits", PBF_ERR_INVALID;
   if (!std::isfinite(p->viscosity) || p->viscosity < 0.0f) return why = "viscosity must be finite and >= 0", PBF_ERR_INVALID;
   if (p->viscosity_iterations < 0 || p->viscosity_iterations > 1000)
     return why = "viscosity_iterations must be in [0, 1000]", PBF_ERR_INVALID;
@@ -391,6 +406,7 @@ int derive(pbf_ctx* c, const pbf_params* p) {
   c->g = g;
   c->full_cells = ncell;
   c->pc = pc;
+  c->gen++;
   c->pc64 = derive64(p);
   c->m_factor = m_factor;
   c->mass = mass;
@@ -405,6 +421,7 @@ int ensure_scenes(pbf_ctx* c, int nscenes) {
   if (need_cells > (long long)INT32_MAX - 2)
     return fail(c, PBF_ERR_UNSUPPORTED, "scenes x grid cells exceed 2^31");
   if (nscenes > c->scene_cap) {
+    c->gen++;
     dfree(c->scene_minmax);
     dfree(c->win);
     c->scene_cap = 0;
@@ -416,6 +433,7 @@ int ensure_scenes(pbf_ctx* c, int nscenes) {
     c->scene_cap = nscenes;
   }
   if (need_cells > c->cell_cap) {
+    c->gen++;
     dfree(c->cell_cnt);
     dfree(c->cell_start);
     c->cell_cap = 0;
@@ -432,6 +450,7 @@ int ensure_scenes(pbf_ctx* c, int nscenes) {
 int ensure_scan_scratch(pbf_ctx* c, long long n) {
   const size_t need = scan_scratch_ints(n);
   if (need <= c->scan_scratch_n) return PBF_OK;
+  c->gen++;
   dfree(c->scan_scratch);
   c->scan_scratch_n = 0;
   PBF_HIP(c, hipMalloc(&c->scan_scratch, sizeof(int) * need));
@@ -457,6 +476,7 @@ int list_per_of(const pbf_ctx* c) {
 
 int ensure_mask(pbf_ctx* c, long long words) {
   if (words <= c->mask_cap) return PBF_OK;
+  c->gen++;
   dfree(c->mask);
   c->mask_cap = 0;
   PBF_HIP(c, hipMalloc(&c->mask, sizeof(uint32_t) * (size_t)words));
@@ -466,6 +486,7 @@ int ensure_mask(pbf_ctx* c, long long words) {
 
 int ensure_list(pbf_ctx* c, long long slots) {
   if (slots <= c->list_cap) return PBF_OK;
+  c->gen++;
   dfree(c->list);
   c->list_cap = 0;
   PBF_HIP(c, hipMalloc(&c->list, sizeof(uint16_t) * (size_t)slots));
@@ -481,6 +502,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   if (st) return st;
   if (n < c->cap_n) return PBF_OK;  // index n holds the padding particle
   const int cap = std::max(((n + 1 + 255) / 256) * 256, 256);
+  c->gen++;
   dfree(c->key); dfree(c->slot); dfree(c->tmp); dfree(c->perm); dfree(c->inv);
   dfree(c->sid); dfree(c->ssid); dfree(c->bpart); dfree(c->nbr_cnt); dfree(c->cnt_orig);
   dfree(c->tid); dfree(c->pid);
@@ -839,14 +861,65 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
 // nsteps fused steps in place; d_rho receives the last step's density.
 // Plain steps keep the state in cell order between the steps (the next sort
 // reads nearly in order); the last step writes the caller's arrays.
+// A step between the first and the last of a plain multi-step call reads and
+// writes only the context's cell-ordered state (the caller's arrays are not
+// touched), so its launch sequence is the same every time: it is captured
+// once into a hipGraph (kernel nodes only) and replayed on the call's stream.
+int replay_middle_step(pbf_ctx* c, int n, const int* scene_off, int nscenes, CellState& cs, hipStream_t st) {
+  auto& g = c->sg;
+  if (!g.exec || g.gen != c->gen || g.n != n || g.nscenes != nscenes || g.off != scene_off ||
+      g.iters != c->p.iterations) {
+    if (g.exec) {
+      (void)hipGraphExecDestroy(g.exec);
+      g.exec = nullptr;
+    }
+    PBF_HIP(c, hipStreamBeginCapture(c->cap_stream, hipStreamCaptureModeRelaxed));
+    CellState from = cs, to{};
+    const int s = step_once(c, nullptr, nullptr, nullptr, nullptr, nullptr, n, scene_off, nscenes, nullptr, nullptr,
+                            c->cap_stream, &from, &to);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->cap_stream, &graph);
+    if (s || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      if (s) return s;
+      return fail(c, PBF_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    if (to.x != cs.x || to.v != cs.v || to.id != cs.id) {
+      (void)hipGraphDestroy(graph);
+      return fail(c, PBF_ERR_HIP, "captured step left its state in other buffers");
+    }
+    const hipError_t ei = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) {
+      g.exec = nullptr;
+      return fail(c, PBF_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    g.gen = c->gen;
+    g.n = n;
+    g.nscenes = nscenes;
+    g.off = scene_off;
+    g.iters = c->p.iterations;
+    c->graph_captures++;
+  }
+  PBF_HIP(c, hipGraphLaunch(g.exec, st));
+  c->graph_replays++;
+  return PBF_OK;
+}
+
 int run_steps(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, const int* scene_off, int nscenes, int nsteps,
               hipStream_t st) {
   const bool keep = nsteps > 1 && c->p.iterations > 0 && (c->p.flags & (PBF_FLAG_BOX | PBF_FLAG_VISCOSITY)) == 0;
+  // the per-pass timers record events around each pass: no replay then
+  const bool graph = keep && nsteps > 2 && !c->profiling && (c->p.flags & PBF_FLAG_EAGER) == 0;
   CellState cs{};
   for (int step = 0; step < nsteps; ++step) {
     const bool last = step + 1 == nsteps;
-    int s = step_once(c, d_x, d_v, d_x, d_v, last ? d_rho : nullptr, n, scene_off, nscenes, nullptr, nullptr, st,
-                      keep && step > 0 ? &cs : nullptr, keep && !last ? &cs : nullptr);
+    int s;
+    if (graph && step > 0 && !last)
+      s = replay_middle_step(c, n, scene_off, nscenes, cs, st);
+    else
+      s = step_once(c, d_x, d_v, d_x, d_v, last ? d_rho : nullptr, n, scene_off, nscenes, nullptr, nullptr, st,
+                    keep && step > 0 ? &cs : nullptr, keep && !last ? &cs : nullptr);
     if (s) return s;
   }
   return record_counters(c, st);
@@ -927,6 +1000,7 @@ int ensure_tape(pbf_ctx* c, int m) {
 // Exchanges the context's search outputs with a slot's arrays: swapped in
 // before a taped step, the step writes into the slot; swapped back after it.
 void swap_slot(pbf_ctx* c, pbf_ctx::TapeSlot& t) {
+  c->gen++;
   std::swap(c->perm, t.perm);
   std::swap(c->ssid, t.ssid);
   std::swap(c->nbr_cnt, t.nbr_cnt);
@@ -1091,6 +1165,7 @@ int pbf_create(const pbf_params* p, pbf_ctx** out) {
       hipMalloc(&c->counters, sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
       hipHostMalloc(&c->h_counters, sizeof(unsigned long long) * C_COUNT) != hipSuccess ||
       hipEventCreateWithFlags(&c->counters_ev, hipEventDisableTiming) != hipSuccess ||
+      hipStreamCreateWithFlags(&c->cap_stream, hipStreamNonBlocking) != hipSuccess ||
       hipMalloc(&c->off1, sizeof(int) * 2) != hipSuccess || hipMalloc(&c->total_cells, sizeof(int)) != hipSuccess ||
       hipMalloc(&c->scan_part, sizeof(int) * (kScanDevBlocks + 1)) != hipSuccess) {
     pbf_destroy(c);
@@ -1127,6 +1202,8 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->counters);
   if (c->h_counters) (void)hipHostFree(c->h_counters);
   if (c->counters_ev) (void)hipEventDestroy(c->counters_ev);
+  if (c->sg.exec) (void)hipGraphExecDestroy(c->sg.exec);
+  if (c->cap_stream) (void)hipStreamDestroy(c->cap_stream);
   for (auto& m : c->marks) {
     (void)hipEventDestroy(m.a);
     (void)hipEventDestroy(m.b);
@@ -1282,7 +1359,7 @@ int pbf_rollout_forward(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const 
   if (bad_n(n) || n == 0 || nsteps < 1 || checkpoint_every < 1 || nscenes < 1 || !d_x || !d_v ||
       (!d_scene_offsets && nscenes != 1))
     return fail(c, PBF_ERR_INVALID, "bad argument");
-  if (c->p.flags != 0)
+  if (c->p.flags & (PBF_FLAG_VISCOSITY | PBF_FLAG_BOX))
     return fail(c, PBF_ERR_UNSUPPORTED, "reverse mode covers the plain PBD step (no viscosity, no box)");
   hipStream_t st = (hipStream_t)stream;
   auto& r = c->roll;
@@ -1411,7 +1488,7 @@ int pbf_rollout_forward_f64(pbf_ctx* c, double* d_x, double* d_v, double* d_rho,
   if (bad_n(n) || n == 0 || nsteps < 1 || checkpoint_every < 1 || nscenes < 1 || !d_x || !d_v ||
       (!d_scene_offsets && nscenes != 1))
     return fail(c, PBF_ERR_INVALID, "bad argument");
-  if (c->p.flags != 0)
+  if (c->p.flags & (PBF_FLAG_VISCOSITY | PBF_FLAG_BOX))
     return fail(c, PBF_ERR_UNSUPPORTED, "reverse mode covers the plain PBD step (no viscosity, no box)");
   hipStream_t st = (hipStream_t)stream;
   auto& r = c->roll64;
@@ -1641,6 +1718,8 @@ int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
   out->density_factor = c->m_factor;
   out->particle_mass = c->mass;
   out->neighbor_threshold = c->pc.thr2;
+  out->graph_captures = c->graph_captures;
+  out->graph_replays = c->graph_replays;
   return PBF_OK;
 }
 
