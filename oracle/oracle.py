@@ -14,10 +14,16 @@ from pathlib import Path
 import numpy as np
 
 _DIR = Path(__file__).resolve().parent
-_LIB_PATH = _DIR / "_build" / "liborc.so"
+# PBF_ORACLE_LIB selects another build of the same sources (the sanitizer
+# build, _build/liborc_san.so)
+_LIB_PATH = Path(os.environ.get("PBF_ORACLE_LIB", _DIR / "_build" / "liborc.so"))
 
 
 def build(force: bool = False) -> Path:
+    if "PBF_ORACLE_LIB" in os.environ:
+        if not _LIB_PATH.exists():
+            raise RuntimeError(f"PBF_ORACLE_LIB={_LIB_PATH} does not exist")
+        return _LIB_PATH
     if force or not _LIB_PATH.exists():
         subprocess.run(["make", "-C", str(_DIR)] + (["-B"] if force else []), check=True,
                        stdout=subprocess.DEVNULL)
