@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 5
+#define PBF_ABI_VERSION 6
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -220,6 +220,22 @@ int pbf_pbd_constrain(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_index,
  * pbf_pbd_constrain. */
 int pbf_pbd_constrain_mass(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_index, const int* d_elems,
                            long long nelems, const float* d_mass_inv, float* d_rho, int n, void* stream);
+
+/* pbf_pbd_constrain without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Same
+ * arguments, semantics and results (to float-atomic summation order) as
+ * pbf_pbd_constrain, but its scratch (iterate positions and displacement
+ * sums) is a fixed array in the library, one copy per device, instead of the
+ * context's reservation: at most PBF_NOALLOC_MAX_PARTICLES particles
+ * (PBF_ERR_UNSUPPORTED beyond). The array holds 28 MiB on every device the
+ * library runs on until the process ends. Every call on one device shares
+ * that device's array: calls that overlap on one device (several streams,
+ * threads or contexts) race and must be serialized by the caller. The
+ * context only supplies the parameters. */
+#define PBF_NOALLOC_MAX_PARTICLES (1 << 20)
+int pbf_pbd_constrain_noalloc(pbf_ctx* ctx, float* d_x, float* d_v, const int* d_index, const int* d_elems,
+                              long long nelems, float* d_rho, int n, void* stream);
 
 /* nsteps of PositionBasedFluidModel::step without viscosity
  * (PositionBasedFluidModel.cpp:83-101): begin -> NeighborQuery on x^n ->

@@ -14,7 +14,7 @@ _LIB = _PKG / "_build" / "libpbf.so"
 
 # PBF_ABI_VERSION of the include/pbf.h this table mirrors; lib() refuses a
 # library built from another version (its entry points may take other arguments)
-ABI_VERSION = 5
+ABI_VERSION = 6
 
 PBF_OK = 0
 PBF_ERR_INVALID = -1
@@ -106,6 +106,7 @@ SIGNATURES = {
     "pbf_density": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_longlong, _VP, C.c_int, _VP]),
     "pbf_pbd_constrain": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_longlong, _VP, C.c_int, _VP]),
     "pbf_pbd_constrain_mass": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_longlong, _VP, _VP, C.c_int, _VP]),
+    "pbf_pbd_constrain_noalloc": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_longlong, _VP, C.c_int, _VP]),
     "pbf_step": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, C.c_int, _VP]),
     "pbf_batch_step": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, _VP]),
     "pbf_rollout_forward": (C.c_int, [_CTX, _VP, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_int, _VP]),

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -14,6 +15,14 @@
 #include "pbf_internal.h"
 
 using namespace pbf;
+
+// Scratch of pbf_pbd_constrain_noalloc: the iterate positions (float4) and the
+// displacement sums (3 floats) of up to PBF_NOALLOC_MAX_PARTICLES particles,
+// 28 MiB on every device the library runs on, held until the process ends.
+// In host code the name is no device address: each device's copy is looked
+// up once (scratch_address) and only that pointer reaches kernels.
+static __device__ __align__(256) unsigned char
+    scratch[(size_t)PBF_NOALLOC_MAX_PARTICLES * (sizeof(float4) + 3 * sizeof(float))];
 
 struct pbf_ctx {
   pbf_params p{};
@@ -160,6 +169,9 @@ struct pbf_ctx {
   bool counters_pending = false;
 
   int last_n = -1;  // particle count of the last neighbor search (-1: none)
+
+  void* noalloc_scratch = nullptr;  // this device's copy of `scratch` (null: lookup failed, noalloc_err says why)
+  std::string noalloc_err;
 
   // per-pass timing
   bool profiling = false;
@@ -766,6 +778,41 @@ int caller_csr(pbf_ctx* c, const int* d_index, const int* d_elems, long long nel
   return PBF_OK;
 }
 
+// Device address of the current device's copy of `scratch`, looked up once per
+// device (hipGetSymbolAddress with that device current) and kept.
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+hipError_t scratch_address(int device, void** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
+// DensityPBD::takeOneIteration x K over a caller list (DensityPBD.cu:306-365),
+// then DP_UpdateVelocity (:368-388): xa holds the caller's x on entry, dpos
+// zeros; minv selects the massInv variant.
+void pbd_csr_rounds(const pbf_ctx* c, float4* xa, float* dpos, const Csr& nb, float* d_x, float* d_v,
+                    float* d_rho, int n, const float* minv, hipStream_t st) {
+  for (int it = 0; it < c->p.iterations; ++it) {
+    const bool last = it + 1 == c->p.iterations;
+    launch_csr_lambda(xa, nb, last ? d_rho : nullptr, c->pc, st, minv);
+    launch_csr_dx_atomic(xa, nb, dpos, c->pc, st, minv);
+    launch_csr_apply(xa, dpos, n, st);
+  }
+  launch_csr_finish(xa, d_x, d_v, n, c->pc.dt, st);
+}
+
 // Samples of the box field (K_DistanceFieldToBox, computed on the device once
 // per box).
 int ensure_box(pbf_ctx* c, hipStream_t st) {
@@ -1173,6 +1220,11 @@ int pbf_create(const pbf_params* p, pbf_ctx** out) {
   }
   std::memset(c->h_counters, 0, sizeof(unsigned long long) * C_COUNT);
   (void)hipMemset(c->counters, 0, sizeof(unsigned long long) * C_COUNT);
+  {
+    const hipError_t e = scratch_address(c->device, &c->noalloc_scratch);
+    if (e != hipSuccess)
+      c->noalloc_err = std::string("scratch lookup on device ") + std::to_string(c->device) + ": " + hipGetErrorString(e);
+  }
   st = ensure_scenes(c, 1);
   if (st) {
     pbf_destroy(c);
@@ -1315,13 +1367,29 @@ int pbf_pbd_constrain(pbf_ctx* c, float* d_x, float* d_v, const int* d_index, co
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   launch_aos_to_f4(d_x, c->xa, n, st);
-  for (int it = 0; it < c->p.iterations; ++it) {
-    const bool last = it + 1 == c->p.iterations;
-    launch_csr_lambda(c->xa, nb, last ? d_rho : nullptr, c->pc, st);
-    launch_csr_dx_atomic(c->xa, nb, c->dpos, c->pc, st);
-    launch_csr_apply(c->xa, c->dpos, n, st);
-  }
-  launch_csr_finish(c->xa, d_x, d_v, n, c->pc.dt, st);
+  pbd_csr_rounds(c, c->xa, c->dpos, nb, d_x, d_v, d_rho, n, nullptr, st);
+  PBF_LAUNCHED(c);
+  return PBF_OK;
+}
+
+int pbf_pbd_constrain_noalloc(pbf_ctx* c, float* d_x, float* d_v, const int* d_index, const int* d_elems,
+                              long long nelems, float* d_rho, int n, void* stream) {
+  if (!c) return PBF_ERR_INVALID;
+  if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_index))) return fail(c, PBF_ERR_INVALID, "bad argument");
+  if (n > PBF_NOALLOC_MAX_PARTICLES)
+    return fail(c, PBF_ERR_UNSUPPORTED,
+                "pbf_pbd_constrain_noalloc: " + std::to_string(n) + " particles exceed its fixed scratch of " +
+                    std::to_string(PBF_NOALLOC_MAX_PARTICLES) + " (use pbf_pbd_constrain)");
+  if (n == 0) return PBF_OK;
+  if (!c->noalloc_scratch) return fail(c, PBF_ERR_HIP, "pbf_pbd_constrain_noalloc: " + c->noalloc_err);
+  Csr nb;
+  int s = caller_csr(c, d_index, d_elems, nelems, n, &nb);
+  if (s) return s;
+  hipStream_t st = (hipStream_t)stream;
+  float4* xa = static_cast<float4*>(c->noalloc_scratch);
+  float* dpos = reinterpret_cast<float*>(xa + PBF_NOALLOC_MAX_PARTICLES);
+  launch_aos_to_f4_clear(d_x, xa, dpos, n, st);
+  pbd_csr_rounds(c, xa, dpos, nb, d_x, d_v, d_rho, n, nullptr, st);
   PBF_LAUNCHED(c);
   return PBF_OK;
 }
@@ -1644,13 +1712,7 @@ int pbf_pbd_constrain_mass(pbf_ctx* c, float* d_x, float* d_v, const int* d_inde
   if (s) return s;
   hipStream_t st = (hipStream_t)stream;
   launch_aos_to_f4(d_x, c->xa, n, st);
-  for (int it = 0; it < c->p.iterations; ++it) {
-    const bool last = it + 1 == c->p.iterations;
-    launch_csr_lambda(c->xa, nb, last ? d_rho : nullptr, c->pc, st, d_mass_inv);
-    launch_csr_dx_atomic(c->xa, nb, c->dpos, c->pc, st, d_mass_inv);
-    launch_csr_apply(c->xa, c->dpos, n, st);
-  }
-  launch_csr_finish(c->xa, d_x, d_v, n, c->pc.dt, st);
+  pbd_csr_rounds(c, c->xa, c->dpos, nb, d_x, d_v, d_rho, n, d_mass_inv, st);
   PBF_LAUNCHED(c);
   return PBF_OK;
 }

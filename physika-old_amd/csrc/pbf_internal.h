@@ -299,6 +299,8 @@ void launch_csr_viscosity(const float4* xa, const Csr& nb, const float4* v_old, 
 void launch_integrate(float* x, float* v, const float* f, int n, const PhysConst& pc, hipStream_t st);
 void launch_csr_density(const float* x, const Csr& nb, float* rho, const PhysConst& pc, hipStream_t st);
 void launch_aos_to_f4(const float* x, float4* xa, int n, hipStream_t st);
+// as launch_aos_to_f4, and dpos[0 .. 3n) := 0
+void launch_aos_to_f4_clear(const float* x, float4* xa, float* dpos, int n, hipStream_t st);
 void launch_csr_lambda(float4* xa, const Csr& nb, float* rho, const PhysConst& pc, hipStream_t st,
                        const float* minv = nullptr);
 void launch_csr_dx_atomic(const float4* xa, const Csr& nb, float* dpos, const PhysConst& pc, hipStream_t st,

@@ -63,6 +63,23 @@ void launch_aos_to_f4(const float* x, float4* xa, int n, hipStream_t st) {
   k_aos_to_f4<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(x, xa, n);
 }
 
+// The copy, and the displacement sums cleared (m_deltaPos.reset(),
+// DensityPBD.cu:315) for scratch that the previous call left as it was.
+__global__ __launch_bounds__(kBlock) void k_aos_to_f4_clear(const float* __restrict__ x, float4* __restrict__ xa,
+                                                            float* __restrict__ dpos, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  xa[i] = load3(x, i);
+  const size_t b = 3 * (size_t)i;
+  dpos[b] = 0.f;
+  dpos[b + 1] = 0.f;
+  dpos[b + 2] = 0.f;
+}
+
+void launch_aos_to_f4_clear(const float* x, float4* xa, float* dpos, int n, hipStream_t st) {
+  k_aos_to_f4_clear<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(x, xa, dpos, n);
+}
+
 // Density and K_ComputeLambdas (DensityPBD.cu:46-90) over a CSR list; lambda
 // goes to xa[i].w. With minv (inverse masses) the massInv variant
 // (DensityPBD.cu:93-131): S = sum_j |g|^2 m_j^-1 + |sum g|^2 m_i^-1.

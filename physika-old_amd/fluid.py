@@ -249,6 +249,20 @@ class Context:
                                                     _ptr(nbr.elements), nbr.elements.numel(), _ptr(rho), n,
                                                     _stream(x.device)), self._ctx)
 
+    def pbd_constrain_noalloc(self, x, v, nbr: "NeighborList", rho: Optional[torch.Tensor]) -> None:
+        """pbf_pbd_constrain_noalloc: pbd_constrain with the library's fixed
+        per-device scratch (at most 2^20 particles; calls on one device must
+        not overlap)."""
+        x = _check_vec3(x, "position")
+        n = x.shape[0]
+        v = _check_vec3(v, "velocity", n, x.device)
+        _check_nbr(nbr, n, x.device)
+        rho = _check_scalar(rho, "rho", n, x.device)
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_pbd_constrain_noalloc(self._ctx, _ptr(x), _ptr(v), _ptr(nbr.index),
+                                                            _ptr(nbr.elements), nbr.elements.numel(), _ptr(rho),
+                                                            n, _stream(x.device)), self._ctx)
+
     def step(self, x, v, rho: Optional[torch.Tensor] = None, nsteps: int = 1) -> None:
         x = _check_vec3(x, "position")
         v = _check_vec3(v, "velocity", x.shape[0], x.device)
