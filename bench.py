@@ -181,14 +181,27 @@ def cpu_baseline(x, steps, threads, runs=3):
             "threads_rule": "OMP_NUM_THREADS when set (the process's CPU share), else all CPUs of its affinity"}
 
 
+TUNE = True  # autotune the launch settings of every measured context (--no-autotune: the defaults)
+
+
+def tune_summary(r):
+    return {k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+
+
 def gpu_rate(torch, fluid, x_np, steps, warmup, runs=3, params=None):
     """Median over `runs` runs, each from the same initial state: `warmup`
     untimed steps and one more step, then `steps` steps in one call timed with
-    HIP events on the stream. Returns (median ms per step, all ms per step,
-    context, x, v)."""
+    HIP events on the stream. Before them (untimed) the autotuner picks the
+    context's launch settings on the initial state; ctx.tune_report holds its
+    report. Returns (median ms per step, all ms per step, context, x, v)."""
     dev = torch.device("cuda:0")
     ctx = fluid.Context(dev, **(params or {}))
     ctx.reserve(len(x_np))
+    ctx.tune_report = None
+    if TUNE:
+        x0 = torch.from_numpy(x_np).to(dev)
+        ctx.tune_report = tune_summary(ctx.autotune(x0, torch.zeros_like(x0), steps=min(max(steps, 2), 5)))
+        del x0
     stream = torch.cuda.current_stream(dev)
     times = []
     for _ in range(runs):
@@ -281,7 +294,7 @@ def config1(torch, fluid, scene, threads):
     return {"workload": "32,768-particle dam-break lattice at (0.10, 0.50, 0.10), seed 1; 100 steps",
             "gpu": {"value": round(n / (ms / 1e3), 1), "unit": "particle-steps/s", "ms_per_step": round(ms, 4),
                     "runs_ms_per_step": [round(t, 4) for t in runs], "graph_replays": replays,
-                    "eager_ms_per_step": round(eager, 4)},
+                    "eager_ms_per_step": round(eager, 4), "tuning": ctx.tune_report},
             "cpu": {"value": round(rate, 1), "cores": used, "seconds": round(dt, 2),
                     "one_thread": round(rate1, 1)}}
 
@@ -308,7 +321,7 @@ def config2b(torch, fluid, scene, steps=200, dense_steps=20):
     return {"workload": "8 Gaussian blobs x 131,072 = 1,048,576 particles, sigma 0.05, seed 3; "
                         f"{steps} steps after 2", "particles": n,
             "value": round(n / (ms / 1e3), 1), "unit": "particle-steps/s", "ms_per_step": round(ms, 4),
-            "runs_ms_per_step": [round(t, 4) for t in runs],
+            "runs_ms_per_step": [round(t, 4) for t in runs], "tuning": ctx.tune_report,
             "out_of_grid_last_step": int(st["out_of_grid"]),
             "dense_phase": {"steps": dense_steps, "value": round(n / (ms_d / 1e3), 1), "ms_per_step": round(ms_d, 4),
                             "runs_ms_per_step": [round(t, 4) for t in runs_d]},
@@ -330,7 +343,7 @@ def shard_share(torch, fluid, scene, steps=200, warmup=5):
     progress(f"shard share: {len(x)} particles, {ms:.4f} ms/step")
     return {"workload": "one of 8 x-slabs of the 1M lattice: 16 owned + 2 x 3 ghost columns (x 64 x 128)",
             "particles": len(x), "owned": owned, "steps": steps, "ms_per_step": round(ms, 4),
-            "runs_ms_per_step": [round(t, 4) for t in runs],
+            "runs_ms_per_step": [round(t, 4) for t in runs], "tuning": ctx.tune_report,
             "owned_particle_steps_per_s": round(owned / (ms / 1e3), 1)}
 
 
@@ -359,6 +372,7 @@ def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
     rb = torch.empty(n, dtype=torch.float32, device=dev)
     cb = fluid.Context(dev)
     cb.reserve(n, 256)
+    tuning = tune_summary(cb.autotune(xb, vb, off, steps=3)) if TUNE else None
     cb.batch_step(xb, vb, off, rb, 3)  # also captures the replayed middle step
     ms_b = timed(lambda k: cb.batch_step(xb, vb, off, rb, k), steps)
     cells = cb.stats()["cells_used"]
@@ -383,7 +397,8 @@ def batched_vs_serial(torch, fluid, scene, steps, serial_steps):
     vs_ = n * serial_steps / (ms_s / 1e3)
     return {"workload": "256 lattice scenes, n_s log-uniform [2048, 65536], seed 4", "scenes": 256,
             "particles": n, "batched": {"value": round(vb_, 1), "unit": "particle-steps/s", "steps": steps,
-                                        "ms_per_step": round(ms_b / steps, 4), "cells_used": cells},
+                                        "ms_per_step": round(ms_b / steps, 4), "cells_used": cells,
+                                        "tuning": tuning},
             "serial": {"value": round(vs_, 1), "unit": "particle-steps/s", "steps": serial_steps,
                        "ms_per_step": round(ms_s / serial_steps, 4),
                        "method": "one pbf_step call per scene and step (the plain per-scene launch loop)"},
@@ -587,9 +602,12 @@ def main():
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable and batched-inverse workloads")
     ap.add_argument("--no-peaks", action="store_true", help="skip the achievable copy / FMA rate probes")
+    ap.add_argument("--no-autotune", action="store_true", help="measure with the default launch settings")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
+    global TUNE
+    TUNE = not args.no_autotune
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not part of this "
               "build", file=sys.stderr)
@@ -673,6 +691,7 @@ def main():
         "passes_launches": {k: v[1] for k, v in pass_ms.items()},
         "step_roofline": whole_step,
         "runs_ms_per_step": [round(t, 4) for t in runs],
+        "tuning": ctx.tune_report,
     }
     if not args.no_configs:
         out["shard_share"] = shard_share(torch, fluid, pkg.scene)

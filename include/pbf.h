@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 6
+#define PBF_ABI_VERSION 7
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -338,6 +338,43 @@ typedef struct pbf_profile {
 int pbf_set_profiling(pbf_ctx* ctx, int enable);
 /* Accumulated timings since profiling was enabled. Blocking. */
 int pbf_get_profile(pbf_ctx* ctx, pbf_profile* out);
+
+/* Launch settings of a context. Not part of the reference (its kernels run in
+ * blocks of BLOCK_SIZE = 64 threads, Core/Utility/cuda_utilities.h:34). None
+ * of them changes a result: every setting gives bit-identical outputs. */
+typedef struct pbf_tuning {
+  int pass_block;          /* threads per workgroup of the density/lambda and
+                              displacement passes: 64, 128, 256 (default) or 512 */
+  int stage_min_particles; /* as pbf_params.stage_min_particles: searches over at
+                              least this many particles stage their neighbor
+                              lists in LDS; < 0 never */
+} pbf_tuning;
+
+int pbf_set_tuning(pbf_ctx* ctx, const pbf_tuning* t);
+int pbf_get_tuning(const pbf_ctx* ctx, pbf_tuning* out);
+
+/* Autotuner. Times every setting of pbf_tuning (4 workgroup sizes x staged
+ * lists on / off) on the caller's workload: a copy of (d_x, d_v), `steps` plain
+ * steps after a warm-up call, the better of two runs, on `stream`; keeps the
+ * fastest (a setting replaces the one before it only when more than 1%
+ * faster, the default first) and applies it to the context. The workload is a
+ * scene (d_scene_offsets NULL, nscenes 1) or a batch (as pbf_batch_step); the
+ * caller's x and v are not changed. With cache_path (may be NULL) the choice
+ * is kept in a JSON file under a key of the device architecture, the particle
+ * count (half octaves), the scene count (octaves), h/d, the iteration count
+ * and the flags; a later call with an equal key applies the stored setting
+ * without timing. Blocking; allocates two copies of the state while timing. */
+typedef struct pbf_tune_report {
+  pbf_tuning best;
+  double best_ms;     /* per step of the chosen setting (timed, or as stored in the cache) */
+  double default_ms;  /* per step of the default setting (0 when read from the cache) */
+  int candidates;     /* settings timed (0 when read from the cache) */
+  int from_cache;     /* 1: the setting came from cache_path */
+  char key[128];      /* the cache key of this workload */
+} pbf_tune_report;
+
+int pbf_autotune(pbf_ctx* ctx, const float* d_x, const float* d_v, const int* d_scene_offsets, int nscenes, int n,
+                 int steps, const char* cache_path, pbf_tune_report* out, void* stream);
 
 const char* pbf_last_error(const pbf_ctx* ctx);
 

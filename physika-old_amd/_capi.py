@@ -14,7 +14,7 @@ _LIB = _PKG / "_build" / "libpbf.so"
 
 # PBF_ABI_VERSION of the include/pbf.h this table mirrors; lib() refuses a
 # library built from another version (its entry points may take other arguments)
-ABI_VERSION = 6
+ABI_VERSION = 7
 
 PBF_OK = 0
 PBF_ERR_INVALID = -1
@@ -87,6 +87,21 @@ class pbf_profile(C.Structure):
     _fields_ = [("ms", C.c_double * 4), ("launches", C.c_longlong * 4)]
 
 
+class pbf_tuning(C.Structure):
+    _fields_ = [("pass_block", C.c_int), ("stage_min_particles", C.c_int)]
+
+
+class pbf_tune_report(C.Structure):
+    _fields_ = [
+        ("best", pbf_tuning),
+        ("best_ms", C.c_double),
+        ("default_ms", C.c_double),
+        ("candidates", C.c_int),
+        ("from_cache", C.c_int),
+        ("key", C.c_char * 128),
+    ]
+
+
 # every symbol include/pbf.h declares, with its ctypes signature
 _VP = C.c_void_p
 _CTX = C.c_void_p
@@ -119,6 +134,10 @@ SIGNATURES = {
     "pbf_get_stats": (C.c_int, [_CTX, C.POINTER(pbf_stats)]),
     "pbf_set_profiling": (C.c_int, [_CTX, C.c_int]),
     "pbf_get_profile": (C.c_int, [_CTX, C.POINTER(pbf_profile)]),
+    "pbf_set_tuning": (C.c_int, [_CTX, C.POINTER(pbf_tuning)]),
+    "pbf_get_tuning": (C.c_int, [_CTX, C.POINTER(pbf_tuning)]),
+    "pbf_autotune": (C.c_int, [_CTX, _VP, _VP, _VP, C.c_int, C.c_int, C.c_int, C.c_char_p,
+                               C.POINTER(pbf_tune_report), _VP]),
     "pbf_last_error": (C.c_char_p, [_CTX]),
 }
 

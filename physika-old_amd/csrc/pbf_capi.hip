@@ -170,6 +170,8 @@ struct pbf_ctx {
 
   int last_n = -1;  // particle count of the last neighbor search (-1: none)
 
+  int pass_block = kBlock;  // threads per workgroup of the PBD passes (pbf_tuning)
+
   void* noalloc_scratch = nullptr;  // this device's copy of `scratch` (null: lookup failed, noalloc_err says why)
   std::string noalloc_err;
 
@@ -778,6 +780,12 @@ int caller_csr(pbf_ctx* c, const int* d_index, const int* d_elems, long long nel
   return PBF_OK;
 }
 
+}  // namespace
+
+int pbf::ctx_fail(pbf_ctx* c, int status, const std::string& msg) { return fail(c, status, msg); }
+
+namespace {
+
 // Device address of the current device's copy of `scratch`, looked up once per
 // device (hipGetSymbolAddress with that device current) and kept.
 constexpr int kMaxDevices = 64;
@@ -874,17 +882,18 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
     float4* dst = P ? (last ? nullptr : P[it + 1]) : bufs[it & 1];
     {
       PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st);
-      launch_pass_density_lambda(nv, src, c->pc, last ? d_rho : nullptr, c->perm, st, aux ? aux[it] : nullptr);
+      launch_pass_density_lambda(nv, src, c->pc, last ? d_rho : nullptr, c->perm, st, aux ? aux[it] : nullptr,
+                                 c->pass_block);
     }
     {
       PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
       if (last && to) {
         launch_pass_displacement(nv, src, dst, c->pc, true, c->x0, c->vs, c->perm, nullptr, nullptr, st, c->vv,
-                                 c->pid);
+                                 c->pid, c->pass_block);
         *to = CellState{dst, c->vv, c->pid};
       } else {
         launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st,
-                                 (last && visc) ? c->vv : nullptr);
+                                 (last && visc) ? c->vv : nullptr, nullptr, c->pass_block);
       }
     }
   }
@@ -1807,6 +1816,25 @@ int pbf_get_profile(pbf_ctx* c, pbf_profile* out) {
     out->ms[i] = c->prof_ms[i];
     out->launches[i] = c->prof_n[i];
   }
+  return PBF_OK;
+}
+
+int pbf_set_tuning(pbf_ctx* c, const pbf_tuning* t) {
+  if (!c || !t) return fail(c, PBF_ERR_INVALID, "null argument");
+  if (t->pass_block != 64 && t->pass_block != 128 && t->pass_block != 256 && t->pass_block != 512)
+    return fail(c, PBF_ERR_INVALID, "pass_block must be 64, 128, 256 or 512");
+  if (t->pass_block != c->pass_block || t->stage_min_particles != c->p.stage_min_particles) {
+    c->pass_block = t->pass_block;
+    c->p.stage_min_particles = t->stage_min_particles;
+    c->gen++;  // a captured step holds the launch shapes and the staging choice
+  }
+  return PBF_OK;
+}
+
+int pbf_get_tuning(const pbf_ctx* c, pbf_tuning* out) {
+  if (!c || !out) return PBF_ERR_INVALID;
+  out->pass_block = c->pass_block;
+  out->stage_min_particles = c->p.stage_min_particles;
   return PBF_OK;
 }
 

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
+
+struct pbf_ctx;
 
 namespace pbf {
 
@@ -231,15 +234,18 @@ void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* 
 // Sums the per-wave statistics of the last neighbor pass (nwaves waves) into
 // counters[C_PAIRS], [C_CANDIDATES], [C_LIST_SLOTS] and [C_LIST_NEED].
 void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* counters, hipStream_t st);
+// block: threads per workgroup, 64, 128, 256 or 512 (the results do not
+// depend on it; each wave of a block has its own stage)
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
-                                float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr);
+                                float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr,
+                                int block = kBlock);
 // last: write x/v to original order, or with v_sorted the positions to xd and
 // the velocities to v_sorted (sorted order, for the viscosity passes or the
 // next step of the call; id_out then receives each particle's original index).
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc,
                               bool last, const float4* x0, const float4* vs, const int* perm,
                               float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr,
-                              int* id_out = nullptr);
+                              int* id_out = nullptr, int block = kBlock);
 void launch_write_back(const float4* xs, const float4* vs, const int* perm, int n, float* x_out,
                        float* v_out, hipStream_t st);
 void launch_invert(const int* perm, int* inv, int n, hipStream_t st);
@@ -311,5 +317,9 @@ constexpr int kFixedMax = 32;
 void launch_nbr_fixed(const NbrView& nv, const int* perm, int limit, int* counts, int* elems, hipStream_t st);
 void launch_csr_apply(float4* xa, float* dpos, int n, hipStream_t st);
 void launch_csr_finish(const float4* xa, float* x, float* v, int n, float dt, hipStream_t st);
+
+// Records msg as the context's last error and returns status (host code of
+// the library's other translation units).
+int ctx_fail(pbf_ctx* c, int status, const std::string& msg);
 
 }  // namespace pbf

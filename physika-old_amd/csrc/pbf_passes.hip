@@ -424,13 +424,14 @@ void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* co
 // (DensityPBD.cu:46-90) fused: one traversal gives rho_i, sum g and sum g.g;
 // lambda_i goes to xs[s].w for pass B. aux (taped steps only) receives
 // (sum_j t^2/r d_ij, 1/(S + eps)) for the adjoint.
-__global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* __restrict__ xs, PhysConst pc,
+template <int B>
+__global__ __launch_bounds__(B) void k_density_lambda(NbrView nv, float4* __restrict__ xs, PhysConst pc,
                                                            float* __restrict__ rho_out,
                                                            const int* __restrict__ perm,
                                                            float4* __restrict__ aux) {
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (s >= nv.n) return;
-  __shared__ float4 stage_buf[kBlock / 64][kStageRecords];
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
   const float4 xi = xs[s];
   float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
   for_each_neighbor_staged<false>(nv, s, xs, stage_buf[threadIdx.x >> 6], [&](int, float4 xj) {
@@ -455,9 +456,20 @@ __global__ __launch_bounds__(kBlock) void k_density_lambda(NbrView nv, float4* _
   if (aux) aux[s] = make_float4(sgx, sgy, sgz, 1.0f / (S + pc.lam_eps));
 }
 
+template <int B>
+void density_lambda_as(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
+                       hipStream_t st, float4* aux) {
+  k_density_lambda<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
+}
+
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                                hipStream_t st, float4* aux) {
-  k_density_lambda<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
+                                hipStream_t st, float4* aux, int block) {
+  switch (block) {
+    case 64: density_lambda_as<64>(nv, xs, pc, rho_out, perm, st, aux); break;
+    case 128: density_lambda_as<128>(nv, xs, pc, rho_out, perm, st, aux); break;
+    case 512: density_lambda_as<512>(nv, xs, pc, rho_out, perm, st, aux); break;
+    default: density_lambda_as<256>(nv, xs, pc, rho_out, perm, st, aux); break;
+  }
 }
 
 // Pass B: K_ComputeDisplacement + K_UpdatePosition (DensityPBD.cu:135-179,
@@ -469,7 +481,8 @@ void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& 
 // the scatter back to original order, or keeps x and v in cell order (with
 // each particle's original index in id_out) for the viscosity passes or the
 // next step of the call.
-__global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float4* __restrict__ xs,
+template <int B>
+__global__ __launch_bounds__(B) void k_displacement(NbrView nv, const float4* __restrict__ xs,
                                                          float4* __restrict__ xd, PhysConst pc, int last,
                                                          const float4* __restrict__ x0,
                                                          const float4* __restrict__ vs,
@@ -490,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
     cy = fmaf(c, p.dy, cy);
     cz = fmaf(c, p.dz, cz);
   };
-  __shared__ float4 stage_buf[kBlock / 64][kStageRecords];
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
   float ax = 0.f, ay = 0.f, az = 0.f;
   for_each_neighbor_staged<true>(nv, s, xs, stage_buf[threadIdx.x >> 6],
                                  [&](int, float4 xj) { pair(xj, ax, ay, az); });
@@ -559,11 +572,23 @@ __global__ __launch_bounds__(kBlock) void k_displacement(NbrView nv, const float
   v_out[3 * i + 2] = vv.z + (nz - xo.z) / pc.dt;
 }
 
+template <int B>
+void displacement_as(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
+                     const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out, hipStream_t st,
+                     float4* v_sorted, int* id_out) {
+  k_displacement<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm, x_out, v_out,
+                                                      v_sorted, id_out);
+}
+
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                               const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out,
-                              hipStream_t st, float4* v_sorted, int* id_out) {
-  k_displacement<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm,
-                                                                 x_out, v_out, v_sorted, id_out);
+                              hipStream_t st, float4* v_sorted, int* id_out, int block) {
+  switch (block) {
+    case 64: displacement_as<64>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
+    case 128: displacement_as<128>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
+    case 512: displacement_as<512>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
+    default: displacement_as<256>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
+  }
 }
 
 // ---------------------------------------------------------------------------

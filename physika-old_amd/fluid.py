@@ -144,6 +144,36 @@ class Context:
     def set_profiling(self, enable: bool) -> None:
         _capi.check(self._lib.pbf_set_profiling(self._ctx, 1 if enable else 0), self._ctx)
 
+    # -- launch settings and the autotuner (no reference counterpart) ----------
+    def tuning(self) -> dict:
+        t = _capi.pbf_tuning()
+        _capi.check(self._lib.pbf_get_tuning(self._ctx, C.byref(t)), self._ctx)
+        return {"pass_block": t.pass_block, "stage_min_particles": t.stage_min_particles}
+
+    def set_tuning(self, pass_block: int = 256, stage_min_particles: int = 0) -> None:
+        t = _capi.pbf_tuning(int(pass_block), int(stage_min_particles))
+        _capi.check(self._lib.pbf_set_tuning(self._ctx, C.byref(t)), self._ctx)
+
+    def autotune(self, x: torch.Tensor, v: torch.Tensor, scene_offsets: Optional[torch.Tensor] = None,
+                 steps: int = 3, cache: Optional[str] = None) -> dict:
+        """pbf_autotune: times every launch setting on (x, v) (not modified),
+        applies the fastest; with `cache` (a JSON file path) the choice is
+        stored and reused for workloads of the same key."""
+        x = _check_vec3(x, "position")
+        v = _check_vec3(v, "velocity", x.shape[0], x.device)
+        nscenes, off = 1, None
+        if scene_offsets is not None:
+            off = _check_scene_offsets(scene_offsets, x.shape[0], x.device, self.validate_offsets)
+            nscenes = off.shape[0] - 1
+        r = _capi.pbf_tune_report()
+        with torch.cuda.device(x.device):
+            _capi.check(self._lib.pbf_autotune(self._ctx, _ptr(x), _ptr(v), _ptr(off), nscenes, x.shape[0],
+                                               int(steps), None if cache is None else str(cache).encode(),
+                                               C.byref(r), _stream(x.device)), self._ctx)
+        return {"pass_block": r.best.pass_block, "stage_min_particles": r.best.stage_min_particles,
+                "best_ms": r.best_ms, "default_ms": r.default_ms, "candidates": r.candidates,
+                "from_cache": bool(r.from_cache), "key": r.key.decode()}
+
     def profile(self) -> dict:
         """Per pass: (summed ms, timed launches) since profiling was enabled."""
         p = _capi.pbf_profile()

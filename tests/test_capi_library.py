@@ -77,7 +77,7 @@ def test_binding_argument_lists_match_header(capi):
 def test_host_only_calls(capi):
     """Calls that touch no device: version, status strings, defaults."""
     L = capi.lib()
-    assert L.pbf_abi_version() == capi.ABI_VERSION == 6
+    assert L.pbf_abi_version() == capi.ABI_VERSION == 7
     assert L.pbf_status_string(0) == b"ok"
     assert L.pbf_status_string(-3) == b"buffer capacity exceeded"
     p = capi.default_params()
@@ -91,7 +91,8 @@ def test_struct_layout_matches_c_compiler(capi, tmp_path):
     """Field offsets and sizes of the ctypes mirrors equal what gcc lays out
     for include/pbf.h."""
     import subprocess
-    structs = {"pbf_params": capi.pbf_params, "pbf_stats": capi.pbf_stats, "pbf_profile": capi.pbf_profile}
+    structs = {"pbf_params": capi.pbf_params, "pbf_stats": capi.pbf_stats, "pbf_profile": capi.pbf_profile,
+               "pbf_tuning": capi.pbf_tuning, "pbf_tune_report": capi.pbf_tune_report}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "pbf.h"', "int main(void){"]
     for sname, cls in structs.items():
         lines.append(f'printf("{sname} %zu\\n", sizeof({sname}));')
