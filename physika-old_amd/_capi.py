@@ -175,6 +175,30 @@ def lib():
     return _lib
 
 
+_native = None
+_NATIVE = _LIB.parent / ("_pbf_torch" + __import__("sysconfig").get_config_var("EXT_SUFFIX"))
+
+
+def native():
+    """The pybind11 layer (_pbf_torch, csrc/pbf_torch.cpp): the reverse mode as
+    a torch C++ autograd Function over the same libpbf.so. Loaded after lib(),
+    so the library it links is the checked one."""
+    global _native
+    if _native is None:
+        lib()
+        if not _NATIVE.exists():
+            raise RuntimeError(f"{_NATIVE} is not built; run `python __graft_entry__.py build`")
+        import importlib.util
+        import torch  # noqa: F401  (libtorch before the extension)
+        spec = importlib.util.spec_from_file_location("_pbf_torch", _NATIVE)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        if mod.abi_version() != ABI_VERSION:
+            raise RuntimeError(f"{_NATIVE} links a library of ABI version {mod.abi_version()}, expected {ABI_VERSION}")
+        _native = mod
+    return _native
+
+
 def check(status: int, ctx=None) -> None:
     if status != PBF_OK:
         msg = lib().pbf_last_error(ctx).decode() if ctx else lib().pbf_status_string(status).decode()

@@ -42,6 +42,11 @@ def _stream(device) -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream(device).cuda_stream)
 
 
+def _stream_int(device) -> int:
+    """The current stream's hipStream_t as an integer (0: the null stream)."""
+    return int(torch.cuda.current_stream(device).cuda_stream)
+
+
 def _check_vec3(t: torch.Tensor, name: str, n: Optional[int] = None, device=None,
                 dtype=torch.float32) -> torch.Tensor:
     """A (n, 3) contiguous device tensor of `dtype`, checked, not copied: the
@@ -333,7 +338,7 @@ class Context:
             _capi.check(self._lib.pbf_box_constrain(self._ctx, _ptr(x), _ptr(v), x.shape[0], _stream(x.device)),
                         self._ctx)
 
-    # -- reverse mode -----------------------------------------------------------
+    # -- reverse mode (through the pybind11 layer, csrc/pbf_torch.cpp) --------
     def rollout_forward(self, x, v, rho: Optional[torch.Tensor] = None, nsteps: int = 1,
                         checkpoint_every: int = 10, scene_offsets: Optional[torch.Tensor] = None) -> int:
         """pbf_rollout_forward (float32 state) or pbf_rollout_forward_f64
@@ -344,32 +349,24 @@ class Context:
         x = _check_vec3(x, "position", dtype=dt)
         v = _check_vec3(v, "velocity", x.shape[0], x.device, dtype=dt)
         rho = _check_scalar(rho, "rho", x.shape[0], x.device, dtype=dt)
-        off, nscenes = None, 1
+        off = None
         if scene_offsets is not None:
             off = _check_scene_offsets(scene_offsets, x.shape[0], x.device, self.validate_offsets)
-            nscenes = off.shape[0] - 1
-        fn = self._lib.pbf_rollout_forward_f64 if f64 else self._lib.pbf_rollout_forward
         with torch.cuda.device(x.device):
-            _capi.check(fn(self._ctx, _ptr(x), _ptr(v), _ptr(rho), _ptr(off), nscenes, x.shape[0], int(nsteps),
-                           int(checkpoint_every), _stream(x.device)), self._ctx)
-        key = "_rollout_token64" if f64 else "_rollout_token"
-        setattr(self, key, getattr(self, key, 0) + 1)
-        return (f64, getattr(self, key))
+            return _capi.native().rollout_forward(int(self._ctx.value or 0), x, v, rho, int(nsteps),
+                                                  int(checkpoint_every), off, _stream_int(x.device))
 
     def rollout_backward(self, xbar, vbar, token: Optional[int] = None) -> None:
         """pbf_rollout_backward (float32) or pbf_rollout_backward_f64 (float64):
-        (dL/dx_N, dL/dv_N) -> (dL/dx_0, dL/dv_0) in place."""
-        f64 = isinstance(xbar, torch.Tensor) and xbar.dtype == torch.float64
-        key = "_rollout_token64" if f64 else "_rollout_token"
-        if token is not None and token != (f64, getattr(self, key, None)):
-            raise RuntimeError("the context ran another rollout of this precision since this one; its checkpoints "
-                               "are gone")
-        dt = torch.float64 if f64 else torch.float32
+        (dL/dx_N, dL/dv_N) -> (dL/dx_0, dL/dv_0) in place. A token from a
+        rollout that is no longer the context's latest of its precision is
+        refused (RuntimeError)."""
+        dt = torch.float64 if isinstance(xbar, torch.Tensor) and xbar.dtype == torch.float64 else torch.float32
         xbar = _check_vec3(xbar, "xbar", dtype=dt)
         vbar = _check_vec3(vbar, "vbar", xbar.shape[0], xbar.device, dtype=dt)
-        fn = self._lib.pbf_rollout_backward_f64 if f64 else self._lib.pbf_rollout_backward
         with torch.cuda.device(xbar.device):
-            _capi.check(fn(self._ctx, _ptr(xbar), _ptr(vbar), _stream(xbar.device)), self._ctx)
+            _capi.native().rollout_backward(int(self._ctx.value or 0), xbar, vbar, int(token or 0),
+                                            _stream_int(xbar.device))
 
 
 class NeighborList:
@@ -840,25 +837,6 @@ class ParticleFluid:
         return 0
 
 
-class _Rollout(torch.autograd.Function):
-    @staticmethod
-    def forward(fctx, x0, v0, ctx, nsteps, checkpoint_every, scene_offsets):
-        x = x0.detach().clone().contiguous()
-        v = v0.detach().clone().contiguous()
-        rho = torch.empty(x.shape[0], dtype=x.dtype, device=x.device)
-        fctx.pbf = ctx
-        fctx.token = ctx.rollout_forward(x, v, rho, nsteps, checkpoint_every, scene_offsets)
-        fctx.mark_non_differentiable(rho)
-        return x, v, rho
-
-    @staticmethod
-    def backward(fctx, gx, gv, grho):
-        xb = torch.zeros_like(gv) if gx is None else gx.detach().clone().contiguous()
-        vb = torch.zeros_like(xb) if gv is None else gv.detach().clone().contiguous()
-        fctx.pbf.rollout_backward(xb, vb, fctx.token)
-        return xb, vb, None, None, None, None
-
-
 def rollout(ctx: Context, x0: torch.Tensor, v0: torch.Tensor, nsteps: int, checkpoint_every: int = 10,
             scene_offsets: Optional[torch.Tensor] = None):
     """Differentiable nsteps of the fluid step (one scene, or a batch delimited
@@ -867,4 +845,9 @@ def rollout(ctx: Context, x0: torch.Tensor, v0: torch.Tensor, nsteps: int, check
     run the single-precision kernels, float64 the double-precision path
     (pbf_rollout_forward_f64). The context keeps one rollout per precision:
     run backward before the next rollout of that precision on it."""
-    return _Rollout.apply(x0, v0, ctx, int(nsteps), int(checkpoint_every), scene_offsets)
+    off = None
+    if scene_offsets is not None:
+        off = _check_scene_offsets(scene_offsets, x0.shape[0], x0.device, ctx.validate_offsets)
+    with torch.cuda.device(x0.device):
+        return tuple(_capi.native().rollout(x0, v0, int(ctx.handle.value or 0), int(nsteps),
+                                            int(checkpoint_every), off, _stream_int(x0.device)))

@@ -2,6 +2,8 @@
 declares; the Python binding's signature table covers exactly those."""
 import ctypes
 import re
+
+import pytest
 from pathlib import Path
 
 ROOT = Path(__file__).resolve().parents[1]
@@ -121,3 +123,16 @@ def test_null_context_rejected(capi):
 def test_header_and_binding_agree_on_the_abi_version(capi):
     m = re.search(r"#define PBF_ABI_VERSION (\d+)", (ROOT / "include" / "pbf.h").read_text())
     assert int(m.group(1)) == capi.ABI_VERSION
+
+
+def test_pybind11_layer_loads(capi):
+    """The pybind11 layer (_pbf_torch) links the same library version and
+    refuses a null context and host tensors before any device call."""
+    import torch
+    m = capi.native()
+    assert m.abi_version() == capi.ABI_VERSION
+    x = torch.zeros((4, 3))
+    with pytest.raises(RuntimeError, match="null pbf context"):
+        m.rollout(x, x, 0, 1, 1, None, 0)
+    with pytest.raises(RuntimeError, match="HIP device"):
+        m.rollout_forward(1, x, x.clone(), None, 1, 1, None, 0)
