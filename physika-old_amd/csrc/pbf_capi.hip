@@ -63,6 +63,8 @@ struct pbf_ctx {
   uint16_t* cnt_a = nullptr;    // per sorted particle: stage-slot entries of its list (staged lists)
   int* oog_rcnt = nullptr;      // per sorted particle: out-of-grid particles that list it
   int* oog_rev = nullptr;       // kOogRev per sorted particle: those particles
+  int* oog_flag = nullptr;      // per original index: out of the grid (cap + 1, zero between sorts)
+  int* oog_pre = nullptr;       // its exclusive scan (cap + 1): a particle's slot in its scene's bucket
   uint32_t* mask = nullptr;
   long long mask_cap = 0;  // words
   uint16_t* list = nullptr;
@@ -523,6 +525,7 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->stage); dfree(c->cnt_a); dfree(c->wave_stat); dfree(c->oog_rcnt); dfree(c->oog_rev);
+  dfree(c->oog_flag); dfree(c->oog_pre);
   c->cap_n = 0;
   const size_t ni = sizeof(int) * (size_t)cap, nf4 = sizeof(float4) * (size_t)cap;
   PBF_HIP(c, hipMalloc(&c->key, ni));
@@ -555,6 +558,9 @@ int ensure_particles(pbf_ctx* c, int n, int nscenes) {
   PBF_HIP(c, hipMalloc(&c->cnt_a, sizeof(uint16_t) * (size_t)cap));
   PBF_HIP(c, hipMalloc(&c->oog_rcnt, ni));
   PBF_HIP(c, hipMalloc(&c->oog_rev, ni * kOogRev));
+  PBF_HIP(c, hipMalloc(&c->oog_flag, ni + sizeof(int)));
+  PBF_HIP(c, hipMemset(c->oog_flag, 0, ni + sizeof(int)));
+  PBF_HIP(c, hipMalloc(&c->oog_pre, ni + sizeof(int)));
   PBF_HIP(c, hipMalloc(&c->wave_stat, sizeof(int4) * nwv));
   c->cap_n = cap;
   st = ensure_mask(c, mask_words_for(c, cap));
@@ -653,6 +659,8 @@ SortBufs sort_of(pbf_ctx* c, const int* scene_off, int nscenes) {
   sb.xc = c->xc;
   sb.oog_rcnt = c->oog_rcnt;
   sb.tid = c->tid;
+  sb.oog_flag = c->oog_flag;
+  sb.oog_pre = c->oog_pre;
   sb.xc_stride = c->cap_n + kXnPad;
   return sb;
 }
@@ -1254,6 +1262,7 @@ int pbf_destroy(pbf_ctx* c) {
   dfree(c->xn); dfree(c->x0); dfree(c->xa); dfree(c->xb); dfree(c->vs); dfree(c->vv); dfree(c->xc);
   dfree(c->dpos); dfree(c->mask_base); dfree(c->list_base); dfree(c->list_jbase); dfree(c->list_mode);
   dfree(c->stage); dfree(c->cnt_a); dfree(c->wave_stat); dfree(c->oog_rcnt); dfree(c->oog_rev);
+  dfree(c->oog_flag); dfree(c->oog_pre);
   dfree(c->mask); dfree(c->list); dfree(c->scan_scratch); dfree(c->bf.d);
   free_slots(c);
   dfree(c->roll.off); dfree(c->roll.ckpt); dfree(c->roll.seg); dfree(c->roll.adj);

@@ -193,6 +193,8 @@ struct SortBufs {
   int xc_stride;
   int* oog_rcnt;          // per sorted particle, zeroed by the reorder
   int* tid;               // original index of each particle of tmp (input in cell order only)
+  int* oog_flag;          // per original index: 1 when out of the grid (n + 1; zero between sorts)
+  int* oog_pre;           // exclusive scan of oog_flag (n + 1)
 };
 
 // Input of a sort: the caller's state in original order (x, v: xyz float

@@ -111,9 +111,13 @@ __device__ __forceinline__ void dev_chunk(int n, int& lo, int& hi) {
   hi = min(n, lo + per * kScanPerBlock);
 }
 
-__global__ __launch_bounds__(1024) void k_scan_dev_reduce(const int* in, const int* n_ptr, int* part) {
+// Length min(*n_ptr, n_max); gate (may be null): the scan runs only when
+// *gate != 0.
+__global__ __launch_bounds__(1024) void k_scan_dev_reduce(const int* in, const int* n_ptr, int n_max, int* part,
+                                                          const unsigned long long* gate) {
   __shared__ int wsum[16];
-  const int n = *n_ptr;
+  if (gate && *gate == 0) return;
+  const int n = min(*n_ptr, n_max);
   int lo, hi;
   dev_chunk(n, lo, hi);
   int acc = 0;
@@ -132,8 +136,10 @@ __global__ __launch_bounds__(1024) void k_scan_dev_reduce(const int* in, const i
 // and the last block writes the grand total. The counts are zeroed once read,
 // so the next sort starts from zero counts without a clearing pass (the array
 // is zeroed when allocated).
-__global__ __launch_bounds__(1024) void k_scan_dev_down(int* in, int* out, const int* n_ptr, const int* part) {
-  const int n = *n_ptr;
+__global__ __launch_bounds__(1024) void k_scan_dev_down(int* in, int* out, const int* n_ptr, int n_max,
+                                                        const int* part, const unsigned long long* gate) {
+  if (gate && *gate == 0) return;
+  const int n = min(*n_ptr, n_max);
   int lo, hi;
   dev_chunk(n, lo, hi);
   int before;
@@ -242,7 +248,7 @@ template <bool kCellOrder>
 __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(SortIn in, int n, GridDev g,
                                                                const int* __restrict__ off, int nscenes,
                                                                int* __restrict__ sid, int* __restrict__ minmax,
-                                                               int* __restrict__ bpart) {
+                                                               int* __restrict__ bpart, int* __restrict__ oog_flag) {
   __shared__ int red[kBoundsBlock / 64][7];
   const int base = blockIdx.x * kBoundsSpan;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -260,6 +266,7 @@ __global__ __launch_bounds__(kBoundsBlock) void k_scene_bounds(SortIn in, int n,
       sid[i] = sc;
       inside = grid_cell(g, in_pos<kCellOrder>(in, i), c[0], c[1], c[2], nonfinite);
       counts += (inside ? 0 : 1) + (nonfinite ? 1 << 16 : 0);
+      if (!inside) oog_flag[kCellOrder ? in.id[i] : i] = 1;
     }
     sq[q] = sc;
     inq[q] = inside;
@@ -476,13 +483,25 @@ __global__ __launch_bounds__(1024) void k_scene_windows(int* __restrict__ minmax
 }
 
 // K_CalculateParticleNumber (GridHash.cu:78-87): key = compact cell or the
-// scene's out-of-grid bucket; slot = arrival order inside the cell.
+// scene's out-of-grid bucket; slot = arrival order inside a cell, and inside a
+// bucket the rank of the particle's original index among the scene's
+// out-of-grid particles (oog_pre, the scan of the bounds pass's flags), so a
+// bucket is in original-index order at any size without atomics. Block 0 sets
+// each bucket's count.
 template <bool kCellOrder>
 __global__ __launch_bounds__(kBlock) void k_bin(SortIn in, int n, GridDev g,
                                                 const int* __restrict__ sid, const SceneWin* __restrict__ win,
                                                 int* __restrict__ key, int* __restrict__ slot,
-                                                int* __restrict__ cell_cnt) {
+                                                int* __restrict__ cell_cnt, const int* __restrict__ scene_off,
+                                                int nscenes, const int* __restrict__ oog_pre,
+                                                const unsigned long long* __restrict__ counters) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (blockIdx.x == 0 && counters[C_OOG] != 0) {
+    for (int s = threadIdx.x; s < nscenes; s += blockDim.x) {
+      const SceneWin w = win[s];
+      cell_cnt[w.cell_base + w.ncell] = oog_pre[min(scene_off[s + 1], n)] - oog_pre[min(scene_off[s], n)];
+    }
+  }
   if (i >= n) return;
   int ci, cj, ck;
   bool nonfinite;
@@ -513,19 +532,7 @@ __global__ __launch_bounds__(kBlock) void k_bin(SortIn in, int n, GridDev g,
   } else if (inside) {
     sl = atomicAdd(&cell_cnt[k], 1);
   }
-  // Out-of-grid particles of one scene all count into its bucket: one atomic
-  // per bucket and wave (same-address atomics serialize in L2).
-  unsigned long long pending = __ballot(!inside);
-  while (pending) {  // uniform: every active lane runs the same iterations
-    const int leader = __ffsll((long long)pending) - 1;
-    const int kb = __shfl(k, leader);
-    const unsigned long long peers = __ballot(!inside && k == kb) & pending;
-    int base = 0;
-    if (lane == leader) base = atomicAdd(&cell_cnt[kb], __popcll(peers));
-    base = __shfl(base, leader);
-    if ((peers >> lane) & 1ull) sl = base + __popcll(peers & ((1ull << lane) - 1ull));
-    pending &= ~peers;
-  }
+  if (!inside) sl = oog_pre[kCellOrder ? in.id[i] : i] - oog_pre[min(scene_off[sid[i]], n)];
   slot[i] = sl;
 }
 
@@ -544,7 +551,8 @@ __global__ __launch_bounds__(kBlock) void k_scatter(const int* __restrict__ key,
 
 // Final position inside the cell = rank of the particle id among the cell's
 // ids (deterministic order, independent of atomic arrival). Cells holding more
-// than kRankMax particles keep arrival order (still a valid cell sort). One
+// than kRankMax particles keep arrival order (still a valid cell sort); a
+// scene's out-of-grid bucket is in original-index order already (k_bin). One
 // thread per slot of the scattered order: slot t's particle lands inside its
 // own cell's range, so the sorted arrays are written nearly in order and only
 // the inputs are read scattered.
@@ -561,7 +569,7 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
                                                     float4* __restrict__ xn,
                                                     float4* __restrict__ x0, float4* __restrict__ vs, PhysConst pc,
                                                     int integrate, float* __restrict__ xc, int xc_stride,
-                                                    int* __restrict__ oog_rcnt) {
+                                                    int* __restrict__ oog_rcnt, const SceneWin* __restrict__ win) {
 #pragma clang fp contract(off)
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t == 0) x0[n] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);  // the padding particle
@@ -571,7 +579,8 @@ __global__ __launch_bounds__(kBlock) void k_reorder(const int* __restrict__ key,
   const int k = key[i];
   const int a = cell_start[k], b = cell_start[k + 1];
   int s = t;
-  if (b - a <= kRankMax) {
+  const SceneWin w = win[sid[i]];
+  if (k != w.cell_base + w.ncell && b - a <= kRankMax) {
     int rank = 0;
     const int* ids = kCellOrder ? tid : tmp;
     int u = a;
@@ -609,20 +618,28 @@ void sort_in(const SortBufs& sb, const GridDev& g, const SortIn& in, int n, floa
   const int nb = (n + kBlock - 1) / kBlock;
   const int nbounds = (n + kBoundsSpan - 1) / kBoundsSpan;
   k_scene_bounds<kCellOrder><<<nbounds, kBoundsBlock, 0, st>>>(in, n, g, sb.scene_off, sb.nscenes, sb.sid,
-                                                               sb.scene_minmax, sb.bpart);
+                                                               sb.scene_minmax, sb.bpart, sb.oog_flag);
   if (sb.nscenes > 1)
     k_scene_reduce<<<sb.nscenes, kReduceBlock, 0, st>>>(sb.scene_off, sb.nscenes, sb.bpart, sb.scene_minmax);
   k_scene_windows<<<1, 1024, 0, st>>>(sb.scene_minmax, sb.nscenes, sb.win, sb.total_cells, sb.bpart, nbounds,
                                       counters);
-  k_bin<kCellOrder><<<nb, kBlock, 0, st>>>(in, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt);
-  k_scan_dev_reduce<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.total_cells, sb.scan_part);
-  k_scan_dev_down<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.cell_start, sb.total_cells, sb.scan_part);
+  // out-of-grid ranks by original index (only when the bounds pass found any;
+  // the last scene offset is n)
+  const unsigned long long* any_oog = counters + C_OOG;
+  const int* n_dev = sb.scene_off + sb.nscenes;
+  k_scan_dev_reduce<<<kScanDevBlocks, 1024, 0, st>>>(sb.oog_flag, n_dev, n, sb.scan_part, any_oog);
+  k_scan_dev_down<<<kScanDevBlocks, 1024, 0, st>>>(sb.oog_flag, sb.oog_pre, n_dev, n, sb.scan_part, any_oog);
+  k_bin<kCellOrder><<<nb, kBlock, 0, st>>>(in, n, g, sb.sid, sb.win, sb.key, sb.slot, sb.cell_cnt, sb.scene_off,
+                                           sb.nscenes, sb.oog_pre, counters);
+  k_scan_dev_reduce<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.total_cells, INT_MAX, sb.scan_part, nullptr);
+  k_scan_dev_down<<<kScanDevBlocks, 1024, 0, st>>>(sb.cell_cnt, sb.cell_start, sb.total_cells, INT_MAX,
+                                                   sb.scan_part, nullptr);
   k_scatter<kCellOrder><<<nb, kBlock, 0, st>>>(sb.key, sb.slot, sb.cell_start, sb.tmp, in.id, sb.tid, n);
   PhysConst pc{};
   if (integrate) pc = *integrate;
   k_reorder<kCellOrder><<<nb, kBlock, 0, st>>>(sb.key, sb.cell_start, sb.tmp, sb.tid, sb.sid, in, n, sb.perm,
                                                sb.ssid, xn, x0, vs, pc, integrate ? 1 : 0, sb.xc, sb.xc_stride,
-                                               sb.oog_rcnt);
+                                               sb.oog_rcnt, sb.win);
 }
 
 void launch_sort(const SortBufs& sb, const GridDev& g, const SortIn& in, int n, float4* xn, float4* x0, float4* vs,

@@ -4,8 +4,9 @@ next sort reads it nearly in order and orders each cell by original index, so
 the steps of one call must give the same bits as the same steps made one call
 at a time, each of which reads and writes the caller's order.
 
-Cells of more than 1024 particles keep arrival order in both forms, so every
-case here keeps each scene's out-of-grid bucket below that.
+A scene's out-of-grid bucket is in original-index order at any size (the
+"oog_many" case holds thousands); in-grid cells of more than 1024 particles
+keep arrival order, and no case here has one.
 
 Every call goes through the C ABI."""
 import numpy as np
@@ -31,6 +32,8 @@ def cell_order_scene(scene, name):
         return scene.jittered_lattice((80, 64, 64), (0.1, 0.1, 0.1), 0.005, 0.1, seed=31)
     if name == "blob":
         return scene.gaussian_blobs(1, 20000, 0.02, seed=23)
+    if name == "oog_many":  # about 9,000 of 11,520 particles below the grid's lower x face
+        return scene.jittered_lattice((20, 24, 24), (-0.085, 0.45, 0.45), 0.005, 0.1, seed=13)
     # out-of-grid particles near the grid's lower x face, next to in-grid ones
     # (5 layers of 12 x 12 below the grid)
     return scene.jittered_lattice((20, 12, 12), (-0.047, 0.45, 0.45), 0.005, 0.1, seed=12)
@@ -59,16 +62,20 @@ def assert_same_bits(a, b):
         np.testing.assert_array_equal(u.view(np.uint32), w.view(np.uint32), err_msg=name)
 
 
-@pytest.mark.parametrize("scene_name", ["lattice", "staged", "blob", "oog"])
+@pytest.mark.parametrize("scene_name", ["lattice", "staged", "blob", "oog", "oog_many"])
 def test_steps_of_one_call_equal_single_steps(fluid, scene, scene_name):
     say(f"6 steps in one call vs 6 calls of one step, {scene_name}")
     x = cell_order_scene(scene, scene_name)
     v = velocities(scene, len(x), 17)
     one, st = run(fluid, x, v, 6, 1)
     many, _ = run(fluid, x, v, 1, 6)
-    assert st["out_of_grid"] < 1024
     if scene_name == "oog":
-        assert st["out_of_grid"] > 0
+        assert 0 < st["out_of_grid"] < 1024
+    if scene_name == "oog_many":
+        say(f"out of the grid: {st['out_of_grid']} of {len(x)}")
+        assert st["out_of_grid"] > 4096
+        again, _ = run(fluid, x, v, 6, 1)  # the same call again: the same bits
+        assert_same_bits(one, again)
     if scene_name == "staged":
         assert st["staged_waves"] > 0
     assert_same_bits(one, many)
