@@ -76,16 +76,25 @@ def roofline_of(bytes_, flops, seconds):
             "frac": round(hbm, 4), "hbm_frac": round(hbm, 4), "valu_frac": round(valu, 4)}
 
 
-def profiled_traffic(kernel):
-    """HBM bytes per dispatch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (copy-calibrated), or None."""
+def profiled_traffic(kernel, block=None, persistent=False):
+    """(HBM bytes per dispatch, profiled kernel name) of `kernel` from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload
+    (copy-calibrated): the instance with the given workgroup size and launch
+    form when profiled, else the kernel's only profiled instance; (None, None)
+    when absent."""
     try:
         for line in TRAFFIC_PROFILE.read_text().splitlines():
             if line.startswith("traffic_json: "):
-                return json.loads(line[len("traffic_json: "):]).get(kernel, {}).get("bytes")
+                table = json.loads(line[len("traffic_json: "):])
+                names = [kernel + ("_p" if persistent else "") + f"<{block}>"] if block else []
+                names += [k for k in table if k == kernel or k.startswith(kernel + "<")
+                          or k.startswith(kernel + "_p<")]
+                for name in names:
+                    if name in table:
+                        return table[name].get("bytes"), name
     except OSError:
-        return None
-    return None
+        pass
+    return None, None
 
 
 def achievable_peaks(torch, pkg_dir, reps=10):
@@ -182,6 +191,7 @@ def cpu_baseline(x, steps, threads, runs=3):
 
 
 TUNE = True  # autotune the launch settings of every measured context (--no-autotune: the defaults)
+FIXED_TUNING = None  # a launch setting given with --tuning, used instead
 
 
 def tune_summary(r):
@@ -198,7 +208,10 @@ def gpu_rate(torch, fluid, x_np, steps, warmup, runs=3, params=None):
     ctx = fluid.Context(dev, **(params or {}))
     ctx.reserve(len(x_np))
     ctx.tune_report = None
-    if TUNE:
+    if FIXED_TUNING:
+        ctx.set_tuning(**FIXED_TUNING)
+        ctx.tune_report = {**FIXED_TUNING, "fixed": True}
+    elif TUNE:
         x0 = torch.from_numpy(x_np).to(dev)
         ctx.tune_report = tune_summary(ctx.autotune(x0, torch.zeros_like(x0), steps=min(max(steps, 2), 5)))
         del x0
@@ -603,11 +616,16 @@ def main():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable and batched-inverse workloads")
     ap.add_argument("--no-peaks", action="store_true", help="skip the achievable copy / FMA rate probes")
     ap.add_argument("--no-autotune", action="store_true", help="measure with the default launch settings")
+    ap.add_argument("--tuning", default="", metavar="BLOCK,STAGE,PERSISTENT",
+                    help="measure with this launch setting instead of tuning (e.g. 128,1,0 for profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
-    global TUNE
-    TUNE = not args.no_autotune
+    global TUNE, FIXED_TUNING
+    TUNE = not args.no_autotune and not args.tuning
+    if args.tuning:
+        FIXED_TUNING = dict(zip(("pass_block", "stage_min_particles", "persistent"),
+                                (int(t) for t in args.tuning.split(","))))
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not part of this "
               "build", file=sys.stderr)
@@ -653,10 +671,14 @@ def main():
     kname = {"density_lambda": "k_density_lambda", "displacement": "k_displacement", "neighbors": "k_neighbors"}[dom]
     kb, kf = kernel_counts(kname, n, pairs)
     roofline = roofline_of(kb, kf, t_dom)
+    tune = ctx.tuning()
+    traffic, traffic_kernel = profiled_traffic(kname, tune["pass_block"] if kname != "k_neighbors" else None,
+                                               bool(tune["persistent"]) and kname != "k_neighbors")
     roofline.update({
         "kernel": kname,
-        "traffic": profiled_traffic(kname),
+        "traffic": traffic,
         "traffic_source": TRAFFIC_PROFILE.name,
+        "traffic_kernel": traffic_kernel,
         "algorithmic_bytes": int(kb), "algorithmic_flops": int(kf),
         "valu_frac_nonpacked": round(kf / t_dom / (FP32_NONPACKED_TFLOPS * 1e12), 4),
         "avg_launch_us": round(pass_ms[dom][0] * 1e3, 2), "launches": pass_ms[dom][1],

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 7
+#define PBF_ABI_VERSION 8
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -348,13 +348,17 @@ typedef struct pbf_tuning {
   int stage_min_particles; /* as pbf_params.stage_min_particles: searches over at
                               least this many particles stage their neighbor
                               lists in LDS; < 0 never */
+  int persistent;          /* 1: the two passes launch as many workgroups as stay
+                              resident on the device, each wave walking
+                              64-particle tiles of its XCD's contiguous share in
+                              a fixed order; 0 (default): one wave per tile */
 } pbf_tuning;
 
 int pbf_set_tuning(pbf_ctx* ctx, const pbf_tuning* t);
 int pbf_get_tuning(const pbf_ctx* ctx, pbf_tuning* out);
 
 /* Autotuner. Times every setting of pbf_tuning (4 workgroup sizes x staged
- * lists on / off) on the caller's workload: a copy of (d_x, d_v), `steps` plain
+ * lists on / off x persistent or not) on the caller's workload: a copy of (d_x, d_v), `steps` plain
  * steps after a warm-up call, the better of two runs, on `stream`; keeps the
  * fastest (a setting replaces the one before it only when more than 1%
  * faster, the default first) and applies it to the context. The workload is a

@@ -173,6 +173,7 @@ struct pbf_ctx {
   int last_n = -1;  // particle count of the last neighbor search (-1: none)
 
   int pass_block = kBlock;  // threads per workgroup of the PBD passes (pbf_tuning)
+  bool pass_persistent = false;  // persistent launches of the PBD passes (pbf_tuning)
 
   void* noalloc_scratch = nullptr;  // this device's copy of `scratch` (null: lookup failed, noalloc_err says why)
   std::string noalloc_err;
@@ -891,17 +892,17 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
     {
       PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st);
       launch_pass_density_lambda(nv, src, c->pc, last ? d_rho : nullptr, c->perm, st, aux ? aux[it] : nullptr,
-                                 c->pass_block);
+                                 c->pass_block, c->pass_persistent);
     }
     {
       PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
       if (last && to) {
         launch_pass_displacement(nv, src, dst, c->pc, true, c->x0, c->vs, c->perm, nullptr, nullptr, st, c->vv,
-                                 c->pid, c->pass_block);
+                                 c->pid, c->pass_block, c->pass_persistent);
         *to = CellState{dst, c->vv, c->pid};
       } else {
         launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st,
-                                 (last && visc) ? c->vv : nullptr, nullptr, c->pass_block);
+                                 (last && visc) ? c->vv : nullptr, nullptr, c->pass_block, c->pass_persistent);
       }
     }
   }
@@ -1832,9 +1833,12 @@ int pbf_set_tuning(pbf_ctx* c, const pbf_tuning* t) {
   if (!c || !t) return fail(c, PBF_ERR_INVALID, "null argument");
   if (t->pass_block != 64 && t->pass_block != 128 && t->pass_block != 256 && t->pass_block != 512)
     return fail(c, PBF_ERR_INVALID, "pass_block must be 64, 128, 256 or 512");
-  if (t->pass_block != c->pass_block || t->stage_min_particles != c->p.stage_min_particles) {
+  if (t->persistent != 0 && t->persistent != 1) return fail(c, PBF_ERR_INVALID, "persistent must be 0 or 1");
+  if (t->pass_block != c->pass_block || t->stage_min_particles != c->p.stage_min_particles ||
+      (t->persistent != 0) != c->pass_persistent) {
     c->pass_block = t->pass_block;
     c->p.stage_min_particles = t->stage_min_particles;
+    c->pass_persistent = t->persistent != 0;
     c->gen++;  // a captured step holds the launch shapes and the staging choice
   }
   return PBF_OK;
@@ -1844,6 +1848,7 @@ int pbf_get_tuning(const pbf_ctx* c, pbf_tuning* out) {
   if (!c || !out) return PBF_ERR_INVALID;
   out->pass_block = c->pass_block;
   out->stage_min_particles = c->p.stage_min_particles;
+  out->persistent = c->pass_persistent ? 1 : 0;
   return PBF_OK;
 }
 

@@ -13,6 +13,8 @@
 // every path.
 #include <climits>
 
+#include <algorithm>
+
 #include "pbf_nbr.h"
 
 namespace pbf {
@@ -424,17 +426,15 @@ void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* co
 // (DensityPBD.cu:46-90) fused: one traversal gives rho_i, sum g and sum g.g;
 // lambda_i goes to xs[s].w for pass B. aux (taped steps only) receives
 // (sum_j t^2/r d_ij, 1/(S + eps)) for the adjoint.
-template <int B>
-__global__ __launch_bounds__(B) void k_density_lambda(NbrView nv, float4* __restrict__ xs, PhysConst pc,
-                                                           float* __restrict__ rho_out,
-                                                           const int* __restrict__ perm,
-                                                           float4* __restrict__ aux) {
-  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+// One particle s of pass A (stage: the wave's LDS region).
+__device__ __forceinline__ void density_lambda_at(const NbrView& nv, int s, float4* __restrict__ xs,
+                                                  const PhysConst& pc, float* __restrict__ rho_out,
+                                                  const int* __restrict__ perm, float4* __restrict__ aux,
+                                                  float4* stage) {
   if (s >= nv.n) return;
-  __shared__ float4 stage_buf[B / 64][kStageRecords];
   const float4 xi = xs[s];
   float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
-  for_each_neighbor_staged<false>(nv, s, xs, stage_buf[threadIdx.x >> 6], [&](int, float4 xj) {
+  for_each_neighbor_staged<false>(nv, s, xs, stage, [&](int, float4 xj) {
 #pragma clang fp contract(off)
     const PairGeom p = pair_geom(xi, xj, pc.inv_h);
     const float t2 = p.t * p.t;
@@ -457,18 +457,76 @@ __global__ __launch_bounds__(B) void k_density_lambda(NbrView nv, float4* __rest
 }
 
 template <int B>
+__global__ __launch_bounds__(B) void k_density_lambda(NbrView nv, float4* __restrict__ xs, PhysConst pc,
+                                                      float* __restrict__ rho_out, const int* __restrict__ perm,
+                                                      float4* __restrict__ aux) {
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
+  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  density_lambda_at(nv, s, xs, pc, rho_out, perm, aux, stage_buf[threadIdx.x >> 6]);
+}
+
+// Persistent form: as many blocks as stay resident; each wave walks the
+// 64-particle tiles of its XCD's contiguous eighth of the particles with a
+// stride of that XCD's resident waves (a static schedule: no atomics). The
+// dispatcher sends block b to XCD b % 8; a grid of fewer than 8 blocks splits
+// the tiles into as many groups as it has blocks, so every tile is walked.
+__device__ __forceinline__ void persistent_tiles(int ntiles, int& first, int& stride, int& end) {
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int groups = nb < 8 ? nb : 8, x = b % groups;
+  const int blocks_x = nb / groups + (x < nb % groups ? 1 : 0);  // blocks of group x
+  const int wpb = blockDim.x >> 6;
+  const int t0 = (int)((long long)ntiles * x / groups), t1 = (int)((long long)ntiles * (x + 1) / groups);
+  first = t0 + (b / groups) * wpb + (threadIdx.x >> 6);
+  stride = blocks_x * wpb;
+  end = t1;
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_density_lambda_p(NbrView nv, float4* __restrict__ xs, PhysConst pc,
+                                                        float* __restrict__ rho_out, const int* __restrict__ perm,
+                                                        float4* __restrict__ aux) {
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
+  int t, stride, end;
+  persistent_tiles((nv.n + 63) >> 6, t, stride, end);
+  for (; t < end; t += stride) {
+    asm volatile("" ::: "memory");  // the last tile's stage reads stay before this tile's stage writes
+    density_lambda_at(nv, (t << 6) + (threadIdx.x & 63), xs, pc, rho_out, perm, aux, stage_buf[threadIdx.x >> 6]);
+  }
+}
+
+// Blocks of a persistent launch: the blocks of `kernel` (B threads) that stay
+// resident on all CUs of the current device, at most one per B particles.
+template <class K>
+int resident_blocks(K kernel, int B) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+      hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, B, 0) != hipSuccess)
+    cus = per_cu = 1;
+  return std::max(cus * per_cu, 1);
+}
+
+int persistent_grid(int resident, int B, int n) { return std::max(std::min(resident, (n + B - 1) / B), 1); }
+
+template <int B>
 void density_lambda_as(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                       hipStream_t st, float4* aux) {
-  k_density_lambda<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
+                       hipStream_t st, float4* aux, bool persistent) {
+  if (persistent) {
+    static const int resident = resident_blocks(k_density_lambda_p<B>, B);  // one device type per process
+    const int nb = persistent_grid(resident, B, nv.n);
+    k_density_lambda_p<B><<<nb, B, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
+  } else {
+    k_density_lambda<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
+  }
 }
 
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                                hipStream_t st, float4* aux, int block) {
+                                hipStream_t st, float4* aux, int block, bool persistent) {
   switch (block) {
-    case 64: density_lambda_as<64>(nv, xs, pc, rho_out, perm, st, aux); break;
-    case 128: density_lambda_as<128>(nv, xs, pc, rho_out, perm, st, aux); break;
-    case 512: density_lambda_as<512>(nv, xs, pc, rho_out, perm, st, aux); break;
-    default: density_lambda_as<256>(nv, xs, pc, rho_out, perm, st, aux); break;
+    case 64: density_lambda_as<64>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
+    case 128: density_lambda_as<128>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
+    case 512: density_lambda_as<512>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
+    default: density_lambda_as<256>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
   }
 }
 
@@ -481,15 +539,12 @@ void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& 
 // the scatter back to original order, or keeps x and v in cell order (with
 // each particle's original index in id_out) for the viscosity passes or the
 // next step of the call.
-template <int B>
-__global__ __launch_bounds__(B) void k_displacement(NbrView nv, const float4* __restrict__ xs,
-                                                         float4* __restrict__ xd, PhysConst pc, int last,
-                                                         const float4* __restrict__ x0,
-                                                         const float4* __restrict__ vs,
-                                                         const int* __restrict__ perm, float* __restrict__ x_out,
-                                                         float* __restrict__ v_out, float4* __restrict__ v_sorted,
-                                                         int* __restrict__ id_out) {
-  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void displacement_at(const NbrView& nv, int s, const float4* __restrict__ xs,
+                                                float4* __restrict__ xd, const PhysConst& pc, int last,
+                                                const float4* __restrict__ x0, const float4* __restrict__ vs,
+                                                const int* __restrict__ perm, float* __restrict__ x_out,
+                                                float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                int* __restrict__ id_out, float4* stage) {
   if (s == 0 && (!last || v_sorted)) xd[nv.pad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
   if (s >= nv.n) return;
   const float4 xi = xs[s];
@@ -503,10 +558,8 @@ __global__ __launch_bounds__(B) void k_displacement(NbrView nv, const float4* __
     cy = fmaf(c, p.dy, cy);
     cz = fmaf(c, p.dz, cz);
   };
-  __shared__ float4 stage_buf[B / 64][kStageRecords];
   float ax = 0.f, ay = 0.f, az = 0.f;
-  for_each_neighbor_staged<true>(nv, s, xs, stage_buf[threadIdx.x >> 6],
-                                 [&](int, float4 xj) { pair(xj, ax, ay, az); });
+  for_each_neighbor_staged<true>(nv, s, xs, stage, [&](int, float4 xj) { pair(xj, ax, ay, az); });
   float fa = 2.0f;
   float bx = 0.f, by = 0.f, bz = 0.f;
   if (nv.counters[C_OOG] != 0) {  // some scene has out-of-grid particles
@@ -572,23 +625,64 @@ __global__ __launch_bounds__(B) void k_displacement(NbrView nv, const float4* __
   v_out[3 * i + 2] = vv.z + (nz - xo.z) / pc.dt;
 }
 
+
+template <int B>
+__global__ __launch_bounds__(B) void k_displacement(NbrView nv, const float4* __restrict__ xs,
+                                                    float4* __restrict__ xd, PhysConst pc, int last,
+                                                    const float4* __restrict__ x0, const float4* __restrict__ vs,
+                                                    const int* __restrict__ perm, float* __restrict__ x_out,
+                                                    float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                    int* __restrict__ id_out) {
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
+  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  displacement_at(nv, s, xs, xd, pc, last, x0, vs, perm, x_out, v_out, v_sorted, id_out,
+                  stage_buf[threadIdx.x >> 6]);
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_displacement_p(NbrView nv, const float4* __restrict__ xs,
+                                                      float4* __restrict__ xd, PhysConst pc, int last,
+                                                      const float4* __restrict__ x0, const float4* __restrict__ vs,
+                                                      const int* __restrict__ perm, float* __restrict__ x_out,
+                                                      float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                      int* __restrict__ id_out) {
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
+  int t, stride, end;
+  persistent_tiles((nv.n + 63) >> 6, t, stride, end);
+  for (; t < end; t += stride) {
+    asm volatile("" ::: "memory");  // the last tile's stage reads stay before this tile's stage writes
+    displacement_at(nv, (t << 6) + (threadIdx.x & 63), xs, xd, pc, last, x0, vs, perm, x_out, v_out, v_sorted,
+                    id_out, stage_buf[threadIdx.x >> 6]);
+  }
+}
+
 template <int B>
 void displacement_as(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                      const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out, hipStream_t st,
-                     float4* v_sorted, int* id_out) {
-  k_displacement<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm, x_out, v_out,
-                                                      v_sorted, id_out);
+                     float4* v_sorted, int* id_out, bool persistent) {
+  if (persistent) {
+    static const int resident = resident_blocks(k_displacement_p<B>, B);  // one device type per process
+    const int nb = persistent_grid(resident, B, nv.n);
+    k_displacement_p<B><<<nb, B, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm, x_out, v_out, v_sorted,
+                                          id_out);
+  } else {
+    k_displacement<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm, x_out, v_out,
+                                                        v_sorted, id_out);
+  }
 }
 
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                               const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out,
-                              hipStream_t st, float4* v_sorted, int* id_out, int block) {
+                              hipStream_t st, float4* v_sorted, int* id_out, int block, bool persistent) {
+#define PBF_DISP(b) \
+  displacement_as<b>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out, persistent)
   switch (block) {
-    case 64: displacement_as<64>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
-    case 128: displacement_as<128>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
-    case 512: displacement_as<512>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
-    default: displacement_as<256>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out); break;
+    case 64: PBF_DISP(64); break;
+    case 128: PBF_DISP(128); break;
+    case 512: PBF_DISP(512); break;
+    default: PBF_DISP(256); break;
   }
+#undef PBF_DISP
 }
 
 // ---------------------------------------------------------------------------

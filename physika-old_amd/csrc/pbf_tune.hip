@@ -50,7 +50,7 @@ std::string tune_key(const pbf_params& p, int n, int nscenes, int device) {
   return buf;
 }
 
-// One entry per line: "key": {"pass_block": B, "stage_min_particles": S, "ms_per_step": T}
+// One entry per line: "key": {"pass_block": B, "stage_min_particles": S, "persistent": P, "ms_per_step": T}
 struct Entry {
   std::string key, body;
 };
@@ -140,10 +140,11 @@ extern "C" int pbf_autotune(pbf_ctx* c, const float* d_x, const float* d_v, cons
   if (cache_path) {
     cache = read_cache(cache_path);
     for (const Entry& e : cache) {
-      double b, sm, ms = 0;
+      double b, sm, ps = 0, ms = 0;
       if (e.key != key || !field(e.body, "pass_block", &b) || !field(e.body, "stage_min_particles", &sm)) continue;
+      (void)field(e.body, "persistent", &ps);
       (void)field(e.body, "ms_per_step", &ms);
-      const pbf_tuning t{(int)b, (int)sm};
+      const pbf_tuning t{(int)b, (int)sm, (int)ps};
       if (pbf_set_tuning(c, &t)) break;  // not a valid setting: time again
       out->best = t;
       out->best_ms = ms;
@@ -160,9 +161,10 @@ extern "C" int pbf_autotune(pbf_ctx* c, const float* d_x, const float* d_v, cons
   const int* off = nscenes > 1 || d_scene_offsets ? d_scene_offsets : nullptr;
 
   // the default first, then the rest; a setting must beat the best so far by 1%
-  std::vector<pbf_tuning> cand{{256, 0}};
-  for (int b : kBlocks)
-    for (int stage : {kStageAlways, kStageNever}) cand.push_back({b, stage});
+  std::vector<pbf_tuning> cand{{256, 0, 0}};
+  for (int persistent : {0, 1})
+    for (int b : kBlocks)
+      for (int stage : {kStageAlways, kStageNever}) cand.push_back({b, stage, persistent});
   pbf_tuning best = cand[0];
   double best_ms = 0;
   for (size_t k = 0; k < cand.size(); ++k) {
@@ -187,9 +189,10 @@ extern "C" int pbf_autotune(pbf_ctx* c, const float* d_x, const float* d_v, cons
   int s = pbf_set_tuning(c, &best);
   if (s) return s;
   if (cache_path) {
-    char body[160];
-    std::snprintf(body, sizeof body, "{\"pass_block\": %d, \"stage_min_particles\": %d, \"ms_per_step\": %.5f}",
-                  best.pass_block, best.stage_min_particles, best_ms);
+    char body[192];
+    std::snprintf(body, sizeof body,
+                  "{\"pass_block\": %d, \"stage_min_particles\": %d, \"persistent\": %d, \"ms_per_step\": %.5f}",
+                  best.pass_block, best.stage_min_particles, best.persistent, best_ms);
     bool found = false;
     for (Entry& e : cache)
       if (e.key == key) {

@@ -153,10 +153,11 @@ class Context:
     def tuning(self) -> dict:
         t = _capi.pbf_tuning()
         _capi.check(self._lib.pbf_get_tuning(self._ctx, C.byref(t)), self._ctx)
-        return {"pass_block": t.pass_block, "stage_min_particles": t.stage_min_particles}
+        return {"pass_block": t.pass_block, "stage_min_particles": t.stage_min_particles,
+                "persistent": t.persistent}
 
-    def set_tuning(self, pass_block: int = 256, stage_min_particles: int = 0) -> None:
-        t = _capi.pbf_tuning(int(pass_block), int(stage_min_particles))
+    def set_tuning(self, pass_block: int = 256, stage_min_particles: int = 0, persistent: int = 0) -> None:
+        t = _capi.pbf_tuning(int(pass_block), int(stage_min_particles), int(persistent))
         _capi.check(self._lib.pbf_set_tuning(self._ctx, C.byref(t)), self._ctx)
 
     def autotune(self, x: torch.Tensor, v: torch.Tensor, scene_offsets: Optional[torch.Tensor] = None,
@@ -176,7 +177,7 @@ class Context:
                                                int(steps), None if cache is None else str(cache).encode(),
                                                C.byref(r), _stream(x.device)), self._ctx)
         return {"pass_block": r.best.pass_block, "stage_min_particles": r.best.stage_min_particles,
-                "best_ms": r.best_ms, "default_ms": r.default_ms, "candidates": r.candidates,
+                "persistent": r.best.persistent, "best_ms": r.best_ms, "default_ms": r.default_ms, "candidates": r.candidates,
                 "from_cache": bool(r.from_cache), "key": r.key.decode()}
 
     def profile(self) -> dict:

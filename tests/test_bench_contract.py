@@ -65,10 +65,11 @@ def test_baseline_threads_follow_the_environment(monkeypatch):
 
 
 def test_committed_traffic_summary_is_read():
-    for k in ("k_density_lambda", "k_displacement"):
-        t = bench.profiled_traffic(k)
-        assert isinstance(t, int) and 1e8 < t < 1e9, (k, t)
-    assert bench.profiled_traffic("no_such_kernel") is None
+    for k in ("k_density_lambda", "k_displacement", "k_neighbors"):
+        t, name = bench.profiled_traffic(k)
+        assert isinstance(t, int) and 1e7 < t < 1e9, (k, t)
+        assert name == k or name.startswith(k + "<") or name.startswith(k + "_p<")
+    assert bench.profiled_traffic("no_such_kernel") == (None, None)
 
 
 def test_rate_probe_library_exports():

@@ -1,7 +1,10 @@
 """ParticleFluid with emitters on the GPU (ParticleFluid.cpp:33-124): the
 particle count grows frame by frame, and every frame equals a direct
 PositionBasedFluidModel step of the concatenated input — the emitters'
-segments in the order added, each one's carried particles then its new ones."""
+segments in the order added, each one's carried particles then its new ones.
+Every tenth frame that direct step is also held against the oracle's step
+(with ImplicitViscosity, as the model runs it) on the same concatenated
+input, to the one-step parity bars."""
 from importlib import import_module
 
 import numpy as np
@@ -16,7 +19,8 @@ def em(pkg):
 
 
 @pytest.mark.gpu
-def test_fluid_with_emitters_equals_direct_steps(fluid, em):
+def test_fluid_with_emitters_equals_direct_steps(fluid, em, oracle):
+    from parity_util import assert_step_parity, oracle_both, say
     dev = torch.device("cuda:0")
     pf = fluid.ParticleFluid()
     direct = fluid.PositionBasedFluidModel()
@@ -46,6 +50,7 @@ def test_fluid_with_emitters_equals_direct_steps(fluid, em):
         counts.append(stepped)
         if stepped == 0:
             continue
+        xi, vi = x_in.cpu().numpy(), v_in.cpu().numpy()  # the model steps x_in and v_in in place
         direct.m_position.setValue(x_in.contiguous())
         direct.m_velocity.setValue(v_in.contiguous())
         direct.m_forceDensity.setValue(torch.zeros_like(x_in))
@@ -54,6 +59,11 @@ def test_fluid_with_emitters_equals_direct_steps(fluid, em):
         assert torch.equal(pf.currentPosition.getValue(), direct.m_position.getValue()), frame
         assert torch.equal(pf.currentVelocity.getValue(), direct.m_velocity.getValue()), frame
         assert torch.equal(pf.model.m_density.getValue(), direct.m_density.getValue()), frame
+        if frame % 10 == 9:
+            say(f"emitter frame {frame}: {stepped} particles vs oracle")
+            got = tuple(a.cpu().numpy() for a in (direct.m_position.getValue(), direct.m_velocity.getValue(),
+                                                  direct.m_density.getValue()))
+            assert_step_parity(got, *oracle_both(oracle, xi, vi, 1, flags=1))
         assert e1.currentPosition.getElementCount() == n1 and e2.currentPosition.getElementCount() == n2
     assert counts[-1] > counts[len(counts) // 2] > 0
     assert torch.isfinite(pf.currentPosition.getValue()).all()

@@ -1,8 +1,8 @@
 """Launch settings (pbf_set_tuning) and the autotuner (pbf_autotune).
 
 Bars: every setting (PBD-pass workgroups of 64, 128, 256 and 512 threads,
-staged lists always or never) gives bit-identical positions, velocities and
-density to the default setting, on one scene, on a staged lattice and on a
+staged lists always or never, ordinary or persistent launches) gives
+bit-identical positions, velocities and density to the default setting, on one scene, on a staged lattice and on a
 batch; the autotuner leaves the caller's state unchanged, applies the
 setting it reports, stores it in its JSON cache and reads it back from there.
 Every call goes through the C ABI."""
@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 from parity_util import dev, say  # noqa: E402
 
-SETTINGS = [(b, s) for b in (64, 128, 256, 512) for s in (1, -1)]
+SETTINGS = [(b, s, p) for p in (0, 1) for b in (64, 128, 256, 512) for s in (1, -1)]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -43,11 +43,12 @@ def run(ctx, x0, v0, nsteps, off=None):
 def assert_settings_identical(fluid, x0, v0, nsteps, off=None, **params):
     ctx = fluid.Context("cuda:0", **params)
     want = run(ctx, x0, v0, nsteps, off)
-    for b, s in SETTINGS:
-        ctx.set_tuning(b, s)
+    for b, s, p in SETTINGS:
+        ctx.set_tuning(b, s, p)
         got = run(ctx, x0, v0, nsteps, off)
         for name, g, w in zip(("x", "v", "rho"), got, want):
-            np.testing.assert_array_equal(g, w, err_msg=f"pass_block {b}, stage_min_particles {s}: {name}")
+            np.testing.assert_array_equal(g, w, err_msg=f"pass_block {b}, stage_min_particles {s}, persistent {p}: "
+                                                        f"{name}")
     return ctx
 
 
@@ -61,7 +62,7 @@ def test_settings_identical_staged_lattice_with_modules(fluid, scene):
     say("launch settings, 327,680-particle lattice (staged lists), 2 steps, viscosity and box")
     x0 = scene.jittered_lattice((80, 64, 64), (0.2, 0.3, 0.2), 0.005, 0.1, seed=2)
     ctx = assert_settings_identical(fluid, x0, np.zeros_like(x0), 2, flags=3)
-    ctx.set_tuning(256, 1)
+    ctx.set_tuning(256, 1, 1)
     run(ctx, x0, np.zeros_like(x0), 1)
     assert ctx.stats()["staged_waves"] > 0
 
@@ -81,9 +82,10 @@ def test_autotune_and_cache(fluid, scene, tmp_path):
     r = ctx.autotune(x, v, steps=3, cache=cache)
     say(f"autotune, 32,768 particles: {r}")
     assert r["candidates"] == 1 + len(SETTINGS) and not r["from_cache"]
-    assert (r["pass_block"], r["stage_min_particles"]) in SETTINGS + [(256, 0)]
+    assert (r["pass_block"], r["stage_min_particles"], r["persistent"]) in SETTINGS + [(256, 0, 0)]
     assert 0 < r["best_ms"] <= r["default_ms"]
-    assert ctx.tuning() == {"pass_block": r["pass_block"], "stage_min_particles": r["stage_min_particles"]}
+    assert ctx.tuning() == {"pass_block": r["pass_block"], "stage_min_particles": r["stage_min_particles"],
+                            "persistent": r["persistent"]}
     np.testing.assert_array_equal(bits(x), x0.view(np.uint32))  # the caller's state is not touched
     np.testing.assert_array_equal(bits(v), np.zeros_like(x0).view(np.uint32))
     stored = json.loads(cache.read_text())
@@ -104,4 +106,6 @@ def test_invalid_tuning_rejected(fluid, capi):
     ctx = fluid.Context("cuda:0")
     with pytest.raises(capi.PBFError):
         ctx.set_tuning(96, 0)
-    assert ctx.tuning() == {"pass_block": 256, "stage_min_particles": 0}
+    with pytest.raises(capi.PBFError):
+        ctx.set_tuning(256, 0, 2)
+    assert ctx.tuning() == {"pass_block": 256, "stage_min_particles": 0, "persistent": 0}
