@@ -609,7 +609,8 @@ def main():
     ap.add_argument("--runs", type=int, default=3, help="timed runs from the same initial state (median)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-configs", action="store_true", help="skip the 32k (config 1) and blob (config 2b) lines")
-    ap.add_argument("--workload", default="lattice_1m", choices=["lattice_1m", "blobs_1m"])
+    ap.add_argument("--workload", default="lattice_1m", choices=["lattice_1m", "blobs_1m", "dam_break_32k"],
+                    help="the main line's workload (the default is the benchmark's; the others are for profiles)")
     ap.add_argument("--batched-steps", type=int, default=100, help="steps of the 256-scene batched comparison")
     ap.add_argument("--serial-steps", type=int, default=10, help="steps of its serial per-scene loop")
     ap.add_argument("--no-batched", action="store_true")

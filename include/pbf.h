@@ -318,9 +318,12 @@ int pbf_viscosity(pbf_ctx* ctx, const float* d_x, float* d_v, const int* d_index
  * velocity reflected and damped by the normal friction. x, v in place. */
 int pbf_box_constrain(pbf_ctx* ctx, float* d_x, float* d_v, int n, void* stream);
 
-/* Per-pass GPU timing (HIP events recorded on the call's stream around each
- * pass of pbf_step; the reference's GTimer, Core/Utility/GTimer.cpp:21-27, has
- * no fluid hook). Not part of the reference interface. */
+/* Per-pass GPU timing of pbf_step (the reference's GTimer,
+ * Core/Utility/GTimer.cpp:21-27, has no fluid hook). Not part of the reference
+ * interface. A one-kernel pass (neighbors, density + lambda, displacement) is
+ * timed by start/stop events attached to its dispatch (hipExtLaunchKernel: the
+ * kernel's own duration); the sort, several kernels, by events recorded on the
+ * call's stream before and after it. */
 typedef enum pbf_pass {
   PBF_PASS_SORT = 0,            /* clear + bin + scan + scatter + reorder/integrate */
   PBF_PASS_NEIGHBORS = 1,       /* neighbor masks (one kernel) */

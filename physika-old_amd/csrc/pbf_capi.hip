@@ -680,23 +680,37 @@ hipEvent_t take_event(pbf_ctx* c) {
 
 int drain_marks(pbf_ctx* c);
 
+// Times one pass while profiling. A pass of several kernels (the sort) is
+// bracketed by events recorded on the stream; a one-kernel pass hands its
+// events to the launch (events(): they time that dispatch itself).
 struct PassTimer {
   pbf_ctx* c;
   int pass;
   hipStream_t st;
-  hipEvent_t a = nullptr;
-  PassTimer(pbf_ctx* c_, int pass_, hipStream_t st_) : c(c_), pass(pass_), st(st_) {
-    if (c->profiling && (a = take_event(c)) != nullptr) (void)hipEventRecord(a, st);
-  }
-  ~PassTimer() {
-    if (!a) return;
-    hipEvent_t b = take_event(c);
-    if (!b) {
-      c->spare.push_back(a);
+  bool attach;
+  KernelEvents ev{nullptr, nullptr};
+  PassTimer(pbf_ctx* c_, int pass_, hipStream_t st_, bool attach_ = false)
+      : c(c_), pass(pass_), st(st_), attach(attach_) {
+    if (!c->profiling || (ev.start = take_event(c)) == nullptr) return;
+    if (attach && (ev.stop = take_event(c)) == nullptr) {
+      c->spare.push_back(ev.start);
+      ev.start = nullptr;
       return;
     }
-    (void)hipEventRecord(b, st);
-    c->marks.push_back({pass, a, b});
+    if (!attach) (void)hipEventRecord(ev.start, st);
+  }
+  const KernelEvents* events() const { return attach && ev.start ? &ev : nullptr; }
+  ~PassTimer() {
+    if (!ev.start) return;
+    if (!attach) {
+      ev.stop = take_event(c);
+      if (!ev.stop) {
+        c->spare.push_back(ev.start);
+        return;
+      }
+      (void)hipEventRecord(ev.stop, st);
+    }
+    c->marks.push_back({pass, ev.start, ev.stop});
     // bound the events in flight: sum the finished ones every 512 passes
     if (c->marks.size() >= 512) (void)drain_marks(c);
   }
@@ -727,11 +741,11 @@ int search(pbf_ctx* c, const SortIn& in, int n, const int* scene_off, int nscene
                 c->counters, st);
   }
   {
-    PassTimer t(c, PBF_PASS_NEIGHBORS, st);
+    PassTimer t(c, PBF_PASS_NEIGHBORS, st, true);
     NbrOut o = out_of(c);
     const int smin = c->p.stage_min_particles == 0 ? kStageMinParticles : c->p.stage_min_particles;
     o.stage_lists = (smin > 0 && n >= smin) ? 1 : 0;
-    launch_neighbors(view_of(c, n), o, c->counters, st);
+    launch_neighbors(view_of(c, n), o, c->counters, st, t.events());
   }
   c->stats_waves = (n + 63) / 64;
   PBF_LAUNCHED(c);
@@ -890,19 +904,20 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
     float4* src = P ? P[it] : (it == 0 ? c->x0 : bufs[(it - 1) & 1]);
     float4* dst = P ? (last ? nullptr : P[it + 1]) : bufs[it & 1];
     {
-      PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st);
+      PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st, true);
       launch_pass_density_lambda(nv, src, c->pc, last ? d_rho : nullptr, c->perm, st, aux ? aux[it] : nullptr,
-                                 c->pass_block, c->pass_persistent);
+                                 c->pass_block, c->pass_persistent, t.events());
     }
     {
-      PassTimer t(c, PBF_PASS_DISPLACEMENT, st);
+      PassTimer t(c, PBF_PASS_DISPLACEMENT, st, true);
       if (last && to) {
         launch_pass_displacement(nv, src, dst, c->pc, true, c->x0, c->vs, c->perm, nullptr, nullptr, st, c->vv,
-                                 c->pid, c->pass_block, c->pass_persistent);
+                                 c->pid, c->pass_block, c->pass_persistent, t.events());
         *to = CellState{dst, c->vv, c->pid};
       } else {
         launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st,
-                                 (last && visc) ? c->vv : nullptr, nullptr, c->pass_block, c->pass_persistent);
+                                 (last && visc) ? c->vv : nullptr, nullptr, c->pass_block, c->pass_persistent,
+                                 t.events());
       }
     }
   }

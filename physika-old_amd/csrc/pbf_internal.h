@@ -9,6 +9,12 @@ struct pbf_ctx;
 
 namespace pbf {
 
+// Start and stop events attached to one kernel dispatch (hipExtLaunchKernel:
+// the events time the kernel itself, with nothing recorded between kernels).
+struct KernelEvents {
+  hipEvent_t start, stop;
+};
+
 // Uniform grid of GridHash::setSpace (Framework/Topology/GridHash.cu:47-75):
 // cell = h, origin = scene_lo - 2 cells, n = ceil((hi - lo)/h) + 1 + 4 per
 // axis. Every scene of a batch uses this geometry, so a particle's cell
@@ -232,7 +238,7 @@ void launch_scene_reset(int* minmax, int nscenes, hipStream_t st);
 void launch_set_offsets(int* off, int n, hipStream_t st);
 // cell_start[0..*total_cells] from src to dst (cap: allocated cells).
 void launch_copy_cells(const int* src, int* dst, const int* total_cells, long long cap, hipStream_t st);
-void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st);
+void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st, const KernelEvents* ev = nullptr);
 // Sums the per-wave statistics of the last neighbor pass (nwaves waves) into
 // counters[C_PAIRS], [C_CANDIDATES], [C_LIST_SLOTS] and [C_LIST_NEED].
 void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* counters, hipStream_t st);
@@ -241,14 +247,15 @@ void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* co
 // depend on neither; each wave of a block has its own stage)
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
                                 float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr,
-                                int block = kBlock, bool persistent = false);
+                                int block = kBlock, bool persistent = false, const KernelEvents* ev = nullptr);
 // last: write x/v to original order, or with v_sorted the positions to xd and
 // the velocities to v_sorted (sorted order, for the viscosity passes or the
 // next step of the call; id_out then receives each particle's original index).
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc,
                               bool last, const float4* x0, const float4* vs, const int* perm,
                               float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr,
-                              int* id_out = nullptr, int block = kBlock, bool persistent = false);
+                              int* id_out = nullptr, int block = kBlock, bool persistent = false,
+                              const KernelEvents* ev = nullptr);
 void launch_write_back(const float4* xs, const float4* vs, const int* perm, int n, float* x_out,
                        float* v_out, hipStream_t st);
 void launch_invert(const int* perm, int* inv, int n, hipStream_t st);

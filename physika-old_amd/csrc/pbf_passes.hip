@@ -15,6 +15,8 @@
 
 #include <algorithm>
 
+#include <hip/hip_ext.h>
+
 #include "pbf_nbr.h"
 
 namespace pbf {
@@ -367,8 +369,13 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   }
 }
 
-void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st) {
-  k_neighbors<<<(nv.n + 63) / 64, 64, 0, st>>>(nv, out, counters);
+void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st,
+                      const KernelEvents* ev) {
+  if (ev)
+    hipExtLaunchKernelGGL(k_neighbors, dim3((nv.n + 63) / 64), dim3(64), 0, st, ev->start, ev->stop, 0, nv, out,
+                          counters);
+  else
+    k_neighbors<<<(nv.n + 63) / 64, 64, 0, st>>>(nv, out, counters);
 }
 
 // One block: sums and maxima over the per-wave statistics.
@@ -508,25 +515,34 @@ int resident_blocks(K kernel, int B) {
 
 int persistent_grid(int resident, int B, int n) { return std::max(std::min(resident, (n + B - 1) / B), 1); }
 
+// A launch, with its dispatch timed by ev when given.
+template <class K, class... Args>
+void launch_timed(K kernel, int grid, int block, hipStream_t st, const KernelEvents* ev, Args... args) {
+  if (ev)
+    hipExtLaunchKernelGGL(kernel, dim3(grid), dim3(block), 0, st, ev->start, ev->stop, 0, args...);
+  else
+    kernel<<<grid, block, 0, st>>>(args...);
+}
+
 template <int B>
 void density_lambda_as(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                       hipStream_t st, float4* aux, bool persistent) {
+                       hipStream_t st, float4* aux, bool persistent, const KernelEvents* ev) {
   if (persistent) {
     static const int resident = resident_blocks(k_density_lambda_p<B>, B);  // one device type per process
-    const int nb = persistent_grid(resident, B, nv.n);
-    k_density_lambda_p<B><<<nb, B, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
+    launch_timed(k_density_lambda_p<B>, persistent_grid(resident, B, nv.n), B, st, ev, nv, xs, pc, rho_out, perm,
+                 aux);
   } else {
-    k_density_lambda<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, pc, rho_out, perm, aux);
+    launch_timed(k_density_lambda<B>, (nv.n + B - 1) / B, B, st, ev, nv, xs, pc, rho_out, perm, aux);
   }
 }
 
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                                hipStream_t st, float4* aux, int block, bool persistent) {
+                                hipStream_t st, float4* aux, int block, bool persistent, const KernelEvents* ev) {
   switch (block) {
-    case 64: density_lambda_as<64>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
-    case 128: density_lambda_as<128>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
-    case 512: density_lambda_as<512>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
-    default: density_lambda_as<256>(nv, xs, pc, rho_out, perm, st, aux, persistent); break;
+    case 64: density_lambda_as<64>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
+    case 128: density_lambda_as<128>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
+    case 512: density_lambda_as<512>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
+    default: density_lambda_as<256>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
   }
 }
 
@@ -659,23 +675,24 @@ __global__ __launch_bounds__(B) void k_displacement_p(NbrView nv, const float4* 
 template <int B>
 void displacement_as(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                      const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out, hipStream_t st,
-                     float4* v_sorted, int* id_out, bool persistent) {
+                     float4* v_sorted, int* id_out, bool persistent, const KernelEvents* ev) {
+  const int l = last ? 1 : 0;
   if (persistent) {
     static const int resident = resident_blocks(k_displacement_p<B>, B);  // one device type per process
-    const int nb = persistent_grid(resident, B, nv.n);
-    k_displacement_p<B><<<nb, B, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm, x_out, v_out, v_sorted,
-                                          id_out);
+    launch_timed(k_displacement_p<B>, persistent_grid(resident, B, nv.n), B, st, ev, nv, xs, xd, pc, l, x0, vs,
+                 perm, x_out, v_out, v_sorted, id_out);
   } else {
-    k_displacement<B><<<(nv.n + B - 1) / B, B, 0, st>>>(nv, xs, xd, pc, last ? 1 : 0, x0, vs, perm, x_out, v_out,
-                                                        v_sorted, id_out);
+    launch_timed(k_displacement<B>, (nv.n + B - 1) / B, B, st, ev, nv, xs, xd, pc, l, x0, vs, perm, x_out, v_out,
+                 v_sorted, id_out);
   }
 }
 
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                               const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out,
-                              hipStream_t st, float4* v_sorted, int* id_out, int block, bool persistent) {
+                              hipStream_t st, float4* v_sorted, int* id_out, int block, bool persistent,
+                              const KernelEvents* ev) {
 #define PBF_DISP(b) \
-  displacement_as<b>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out, persistent)
+  displacement_as<b>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out, persistent, ev)
   switch (block) {
     case 64: PBF_DISP(64); break;
     case 128: PBF_DISP(128); break;
