@@ -127,7 +127,7 @@ __device__ __forceinline__ float4 load3(const float* x, int i) {
 // Pair geometry with t = max(1 - r/h, 0):
 //   W(r)  = 15/(pi h^3) t^3   (Dynamics/ParticleSystem/Kernel.h:36-45)
 //   Grad  = -45/(pi h^3) t^2  (Kernel.h:47-57)
-// r from v_rsq_f32 (within 2 ulp); r == 0 (self) gives t = 1.
+// r from v_rsq_f32 (within 2 ulp); a particle paired with itself gets t = 1.
 struct PairGeom {
   float dx, dy, dz, r, rinv, t;
 };
@@ -141,10 +141,16 @@ __device__ __forceinline__ PairGeom pair_geom(float4 xi, float4 xj, float inv_h)
   p.dx = xi.x - xj.x;
   p.dy = xi.y - xj.y;
   p.dz = xi.z - xj.z;
-  const float r2 = fmaf(p.dz, p.dz, fmaf(p.dy, p.dy, p.dx * p.dx));
-  p.rinv = __builtin_amdgcn_rsqf(fmaxf(r2, 1e-30f));
+  // r2 >= 1e-30 (a particle paired with itself has r2 = 1e-30, r ~ 1e-15,
+  // t = 1), so the reciprocal square root stays finite. For two distinct
+  // particles the term changes r2 only when dx*dx lies exactly halfway between
+  // two floats (that tie then rounds up).
+  const float r2 = fmaf(p.dz, p.dz, fmaf(p.dy, p.dy, fmaf(p.dx, p.dx, 1e-30f)));
+  p.rinv = __builtin_amdgcn_rsqf(r2);
   p.r = r2 * p.rinv;
-  p.t = fmaxf(fmaf(-p.r, inv_h, 1.0f), 0.0f);
+  // t = max(1 - r/h, 0); r >= 0 keeps 1 - r/h <= 1, so the clamp to [0, 1]
+  // (the multiply-add's clamp bit) is the same value
+  p.t = __builtin_amdgcn_fmed3f(fmaf(-p.r, inv_h, 1.0f), 0.0f, 1.0f);
   return p;
 }
 
