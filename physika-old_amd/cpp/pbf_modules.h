@@ -426,6 +426,24 @@ class PositionBasedFluidModel {
     m_integrator->end();
   }
 
+  // Launch settings (no reference counterpart): pbf_autotune on the current
+  // state (not changed), optionally kept in the JSON file cache_path; the
+  // chosen setting applies to every later step of this model.
+  pbf_tune_report autotune(int steps = 3, const char* cache_path = nullptr) {
+    pbf_tune_report r;
+    Context::check(pbf_autotune(ctx_->get(), (const float*)m_position.getValue().getDataPtr(),
+                                (const float*)m_velocity.getValue().getDataPtr(), nullptr, 1,
+                                m_position.getElementCount(), steps, cache_path, &r, nullptr),
+                   ctx_->get());
+    return r;
+  }
+  void setTuning(const pbf_tuning& t) { Context::check(pbf_set_tuning(ctx_->get(), &t), ctx_->get()); }
+  pbf_tuning getTuning() const {
+    pbf_tuning t;
+    Context::check(pbf_get_tuning(ctx_->get(), &t), ctx_->get());
+    return t;
+  }
+
   std::shared_ptr<Context> context() { return ctx_; }
   VarField<Real> m_smoothingLength;
   DeviceArrayField<Coord> m_position, m_velocity, m_forceDensity;

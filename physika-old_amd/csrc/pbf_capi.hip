@@ -199,12 +199,16 @@ int fail(pbf_ctx* c, int status, const std::string& msg) {
   return status;
 }
 
+// A failed call's error is also HIP's last error: it is cleared here, so that
+// the launch check of a later call (hipGetLastError) does not report it again.
 #define PBF_HIP(ctx, call)                                                                           \
   do {                                                                                               \
     hipError_t e_ = (call);                                                                          \
-    if (e_ != hipSuccess)                                                                            \
+    if (e_ != hipSuccess) {                                                                          \
+      (void)hipGetLastError();                                                                       \
       return fail(ctx, e_ == hipErrorOutOfMemory ? PBF_ERR_ALLOC : PBF_ERR_HIP,                      \
                   std::string(#call) + ": " + hipGetErrorString(e_));                               \
+    }                                                                                                \
   } while (0)
 
 #define PBF_LAUNCHED(ctx)                                                                            \

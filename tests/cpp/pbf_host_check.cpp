@@ -122,6 +122,15 @@ int main() {
   refused &= pbf_box_constrain(nullptr, &f, &f, 1, nullptr) == PBF_ERR_INVALID;
   refused &= pbf_set_profiling(nullptr, 1) == PBF_ERR_INVALID;
   refused &= pbf_get_profile(nullptr, &prof) == PBF_ERR_INVALID;
+  refused &= pbf_pbd_constrain_noalloc(nullptr, &f, &f, &i, &i, 1, &f, 1, nullptr) == PBF_ERR_INVALID;
+  refused &= pbf_rollout_forward_f64(nullptr, nullptr, nullptr, nullptr, nullptr, 1, 1, 1, 1, nullptr) ==
+             PBF_ERR_INVALID;
+  refused &= pbf_rollout_backward_f64(nullptr, nullptr, nullptr, nullptr) == PBF_ERR_INVALID;
+  pbf_tuning tun{256, 0, 0};
+  pbf_tune_report rep;
+  refused &= pbf_set_tuning(nullptr, &tun) == PBF_ERR_INVALID;
+  refused &= pbf_get_tuning(nullptr, &tun) == PBF_ERR_INVALID;
+  refused &= pbf_autotune(nullptr, &f, &f, nullptr, 1, 1, 1, nullptr, &rep, nullptr) == PBF_ERR_INVALID;
   check(refused, "null context refused by every entry point");
   check(pbf_last_error(nullptr) != nullptr, "last error of a null context");
 

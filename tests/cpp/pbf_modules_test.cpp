@@ -71,6 +71,28 @@ int main() {
     report(dx <= 1e-6, "fused step == module sequence, positions", dx);
     report(dv <= 1e-3, "fused step == module sequence, velocities", dv);
 
+    // the autotuner leaves the state alone; any setting it applies gives the same bits
+    PositionBasedFluidModel tuned, plain;
+    tuned.initialize();
+    plain.initialize();
+    for (auto* m : {&tuned, &plain}) {
+      m->m_position.getValue().assign(x0);
+      m->m_velocity.getValue().assign(v0);
+      m->m_forceDensity.setElementCount(n);
+    }
+    const pbf_tune_report tr = tuned.autotune(2);
+    const double moved = max_diff(tuned.m_position.getValue().toHost(), x0);
+    const pbf_tuning now = tuned.getTuning();
+    report(moved == 0.0 && tr.candidates == 17 && now.pass_block == tr.best.pass_block &&
+               now.persistent == tr.best.persistent,
+           "autotune: state unchanged, setting applied", (double)tr.best.pass_block);
+    for (int s = 0; s < 2; ++s) {
+      tuned.step(0.001f);
+      plain.step(0.001f);
+    }
+    const double dt_bits = max_diff(tuned.m_position.getValue().toHost(), plain.m_position.getValue().toHost());
+    report(dt_bits == 0.0, "tuned setting == default setting, positions", dt_bits);
+
     // dynamic and limited neighbor rows
     auto ctx = std::make_shared<Context>();
     NeighborQuery nq(ctx);

@@ -22,7 +22,7 @@ def test_cpp_module_mirror():
     print(r.stdout)
     lines = [ln for ln in r.stdout.splitlines() if ln.startswith(("PASS", "FAIL"))]
     assert r.returncode == 0, r.stdout + r.stderr
-    assert len(lines) == 5 and all(ln.startswith("PASS") for ln in lines)
+    assert len(lines) == 7 and all(ln.startswith("PASS") for ln in lines)
 
 
 def test_cpp_module_mirror_is_built():

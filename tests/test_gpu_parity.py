@@ -364,6 +364,29 @@ def test_invalid_params_rejected(fluid, capi):
 
 
 # ------------------------------------------------------- full-size properties
+def test_allocation_failure_recovers(fluid, capi, scene):
+    """A reservation the device cannot hold (list_slots = 2^30 per particle:
+    about 2 TB) fails with PBF_ERR_ALLOC before any kernel runs; after the
+    parameter is set back, the same context steps again and gives the bits of
+    a fresh context."""
+    say("allocation failure: list reservation of 2^30 slots per particle, then recovery")
+    x = scene.jittered_lattice((10, 10, 10), (0.3, 0.5, 0.3), 0.005, 0.1, seed=3)
+    ctx = fluid.Context("cuda:0", list_slots=1 << 30)
+    xd, vd = dev(x), dev(np.zeros_like(x))
+    with pytest.raises(capi.PBFError) as e:
+        ctx.step(xd, vd, None, 1)
+    assert e.value.status == capi.PBF_ERR_ALLOC and "hipMalloc" in str(e.value)
+    np.testing.assert_array_equal(xd.cpu().numpy(), x)  # nothing ran
+    ctx.set_params(list_slots=0)
+    got = []
+    for c in (ctx, fluid.Context("cuda:0")):
+        xd, vd = dev(x), dev(np.zeros_like(x))
+        c.step(xd, vd, None, 3)
+        got.append((xd.cpu().numpy(), vd.cpu().numpy()))
+    np.testing.assert_array_equal(got[0][0].view(np.uint32), got[1][0].view(np.uint32))
+    np.testing.assert_array_equal(got[0][1].view(np.uint32), got[1][1].view(np.uint32))
+
+
 def test_million_particle_neighbor_checksums(fluid, oracle, scene):
     say("1,048,576-particle lattice: per-row neighbor checksums vs oracle")
     x = scene.lattice_1m()
