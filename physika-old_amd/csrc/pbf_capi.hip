@@ -11,6 +11,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "pbf.h"
 #include "pbf_internal.h"
 
@@ -684,17 +686,36 @@ hipEvent_t take_event(pbf_ctx* c) {
 
 int drain_marks(pbf_ctx* c);
 
+// A roctx range (rocprofv3 --marker-trace shows the entry points and passes);
+// a push without an attached tool costs a flag test.
+struct Marker {
+  explicit Marker(const char* name) { roctxRangePush(name); }
+  ~Marker() { roctxRangePop(); }
+  Marker(const Marker&) = delete;
+  Marker& operator=(const Marker&) = delete;
+};
+
+const char* pass_name(int pass) {
+  switch (pass) {
+    case PBF_PASS_SORT: return "pbf sort";
+    case PBF_PASS_NEIGHBORS: return "pbf neighbors";
+    case PBF_PASS_DENSITY_LAMBDA: return "pbf density+lambda";
+    default: return "pbf displacement";
+  }
+}
+
 // Times one pass while profiling. A pass of several kernels (the sort) is
 // bracketed by events recorded on the stream; a one-kernel pass hands its
 // events to the launch (events(): they time that dispatch itself).
 struct PassTimer {
+  Marker mark;
   pbf_ctx* c;
   int pass;
   hipStream_t st;
   bool attach;
   KernelEvents ev{nullptr, nullptr};
   PassTimer(pbf_ctx* c_, int pass_, hipStream_t st_, bool attach_ = false)
-      : c(c_), pass(pass_), st(st_), attach(attach_) {
+      : mark(pass_name(pass_)), c(c_), pass(pass_), st(st_), attach(attach_) {
     if (!c->profiling || (ev.start = take_event(c)) == nullptr) return;
     if (attach && (ev.stop = take_event(c)) == nullptr) {
       c->spare.push_back(ev.start);
@@ -1344,6 +1365,7 @@ int pbf_reserve_batch(pbf_ctx* c, int n, int nscenes) {
 
 int pbf_neighbor_query(pbf_ctx* c, const float* d_x, int n, int* d_index, int* d_elems, long long cap,
                        long long* total, void* stream) {
+  Marker mark("pbf_neighbor_query");
   if (!c) return PBF_ERR_INVALID;
   if (bad_n(n) || !total || (n > 0 && (!d_x || !d_index)) || cap < 0) return fail(c, PBF_ERR_INVALID, "bad argument");
   hipStream_t st = (hipStream_t)stream;
@@ -1395,6 +1417,7 @@ int pbf_density(pbf_ctx* c, const float* d_x, const int* d_index, const int* d_e
 
 int pbf_pbd_constrain(pbf_ctx* c, float* d_x, float* d_v, const int* d_index, const int* d_elems, long long nelems,
                       float* d_rho, int n, void* stream) {
+  Marker mark("pbf_pbd_constrain");
   if (!c) return PBF_ERR_INVALID;
   if (bad_n(n) || (n > 0 && (!d_x || !d_v || !d_index))) return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0) return PBF_OK;
@@ -1433,6 +1456,7 @@ int pbf_pbd_constrain_noalloc(pbf_ctx* c, float* d_x, float* d_v, const int* d_i
 }
 
 int pbf_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, int nsteps, void* stream) {
+  Marker mark("pbf_step");
   if (!c) return PBF_ERR_INVALID;
   if (bad_n(n) || nsteps < 0 || (n > 0 && (!d_x || !d_v))) return fail(c, PBF_ERR_INVALID, "bad argument");
   if (n == 0 || nsteps == 0) return PBF_OK;
@@ -1447,6 +1471,7 @@ int pbf_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, int n, int nsteps
 
 int pbf_batch_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const int* d_scene_offsets, int nscenes, int n,
                    int nsteps, void* stream) {
+  Marker mark("pbf_batch_step");
   if (!c) return PBF_ERR_INVALID;
   if (bad_n(n) || nsteps < 0 || nscenes < 1 || !d_scene_offsets || (n > 0 && (!d_x || !d_v)))
     return fail(c, PBF_ERR_INVALID, "bad argument");
@@ -1461,6 +1486,7 @@ int pbf_batch_step(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const int* 
 
 int pbf_rollout_forward(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const int* d_scene_offsets, int nscenes,
                         int n, int nsteps, int checkpoint_every, void* stream) {
+  Marker mark("pbf_rollout_forward");
   if (!c) return PBF_ERR_INVALID;
   if (bad_n(n) || n == 0 || nsteps < 1 || checkpoint_every < 1 || nscenes < 1 || !d_x || !d_v ||
       (!d_scene_offsets && nscenes != 1))
@@ -1512,6 +1538,7 @@ int pbf_rollout_forward(pbf_ctx* c, float* d_x, float* d_v, float* d_rho, const 
 }
 
 int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream) {
+  Marker mark("pbf_rollout_backward");
   if (!c) return PBF_ERR_INVALID;
   if (!d_xbar || !d_vbar) return fail(c, PBF_ERR_INVALID, "bad argument");
   auto& r = c->roll;
@@ -1590,6 +1617,7 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
 
 int pbf_rollout_forward_f64(pbf_ctx* c, double* d_x, double* d_v, double* d_rho, const int* d_scene_offsets,
                             int nscenes, int n, int nsteps, int checkpoint_every, void* stream) {
+  Marker mark("pbf_rollout_forward_f64");
   if (!c) return PBF_ERR_INVALID;
   if (bad_n(n) || n == 0 || nsteps < 1 || checkpoint_every < 1 || nscenes < 1 || !d_x || !d_v ||
       (!d_scene_offsets && nscenes != 1))
@@ -1636,6 +1664,7 @@ int pbf_rollout_forward_f64(pbf_ctx* c, double* d_x, double* d_v, double* d_rho,
 }
 
 int pbf_rollout_backward_f64(pbf_ctx* c, double* d_xbar, double* d_vbar, void* stream) {
+  Marker mark("pbf_rollout_backward_f64");
   if (!c) return PBF_ERR_INVALID;
   if (!d_xbar || !d_vbar) return fail(c, PBF_ERR_INVALID, "bad argument");
   auto& r = c->roll64;

@@ -12,7 +12,7 @@ sanitize: $(SAN)/pbf_host_check
 
 $(SAN)/libpbf_san.so: $(SRC) $(HDR)
 	@mkdir -p $(SAN)
-	$(HIPCC) $(subst -O3,-O1 -g,$(HIPFLAGS)) $(SANHOST) -shared $(SRC) -o $@
+	$(HIPCC) $(subst -O3,-O1 -g,$(HIPFLAGS)) $(SANHOST) -shared $(SRC) -o $@ $(LDLIBS)
 
 $(SAN)/pbf_host_check: ../../tests/cpp/pbf_host_check.cpp ../../include/pbf.h $(SAN)/libpbf_san.so
 	$(CLANGXX) -O1 -g -std=c++17 -Wall -I../../include -fsanitize=address,undefined -fno-sanitize-recover=all \
