@@ -223,7 +223,7 @@ constexpr int kMaxParticles = 1 << 27;
 // candidate group of 8 may read past a run's end.
 constexpr int kXnPad = 32;
 // Blocks of the device-length scan.
-constexpr int kScanDevBlocks = 512;
+constexpr int kScanDevBlocks = 128;
 
 size_t scan_scratch_ints(long long n);
 size_t bounds_part_ints(long long n);
