@@ -368,9 +368,10 @@ int pbf_get_tuning(const pbf_ctx* ctx, pbf_tuning* out);
  * scene (d_scene_offsets NULL, nscenes 1) or a batch (as pbf_batch_step); the
  * caller's x and v are not changed. With cache_path (may be NULL) the choice
  * is kept in a JSON file under a key of the device architecture, the particle
- * count (half octaves), the scene count (octaves), h/d, the iteration count
- * and the flags; a later call with an equal key applies the stored setting
- * without timing. Blocking; allocates two copies of the state while timing. */
+ * count (half octaves), the scene count (octaves), the mean neighbor count
+ * (octaves, from one step of the copy), h/d, the iteration count and the
+ * flags; a later call with an equal key applies the stored setting without
+ * timing. Blocking; allocates two copies of the state while it runs. */
 typedef struct pbf_tune_report {
   pbf_tuning best;
   double best_ms;     /* per step of the chosen setting (timed, or as stored in the cache) */

@@ -5,6 +5,7 @@
 // blocks of BLOCK_SIZE = 64 (Core/Utility/cuda_utilities.h:34).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -35,8 +36,10 @@ struct DeviceCopy {
 };
 
 // The cache key: device architecture, particle count in half octaves, scene
-// count in octaves, h/d, iterations and the module flags.
-std::string tune_key(const pbf_params& p, int n, int nscenes, int device) {
+// count in octaves, mean neighbors per particle in octaves (a lattice and a
+// dense blob of one size prefer different settings), h/d, iterations and the
+// module flags.
+std::string tune_key(const pbf_params& p, int n, int nscenes, double mean_nbrs, int device) {
   hipDeviceProp_t prop;
   std::string arch = "unknown";
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) {
@@ -44,9 +47,10 @@ std::string tune_key(const pbf_params& p, int n, int nscenes, int device) {
     arch = arch.substr(0, arch.find(':'));
   }
   char buf[128];
-  std::snprintf(buf, sizeof buf, "%s n2^%.1f s2^%d h/d%.2f K%d f%u", arch.c_str(), std::floor(std::log2(n) * 2.0) / 2.0,
-                (int)std::floor(std::log2(nscenes)), p.smoothing_length / p.sampling_distance, p.iterations,
-                p.flags & (PBF_FLAG_VISCOSITY | PBF_FLAG_BOX));
+  std::snprintf(buf, sizeof buf, "%s n2^%.1f s2^%d k2^%d h/d%.2f K%d f%u", arch.c_str(),
+                std::floor(std::log2(n) * 2.0) / 2.0, (int)std::floor(std::log2(nscenes)),
+                (int)std::floor(std::log2(std::max(mean_nbrs, 1.0))), p.smoothing_length / p.sampling_distance,
+                p.iterations, p.flags & (PBF_FLAG_VISCOSITY | PBF_FLAG_BOX));
   return buf;
 }
 
@@ -133,7 +137,28 @@ extern "C" int pbf_autotune(pbf_ctx* c, const float* d_x, const float* d_v, cons
   int device = 0;
   if (pbf_get_params(c, &p) || pbf_get_tuning(c, &start) || hipGetDevice(&device) != hipSuccess)
     return pbf::ctx_fail(c, PBF_ERR_HIP, "pbf_autotune: context query failed");
-  const std::string key = tune_key(p, n, nscenes, device);
+  DeviceCopy w;
+  const size_t bytes = sizeof(float) * 3 * (size_t)n;
+  if (hipMalloc(&w.x, bytes) != hipSuccess || hipMalloc(&w.v, bytes) != hipSuccess ||
+      hipEventCreate(&w.e0) != hipSuccess || hipEventCreate(&w.e1) != hipSuccess) {
+    (void)hipGetLastError();
+    return pbf::ctx_fail(c, PBF_ERR_ALLOC, "pbf_autotune: no memory for a copy of the workload");
+  }
+  const int* off = nscenes > 1 || d_scene_offsets ? d_scene_offsets : nullptr;
+  // one step of a copy: the neighbor count of the workload for the key
+  double mean_nbrs = 0;
+  {
+    if (hipMemcpyAsync(w.x, d_x, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(w.v, d_v, bytes, hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return pbf::ctx_fail(c, PBF_ERR_HIP, "pbf_autotune: copy of the workload failed");
+    const int s = off ? pbf_batch_step(c, w.x, w.v, nullptr, off, nscenes, n, 1, st)
+                      : pbf_step(c, w.x, w.v, nullptr, n, 1, st);
+    if (s) return s;
+    pbf_stats stats;
+    if (pbf_get_stats(c, &stats)) return pbf::ctx_fail(c, PBF_ERR_HIP, "pbf_autotune: statistics");
+    mean_nbrs = (double)stats.neighbor_pairs / n;
+  }
+  const std::string key = tune_key(p, n, nscenes, mean_nbrs, device);
   std::snprintf(out->key, sizeof out->key, "%s", key.c_str());
 
   std::vector<Entry> cache;
@@ -152,13 +177,6 @@ extern "C" int pbf_autotune(pbf_ctx* c, const float* d_x, const float* d_v, cons
       return PBF_OK;
     }
   }
-
-  DeviceCopy w;
-  const size_t bytes = sizeof(float) * 3 * (size_t)n;
-  if (hipMalloc(&w.x, bytes) != hipSuccess || hipMalloc(&w.v, bytes) != hipSuccess ||
-      hipEventCreate(&w.e0) != hipSuccess || hipEventCreate(&w.e1) != hipSuccess)
-    return pbf::ctx_fail(c, PBF_ERR_ALLOC, "pbf_autotune: no memory for a copy of the workload");
-  const int* off = nscenes > 1 || d_scene_offsets ? d_scene_offsets : nullptr;
 
   // the default first, then the rest; a setting must beat the best so far by 1%
   std::vector<pbf_tuning> cand{{256, 0, 0}};

@@ -100,6 +100,12 @@ def test_autotune_and_cache(fluid, scene, tmp_path):
     r3 = fluid.Context("cuda:0").autotune(xs, torch.zeros_like(xs), steps=2, cache=cache)
     assert r3["key"] != r["key"] and not r3["from_cache"]
     assert set(json.loads(cache.read_text())) == {r["key"], r3["key"]}
+    # a dense blob of the same size has its own key: the mean neighbor count is part of it
+    xb = dev(scene.gaussian_blobs(1, 32768, 0.02, seed=3))
+    r4 = fluid.Context("cuda:0").autotune(xb, torch.zeros_like(xb), steps=2, cache=cache)
+    say(f"keys: lattice {r['key']}, blob {r4['key']}")
+    assert " k2^6 " in r["key"] and r4["key"] != r["key"] and not r4["from_cache"]
+    assert r4["key"].split(" k2^")[0] == r["key"].split(" k2^")[0]
 
 
 def test_invalid_tuning_rejected(fluid, capi):
