@@ -15,6 +15,8 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 #include "pbf.h"
 #include "pbf_internal.h"
 
@@ -86,7 +88,9 @@ bool field(const std::string& body, const char* name, double* v) {
 }
 
 bool write_cache(const char* path, const std::vector<Entry>& entries) {
-  const std::string tmp = std::string(path) + ".tmp";
+  // a file of this process, renamed over the cache (rename is atomic: a reader
+  // sees the old or the new file; two processes tuning at once keep the last)
+  const std::string tmp = std::string(path) + ".tmp" + std::to_string((long)getpid());
   {
     std::ofstream f(tmp, std::ios::trunc);
     if (!f) return false;

@@ -850,5 +850,10 @@ def rollout(ctx: Context, x0: torch.Tensor, v0: torch.Tensor, nsteps: int, check
     if scene_offsets is not None:
         off = _check_scene_offsets(scene_offsets, x0.shape[0], x0.device, ctx.validate_offsets)
     with torch.cuda.device(x0.device):
-        return tuple(_capi.native().rollout(x0, v0, int(ctx.handle.value or 0), int(nsteps),
-                                            int(checkpoint_every), off, _stream_int(x0.device)))
+        out = tuple(_capi.native().rollout(x0, v0, int(ctx.handle.value or 0), int(nsteps),
+                                           int(checkpoint_every), off, _stream_int(x0.device)))
+    if out[0].grad_fn is not None:
+        # the backward node holds the context's handle only: keep the context
+        # (and its checkpoints) alive as long as the node
+        out[0].grad_fn.metadata["pbf_context"] = ctx
+    return out

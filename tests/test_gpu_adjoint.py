@@ -147,6 +147,25 @@ def test_rollout_adjoint_staged_lists_identical(fluid, scene):
         np.testing.assert_array_equal(u, w)
 
 
+def test_rollout_keeps_its_context_alive(fluid, scene):
+    """fluid.rollout on a context the caller does not keep: the graph holds
+    it until backward, and the gradient equals the one of a kept context."""
+    import gc
+    say("rollout on a dropped context: backward after garbage collection")
+    x0 = compressed_block(scene, seed=23)
+    grads = []
+    for keep in (True, False):
+        ctx = fluid.Context("cuda:0")
+        v0 = torch.zeros((len(x0), 3), dtype=torch.float32, device="cuda:0", requires_grad=True)
+        xN, vN, _ = fluid.rollout(ctx, dev(x0), v0, 3, 2)
+        if not keep:
+            del ctx
+            gc.collect()
+        (xN * xN).sum().backward()
+        grads.append(v0.grad.cpu().numpy())
+    np.testing.assert_array_equal(grads[0], grads[1])
+
+
 def test_batched_rollout_gradients_equal_per_scene(fluid, scene):
     say("batched rollout adjoint vs per-scene rollouts, 3 scenes, 4 steps")
     scenes = [compressed_block(scene, (5, 6, 7), 30), compressed_block(scene, (4, 4, 4), 31),
