@@ -27,7 +27,7 @@ sys.path.insert(0, str(ROOT))
 METRIC = "particle-steps/s at 1M PBD fluid; batched vs serial; bwd/fwd time ratio"
 HBM_PEAK_GBS = 8000.0
 FP32_PEAK_TFLOPS = 157.3      # MI355X published FP32 vector peak (packed issue)
-FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32, which this build disables
+FP32_NONPACKED_TFLOPS = 78.6  # the same without packed FP32, which the library's build disables
 
 # Algorithmic traffic and flops (fp32, the reference's pass structure; neighbor
 # data count as on-chip, so no list bytes):

@@ -142,7 +142,7 @@ template <class R> SDF<R> box_sdf(const R lo[3], const R hi[3], R spacing, bool 
 template <class R> void sdf_distance(const SDF<R>& f, const V3<R>& p, R& d, V3<R>& normal);
 // The jitter of K_ConstrainSDF: 1e-4 times a uniform in (0, 1] drawn per
 // particle id. The reference draws it with curand (RandNumber(pId),
-// Core/Utility/CudaRand.h); this build uses its own counter-based draw
+// Core/Utility/CudaRand.h); the library and this oracle use a counter-based draw
 // (SplitMix64 of the id, top 24 bits), so jittered positions differ from the
 // reference's by at most 1e-4 m along the wall normal.
 float box_jitter(int id);

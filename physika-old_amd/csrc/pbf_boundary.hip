@@ -159,7 +159,7 @@ __device__ void field_distance(const BoxField& f, float px, float py, float pz, 
 }
 
 // 1e-4 times a uniform in (0, 1] per particle id: SplitMix64 of id + 1, top 24
-// bits (the reference's draw is curand's, which this build does not
+// bits (the reference's draw is curand's, which the library does not
 // reproduce).
 __device__ __forceinline__ float box_jitter(int id) {
   unsigned long long z = ((unsigned long long)(unsigned)id + 1ull) * 0x9E3779B97F4A7C15ull;
