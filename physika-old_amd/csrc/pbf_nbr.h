@@ -36,19 +36,20 @@ __device__ __forceinline__ void stencil_x(const GridDev& g, int fi, float frac, 
 // fi+xlo..fi+xhi of row (cj+dj, ck+dk) with dk = r/3-1, dj = r%3-1 for
 // r = run_of(q), clipped to the scene's window (cells outside it hold no
 // particle, and the window lies inside the grid, so this is the reference's
-// in-grid stencil).
+// in-grid stencil). Branch-free: a run outside the window reads entry 0 of
+// cell_start (always allocated) and gets a = len = 0, so the loads of all
+// nine runs issue together and a wave waits on them once, not once per run.
 __device__ __forceinline__ void run_range(const int* __restrict__ cell_start, const SceneWin& w, int fi, int cj,
                                           int ck, int xlo, int xhi, int q, int& a, int& len) {
   const int r = run_of(q);
   const int lj = cj + (r % 3) - 1 - w.lo[1], lk = ck + (r / 3) - 1 - w.lo[2];
-  a = 0;
-  len = 0;
-  if (lj < 0 || lj >= w.dim[1] || lk < 0 || lk >= w.dim[2]) return;
   const int i0 = max(fi + xlo, w.lo[0]), i1 = min(fi + xhi, w.lo[0] + w.dim[0] - 1);
-  if (i0 > i1) return;
+  const bool ok = lj >= 0 && lj < w.dim[1] && lk >= 0 && lk < w.dim[2] && i0 <= i1;
   const int row = w.cell_base + w.dim[0] * (lj + w.dim[1] * lk) - w.lo[0];
-  a = cell_start[row + i0];
-  len = cell_start[row + i1 + 1] - a;
+  const int a0 = cell_start[ok ? row + i0 : 0];
+  const int a1 = cell_start[ok ? row + i1 + 1 : 0];
+  a = ok ? a0 : 0;
+  len = ok ? a1 - a0 : 0;
 }
 
 __device__ __forceinline__ bool in_window(const SceneWin& w, int ci, int cj, int ck) {
@@ -363,26 +364,39 @@ __device__ __forceinline__ void list_loop_pairs(int cnt, int nfull, const uint4*
   }
 }
 
+// The neighbor pass's per-wave words that select a wave's list path, uniform
+// over the wave (scalar registers). The neighbor pass writes all three for
+// every wave, so they are read unconditionally and together: no load waits on
+// another (a latency chain that shows when waves are few).
+struct WaveList {
+  long long lb;  // list base in 16-bit slots; negative: no list
+  int lm;        // list mode | groups every lane fills << 2
+  int jb;        // index base of 16-bit offsets
+};
+__device__ __forceinline__ WaveList wave_list(const NbrView& nv, int wave) {
+  return WaveList{nv.list_base[wave], nv.list_mode[wave], nv.list_jbase[wave]};
+}
+
 // Calls body(j, gather(j)) for every neighbor j of sorted particle s, in
 // ascending j. The list is used when the wave has one; otherwise (reservation
 // exhausted) the words come from the masks, or are recomputed from xn.
 // kPadded (list path only): the body also runs on up to three entries of the
 // padding particle nv.pad, whose position lies far outside every stencil; a
 // body whose terms vanish with the kernel's t = 0 (and are finite) adds exact
-// zeros for them.
+// zeros for them. wl: the words of s's wave (wave_list).
 template <bool kPadded = false, class GatherFn, class Body>
-__device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, GatherFn&& gather, Body&& body) {
-  // per-wave values, uniform over the wave's lanes (scalar registers)
+__device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, const WaveList& wl, GatherFn&& gather,
+                                                  Body&& body) {
   const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
-  const long long lb = nv.list_base[wave];
+  const long long lb = wl.lb;
+  const int cnt = nv.nbr_cnt[s];  // issued with the wave's words (unused on the mask path)
   if (lb >= 0) {
-    const int cnt = nv.nbr_cnt[s];
     const int past = kPadded ? nv.pad : s;
-    const int lm = nv.list_mode[wave];
+    const int lm = wl.lm;
     const int nfull = lm >> 2;
     if ((lm & 3) == kListDelta16) {
       const uint4* L = reinterpret_cast<const uint4*>(nv.list + lb) + lane;
-      const int jb = nv.list_jbase[wave];
+      const int jb = wl.jb;
       list_loop_pairs<kPadded>(
           cnt, nfull, L,
           [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cnt, past); }, gather,
@@ -396,7 +410,7 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
       const int ka = st[7], ca = nv.cnt_a[s], cb = cnt - ca;
       list_loop_pairs<kPadded>(
           ca, 0, L, [&](uint2 e, int b, auto) { return decode16_staged<false>(e, m, b, ca, past); }, gather, body);
-      const int jb = nv.list_jbase[wave];
+      const int jb = wl.jb;
       list_loop_pairs<kPadded>(
           cb, nfull, L + ((long long)(ka >> 3) << 6),
           [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cb, past); }, gather,
@@ -434,11 +448,22 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, Gath
   }
 }
 
+// for_each_neighbor reading the wave's words itself.
+template <bool kPadded = false, class GatherFn, class Body>
+__device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, GatherFn&& gather, Body&& body) {
+  for_each_neighbor<kPadded>(nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)), gather, body);
+}
+
 // for_each_neighbor with the neighbor's position (kW: with its w component).
+template <bool kW, bool kPadded = false, class Body>
+__device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const WaveList& wl,
+                                                    const float4* __restrict__ xs, Body&& body) {
+  for_each_neighbor<kPadded>(nv, s, wl, [&](int j) { return load_pos<kW>(xs, j); }, body);
+}
 template <bool kW, bool kPadded = false, class Body>
 __device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const float4* __restrict__ xs,
                                                     Body&& body) {
-  for_each_neighbor<kPadded>(nv, s, [&](int j) { return load_pos<kW>(xs, j); }, body);
+  for_each_neighbor_x<kW, kPadded>(nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)), xs, body);
 }
 
 // LDS records a wave needs for for_each_neighbor_staged.
@@ -472,13 +497,15 @@ template <bool kW, class Body>
 __device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int s, const float4* __restrict__ xs,
                                                          float4* buf, Body&& body) {
   const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
-  const long long lb = nv.list_base[wave];
-  const int lm = lb >= 0 ? nv.list_mode[wave] : 0;
+  const WaveList wl = wave_list(nv, wave);
+  const long long lb = wl.lb;
+  const int lm = wl.lm;
   if (lb < 0 || (lm & 3) != kListStaged) {
-    for_each_neighbor_x<kW, true>(nv, s, xs, body);
+    for_each_neighbor_x<kW, true>(nv, s, wl, xs, body);
     return;
   }
   const int* st = nv.stage + kStageInts * wave;
+  const int cb = nv.nbr_cnt[s] - nv.cnt_a[s];  // loads issued before the stage copy
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int a = st[q], p0 = st[3 + q], len = st[4 + q] - p0;
@@ -496,8 +523,7 @@ __device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int 
   list_loop_pairs<true>(
       ka, ka >> 2, L, [&](uint2 e, int b, auto) { return decode16_bytes<true>(e, lds, b, ka, 0); },
       [&](int a) { return lds_load_pos<kW>((uint32_t)a); }, body);
-  const int cb = nv.nbr_cnt[s] - nv.cnt_a[s];
-  const uint32_t jb16 = (uint32_t)nv.list_jbase[wave] << 4;
+  const uint32_t jb16 = (uint32_t)wl.jb << 4;
   const int past = (int)((uint32_t)nv.pad << 4);
   list_loop_pairs<true>(
       cb, lm >> 2, L + ((long long)(ka >> 3) << 6),

@@ -184,12 +184,13 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   int words = 0, cand = 0, jlo = INT_MAX, jhi = -1;
   bool oog_lane = false;
   if (active) {
+    // the scene window's loads first: they do not wait on the position
+    const SceneWin win = nv.win[nv.ssid[s]];
     xi = nv.xn[s];
     int ci, cj, ck, xlo, xhi;
     float frac;
     cell_of(nv.g, xi.x, xi.y, xi.z, ci, cj, ck, frac);
     stencil_x(nv.g, ci, frac, xlo, xhi);
-    const SceneWin win = nv.win[nv.ssid[s]];
 #pragma unroll
     for (int q = 0; q < 9; ++q) {
       run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, q, ra[q], rl[q]);
