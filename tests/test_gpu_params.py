@@ -8,9 +8,11 @@ Cases:
   per lane than the neighbor pass keeps in LDS;
 - one iteration with a four times longer time step and gravity along z;
 - h/d = 1.8 with 4 iterations.
-Bars: neighbor sets bit-exact; one step within the parity bars (x 1e-6 m,
-v 1e-3 m/s, rho 1e-5 relative, each against FP64 with twice the FP32 oracle's
-own distance from FP64 as the floor). Every call goes through the C ABI."""
+Bars: neighbor sets bit-exact; one step of the fused step and of the module
+sequence (NeighborQuery::compute, integrate, DensityPBD::constrain) within the
+parity bars (x 1e-6 m, v 1e-3 m/s, rho 1e-5 relative, each against FP64 with
+twice the FP32 oracle's own distance from FP64 as the floor). Every call goes
+through the C ABI."""
 import numpy as np
 import pytest
 
@@ -54,4 +56,8 @@ def test_parameters_vs_oracle(fluid, oracle, scene, case):
 
     model = fluid.PositionBasedFluidModel(h, "cuda:0", viscosity=False, **gkw)
     gpu = model.step_arrays(x, v, 1)
-    assert_step_parity(gpu, *oracle_both(oracle, x, v, 1, **okw))
+    ref = oracle_both(oracle, x, v, 1, **okw)
+    assert_step_parity(gpu, *ref)
+    # the module sequence (NeighborQuery::compute, integrate, DensityPBD::constrain) at the same parameters
+    say(f"{case}: module path")
+    assert_step_parity(model.step_arrays(x, v, 1, modules=True), *ref)
