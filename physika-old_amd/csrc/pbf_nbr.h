@@ -261,6 +261,29 @@ __device__ __forceinline__ Group4 decode32(int4 e, int b, int cnt, int past) {
   return g;
 }
 
+// Parity of an entry's index in the neighbor order (the canonical order:
+// runs in visit order, ascending sorted index). The PBD passes keep one partial
+// sum per parity and add the two at the end, so that a lane pair splitting a
+// list by parity and a single lane walking all of it sum identically. A group
+// of four starts at an even index on every path below except the part of a
+// staged list after its stage slots, which starts at the particle's stage count;
+// the loops tell the body about that shift with flip (flip(true): the parities
+// trade places until the next flip(true)).
+using EvenEntry = std::integral_constant<int, 0>;
+using OddEntry = std::integral_constant<int, 1>;
+
+// body(j, a, parity) on the four entries of group b (guarded unless full).
+template <bool kPadded, class T, class Full, class Body>
+__device__ __forceinline__ void group_bodies(int b, Full, int cnt, const Group4& g, const T (&a)[4], Body& body) {
+  auto one = [&](int u, auto par) {
+    if (Full::value || kPadded || (b << 2) + u < cnt) body(g.j[u], a[u], par);
+  };
+  one(0, EvenEntry{});
+  one(1, OddEntry{});
+  one(2, EvenEntry{});
+  one(3, OddEntry{});
+}
+
 // Counted loop over one lane's list, software-pipelined two groups deep:
 // while the bodies of group b run, the gathered data of group b+1 and the
 // indices of group b+2 are in flight (one wait per group). gather(j) loads
@@ -296,9 +319,7 @@ __device__ __forceinline__ void list_loop(int cnt, int nfull, LoadFn&& load_grou
     // them to their uses in the next one and every group waits on its own
     // gathers
     asm volatile("" ::: "memory");
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (decltype(full)::value || kPadded || (b << 2) + u < cnt) body(gc.j[u], ac[u]);
+    group_bodies<kPadded>(b, full, cnt, gc, ac, body);
   };
   int b = 0;
   for (; b + 2 < nfull; b += 2) {
@@ -332,9 +353,7 @@ __device__ __forceinline__ void list_loop_pairs(int cnt, int nfull, const uint4*
 #pragma unroll
   for (int u = 0; u < 4; ++u) a0[u] = gather(g0.j[u]);
   auto bodies = [&](int b, auto full, const Group4& g, const T(&a)[4]) {
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (decltype(full)::value || kPadded || (b << 2) + u < cnt) body(g.j[u], a[u]);
+    group_bodies<kPadded>(b, full, cnt, g, a, body);
   };
   auto even = [&](int b, auto full) {
     g1 = decode(make_uint2(pc.z, pc.w), b + 1, full);
@@ -377,16 +396,17 @@ __device__ __forceinline__ WaveList wave_list(const NbrView& nv, int wave) {
   return WaveList{nv.list_base[wave], nv.list_mode[wave], nv.list_jbase[wave]};
 }
 
-// Calls body(j, gather(j)) for every neighbor j of sorted particle s, in
-// ascending j. The list is used when the wave has one; otherwise (reservation
-// exhausted) the words come from the masks, or are recomputed from xn.
-// kPadded (list path only): the body also runs on up to three entries of the
-// padding particle nv.pad, whose position lies far outside every stencil; a
-// body whose terms vanish with the kernel's t = 0 (and are finite) adds exact
-// zeros for them. wl: the words of s's wave (wave_list).
-template <bool kPadded = false, class GatherFn, class Body>
-__device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, const WaveList& wl, GatherFn&& gather,
-                                                  Body&& body) {
+// Calls body(j, gather(j), parity) for every neighbor j of sorted particle s,
+// in the canonical order (parity: EvenEntry or OddEntry, the entry's index mod
+// 2, with flip as described at EvenEntry). The list is used when the wave has
+// one; otherwise (reservation exhausted) the words come from the masks, or are
+// recomputed from xn. kPadded (list path only): the body also runs on up to
+// three entries of the padding particle nv.pad, whose position lies far outside
+// every stencil; a body whose terms vanish with the kernel's t = 0 (and are
+// finite) adds exact zeros for them. wl: the words of s's wave (wave_list).
+template <bool kPadded = false, class GatherFn, class Body, class Flip>
+__device__ __forceinline__ void for_each_neighbor_p(const NbrView& nv, int s, const WaveList& wl, GatherFn&& gather,
+                                                    Body&& body, Flip&& flip) {
   const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
   const long long lb = wl.lb;
   const int cnt = nv.nbr_cnt[s];  // issued with the wave's words (unused on the mask path)
@@ -411,10 +431,12 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, cons
       list_loop_pairs<kPadded>(
           ca, 0, L, [&](uint2 e, int b, auto) { return decode16_staged<false>(e, m, b, ca, past); }, gather, body);
       const int jb = wl.jb;
+      flip((ca & 1) != 0);  // the offsets part starts at index ca
       list_loop_pairs<kPadded>(
           cb, nfull, L + ((long long)(ka >> 3) << 6),
           [&](uint2 e, int b, auto full) { return decode16<decltype(full)::value>(e, jb, b, cb, past); }, gather,
           body);
+      flip((ca & 1) != 0);
     } else {
       const int4* L = reinterpret_cast<const int4*>(nv.list + lb) + lane;
       list_loop<kPadded, int4>(
@@ -430,7 +452,7 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, cons
   stencil_x(nv.g, ci, frac, xlo, xhi);
   const SceneWin win = nv.win[nv.ssid[s]];
   const long long mb = nv.mask_base[wave];
-  int wofs = 0;
+  int wofs = 0, k = 0;
   for (int q = 0; q < 9; ++q) {
     int ra, rl;
     run_range(nv.cell_start, win, ci, cj, ck, xlo, xhi, q, ra, rl);
@@ -441,29 +463,31 @@ __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, cons
                               : test_word(nv.xn + jb, xni, min(32, rl - (w << 5)), nv.thr2);
       while (bits) {
         const int j = jb + pop_first(bits);
-        body(j, gather(j));
+        if (k & 1)
+          body(j, gather(j), OddEntry{});
+        else
+          body(j, gather(j), EvenEntry{});
+        ++k;
       }
     }
     wofs += nw;
   }
 }
 
-// for_each_neighbor reading the wave's words itself.
+// for_each_neighbor_p for a body with one running sum: body(j, gather(j)) in
+// the canonical order.
 template <bool kPadded = false, class GatherFn, class Body>
 __device__ __forceinline__ void for_each_neighbor(const NbrView& nv, int s, GatherFn&& gather, Body&& body) {
-  for_each_neighbor<kPadded>(nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)), gather, body);
+  for_each_neighbor_p<kPadded>(
+      nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)), gather,
+      [&](int j, const auto& a, auto) { body(j, a); }, [](bool) {});
 }
 
 // for_each_neighbor with the neighbor's position (kW: with its w component).
 template <bool kW, bool kPadded = false, class Body>
-__device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const WaveList& wl,
-                                                    const float4* __restrict__ xs, Body&& body) {
-  for_each_neighbor<kPadded>(nv, s, wl, [&](int j) { return load_pos<kW>(xs, j); }, body);
-}
-template <bool kW, bool kPadded = false, class Body>
 __device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, const float4* __restrict__ xs,
                                                     Body&& body) {
-  for_each_neighbor_x<kW, kPadded>(nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)), xs, body);
+  for_each_neighbor<kPadded>(nv, s, [&](int j) { return load_pos<kW>(xs, j); }, body);
 }
 
 // LDS records a wave needs for for_each_neighbor_staged.
@@ -485,27 +509,28 @@ __device__ __forceinline__ float4 lds_load_pos(uint32_t a) {
   return make_float4(v.x, v.y, v.z, 0.0f);
 }
 
-// for_each_neighbor_x<kW, true> for the PBD passes: a staged wave copies its
-// stage from xs into its LDS region buf (kStageRecords records; coalesced
-// loads) and reads the neighbors of its first three runs from there (the
-// padding particle sits at slot kStagePad), so only the others are gathered
-// through the vector-memory path. Other waves take for_each_neighbor_x. The
-// entries decode straight into byte addresses (LDS addresses of stage
-// records, byte offsets into xs), which the body receives as j (the PBD bodies
-// ignore it). The sums run in the same neighbor order as on every other path.
-template <bool kW, class Body>
+// for_each_neighbor_p with the neighbor's position (kW: with its w
+// component, padded lists) for the PBD passes: a staged wave copies its stage
+// from xs into its LDS region buf (kStageRecords records; coalesced loads) and
+// reads the neighbors of its first three runs from there (the padding particle
+// sits at slot kStagePad), so only the others are gathered through the
+// vector-memory path. Other waves take for_each_neighbor_p. The entries decode
+// straight into byte addresses (LDS addresses of stage records, byte offsets
+// into xs), which the body receives as j (the PBD bodies ignore it). Every
+// path calls the body in the same order with the same parities.
+template <bool kW, class Body, class Flip>
 __device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int s, const float4* __restrict__ xs,
-                                                         float4* buf, Body&& body) {
+                                                         float4* buf, Body&& body, Flip&& flip) {
   const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
   const WaveList wl = wave_list(nv, wave);
   const long long lb = wl.lb;
   const int lm = wl.lm;
   if (lb < 0 || (lm & 3) != kListStaged) {
-    for_each_neighbor_x<kW, true>(nv, s, wl, xs, body);
+    for_each_neighbor_p<true>(nv, s, wl, [&](int j) { return load_pos<kW>(xs, j); }, body, flip);
     return;
   }
   const int* st = nv.stage + kStageInts * wave;
-  const int cb = nv.nbr_cnt[s] - nv.cnt_a[s];  // loads issued before the stage copy
+  const int ca = nv.cnt_a[s], cb = nv.nbr_cnt[s] - ca;  // loads issued before the stage copy
 #pragma unroll
   for (int q = 0; q < 3; ++q) {
     const int a = st[q], p0 = st[3 + q], len = st[4 + q] - p0;
@@ -525,10 +550,12 @@ __device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int 
       [&](int a) { return lds_load_pos<kW>((uint32_t)a); }, body);
   const uint32_t jb16 = (uint32_t)wl.jb << 4;
   const int past = (int)((uint32_t)nv.pad << 4);
+  flip((ca & 1) != 0);  // the offsets part starts at index ca
   list_loop_pairs<true>(
       cb, lm >> 2, L + ((long long)(ka >> 3) << 6),
       [&](uint2 e, int b, auto full) { return decode16_bytes<decltype(full)::value>(e, jb16, b, cb, past); },
       [&](int o) { return load_pos_bytes<kW>(xs, (uint32_t)o); }, body);
+  flip((ca & 1) != 0);
 }
 
 }  // namespace pbf

@@ -430,6 +430,13 @@ void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* co
 // ---------------------------------------------------------------------------
 // PBD passes (DensityPBD::takeOneIteration, DensityPBD.cu:306-365).
 
+// Exchanges a[0] and a[1] when c holds (per lane: two selects, no branch).
+__device__ __forceinline__ void swap_if(bool c, float (&a)[2]) {
+  const float x = c ? a[1] : a[0], y = c ? a[0] : a[1];
+  a[0] = x;
+  a[1] = y;
+}
+
 // Pass A: K_ComputeDensity (SummationDensity.cu:13-36) and K_ComputeLambdas
 // (DensityPBD.cu:46-90) fused: one traversal gives rho_i, sum g and sum g.g;
 // lambda_i goes to xs[s].w for pass B. aux (taped steps only) receives
@@ -441,20 +448,34 @@ __device__ __forceinline__ void density_lambda_at(const NbrView& nv, int s, floa
                                                   float4* stage) {
   if (s >= nv.n) return;
   const float4 xi = xs[s];
-  float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
-  for_each_neighbor_staged<false>(nv, s, xs, stage, [&](int, float4 xj) {
+  // one partial sum per entry parity, added at the end
+  float rho2[2] = {0.f, 0.f}, sgx2[2] = {0.f, 0.f}, sgy2[2] = {0.f, 0.f}, sgz2[2] = {0.f, 0.f},
+        sgg2[2] = {0.f, 0.f};
+  for_each_neighbor_staged<false>(
+      nv, s, xs, stage,
+      [&](int, float4 xj, auto par) {
 #pragma clang fp contract(off)
-    const PairGeom p = pair_geom(xi, xj, pc.inv_h);
-    const float t2 = p.t * p.t;
-    rho = fmaf(t2, p.t, rho);
-    // |g| / |cG| = t^2 (|x_i - x_j| / r = 1); no gradient closer than pair_eps
-    const float tg = p.r > pc.pair_eps ? t2 : 0.0f;
-    const float gq = tg * p.rinv;
-    sgx = fmaf(gq, p.dx, sgx);
-    sgy = fmaf(gq, p.dy, sgy);
-    sgz = fmaf(gq, p.dz, sgz);
-    sgg = fmaf(tg, tg, sgg);
-  });
+        constexpr int e = decltype(par)::value;
+        const PairGeom p = pair_geom(xi, xj, pc.inv_h);
+        const float t2 = p.t * p.t;
+        rho2[e] = fmaf(t2, p.t, rho2[e]);
+        // |g| / |cG| = t^2 (|x_i - x_j| / r = 1); no gradient closer than pair_eps
+        const float tg = p.r > pc.pair_eps ? t2 : 0.0f;
+        const float gq = tg * p.rinv;
+        sgx2[e] = fmaf(gq, p.dx, sgx2[e]);
+        sgy2[e] = fmaf(gq, p.dy, sgy2[e]);
+        sgz2[e] = fmaf(gq, p.dz, sgz2[e]);
+        sgg2[e] = fmaf(tg, tg, sgg2[e]);
+      },
+      [&](bool odd) {
+        swap_if(odd, rho2);
+        swap_if(odd, sgx2);
+        swap_if(odd, sgy2);
+        swap_if(odd, sgz2);
+        swap_if(odd, sgg2);
+      });
+  float rho = rho2[0] + rho2[1];
+  const float sgx = sgx2[0] + sgx2[1], sgy = sgy2[0] + sgy2[1], sgz = sgz2[0] + sgz2[1], sgg = sgg2[0] + sgg2[1];
   rho *= pc.cW;
   const float S = (pc.cG * pc.cG) * (sgg + (sgx * sgx + sgy * sgy + sgz * sgz));
   float lam = -(rho - pc.lam_rest) / (S + pc.lam_eps);
@@ -575,8 +596,20 @@ __device__ __forceinline__ void displacement_at(const NbrView& nv, int s, const 
     cy = fmaf(c, p.dy, cy);
     cz = fmaf(c, p.dz, cz);
   };
-  float ax = 0.f, ay = 0.f, az = 0.f;
-  for_each_neighbor_staged<true>(nv, s, xs, stage, [&](int, float4 xj) { pair(xj, ax, ay, az); });
+  // one partial sum per entry parity, added at the end
+  float ax2[2] = {0.f, 0.f}, ay2[2] = {0.f, 0.f}, az2[2] = {0.f, 0.f};
+  for_each_neighbor_staged<true>(
+      nv, s, xs, stage,
+      [&](int, float4 xj, auto par) {
+        constexpr int e = decltype(par)::value;
+        pair(xj, ax2[e], ay2[e], az2[e]);
+      },
+      [&](bool odd) {
+        swap_if(odd, ax2);
+        swap_if(odd, ay2);
+        swap_if(odd, az2);
+      });
+  const float ax = ax2[0] + ax2[1], ay = ay2[0] + ay2[1], az = az2[0] + az2[1];
   float fa = 2.0f;
   float bx = 0.f, by = 0.f, bz = 0.f;
   if (nv.counters[C_OOG] != 0) {  // some scene has out-of-grid particles
