@@ -617,7 +617,7 @@ def main():
     ap.add_argument("--no-diff", action="store_true", help="skip the differentiable and batched-inverse workloads")
     ap.add_argument("--no-peaks", action="store_true", help="skip the achievable copy / FMA rate probes")
     ap.add_argument("--no-autotune", action="store_true", help="measure with the default launch settings")
-    ap.add_argument("--tuning", default="", metavar="BLOCK,STAGE,PERSISTENT",
+    ap.add_argument("--tuning", default="", metavar="BLOCK,STAGE,PERSISTENT[,LANES]",
                     help="measure with this launch setting instead of tuning (e.g. 128,1,0 for profiled runs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 256 MiB device copy (reference kernel for FETCH_SIZE/WRITE_SIZE counters)")
@@ -625,11 +625,11 @@ def main():
     global TUNE, FIXED_TUNING
     TUNE = not args.no_autotune and not args.tuning
     if args.tuning:
-        FIXED_TUNING = dict(zip(("pass_block", "stage_min_particles", "persistent"),
+        FIXED_TUNING = dict(zip(("pass_block", "stage_min_particles", "persistent", "pass_lanes"),
                                 (int(t) for t in args.tuning.split(","))))
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is not part of this "
-              "build", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for {args.gpus}); the multi-GPU path is a design, not built",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -673,7 +673,10 @@ def main():
     kb, kf = kernel_counts(kname, n, pairs)
     roofline = roofline_of(kb, kf, t_dom)
     tune = ctx.tuning()
-    traffic, traffic_kernel = profiled_traffic(kname, tune["pass_block"] if kname != "k_neighbors" else None,
+    # the lane-pair form of a pass (pass_lanes 2, or automatic up to 131,072 particles without persistent launches)
+    pair_lanes = tune["pass_lanes"] == 2 or (tune["pass_lanes"] == 0 and not tune["persistent"] and n <= 131072)
+    traffic, traffic_kernel = profiled_traffic(kname + ("2" if pair_lanes and kname != "k_neighbors" else ""),
+                                               tune["pass_block"] if kname != "k_neighbors" else None,
                                                bool(tune["persistent"]) and kname != "k_neighbors")
     roofline.update({
         "kernel": kname,

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define PBF_ABI_VERSION 8
+#define PBF_ABI_VERSION 9
 
 typedef enum pbf_status {
   PBF_OK = 0,
@@ -355,7 +355,16 @@ typedef struct pbf_tuning {
                               resident on the device, each wave walking
                               64-particle tiles of its XCD's contiguous share in
                               a fixed order; 0 (default): one wave per tile */
+  int pass_lanes;          /* lanes per particle in the two passes: 1; 2 (a lane
+                              pair splits each neighbor list by the parity of
+                              the entries' order and adds the two sums, twice
+                              the waves for small searches; not with
+                              persistent); 0 (default): 2 for searches over at
+                              most PBF_TWO_LANE_MAX_PARTICLES particles without
+                              persistent launches, else 1 */
 } pbf_tuning;
+
+#define PBF_TWO_LANE_MAX_PARTICLES 131072
 
 int pbf_set_tuning(pbf_ctx* ctx, const pbf_tuning* t);
 int pbf_get_tuning(const pbf_ctx* ctx, pbf_tuning* out);

@@ -14,7 +14,7 @@ _LIB = _PKG / "_build" / "libpbf.so"
 
 # PBF_ABI_VERSION of the include/pbf.h this table mirrors; lib() refuses a
 # library built from another version (its entry points may take other arguments)
-ABI_VERSION = 8
+ABI_VERSION = 9
 
 PBF_OK = 0
 PBF_ERR_INVALID = -1
@@ -88,7 +88,8 @@ class pbf_profile(C.Structure):
 
 
 class pbf_tuning(C.Structure):
-    _fields_ = [("pass_block", C.c_int), ("stage_min_particles", C.c_int), ("persistent", C.c_int)]
+    _fields_ = [("pass_block", C.c_int), ("stage_min_particles", C.c_int), ("persistent", C.c_int),
+                ("pass_lanes", C.c_int)]
 
 
 class pbf_tune_report(C.Structure):

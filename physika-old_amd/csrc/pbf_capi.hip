@@ -176,6 +176,7 @@ struct pbf_ctx {
 
   int pass_block = kBlock;  // threads per workgroup of the PBD passes (pbf_tuning)
   bool pass_persistent = false;  // persistent launches of the PBD passes (pbf_tuning)
+  int pass_lanes = 0;            // lanes per particle in the PBD passes, 0 automatic (pbf_tuning)
 
   void* noalloc_scratch = nullptr;  // this device's copy of `scratch` (null: lookup failed, noalloc_err says why)
   std::string noalloc_err;
@@ -886,6 +887,15 @@ int ensure_box(pbf_ctx* c, hipStream_t st) {
   return PBF_OK;
 }
 
+// Lanes per particle of the PBD passes for a search over n particles: the
+// context's setting, or automatically a lane pair up to
+// PBF_TWO_LANE_MAX_PARTICLES (fewer waves than SIMDs to fill otherwise) and
+// one lane above, or with persistent launches (which have no lane-pair form).
+int pass_lanes_for(const pbf_ctx* c, int n) {
+  if (c->pass_lanes != 0) return c->pass_lanes;
+  return (!c->pass_persistent && n <= PBF_TWO_LANE_MAX_PARTICLES) ? 2 : 1;
+}
+
 // One fused step on a batch of scenes (scene_off: device offsets): the state
 // (x_in, v_in) advances into (x_out, v_out), which may alias the inputs. With
 // a tape (P: K iterate arrays, P[0] = the integrated positions; aux: K arrays)
@@ -922,6 +932,7 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
   s = search(c, in, n, scene_off, nscenes, true, st);
   if (s) return s;
   NbrView nv = view_of(c, n);
+  const int lanes = pass_lanes_for(c, n);
   float4* bufs[2] = {c->xa, c->xb};
   if (K == 0 && !visc) launch_write_back(c->x0, c->vs, c->perm, n, x_out, v_out, st);
   for (int it = 0; it < K; ++it) {
@@ -931,18 +942,18 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
     {
       PassTimer t(c, PBF_PASS_DENSITY_LAMBDA, st, true);
       launch_pass_density_lambda(nv, src, c->pc, last ? d_rho : nullptr, c->perm, st, aux ? aux[it] : nullptr,
-                                 c->pass_block, c->pass_persistent, t.events());
+                                 c->pass_block, c->pass_persistent, lanes, t.events());
     }
     {
       PassTimer t(c, PBF_PASS_DISPLACEMENT, st, true);
       if (last && to) {
         launch_pass_displacement(nv, src, dst, c->pc, true, c->x0, c->vs, c->perm, nullptr, nullptr, st, c->vv,
-                                 c->pid, c->pass_block, c->pass_persistent, t.events());
+                                 c->pid, c->pass_block, c->pass_persistent, lanes, t.events());
         *to = CellState{dst, c->vv, c->pid};
       } else {
         launch_pass_displacement(nv, src, dst, c->pc, last, c->x0, c->vs, c->perm, x_out, v_out, st,
                                  (last && visc) ? c->vv : nullptr, nullptr, c->pass_block, c->pass_persistent,
-                                 t.events());
+                                 lanes, t.events());
       }
     }
   }
@@ -1882,11 +1893,15 @@ int pbf_set_tuning(pbf_ctx* c, const pbf_tuning* t) {
   if (t->pass_block != 64 && t->pass_block != 128 && t->pass_block != 256 && t->pass_block != 512)
     return fail(c, PBF_ERR_INVALID, "pass_block must be 64, 128, 256 or 512");
   if (t->persistent != 0 && t->persistent != 1) return fail(c, PBF_ERR_INVALID, "persistent must be 0 or 1");
+  if (t->pass_lanes < 0 || t->pass_lanes > 2) return fail(c, PBF_ERR_INVALID, "pass_lanes must be 0, 1 or 2");
+  if (t->pass_lanes == 2 && t->persistent != 0)
+    return fail(c, PBF_ERR_UNSUPPORTED, "pass_lanes 2 has no persistent form");
   if (t->pass_block != c->pass_block || t->stage_min_particles != c->p.stage_min_particles ||
-      (t->persistent != 0) != c->pass_persistent) {
+      (t->persistent != 0) != c->pass_persistent || t->pass_lanes != c->pass_lanes) {
     c->pass_block = t->pass_block;
     c->p.stage_min_particles = t->stage_min_particles;
     c->pass_persistent = t->persistent != 0;
+    c->pass_lanes = t->pass_lanes;
     c->gen++;  // a captured step holds the launch shapes and the staging choice
   }
   return PBF_OK;
@@ -1897,6 +1912,7 @@ int pbf_get_tuning(const pbf_ctx* c, pbf_tuning* out) {
   out->pass_block = c->pass_block;
   out->stage_min_particles = c->p.stage_min_particles;
   out->persistent = c->pass_persistent ? 1 : 0;
+  out->pass_lanes = c->pass_lanes;
   return PBF_OK;
 }
 

@@ -243,11 +243,13 @@ void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* 
 // counters[C_PAIRS], [C_CANDIDATES], [C_LIST_SLOTS] and [C_LIST_NEED].
 void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* counters, hipStream_t st);
 // block: threads per workgroup, 64, 128, 256 or 512; persistent: as many
-// blocks as stay resident, each wave walking 64-particle tiles (the results
-// depend on neither; each wave of a block has its own stage)
+// blocks as stay resident, each wave walking 64-particle tiles; lanes: 1, or 2
+// for a lane pair per particle (not persistent). The results depend on none of
+// them; each wave of a block has its own stage.
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc,
                                 float* rho_out, const int* perm, hipStream_t st, float4* aux = nullptr,
-                                int block = kBlock, bool persistent = false, const KernelEvents* ev = nullptr);
+                                int block = kBlock, bool persistent = false, int lanes = 1,
+                                const KernelEvents* ev = nullptr);
 // last: write x/v to original order, or with v_sorted the positions to xd and
 // the velocities to v_sorted (sorted order, for the viscosity passes or the
 // next step of the call; id_out then receives each particle's original index).
@@ -255,7 +257,7 @@ void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, c
                               bool last, const float4* x0, const float4* vs, const int* perm,
                               float* x_out, float* v_out, hipStream_t st, float4* v_sorted = nullptr,
                               int* id_out = nullptr, int block = kBlock, bool persistent = false,
-                              const KernelEvents* ev = nullptr);
+                              int lanes = 1, const KernelEvents* ev = nullptr);
 void launch_write_back(const float4* xs, const float4* vs, const int* perm, int n, float* x_out,
                        float* v_out, hipStream_t st);
 void launch_invert(const int* perm, int* inv, int n, hipStream_t st);

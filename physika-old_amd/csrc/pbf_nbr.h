@@ -490,6 +490,26 @@ __device__ __forceinline__ void for_each_neighbor_x(const NbrView& nv, int s, co
   for_each_neighbor<kPadded>(nv, s, [&](int j) { return load_pos<kW>(xs, j); }, body);
 }
 
+// A lane of a lane pair takes every other entry of a 16-bit list: halfword
+// hh of each 32-bit word of a 16-byte chunk, i.e. entries 8c + 2u + hh (u =
+// 0..3) of chunk c, decoded into byte addresses (entry * 16 + base; sh =
+// 16 hh). Entry k = 4c + u of the lane at or past cnt (the lane's count)
+// becomes past.
+template <bool kFull>
+__device__ __forceinline__ Group4 decode_half_bytes(uint4 e, uint32_t sh, uint32_t base, int c, int cnt, int past) {
+  const int k = c << 2;
+  Group4 g;
+  g.j[0] = entry_bytes<false>(e.x >> sh, base);
+  g.j[1] = entry_bytes<false>(e.y >> sh, base);
+  g.j[2] = entry_bytes<false>(e.z >> sh, base);
+  g.j[3] = entry_bytes<false>(e.w >> sh, base);
+  if (!kFull) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g.j[u] = k + u < cnt ? g.j[u] : past;
+  }
+  return g;
+}
+
 // LDS records a wave needs for for_each_neighbor_staged.
 constexpr int kStageRecords = kStageCap + 1;
 
@@ -556,6 +576,78 @@ __device__ __forceinline__ void for_each_neighbor_staged(const NbrView& nv, int 
       [&](uint2 e, int b, auto full) { return decode16_bytes<decltype(full)::value>(e, jb16, b, cb, past); },
       [&](int o) { return load_pos_bytes<kW>(xs, (uint32_t)o); }, body);
   flip((ca & 1) != 0);
+}
+
+// for_each_neighbor_staged for a lane pair: lane h (0 or 1) of the pair that
+// holds particle s calls body(x_j) for the entries of s's list whose index in
+// the canonical order has parity h, in that order (the one-lane passes' even
+// and odd partial sums, one per lane). Entries past a lane's count and the
+// stage pads add exact zeros (the padding particle). buf: this hardware wave's
+// own stage region (both waves of a list copy its stage). Returns false, with
+// nothing run, when s's wave has no 16-bit or staged list; the caller then
+// runs the list on one lane of the pair.
+template <bool kW, class Body>
+__device__ __forceinline__ bool for_each_neighbor_half(const NbrView& nv, int s, int h, const float4* __restrict__ xs,
+                                                       float4* buf, Body&& body) {
+  const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
+  const WaveList wl = wave_list(nv, wave);
+  const int mode = wl.lm & 3;
+  if (wl.lb < 0 || (mode != kListDelta16 && mode != kListStaged)) return false;
+  const uint4* L = reinterpret_cast<const uint4*>(nv.list + wl.lb) + lane;
+  const int cnt = nv.nbr_cnt[s];
+  const uint32_t jb16 = (uint32_t)wl.jb << 4;
+  const int past = (int)((uint32_t)nv.pad << 4);
+  auto each = [&](int, const float4& xj, auto) { body(xj); };
+  auto gather = [&](int o) { return load_pos_bytes<kW>(xs, (uint32_t)o); };
+  // groups of the lane that every lane of the wave fills: half the list's
+  // (a lane's count is at least half its particle's, rounded down)
+  const int nfull = (wl.lm >> 2) >> 1;
+  if (mode == kListDelta16) {
+    const int ch = (cnt - h + 1) >> 1;
+    const uint32_t sh = (uint32_t)h << 4;
+    list_loop<true, uint4>(
+        ch, nfull, [&](int c) { return L[(long long)c << 6]; },
+        [&](uint4 e, int c, auto full) {
+          return decode_half_bytes<decltype(full)::value>(e, sh, jb16, c, ch, past);
+        },
+        gather, each);
+    return true;
+  }
+  const int* st = nv.stage + kStageInts * wave;
+  const int ca = nv.cnt_a[s], cb = cnt - ca;  // loads issued before the stage copy
+  const int hl = (int)(threadIdx.x & 63);    // lane of this hardware wave
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const int a = st[q], p0 = st[3 + q], len = st[4 + q] - p0;
+    for (int t = hl; t < len; t += 64) buf[p0 + t] = xs[a + t];
+  }
+  if (hl == 0) buf[kStagePad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
+  // LDS operations of one wave complete in order: keep the reads below after
+  // the writes above
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_wave_barrier();
+  // stage part: ka entries (a multiple of 8, real ones first, then pads), so
+  // every chunk is full and the lane's entries are ka / 2
+  const int ka = st[7];
+  const uint32_t lds = __builtin_amdgcn_readfirstlane(lds_address(buf));
+  const uint32_t sh1 = (uint32_t)h << 4;
+  list_loop<true, uint4>(
+      ka >> 1, ka >> 3, [&](int c) { return L[(long long)c << 6]; },
+      [&](uint4 e, int c, auto) { return decode_half_bytes<true>(e, sh1, lds, c, ka >> 1, 0); },
+      [&](int a) { return lds_load_pos<kW>((uint32_t)a); }, each);
+  // offsets part: its entry p has index ca + p, so the lane takes p of parity
+  // h ^ (ca & 1)
+  const int hh = h ^ (ca & 1);
+  const int ch = (cb - hh + 1) >> 1;
+  const uint32_t sh2 = (uint32_t)hh << 4;
+  const uint4* L2 = L + ((long long)(ka >> 3) << 6);
+  list_loop<true, uint4>(
+      ch, nfull, [&](int c) { return L2[(long long)c << 6]; },
+      [&](uint4 e, int c, auto full) {
+        return decode_half_bytes<decltype(full)::value>(e, sh2, jb16, c, ch, past);
+      },
+      gather, each);
+  return true;
 }
 
 }  // namespace pbf

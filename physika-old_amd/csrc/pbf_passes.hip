@@ -430,17 +430,92 @@ void launch_wave_stats(const int4* wave_stat, int nwaves, unsigned long long* co
 // ---------------------------------------------------------------------------
 // PBD passes (DensityPBD::takeOneIteration, DensityPBD.cu:306-365).
 
-// Exchanges a[0] and a[1] when c holds (per lane: two selects, no branch).
-__device__ __forceinline__ void swap_if(bool c, float (&a)[2]) {
-  const float x = c ? a[1] : a[0], y = c ? a[0] : a[1];
-  a[0] = x;
-  a[1] = y;
-}
-
 // Pass A: K_ComputeDensity (SummationDensity.cu:13-36) and K_ComputeLambdas
 // (DensityPBD.cu:46-90) fused: one traversal gives rho_i, sum g and sum g.g;
 // lambda_i goes to xs[s].w for pass B. aux (taped steps only) receives
 // (sum_j t^2/r d_ij, 1/(S + eps)) for the adjoint.
+
+// Running sums of pass A for one particle (or one entry parity of it).
+struct DensitySums {
+  float rho = 0.f, sgx = 0.f, sgy = 0.f, sgz = 0.f, sgg = 0.f;
+};
+
+// Adds neighbor xj's terms to a.
+__device__ __forceinline__ void density_term(DensitySums& a, float4 xi, float4 xj, const PhysConst& pc) {
+#pragma clang fp contract(off)
+  const PairGeom p = pair_geom(xi, xj, pc.inv_h);
+  const float t2 = p.t * p.t;
+  a.rho = fmaf(t2, p.t, a.rho);
+  // |g| / |cG| = t^2 (|x_i - x_j| / r = 1); no gradient closer than pair_eps
+  const float tg = p.r > pc.pair_eps ? t2 : 0.0f;
+  const float gq = tg * p.rinv;
+  a.sgx = fmaf(gq, p.dx, a.sgx);
+  a.sgy = fmaf(gq, p.dy, a.sgy);
+  a.sgz = fmaf(gq, p.dz, a.sgz);
+  a.sgg = fmaf(tg, tg, a.sgg);
+}
+
+// The sums of the even and the odd entries added, the one order every path uses.
+__device__ __forceinline__ DensitySums add_parities(const DensitySums& e, const DensitySums& o) {
+  DensitySums t;
+  t.rho = e.rho + o.rho;
+  t.sgx = e.sgx + o.sgx;
+  t.sgy = e.sgy + o.sgy;
+  t.sgz = e.sgz + o.sgz;
+  t.sgg = e.sgg + o.sgg;
+  return t;
+}
+
+// Exchanges a and b when c holds (per lane: selects, no branch).
+__device__ __forceinline__ void swap_if(bool c, float& a, float& b) {
+  const float x = c ? b : a, y = c ? a : b;
+  a = x;
+  b = y;
+}
+__device__ __forceinline__ void swap_if(bool c, DensitySums& a, DensitySums& b) {
+  swap_if(c, a.rho, b.rho);
+  swap_if(c, a.sgx, b.sgx);
+  swap_if(c, a.sgy, b.sgy);
+  swap_if(c, a.sgz, b.sgz);
+  swap_if(c, a.sgg, b.sgg);
+}
+
+// The odd lane's sums of a lane pair, read by both lanes.
+__device__ __forceinline__ DensitySums partner_sums(const DensitySums& a) {
+  DensitySums o;
+  o.rho = __shfl_xor(a.rho, 1);
+  o.sgx = __shfl_xor(a.sgx, 1);
+  o.sgy = __shfl_xor(a.sgy, 1);
+  o.sgz = __shfl_xor(a.sgz, 1);
+  o.sgg = __shfl_xor(a.sgg, 1);
+  return o;
+}
+
+// lambda_i from the particle's total sums, into xs[s].w; rho_i and the
+// adjoint's sums when asked for.
+__device__ __forceinline__ void density_lambda_finish(int s, float4* __restrict__ xs, const PhysConst& pc,
+                                                      float* __restrict__ rho_out, const int* __restrict__ perm,
+                                                      float4* __restrict__ aux, const DensitySums& t) {
+  // nothing contracted, so every kernel that inlines this rounds it alike
+#pragma clang fp contract(off)
+  const float rho = t.rho * pc.cW;
+  const float S = (pc.cG * pc.cG) * (t.sgg + (t.sgx * t.sgx + t.sgy * t.sgy + t.sgz * t.sgz));
+  float lam = -(rho - pc.lam_rest) / (S + pc.lam_eps);
+  lam = lam > 0.0f ? 0.0f : lam;
+  reinterpret_cast<float*>(xs + s)[3] = lam;
+  if (rho_out) rho_out[perm[s]] = rho;
+  if (aux) aux[s] = make_float4(t.sgx, t.sgy, t.sgz, 1.0f / (S + pc.lam_eps));
+}
+
+// The whole list of s on one lane: one partial sum per entry parity.
+template <class Traverse>
+__device__ __forceinline__ DensitySums density_one_lane(float4 xi, const PhysConst& pc, Traverse&& traverse) {
+  DensitySums a[2];
+  traverse([&](int, float4 xj, auto par) { density_term(a[decltype(par)::value], xi, xj, pc); },
+           [&](bool odd) { swap_if(odd, a[0], a[1]); });
+  return add_parities(a[0], a[1]);
+}
+
 // One particle s of pass A (stage: the wave's LDS region).
 __device__ __forceinline__ void density_lambda_at(const NbrView& nv, int s, float4* __restrict__ xs,
                                                   const PhysConst& pc, float* __restrict__ rho_out,
@@ -448,41 +523,34 @@ __device__ __forceinline__ void density_lambda_at(const NbrView& nv, int s, floa
                                                   float4* stage) {
   if (s >= nv.n) return;
   const float4 xi = xs[s];
-  // one partial sum per entry parity, added at the end
-  float rho2[2] = {0.f, 0.f}, sgx2[2] = {0.f, 0.f}, sgy2[2] = {0.f, 0.f}, sgz2[2] = {0.f, 0.f},
-        sgg2[2] = {0.f, 0.f};
-  for_each_neighbor_staged<false>(
-      nv, s, xs, stage,
-      [&](int, float4 xj, auto par) {
-#pragma clang fp contract(off)
-        constexpr int e = decltype(par)::value;
-        const PairGeom p = pair_geom(xi, xj, pc.inv_h);
-        const float t2 = p.t * p.t;
-        rho2[e] = fmaf(t2, p.t, rho2[e]);
-        // |g| / |cG| = t^2 (|x_i - x_j| / r = 1); no gradient closer than pair_eps
-        const float tg = p.r > pc.pair_eps ? t2 : 0.0f;
-        const float gq = tg * p.rinv;
-        sgx2[e] = fmaf(gq, p.dx, sgx2[e]);
-        sgy2[e] = fmaf(gq, p.dy, sgy2[e]);
-        sgz2[e] = fmaf(gq, p.dz, sgz2[e]);
-        sgg2[e] = fmaf(tg, tg, sgg2[e]);
-      },
-      [&](bool odd) {
-        swap_if(odd, rho2);
-        swap_if(odd, sgx2);
-        swap_if(odd, sgy2);
-        swap_if(odd, sgz2);
-        swap_if(odd, sgg2);
-      });
-  float rho = rho2[0] + rho2[1];
-  const float sgx = sgx2[0] + sgx2[1], sgy = sgy2[0] + sgy2[1], sgz = sgz2[0] + sgz2[1], sgg = sgg2[0] + sgg2[1];
-  rho *= pc.cW;
-  const float S = (pc.cG * pc.cG) * (sgg + (sgx * sgx + sgy * sgy + sgz * sgz));
-  float lam = -(rho - pc.lam_rest) / (S + pc.lam_eps);
-  lam = lam > 0.0f ? 0.0f : lam;
-  reinterpret_cast<float*>(xs + s)[3] = lam;
-  if (rho_out) rho_out[perm[s]] = rho;
-  if (aux) aux[s] = make_float4(sgx, sgy, sgz, 1.0f / (S + pc.lam_eps));
+  const DensitySums t = density_one_lane(xi, pc, [&](auto&& body, auto&& flip) {
+    for_each_neighbor_staged<false>(nv, s, xs, stage, body, flip);
+  });
+  density_lambda_finish(s, xs, pc, rho_out, perm, aux, t);
+}
+
+// Pass A with a lane pair per particle (lane h = 0 the even entries, 1 the
+// odd ones; stage: the hardware wave's LDS region). Same results as
+// density_lambda_at, bit for bit.
+__device__ __forceinline__ void density_lambda_at2(const NbrView& nv, int s, int h, float4* __restrict__ xs,
+                                                   const PhysConst& pc, float* __restrict__ rho_out,
+                                                   const int* __restrict__ perm, float4* __restrict__ aux,
+                                                   float4* stage) {
+  if (s >= nv.n) return;  // both lanes of the pair
+  const float4 xi = xs[s];
+  DensitySums a;
+  if (for_each_neighbor_half<false>(nv, s, h, xs, stage, [&](float4 xj) { density_term(a, xi, xj, pc); })) {
+    const DensitySums o = partner_sums(a);
+    if (h == 0) density_lambda_finish(s, xs, pc, rho_out, perm, aux, add_parities(a, o));
+    return;
+  }
+  if (h != 0) return;
+  // neither a 16-bit nor a staged list: the even lane takes the whole list
+  const DensitySums t = density_one_lane(xi, pc, [&](auto&& body, auto&& flip) {
+    for_each_neighbor_p<true>(nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)),
+                              [&](int j) { return load_pos<false>(xs, j); }, body, flip);
+  });
+  density_lambda_finish(s, xs, pc, rho_out, perm, aux, t);
 }
 
 template <int B>
@@ -492,6 +560,17 @@ __global__ __launch_bounds__(B) void k_density_lambda(NbrView nv, float4* __rest
   __shared__ float4 stage_buf[B / 64][kStageRecords];
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   density_lambda_at(nv, s, xs, pc, rho_out, perm, aux, stage_buf[threadIdx.x >> 6]);
+}
+
+// Lane-pair form: thread g of the grid takes entry parity g & 1 of particle
+// g >> 1, so a wave holds 32 particles of one 64-particle list wave.
+template <int B>
+__global__ __launch_bounds__(B) void k_density_lambda2(NbrView nv, float4* __restrict__ xs, PhysConst pc,
+                                                       float* __restrict__ rho_out, const int* __restrict__ perm,
+                                                       float4* __restrict__ aux) {
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
+  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  density_lambda_at2(nv, g >> 1, g & 1, xs, pc, rho_out, perm, aux, stage_buf[threadIdx.x >> 6]);
 }
 
 // Persistent form: as many blocks as stay resident; each wave walks the
@@ -548,8 +627,10 @@ void launch_timed(K kernel, int grid, int block, hipStream_t st, const KernelEve
 
 template <int B>
 void density_lambda_as(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                       hipStream_t st, float4* aux, bool persistent, const KernelEvents* ev) {
-  if (persistent) {
+                       hipStream_t st, float4* aux, bool persistent, int lanes, const KernelEvents* ev) {
+  if (lanes == 2) {
+    launch_timed(k_density_lambda2<B>, (int)((2ll * nv.n + B - 1) / B), B, st, ev, nv, xs, pc, rho_out, perm, aux);
+  } else if (persistent) {
     static const int resident = resident_blocks(k_density_lambda_p<B>, B);  // one device type per process
     launch_timed(k_density_lambda_p<B>, persistent_grid(resident, B, nv.n), B, st, ev, nv, xs, pc, rho_out, perm,
                  aux);
@@ -559,12 +640,13 @@ void density_lambda_as(const NbrView& nv, float4* xs, const PhysConst& pc, float
 }
 
 void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& pc, float* rho_out, const int* perm,
-                                hipStream_t st, float4* aux, int block, bool persistent, const KernelEvents* ev) {
+                                hipStream_t st, float4* aux, int block, bool persistent, int lanes,
+                                const KernelEvents* ev) {
   switch (block) {
-    case 64: density_lambda_as<64>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
-    case 128: density_lambda_as<128>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
-    case 512: density_lambda_as<512>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
-    default: density_lambda_as<256>(nv, xs, pc, rho_out, perm, st, aux, persistent, ev); break;
+    case 64: density_lambda_as<64>(nv, xs, pc, rho_out, perm, st, aux, persistent, lanes, ev); break;
+    case 128: density_lambda_as<128>(nv, xs, pc, rho_out, perm, st, aux, persistent, lanes, ev); break;
+    case 512: density_lambda_as<512>(nv, xs, pc, rho_out, perm, st, aux, persistent, lanes, ev); break;
+    default: density_lambda_as<256>(nv, xs, pc, rho_out, perm, st, aux, persistent, lanes, ev); break;
   }
 }
 
@@ -577,41 +659,64 @@ void launch_pass_density_lambda(const NbrView& nv, float4* xs, const PhysConst& 
 // the scatter back to original order, or keeps x and v in cell order (with
 // each particle's original index in id_out) for the viscosity passes or the
 // next step of the call.
-__device__ __forceinline__ void displacement_at(const NbrView& nv, int s, const float4* __restrict__ xs,
-                                                float4* __restrict__ xd, const PhysConst& pc, int last,
-                                                const float4* __restrict__ x0, const float4* __restrict__ vs,
-                                                const int* __restrict__ perm, float* __restrict__ x_out,
-                                                float* __restrict__ v_out, float4* __restrict__ v_sorted,
-                                                int* __restrict__ id_out, float4* stage) {
-  if (s == 0 && (!last || v_sorted)) xd[nv.pad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
-  if (s >= nv.n) return;
-  const float4 xi = xs[s];
-  const float li = xi.w;
-  // sum over the listed j of (lambda_i + lambda_j) t^2 / r (x_i - x_j)
-  auto pair = [&](float4 xj, float& cx, float& cy, float& cz) {
+// Running sums of pass B for one particle (or one entry parity of it).
+struct DispSums {
+  float x = 0.f, y = 0.f, z = 0.f;
+};
+
+// Adds (lambda_i + lambda_j) t^2 / r (x_i - x_j) of neighbor xj to a.
+__device__ __forceinline__ void disp_term(DispSums& a, float4 xi, float4 xj, const PhysConst& pc) {
 #pragma clang fp contract(off)
-    const PairGeom p = pair_geom(xi, xj, pc.inv_h);
-    const float c = p.r > pc.pair_eps ? (li + xj.w) * (p.t * p.t) * p.rinv : 0.0f;
-    cx = fmaf(c, p.dx, cx);
-    cy = fmaf(c, p.dy, cy);
-    cz = fmaf(c, p.dz, cz);
-  };
-  // one partial sum per entry parity, added at the end
-  float ax2[2] = {0.f, 0.f}, ay2[2] = {0.f, 0.f}, az2[2] = {0.f, 0.f};
-  for_each_neighbor_staged<true>(
-      nv, s, xs, stage,
-      [&](int, float4 xj, auto par) {
-        constexpr int e = decltype(par)::value;
-        pair(xj, ax2[e], ay2[e], az2[e]);
-      },
-      [&](bool odd) {
-        swap_if(odd, ax2);
-        swap_if(odd, ay2);
-        swap_if(odd, az2);
-      });
-  const float ax = ax2[0] + ax2[1], ay = ay2[0] + ay2[1], az = az2[0] + az2[1];
+  const PairGeom p = pair_geom(xi, xj, pc.inv_h);
+  const float c = p.r > pc.pair_eps ? (xi.w + xj.w) * (p.t * p.t) * p.rinv : 0.0f;
+  a.x = fmaf(c, p.dx, a.x);
+  a.y = fmaf(c, p.dy, a.y);
+  a.z = fmaf(c, p.dz, a.z);
+}
+
+__device__ __forceinline__ DispSums add_parities(const DispSums& e, const DispSums& o) {
+  DispSums t;
+  t.x = e.x + o.x;
+  t.y = e.y + o.y;
+  t.z = e.z + o.z;
+  return t;
+}
+
+__device__ __forceinline__ void swap_if(bool c, DispSums& a, DispSums& b) {
+  swap_if(c, a.x, b.x);
+  swap_if(c, a.y, b.y);
+  swap_if(c, a.z, b.z);
+}
+
+__device__ __forceinline__ DispSums partner_sums(const DispSums& a) {
+  DispSums o;
+  o.x = __shfl_xor(a.x, 1);
+  o.y = __shfl_xor(a.y, 1);
+  o.z = __shfl_xor(a.z, 1);
+  return o;
+}
+
+template <class Traverse>
+__device__ __forceinline__ DispSums disp_one_lane(float4 xi, const PhysConst& pc, Traverse&& traverse) {
+  DispSums a[2];
+  traverse([&](int, float4 xj, auto par) { disp_term(a[decltype(par)::value], xi, xj, pc); },
+           [&](bool odd) { swap_if(odd, a[0], a[1]); });
+  return add_parities(a[0], a[1]);
+}
+
+// The rest of pass B for particle s from its list's total sums a: the pairs of
+// out-of-grid particles, the new position and, in the last iteration, the
+// velocity.
+__device__ __forceinline__ void displacement_finish(const NbrView& nv, int s, const float4* __restrict__ xs,
+                                                    float4* __restrict__ xd, const PhysConst& pc, int last,
+                                                    const float4* __restrict__ x0, const float4* __restrict__ vs,
+                                                    const int* __restrict__ perm, float* __restrict__ x_out,
+                                                    float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                    int* __restrict__ id_out, float4 xi, const DispSums& a) {
+  // nothing contracted, so every kernel that inlines this rounds it alike
+#pragma clang fp contract(off)
   float fa = 2.0f;
-  float bx = 0.f, by = 0.f, bz = 0.f;
+  DispSums b;  // pairs of out-of-grid particles that list s
   if (nv.counters[C_OOG] != 0) {  // some scene has out-of-grid particles
     const SceneWin win = nv.win[nv.ssid[s]];
     const int ob = win.cell_base + win.ncell;  // the scene's out-of-grid bucket
@@ -631,7 +736,7 @@ __device__ __forceinline__ void displacement_at(const NbrView& nv, int s, const 
             const int c = rv[w];
             k = (c > prev && c < k) ? c : k;
           }
-          pair(xs[k], bx, by, bz);
+          disp_term(b, xi, xs[k], pc);
           prev = k;
         }
       } else {  // listed by more than kOogRev: scan the scene's out-of-grid particles
@@ -644,15 +749,15 @@ __device__ __forceinline__ void displacement_at(const NbrView& nv, int s, const 
           cell_of(nv.g, xnk.x, xnk.y, xnk.z, ki, kj, kk);
           if (abs((ki >> 1) - (ci >> 1)) <= 1 && abs(kj - cj) <= 1 && abs(kk - ck) <= 1 &&
               within(xnk, xni, nv.thr2))
-            pair(xs[k], bx, by, bz);
+            disp_term(b, xi, xs[k], pc);
         }
       }
     }
   }
   const float scale = pc.corr * pc.cG;
-  const float nx = xi.x + scale * (fa * ax + bx);
-  const float ny = xi.y + scale * (fa * ay + by);
-  const float nz = xi.z + scale * (fa * az + bz);
+  const float nx = xi.x + scale * (fa * a.x + b.x);
+  const float ny = xi.y + scale * (fa * a.y + b.y);
+  const float nz = xi.z + scale * (fa * a.z + b.z);
   if (!last) {
     xd[s] = make_float4(nx, ny, nz, 0.0f);
     return;
@@ -676,6 +781,50 @@ __device__ __forceinline__ void displacement_at(const NbrView& nv, int s, const 
 }
 
 
+
+__device__ __forceinline__ void displacement_at(const NbrView& nv, int s, const float4* __restrict__ xs,
+                                                float4* __restrict__ xd, const PhysConst& pc, int last,
+                                                const float4* __restrict__ x0, const float4* __restrict__ vs,
+                                                const int* __restrict__ perm, float* __restrict__ x_out,
+                                                float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                int* __restrict__ id_out, float4* stage) {
+  if (s == 0 && (!last || v_sorted)) xd[nv.pad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
+  if (s >= nv.n) return;
+  const float4 xi = xs[s];
+  // sum over the listed j of (lambda_i + lambda_j) t^2 / r (x_i - x_j)
+  const DispSums a = disp_one_lane(xi, pc, [&](auto&& body, auto&& flip) {
+    for_each_neighbor_staged<true>(nv, s, xs, stage, body, flip);
+  });
+  displacement_finish(nv, s, xs, xd, pc, last, x0, vs, perm, x_out, v_out, v_sorted, id_out, xi, a);
+}
+
+// Pass B with a lane pair per particle, as density_lambda_at2.
+__device__ __forceinline__ void displacement_at2(const NbrView& nv, int s, int h, const float4* __restrict__ xs,
+                                                 float4* __restrict__ xd, const PhysConst& pc, int last,
+                                                 const float4* __restrict__ x0, const float4* __restrict__ vs,
+                                                 const int* __restrict__ perm, float* __restrict__ x_out,
+                                                 float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                 int* __restrict__ id_out, float4* stage) {
+  if (s == 0 && h == 0 && (!last || v_sorted)) xd[nv.pad] = make_float4(kPadCoord, kPadCoord, kPadCoord, 0.0f);
+  if (s >= nv.n) return;  // both lanes of the pair
+  const float4 xi = xs[s];
+  DispSums a;
+  if (for_each_neighbor_half<true>(nv, s, h, xs, stage, [&](float4 xj) { disp_term(a, xi, xj, pc); })) {
+    const DispSums o = partner_sums(a);
+    if (h == 0)
+      displacement_finish(nv, s, xs, xd, pc, last, x0, vs, perm, x_out, v_out, v_sorted, id_out, xi,
+                          add_parities(a, o));
+    return;
+  }
+  if (h != 0) return;
+  // neither a 16-bit nor a staged list: the even lane takes the whole list
+  const DispSums t = disp_one_lane(xi, pc, [&](auto&& body, auto&& flip) {
+    for_each_neighbor_p<true>(nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)),
+                              [&](int j) { return load_pos<true>(xs, j); }, body, flip);
+  });
+  displacement_finish(nv, s, xs, xd, pc, last, x0, vs, perm, x_out, v_out, v_sorted, id_out, xi, t);
+}
+
 template <int B>
 __global__ __launch_bounds__(B) void k_displacement(NbrView nv, const float4* __restrict__ xs,
                                                     float4* __restrict__ xd, PhysConst pc, int last,
@@ -687,6 +836,19 @@ __global__ __launch_bounds__(B) void k_displacement(NbrView nv, const float4* __
   const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   displacement_at(nv, s, xs, xd, pc, last, x0, vs, perm, x_out, v_out, v_sorted, id_out,
                   stage_buf[threadIdx.x >> 6]);
+}
+
+template <int B>
+__global__ __launch_bounds__(B) void k_displacement2(NbrView nv, const float4* __restrict__ xs,
+                                                     float4* __restrict__ xd, PhysConst pc, int last,
+                                                     const float4* __restrict__ x0, const float4* __restrict__ vs,
+                                                     const int* __restrict__ perm, float* __restrict__ x_out,
+                                                     float* __restrict__ v_out, float4* __restrict__ v_sorted,
+                                                     int* __restrict__ id_out) {
+  __shared__ float4 stage_buf[B / 64][kStageRecords];
+  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  displacement_at2(nv, g >> 1, g & 1, xs, xd, pc, last, x0, vs, perm, x_out, v_out, v_sorted, id_out,
+                   stage_buf[threadIdx.x >> 6]);
 }
 
 template <int B>
@@ -709,9 +871,12 @@ __global__ __launch_bounds__(B) void k_displacement_p(NbrView nv, const float4* 
 template <int B>
 void displacement_as(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                      const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out, hipStream_t st,
-                     float4* v_sorted, int* id_out, bool persistent, const KernelEvents* ev) {
+                     float4* v_sorted, int* id_out, bool persistent, int lanes, const KernelEvents* ev) {
   const int l = last ? 1 : 0;
-  if (persistent) {
+  if (lanes == 2) {
+    launch_timed(k_displacement2<B>, (int)((2ll * nv.n + B - 1) / B), B, st, ev, nv, xs, xd, pc, l, x0, vs, perm,
+                 x_out, v_out, v_sorted, id_out);
+  } else if (persistent) {
     static const int resident = resident_blocks(k_displacement_p<B>, B);  // one device type per process
     launch_timed(k_displacement_p<B>, persistent_grid(resident, B, nv.n), B, st, ev, nv, xs, xd, pc, l, x0, vs,
                  perm, x_out, v_out, v_sorted, id_out);
@@ -724,9 +889,9 @@ void displacement_as(const NbrView& nv, const float4* xs, float4* xd, const Phys
 void launch_pass_displacement(const NbrView& nv, const float4* xs, float4* xd, const PhysConst& pc, bool last,
                               const float4* x0, const float4* vs, const int* perm, float* x_out, float* v_out,
                               hipStream_t st, float4* v_sorted, int* id_out, int block, bool persistent,
-                              const KernelEvents* ev) {
+                              int lanes, const KernelEvents* ev) {
 #define PBF_DISP(b) \
-  displacement_as<b>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out, persistent, ev)
+  displacement_as<b>(nv, xs, xd, pc, last, x0, vs, perm, x_out, v_out, st, v_sorted, id_out, persistent, lanes, ev)
   switch (block) {
     case 64: PBF_DISP(64); break;
     case 128: PBF_DISP(128); break;

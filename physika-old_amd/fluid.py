@@ -154,10 +154,11 @@ class Context:
         t = _capi.pbf_tuning()
         _capi.check(self._lib.pbf_get_tuning(self._ctx, C.byref(t)), self._ctx)
         return {"pass_block": t.pass_block, "stage_min_particles": t.stage_min_particles,
-                "persistent": t.persistent}
+                "persistent": t.persistent, "pass_lanes": t.pass_lanes}
 
-    def set_tuning(self, pass_block: int = 256, stage_min_particles: int = 0, persistent: int = 0) -> None:
-        t = _capi.pbf_tuning(int(pass_block), int(stage_min_particles), int(persistent))
+    def set_tuning(self, pass_block: int = 256, stage_min_particles: int = 0, persistent: int = 0,
+                   pass_lanes: int = 0) -> None:
+        t = _capi.pbf_tuning(int(pass_block), int(stage_min_particles), int(persistent), int(pass_lanes))
         _capi.check(self._lib.pbf_set_tuning(self._ctx, C.byref(t)), self._ctx)
 
     def autotune(self, x: torch.Tensor, v: torch.Tensor, scene_offsets: Optional[torch.Tensor] = None,
@@ -177,7 +178,8 @@ class Context:
                                                int(steps), None if cache is None else str(cache).encode(),
                                                C.byref(r), _stream(x.device)), self._ctx)
         return {"pass_block": r.best.pass_block, "stage_min_particles": r.best.stage_min_particles,
-                "persistent": r.best.persistent, "best_ms": r.best_ms, "default_ms": r.default_ms, "candidates": r.candidates,
+                "persistent": r.best.persistent, "pass_lanes": r.best.pass_lanes, "best_ms": r.best_ms,
+                "default_ms": r.default_ms, "candidates": r.candidates,
                 "from_cache": bool(r.from_cache), "key": r.key.decode()}
 
     def profile(self) -> dict:

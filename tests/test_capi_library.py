@@ -79,7 +79,7 @@ def test_binding_argument_lists_match_header(capi):
 def test_host_only_calls(capi):
     """Calls that touch no device: version, status strings, defaults."""
     L = capi.lib()
-    assert L.pbf_abi_version() == capi.ABI_VERSION == 8
+    assert L.pbf_abi_version() == capi.ABI_VERSION == 9
     assert L.pbf_status_string(0) == b"ok"
     assert L.pbf_status_string(-3) == b"buffer capacity exceeded"
     p = capi.default_params()

@@ -1,11 +1,12 @@
 """Launch settings (pbf_set_tuning) and the autotuner (pbf_autotune).
 
 Bars: every setting (PBD-pass workgroups of 64, 128, 256 and 512 threads,
-staged lists always or never, ordinary or persistent launches) gives
-bit-identical positions, velocities and density to the default setting, on one scene, on a staged lattice and on a
-batch; the autotuner leaves the caller's state unchanged, applies the
-setting it reports, stores it in its JSON cache and reads it back from there.
-Every call goes through the C ABI."""
+staged lists always or never, ordinary or persistent launches, one lane or a
+lane pair per particle) gives bit-identical positions, velocities and density
+to the default setting, on one scene, on a staged lattice and on a batch; the
+autotuner leaves the caller's state unchanged, applies the setting it reports,
+stores it in its JSON cache and reads it back from there. Every call goes
+through the C ABI."""
 import json
 
 import numpy as np
@@ -18,6 +19,8 @@ pytestmark = pytest.mark.gpu
 from parity_util import dev, say  # noqa: E402
 
 SETTINGS = [(b, s, p) for p in (0, 1) for b in (64, 128, 256, 512) for s in (1, -1)]
+# one lane and a lane pair per particle, forced (the default picks by size)
+LANE_SETTINGS = [(b, s, 0, lanes) for lanes in (1, 2) for b in (64, 256) for s in (1, -1)]
 
 
 @pytest.fixture(scope="module", autouse=True)
@@ -43,12 +46,12 @@ def run(ctx, x0, v0, nsteps, off=None):
 def assert_settings_identical(fluid, x0, v0, nsteps, off=None, **params):
     ctx = fluid.Context("cuda:0", **params)
     want = run(ctx, x0, v0, nsteps, off)
-    for b, s, p in SETTINGS:
-        ctx.set_tuning(b, s, p)
+    for setting in [t + (0,) for t in SETTINGS] + LANE_SETTINGS:
+        ctx.set_tuning(*setting)
         got = run(ctx, x0, v0, nsteps, off)
         for name, g, w in zip(("x", "v", "rho"), got, want):
-            np.testing.assert_array_equal(g, w, err_msg=f"pass_block {b}, stage_min_particles {s}, persistent {p}: "
-                                                        f"{name}")
+            np.testing.assert_array_equal(g, w, err_msg=f"pass_block, stage_min_particles, persistent, pass_lanes "
+                                                        f"{setting}: {name}")
     return ctx
 
 
@@ -85,7 +88,7 @@ def test_autotune_and_cache(fluid, scene, tmp_path):
     assert (r["pass_block"], r["stage_min_particles"], r["persistent"]) in SETTINGS + [(256, 0, 0)]
     assert 0 < r["best_ms"] <= r["default_ms"]
     assert ctx.tuning() == {"pass_block": r["pass_block"], "stage_min_particles": r["stage_min_particles"],
-                            "persistent": r["persistent"]}
+                            "persistent": r["persistent"], "pass_lanes": 0}
     np.testing.assert_array_equal(bits(x), x0.view(np.uint32))  # the caller's state is not touched
     np.testing.assert_array_equal(bits(v), np.zeros_like(x0).view(np.uint32))
     stored = json.loads(cache.read_text())
@@ -114,4 +117,8 @@ def test_invalid_tuning_rejected(fluid, capi):
         ctx.set_tuning(96, 0)
     with pytest.raises(capi.PBFError):
         ctx.set_tuning(256, 0, 2)
-    assert ctx.tuning() == {"pass_block": 256, "stage_min_particles": 0, "persistent": 0}
+    with pytest.raises(capi.PBFError):
+        ctx.set_tuning(256, 0, 0, 3)
+    with pytest.raises(capi.PBFError):  # the lane pair has no persistent form
+        ctx.set_tuning(256, 0, 1, 2)
+    assert ctx.tuning() == {"pass_block": 256, "stage_min_particles": 0, "persistent": 0, "pass_lanes": 0}
