@@ -673,8 +673,7 @@ def main():
     kb, kf = kernel_counts(kname, n, pairs)
     roofline = roofline_of(kb, kf, t_dom)
     tune = ctx.tuning()
-    # the lane-pair form of a pass (pass_lanes 2, or automatic up to 131,072 particles without persistent launches)
-    pair_lanes = tune["pass_lanes"] == 2 or (tune["pass_lanes"] == 0 and not tune["persistent"] and n <= 131072)
+    pair_lanes = ctx.stats()["pass_lanes"] == 2  # the lane-pair form of the passes ran
     traffic, traffic_kernel = profiled_traffic(kname + ("2" if pair_lanes and kname != "k_neighbors" else ""),
                                                tune["pass_block"] if kname != "k_neighbors" else None,
                                                bool(tune["persistent"]) and kname != "k_neighbors")

@@ -138,6 +138,7 @@ typedef struct pbf_stats {
   float neighbor_threshold;      /* squared-distance bound T: sqrtf(s) < h  <=>  s < T */
   long long graph_captures;      /* hipGraph captures of a multi-step call's middle step (context lifetime) */
   long long graph_replays;       /* middle steps replayed from that graph (context lifetime) */
+  int pass_lanes;                /* lanes per particle of the last step's PBD passes (pbf_tuning) */
 } pbf_stats;
 
 typedef struct pbf_ctx pbf_ctx;
@@ -359,12 +360,13 @@ typedef struct pbf_tuning {
                               pair splits each neighbor list by the parity of
                               the entries' order and adds the two sums, twice
                               the waves for small searches; not with
-                              persistent); 0 (default): 2 for searches over at
-                              most PBF_TWO_LANE_MAX_PARTICLES particles without
-                              persistent launches, else 1 */
+                              persistent); 0 (default): 2 without persistent
+                              launches when the search keeps unstaged lists or
+                              covers at most PBF_TWO_LANE_MAX_PARTICLES
+                              particles, else 1 */
 } pbf_tuning;
 
-#define PBF_TWO_LANE_MAX_PARTICLES 131072
+#define PBF_TWO_LANE_MAX_PARTICLES 196608
 
 int pbf_set_tuning(pbf_ctx* ctx, const pbf_tuning* t);
 int pbf_get_tuning(const pbf_ctx* ctx, pbf_tuning* out);

@@ -77,6 +77,7 @@ class pbf_stats(C.Structure):
         ("neighbor_threshold", C.c_float),
         ("graph_captures", C.c_longlong),
         ("graph_replays", C.c_longlong),
+        ("pass_lanes", C.c_int),
     ]
 
 

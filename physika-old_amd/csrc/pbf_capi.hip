@@ -177,6 +177,7 @@ struct pbf_ctx {
   int pass_block = kBlock;  // threads per workgroup of the PBD passes (pbf_tuning)
   bool pass_persistent = false;  // persistent launches of the PBD passes (pbf_tuning)
   int pass_lanes = 0;            // lanes per particle in the PBD passes, 0 automatic (pbf_tuning)
+  int last_lanes = 0;            // lanes per particle of the last step's PBD passes
 
   void* noalloc_scratch = nullptr;  // this device's copy of `scratch` (null: lookup failed, noalloc_err says why)
   std::string noalloc_err;
@@ -493,6 +494,13 @@ long long list_slots_for(const pbf_ctx* c, long long n) {
 }
 
 // 16-bit list slots per particle of every wave's fixed list region.
+// Whether a search over n particles stages neighbor lists in LDS
+// (stage_min_particles: 0 the library's threshold, < 0 never).
+bool stages_lists(const pbf_ctx* c, int n) {
+  const int smin = c->p.stage_min_particles == 0 ? kStageMinParticles : c->p.stage_min_particles;
+  return smin > 0 && n >= smin;
+}
+
 int list_per_of(const pbf_ctx* c) {
   const long long waves = ((long long)c->cap_n + 63) / 64;
   return waves > 0 ? (int)std::min<long long>(c->list_cap / (64 * waves), INT_MAX) : 0;
@@ -769,8 +777,7 @@ int search(pbf_ctx* c, const SortIn& in, int n, const int* scene_off, int nscene
   {
     PassTimer t(c, PBF_PASS_NEIGHBORS, st, true);
     NbrOut o = out_of(c);
-    const int smin = c->p.stage_min_particles == 0 ? kStageMinParticles : c->p.stage_min_particles;
-    o.stage_lists = (smin > 0 && n >= smin) ? 1 : 0;
+    o.stage_lists = stages_lists(c, n) ? 1 : 0;
     launch_neighbors(view_of(c, n), o, c->counters, st, t.events());
   }
   c->stats_waves = (n + 63) / 64;
@@ -888,12 +895,14 @@ int ensure_box(pbf_ctx* c, hipStream_t st) {
 }
 
 // Lanes per particle of the PBD passes for a search over n particles: the
-// context's setting, or automatically a lane pair up to
-// PBF_TWO_LANE_MAX_PARTICLES (fewer waves than SIMDs to fill otherwise) and
-// one lane above, or with persistent launches (which have no lane-pair form).
+// context's setting, or automatically a lane pair where the search keeps
+// unstaged lists or covers at most PBF_TWO_LANE_MAX_PARTICLES (measured: both
+// waves of a list copying its stage cost more than the pair saves above that),
+// one lane otherwise and with persistent launches (no lane-pair form).
 int pass_lanes_for(const pbf_ctx* c, int n) {
   if (c->pass_lanes != 0) return c->pass_lanes;
-  return (!c->pass_persistent && n <= PBF_TWO_LANE_MAX_PARTICLES) ? 2 : 1;
+  if (c->pass_persistent) return 1;
+  return (!stages_lists(c, n) || n <= PBF_TWO_LANE_MAX_PARTICLES) ? 2 : 1;
 }
 
 // One fused step on a batch of scenes (scene_off: device offsets): the state
@@ -933,6 +942,7 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
   if (s) return s;
   NbrView nv = view_of(c, n);
   const int lanes = pass_lanes_for(c, n);
+  c->last_lanes = lanes;
   float4* bufs[2] = {c->xa, c->xb};
   if (K == 0 && !visc) launch_write_back(c->x0, c->vs, c->perm, n, x_out, v_out, st);
   for (int it = 0; it < K; ++it) {
@@ -1860,6 +1870,7 @@ int pbf_get_stats(pbf_ctx* c, pbf_stats* out) {
   out->neighbor_threshold = c->pc.thr2;
   out->graph_captures = c->graph_captures;
   out->graph_replays = c->graph_replays;
+  out->pass_lanes = c->last_lanes;
   return PBF_OK;
 }
 

@@ -49,6 +49,8 @@ def assert_settings_identical(fluid, x0, v0, nsteps, off=None, **params):
     for setting in [t + (0,) for t in SETTINGS] + LANE_SETTINGS:
         ctx.set_tuning(*setting)
         got = run(ctx, x0, v0, nsteps, off)
+        if setting[3]:
+            assert ctx.stats()["pass_lanes"] == setting[3]
         for name, g, w in zip(("x", "v", "rho"), got, want):
             np.testing.assert_array_equal(g, w, err_msg=f"pass_block, stage_min_particles, persistent, pass_lanes "
                                                         f"{setting}: {name}")
@@ -58,7 +60,10 @@ def assert_settings_identical(fluid, x0, v0, nsteps, off=None, **params):
 def test_settings_identical_dam_break(fluid, scene):
     say("launch settings, 32,768-particle dam break, 3 steps")
     x0 = scene.jittered_lattice((32, 32, 32), (0.10, 0.50, 0.10), 0.005, 0.1, seed=1)
-    assert_settings_identical(fluid, x0, np.zeros_like(x0), 3)
+    ctx = assert_settings_identical(fluid, x0, np.zeros_like(x0), 3)
+    ctx.set_tuning(256, 0, 0)  # automatic lanes: a small search takes the lane pair
+    run(ctx, x0, np.zeros_like(x0), 1)
+    assert ctx.stats()["pass_lanes"] == 2
 
 
 def test_settings_identical_staged_lattice_with_modules(fluid, scene):
@@ -67,7 +72,13 @@ def test_settings_identical_staged_lattice_with_modules(fluid, scene):
     ctx = assert_settings_identical(fluid, x0, np.zeros_like(x0), 2, flags=3)
     ctx.set_tuning(256, 1, 1)
     run(ctx, x0, np.zeros_like(x0), 1)
-    assert ctx.stats()["staged_waves"] > 0
+    assert ctx.stats()["staged_waves"] > 0 and ctx.stats()["pass_lanes"] == 1
+    ctx.set_tuning(256, 1, 0)  # automatic lanes: a staged search above 196,608 particles keeps one lane
+    run(ctx, x0, np.zeros_like(x0), 1)
+    assert ctx.stats()["pass_lanes"] == 1
+    ctx.set_tuning(256, -1, 0)  # ... and an unstaged one takes the lane pair
+    run(ctx, x0, np.zeros_like(x0), 1)
+    assert ctx.stats()["pass_lanes"] == 2
 
 
 def test_settings_identical_batch(fluid, scene):
