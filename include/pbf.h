@@ -372,7 +372,8 @@ int pbf_set_tuning(pbf_ctx* ctx, const pbf_tuning* t);
 int pbf_get_tuning(const pbf_ctx* ctx, pbf_tuning* out);
 
 /* Autotuner. Times every setting of pbf_tuning (4 workgroup sizes x staged
- * lists on / off x persistent or not) on the caller's workload: a copy of (d_x, d_v), `steps` plain
+ * lists on / off x persistent or not, pass_lanes automatic: the applied
+ * setting has pass_lanes 0) on the caller's workload: a copy of (d_x, d_v), `steps` plain
  * steps after a warm-up call, the better of two runs, on `stream`; keeps the
  * fastest (a setting replaces the one before it only when more than 1%
  * faster, the default first) and applies it to the context. The workload is a
