@@ -88,6 +88,28 @@ def test_settings_identical_batch(fluid, scene):
     assert_settings_identical(fluid, b.pos, np.zeros_like(b.pos), 2, off)
 
 
+def test_lane_pair_identical_on_32bit_lists(fluid, scene):
+    """A slab 8 lattice layers thick whose cell planes hold 41k particles each:
+    975 of its 2,048 waves reach over more than 65,535 sorted indices and keep
+    32-bit lists (a numpy restatement of the sort and the runs counts them),
+    which the lane pair runs on one lane of the pair. One lane and the pair
+    give the same bits, with and without staged lists."""
+    x0 = scene.jittered_lattice((128, 128, 8), (0.18, 0.18, 0.4), 0.005, 0.1, seed=6)
+    say(f"lane pair vs one lane, {len(x0)}-particle slab (32-bit lists), 2 steps")
+    ctx = fluid.Context("cuda:0")
+    want = None
+    for stage in (-1, 1):
+        for lanes in (1, 2):
+            ctx.set_tuning(256, stage, 0, lanes)
+            got = run(ctx, x0, np.zeros_like(x0), 2)
+            assert ctx.stats()["pass_lanes"] == lanes
+            if want is None:
+                want = got
+                continue
+            for name, g, w in zip(("x", "v", "rho"), got, want):
+                np.testing.assert_array_equal(g, w, err_msg=f"stage {stage}, lanes {lanes}: {name}")
+
+
 def test_autotune_and_cache(fluid, scene, tmp_path):
     x0 = scene.jittered_lattice((32, 32, 32), (0.10, 0.50, 0.10), 0.005, 0.1, seed=1)
     x, v = dev(x0), dev(np.zeros_like(x0))
