@@ -1619,7 +1619,8 @@ int pbf_rollout_backward(pbf_ctx* c, float* d_xbar, float* d_vbar, void* stream)
     for (int q = m - 1; q >= 0; --q) {
       auto& t = c->slots[q];
       slot_arrays(t, t.x0);
-      launch_adjoint_step(slot_view(c, t, n), P.data(), aux.data(), K, c->pc, t.perm, d_xbar, d_vbar, ab, st);
+      launch_adjoint_step(slot_view(c, t, n), P.data(), aux.data(), K, c->pc, t.perm, d_xbar, d_vbar, ab, st,
+                          pass_lanes_for(c, n));
       PBF_LAUNCHED(c);
     }
   }

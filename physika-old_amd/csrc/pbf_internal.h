@@ -279,7 +279,8 @@ struct AdjBufs {
   int* flag;    // set when a taped step had out-of-grid particles
 };
 void launch_adjoint_step(const NbrView& nv, float4* const* P, float4* const* aux, int K, const PhysConst& pc,
-                         const int* perm, float* xbar, float* vbar, const AdjBufs& ab, hipStream_t st);
+                         const int* perm, float* xbar, float* vbar, const AdjBufs& ab, hipStream_t st,
+                         int lanes = 1);
 
 // Double-precision step and adjoint (reverse mode against FP64 references):
 // the constants of PhysConst in double, computed from the parameters

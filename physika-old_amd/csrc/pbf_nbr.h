@@ -650,4 +650,82 @@ __device__ __forceinline__ bool for_each_neighbor_half(const NbrView& nv, int s,
   return true;
 }
 
+// The lane-pair decode by index: halfword hh of each word of a 16-byte chunk
+// (entries 8c + 2u + hh), as sorted indices jb + entry (16-bit offsets) or
+// through the stage map (stage slots); entry k = 4c + u of the lane at or past
+// cnt becomes past.
+template <bool kFull>
+__device__ __forceinline__ Group4 decode_half_idx(uint4 e, uint32_t sh, int jb, int c, int cnt, int past) {
+  const int k = c << 2;
+  Group4 g;
+  g.j[0] = jb + (int)((e.x >> sh) & 0xffffu);
+  g.j[1] = jb + (int)((e.y >> sh) & 0xffffu);
+  g.j[2] = jb + (int)((e.z >> sh) & 0xffffu);
+  g.j[3] = jb + (int)((e.w >> sh) & 0xffffu);
+  if (!kFull) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) g.j[u] = k + u < cnt ? g.j[u] : past;
+  }
+  return g;
+}
+__device__ __forceinline__ Group4 decode_half_staged(uint4 e, uint32_t sh, const StageMap& m, int c, int cnt,
+                                                     int past) {
+  const int k = c << 2;
+  Group4 g;
+  g.j[0] = m.index_of((int)((e.x >> sh) & 0xffffu));
+  g.j[1] = m.index_of((int)((e.y >> sh) & 0xffffu));
+  g.j[2] = m.index_of((int)((e.z >> sh) & 0xffffu));
+  g.j[3] = m.index_of((int)((e.w >> sh) & 0xffffu));
+#pragma unroll
+  for (int u = 0; u < 4; ++u) g.j[u] = k + u < cnt ? g.j[u] : past;
+  return g;
+}
+
+// for_each_neighbor for a lane pair, by index: lane h (0 or 1) of the pair
+// that holds particle s calls body(j, gather(j)) for the entries of s's list
+// whose index in the canonical order has parity h, in that order (entries past
+// the lane's count and the stage pads are not visited). Returns false, with
+// nothing run, when s's wave has no 16-bit or staged list.
+template <class GatherFn, class Body>
+__device__ __forceinline__ bool for_each_neighbor_half_idx(const NbrView& nv, int s, int h, GatherFn&& gather,
+                                                           Body&& body) {
+  const int wave = __builtin_amdgcn_readfirstlane(s >> 6), lane = s & 63;
+  const WaveList wl = wave_list(nv, wave);
+  const int mode = wl.lm & 3;
+  if (wl.lb < 0 || (mode != kListDelta16 && mode != kListStaged)) return false;
+  const uint4* L = reinterpret_cast<const uint4*>(nv.list + wl.lb) + lane;
+  const int cnt = nv.nbr_cnt[s];
+  const int nfull = (wl.lm >> 2) >> 1;  // as in for_each_neighbor_half
+  auto each = [&](int j, const auto& a, auto) { body(j, a); };
+  if (mode == kListDelta16) {
+    const int ch = (cnt - h + 1) >> 1;
+    const uint32_t sh = (uint32_t)h << 4;
+    list_loop<false, uint4>(
+        ch, nfull, [&](int c) { return L[(long long)c << 6]; },
+        [&](uint4 e, int c, auto full) { return decode_half_idx<decltype(full)::value>(e, sh, wl.jb, c, ch, s); },
+        gather, each);
+    return true;
+  }
+  // stage slots of the first ca entries (the pads after them are not visited),
+  // then offsets from index ca on
+  const int* st = nv.stage + kStageInts * wave;
+  StageMap m;
+  m.load(st);
+  const int ka = st[7], ca = nv.cnt_a[s], cb = cnt - ca;
+  const int c1 = (ca - h + 1) >> 1;
+  const uint32_t sh1 = (uint32_t)h << 4;
+  list_loop<false, uint4>(
+      c1, 0, [&](int c) { return L[(long long)c << 6]; },
+      [&](uint4 e, int c, auto) { return decode_half_staged(e, sh1, m, c, c1, s); }, gather, each);
+  const int hh = h ^ (ca & 1);
+  const int ch = (cb - hh + 1) >> 1;
+  const uint32_t sh2 = (uint32_t)hh << 4;
+  const uint4* L2 = L + ((long long)(ka >> 3) << 6);
+  list_loop<false, uint4>(
+      ch, nfull, [&](int c) { return L2[(long long)c << 6]; },
+      [&](uint4 e, int c, auto full) { return decode_half_idx<decltype(full)::value>(e, sh2, wl.jb, c, ch, s); },
+      gather, each);
+  return true;
+}
+
 }  // namespace pbf

@@ -147,6 +147,29 @@ def test_rollout_adjoint_staged_lists_identical(fluid, scene):
         np.testing.assert_array_equal(u, w)
 
 
+def test_rollout_adjoint_lane_pair_identical(fluid, scene):
+    """The forward and adjoint passes with a lane pair per particle (each lane
+    summing every other neighbor) give the positions and gradients of one lane
+    per particle bit for bit, on unstaged and staged lists."""
+    say("rollout adjoint, lane pair vs one lane")
+    x0 = scene.jittered_lattice((30, 8, 8), (0.3, 0.5, 0.3), 0.0046, 0.1, seed=23)
+    n = len(x0)
+    c = scene.uniform01(24, 3 * n).reshape(n, 3) - 0.5
+    for smin in (-1, 1):
+        outs = []
+        for lanes in (1, 2):
+            ctx = fluid.Context("cuda:0", list_slots=512, mask_words=64)
+            ctx.set_tuning(256, smin, 0, lanes)
+            x, v = dev(x0), dev(np.zeros_like(x0))
+            ctx.rollout_forward(x, v, None, 4, 2)
+            assert ctx.stats()["pass_lanes"] == lanes
+            xb, vb = dev(c), dev(np.zeros_like(c))
+            ctx.rollout_backward(xb, vb)
+            outs.append((x.cpu().numpy(), xb.cpu().numpy(), vb.cpu().numpy()))
+        for u, w in zip(*outs):
+            np.testing.assert_array_equal(u, w, err_msg=f"stage_min_particles {smin}")
+
+
 def test_rollout_keeps_its_context_alive(fluid, scene):
     """fluid.rollout on a context the caller does not keep: the graph holds
     it until backward, and the gradient equals the one of a kept context."""
