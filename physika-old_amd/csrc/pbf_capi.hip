@@ -1200,8 +1200,9 @@ int step64_once(pbf_ctx* c, const double* x_in, const double* v_in, double* x_ou
     for (int k = 1; k < K; ++k) Pu[k] = r.w4 + (size_t)(1 + ((k - 1) & 1)) * cap;
     P = Pu.data();
   }
+  c->last_lanes = pass_lanes_for(c, n);
   launch64_step(view_of(c, n), c->perm, x_in, v_in, x_out, v_out, rho, P, aux, K, vs, c->pc64, c->counters, r.flag,
-                st);
+                st, c->last_lanes);
   PBF_LAUNCHED(c);
   return PBF_OK;
 }
@@ -1759,7 +1760,8 @@ int pbf_rollout_backward_f64(pbf_ctx* c, double* d_xbar, double* d_vbar, void* s
     for (int q = m - 1; q >= 0; --q) {
       auto& t = c->slots[q];
       slot_arrays(t);
-      launch64_adjoint_step(slot_view(c, t, n), P.data(), aux.data(), K, c->pc64, t.perm, d_xbar, d_vbar, ab, st);
+      launch64_adjoint_step(slot_view(c, t, n), P.data(), aux.data(), K, c->pc64, t.perm, d_xbar, d_vbar, ab, st,
+                            pass_lanes_for(c, n));
       PBF_LAUNCHED(c);
     }
   }

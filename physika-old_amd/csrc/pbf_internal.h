@@ -301,9 +301,11 @@ void launch64_round(const double* x, float* xf, int n, hipStream_t st);
 // when the search had particles outside the grid.
 void launch64_step(const NbrView& nv, const int* perm, const double* x_in, const double* v_in, double* x_out,
                    double* v_out, double* rho, double4* const* P, double4* const* aux, int K, double4* vs,
-                   const PhysConst64& pc, const unsigned long long* counters, int* flag, hipStream_t st);
+                   const PhysConst64& pc, const unsigned long long* counters, int* flag, hipStream_t st,
+                   int lanes = 1);
 void launch64_adjoint_step(const NbrView& nv, double4* const* P, double4* const* aux, int K, const PhysConst64& pc,
-                           const int* perm, double* xbar, double* vbar, const AdjBufs64& ab, hipStream_t st);
+                           const int* perm, double* xbar, double* vbar, const AdjBufs64& ab, hipStream_t st,
+                           int lanes = 1);
 
 // Box boundary and viscosity.
 void launch_box_sdf(const BoxField& f, const float lo[3], const float hi[3], hipStream_t st);

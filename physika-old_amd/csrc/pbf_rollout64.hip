@@ -89,30 +89,93 @@ __global__ __launch_bounds__(kBlock) void k64_begin(int n, const int* __restrict
   vs[s] = make_double4(vx, vy, vz, 0.0);
 }
 
+// Sums of a pass over a particle's neighbors: N doubles, kept as one partial
+// sum per entry parity in the neighbor order and added at the end, as the
+// float passes do, so that one lane and a lane pair give the same bits.
+template <int N>
+struct SumsD {
+  double v[N] = {};
+};
+
+// The whole list of s on one lane: term(sums, gather(j)) per neighbor.
+template <int N, class Gather, class Term>
+__device__ __forceinline__ SumsD<N> sums64_one_lane(const NbrView& nv, int s, Gather&& gather, Term&& term) {
+  SumsD<N> a[2];
+  for_each_neighbor_p(
+      nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)), gather,
+      [&](int, const auto& q, auto par) { term(a[decltype(par)::value], q); },
+      [&](bool odd) {
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+          const double x = odd ? a[1].v[i] : a[0].v[i], y = odd ? a[0].v[i] : a[1].v[i];
+          a[0].v[i] = x;
+          a[1].v[i] = y;
+        }
+      });
+  SumsD<N> t;
+#pragma unroll
+  for (int i = 0; i < N; ++i) t.v[i] = a[0].v[i] + a[1].v[i];
+  return t;
+}
+
+// Lane h of the pair holding s sums the entries of parity h; t receives the
+// even + odd sums (on both lanes). False, with nothing run, where s's wave has
+// neither a 16-bit nor a staged list.
+template <int N, class Gather, class Term>
+__device__ __forceinline__ bool sums64_lane_pair(const NbrView& nv, int s, int h, Gather&& gather, Term&& term,
+                                                 SumsD<N>& t) {
+  SumsD<N> a;
+  if (!for_each_neighbor_half_idx(nv, s, h, gather, [&](int, const auto& q) { term(a, q); })) return false;
+#pragma unroll
+  for (int i = 0; i < N; ++i) {
+    const double o = __shfl_xor(a.v[i], 1);
+    t.v[i] = h == 0 ? a.v[i] + o : o + a.v[i];
+  }
+  return true;
+}
+
+// The sums of a pass for sorted particle s: thread g of the grid takes particle
+// g (kPair false) or entry parity g & 1 of particle g >> 1 (kPair: a lane pair
+// per particle, for small searches). Returns false on the lanes that must not
+// write s's results (past the end, or the odd lane of a pair).
+template <bool kPair, int N, class Gather, class Term>
+__device__ __forceinline__ bool sums64_of(const NbrView& nv, int s, int h, Gather&& gather, Term&& term,
+                                          SumsD<N>& t) {
+  if (kPair && sums64_lane_pair<N>(nv, s, h, gather, term, t)) return h == 0;
+  if (h != 0) return false;
+  t = sums64_one_lane<N>(nv, s, gather, term);
+  return true;
+}
+
 // Density and lambda; lambda goes to P[s].w, (G, 1/(S + eps)) to aux (taped
 // steps), the density of the last iteration to rho_out (original order).
+template <bool kPair>
 __global__ __launch_bounds__(kBlock) void k64_density_lambda(NbrView nv, double4* __restrict__ P, PhysConst64 pc,
                                                              double4* __restrict__ aux, double* __restrict__ rho_out,
                                                              const int* __restrict__ perm) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = kPair ? g >> 1 : g, h = kPair ? g & 1 : 0;
   if (s >= nv.n) return;
   const double4 ps = P[s];
-  double rho = 0.0, sgx = 0.0, sgy = 0.0, sgz = 0.0, sgg = 0.0;
-  for_each_neighbor(
-      nv, s, [&](int j) { return P[j]; },
-      [&](int, const double4& pj) {
-        const PairGeom64 g = pair_geom64(ps, pj, pc.inv_h);
-        const double t2 = g.t * g.t;
-        rho += t2 * g.t;
-        const double tg = g.r > pc.pair_eps ? t2 : 0.0;
-        const double u = tg * g.rinv;
-        sgx += u * g.dx;
-        sgy += u * g.dy;
-        sgz += u * g.dz;
-        sgg += tg * tg;
-      });
-  rho *= pc.cW;
+  SumsD<5> t;  // rho, G, |g|^2
+  if (!sums64_of<kPair, 5>(
+          nv, s, h, [&](int j) { return P[j]; },
+          [&](auto& a, const double4& pj) {
+            const PairGeom64 q = pair_geom64(ps, pj, pc.inv_h);
+            const double t2 = q.t * q.t;
+            a.v[0] += t2 * q.t;
+            const double tg = q.r > pc.pair_eps ? t2 : 0.0;
+            const double u = tg * q.rinv;
+            a.v[1] += u * q.dx;
+            a.v[2] += u * q.dy;
+            a.v[3] += u * q.dz;
+            a.v[4] += tg * tg;
+          },
+          t))
+    return;
+  const double rho = t.v[0] * pc.cW;
+  const double sgx = t.v[1], sgy = t.v[2], sgz = t.v[3], sgg = t.v[4];
   const double S = (pc.cG * pc.cG) * (sgg + (sgx * sgx + sgy * sgy + sgz * sgz));
   double lam = -(rho - pc.lam_rest) / (S + pc.lam_eps);
   lam = lam > 0.0 ? 0.0 : lam;
@@ -123,6 +186,7 @@ __global__ __launch_bounds__(kBlock) void k64_density_lambda(NbrView nv, double4
 
 // Displacement p' = p + 2 corr cG sum_j (lam_i + lam_j) u d; the last
 // iteration writes x' = p_K and v' = v* + (p_K - x*)/dt in original order.
+template <bool kPair>
 __global__ __launch_bounds__(kBlock) void k64_displacement(NbrView nv, const double4* __restrict__ P,
                                                            double4* __restrict__ Pn, PhysConst64 pc, int last,
                                                            const double4* __restrict__ P0,
@@ -130,19 +194,23 @@ __global__ __launch_bounds__(kBlock) void k64_displacement(NbrView nv, const dou
                                                            const int* __restrict__ perm, double* __restrict__ x_out,
                                                            double* __restrict__ v_out) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = kPair ? g >> 1 : g, h = kPair ? g & 1 : 0;
   if (s >= nv.n) return;
   const double4 ps = P[s];
-  double ax = 0.0, ay = 0.0, az = 0.0;
-  for_each_neighbor(
-      nv, s, [&](int j) { return P[j]; },
-      [&](int, const double4& pj) {
-        const PairGeom64 g = pair_geom64(ps, pj, pc.inv_h);
-        const double c = g.r > pc.pair_eps ? (ps.w + pj.w) * (g.t * g.t) * g.rinv : 0.0;
-        ax += c * g.dx;
-        ay += c * g.dy;
-        az += c * g.dz;
-      });
+  SumsD<3> t;
+  if (!sums64_of<kPair, 3>(
+          nv, s, h, [&](int j) { return P[j]; },
+          [&](auto& a, const double4& pj) {
+            const PairGeom64 q = pair_geom64(ps, pj, pc.inv_h);
+            const double c = q.r > pc.pair_eps ? (ps.w + pj.w) * (q.t * q.t) * q.rinv : 0.0;
+            a.v[0] += c * q.dx;
+            a.v[1] += c * q.dy;
+            a.v[2] += c * q.dz;
+          },
+          t))
+    return;
+  const double ax = t.v[0], ay = t.v[1], az = t.v[2];
   const double scale = pc.corr * pc.cG;
   const double nx = ps.x + scale * (2.0 * ax), ny = ps.y + scale * (2.0 * ay), nz = ps.z + scale * (2.0 * az);
   if (!last) {
@@ -188,23 +256,27 @@ __global__ __launch_bounds__(kBlock) void k64_adj_begin(int n, const int* __rest
 
 // lam_bar_i = 2 corr cG sum_j u (f_i - f_j) . d; rho_bar, S_bar where lambda
 // was not clamped.
+template <bool kPair>
 __global__ __launch_bounds__(kBlock) void k64_adj_lambda(NbrView nv, const double4* __restrict__ P,
                                                          const double4* __restrict__ aux,
                                                          const double4* __restrict__ f, double4* __restrict__ q1,
                                                          double4* __restrict__ q2, PhysConst64 pc) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = kPair ? g >> 1 : g, h = kPair ? g & 1 : 0;
   if (s >= nv.n) return;
   const double4 ps = P[s], as = f[s];
-  double acc = 0.0;
-  for_each_neighbor(
-      nv, s, [&](int j) { return D4x2{P[j], f[j]}; },
-      [&](int, const D4x2& q) {
-        const PairGeom64 g = pair_geom64(ps, q.a, pc.inv_h);
-        const double u = g.r > pc.pair_eps ? g.t * g.t * g.rinv : 0.0;
-        acc += u * ((as.x - q.b.x) * g.dx + (as.y - q.b.y) * g.dy + (as.z - q.b.z) * g.dz);
-      });
-  const double lam_bar = (2.0 * pc.corr * pc.cG) * acc;
+  SumsD<1> t;
+  if (!sums64_of<kPair, 1>(
+          nv, s, h, [&](int j) { return D4x2{P[j], f[j]}; },
+          [&](auto& a, const D4x2& q) {
+            const PairGeom64 r = pair_geom64(ps, q.a, pc.inv_h);
+            const double u = r.r > pc.pair_eps ? r.t * r.t * r.rinv : 0.0;
+            a.v[0] += u * ((as.x - q.b.x) * r.dx + (as.y - q.b.y) * r.dy + (as.z - q.b.z) * r.dz);
+          },
+          t))
+    return;
+  const double lam_bar = (2.0 * pc.corr * pc.cG) * t.v[0];
   const double4 ax = aux[s];
   const bool active = ps.w < 0.0;
   const double rho_bar = active ? -lam_bar * ax.w : 0.0;
@@ -214,6 +286,7 @@ __global__ __launch_bounds__(kBlock) void k64_adj_lambda(NbrView nv, const doubl
 }
 
 // e_k = e_{k+1} + sum_j [J(d) w + (rho_bar_i + rho_bar_j) psi(d)], f = e + c.
+template <bool kPair>
 __global__ __launch_bounds__(kBlock) void k64_adj_pos(NbrView nv, const double4* __restrict__ P,
                                                       const double4* __restrict__ aux,
                                                       const double4* __restrict__ q1,
@@ -222,39 +295,42 @@ __global__ __launch_bounds__(kBlock) void k64_adj_pos(NbrView nv, const double4*
                                                       const double4* __restrict__ c, double4* __restrict__ e_out,
                                                       double4* __restrict__ f_out, PhysConst64 pc) {
 #pragma clang fp contract(off)
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int s = kPair ? g >> 1 : g, h = kPair ? g & 1 : 0;
   if (s >= nv.n) return;
   const double4 ps = P[s], q1s = q1[s], q2s = q2[s], ax = aux[s];
   const double Gx = pc.cG * ax.x, Gy = pc.cG * ax.y, Gz = pc.cG * ax.z;
   const double C = 2.0 * pc.corr;
   const double cpsi = -3.0 * pc.cW * pc.inv_h;
-  double bx = 0.0, by = 0.0, bz = 0.0;
-  for_each_neighbor(
-      nv, s, [&](int j) { return D4x3{P[j], q1[j], q2[j]}; },
-      [&](int, const D4x3& q) {
-        const PairGeom64 g = pair_geom64(ps, q.a, pc.inv_h);
-        const double t2 = g.t * g.t;
-        const double cr = (q1s.w + q.b.w) * cpsi * t2 * g.rinv;
-        bx += cr * g.dx;
-        by += cr * g.dy;
-        bz += cr * g.dz;
-        if (g.r > pc.pair_eps) {
-          const double u = t2 * g.rinv;
-          const double cg = pc.cG * u;
-          const double gx = cg * g.dx, gy = cg * g.dy, gz = cg * g.dz;
-          const double cl = C * (ps.w + q.a.w);
-          const double wx = cl * (q1s.x - q.b.x) + 2.0 * q2s.w * (gx + Gx) + 2.0 * (q.c.w * gx - q.c.x);
-          const double wy = cl * (q1s.y - q.b.y) + 2.0 * q2s.w * (gy + Gy) + 2.0 * (q.c.w * gy - q.c.y);
-          const double wz = cl * (q1s.z - q.b.z) + 2.0 * q2s.w * (gz + Gz) + 2.0 * (q.c.w * gz - q.c.z);
-          const double dw = g.dx * wx + g.dy * wy + g.dz * wz;
-          const double k2 = (2.0 * g.t * pc.inv_h + u) * g.rinv * g.rinv * dw;
-          bx += pc.cG * (u * wx - k2 * g.dx);
-          by += pc.cG * (u * wy - k2 * g.dy);
-          bz += pc.cG * (u * wz - k2 * g.dz);
-        }
-      });
+  SumsD<3> t;
+  if (!sums64_of<kPair, 3>(
+          nv, s, h, [&](int j) { return D4x3{P[j], q1[j], q2[j]}; },
+          [&](auto& a, const D4x3& q) {
+            const PairGeom64 r = pair_geom64(ps, q.a, pc.inv_h);
+            const double t2 = r.t * r.t;
+            const double cr = (q1s.w + q.b.w) * cpsi * t2 * r.rinv;
+            a.v[0] += cr * r.dx;
+            a.v[1] += cr * r.dy;
+            a.v[2] += cr * r.dz;
+            if (r.r > pc.pair_eps) {
+              const double u = t2 * r.rinv;
+              const double cg = pc.cG * u;
+              const double gx = cg * r.dx, gy = cg * r.dy, gz = cg * r.dz;
+              const double cl = C * (ps.w + q.a.w);
+              const double wx = cl * (q1s.x - q.b.x) + 2.0 * q2s.w * (gx + Gx) + 2.0 * (q.c.w * gx - q.c.x);
+              const double wy = cl * (q1s.y - q.b.y) + 2.0 * q2s.w * (gy + Gy) + 2.0 * (q.c.w * gy - q.c.y);
+              const double wz = cl * (q1s.z - q.b.z) + 2.0 * q2s.w * (gz + Gz) + 2.0 * (q.c.w * gz - q.c.z);
+              const double dw = r.dx * wx + r.dy * wy + r.dz * wz;
+              const double k2 = (2.0 * r.t * pc.inv_h + u) * r.rinv * r.rinv * dw;
+              a.v[0] += pc.cG * (u * wx - k2 * r.dx);
+              a.v[1] += pc.cG * (u * wy - k2 * r.dy);
+              a.v[2] += pc.cG * (u * wz - k2 * r.dz);
+            }
+          },
+          t))
+    return;
   const double4 es = e_in[s], cs = c[s];
-  const double ex = es.x + bx, ey = es.y + by, ez = es.z + bz;
+  const double ex = es.x + t.v[0], ey = es.y + t.v[1], ez = es.z + t.v[2];
   e_out[s] = make_double4(ex, ey, ez, 0.0);
   f_out[s] = make_double4(ex + cs.x, ey + cs.y, ez + cs.z, 0.0);
 }
@@ -282,8 +358,8 @@ void launch64_round(const double* x, float* xf, int n, hipStream_t st) {
 
 void launch64_step(const NbrView& nv, const int* perm, const double* x_in, const double* v_in, double* x_out,
                    double* v_out, double* rho, double4* const* P, double4* const* aux, int K, double4* vs,
-                   const PhysConst64& pc, const unsigned long long* counters, int* flag, hipStream_t st) {
-  const int n = nv.n, nb = blocks_of(n);
+                   const PhysConst64& pc, const unsigned long long* counters, int* flag, hipStream_t st, int lanes) {
+  const int n = nv.n, nb = blocks_of(n), nbp = (int)((2ll * n + kBlock - 1) / kBlock);
   k64_begin<<<nb, kBlock, 0, st>>>(n, perm, x_in, v_in, pc, P[0], vs, counters, flag);
   if (K == 0) {
     k64_write<<<nb, kBlock, 0, st>>>(n, perm, P[0], vs, x_out, v_out);
@@ -291,21 +367,35 @@ void launch64_step(const NbrView& nv, const int* perm, const double* x_in, const
   }
   for (int k = 0; k < K; ++k) {
     const bool last = k + 1 == K;
-    k64_density_lambda<<<nb, kBlock, 0, st>>>(nv, P[k], pc, aux ? aux[k] : nullptr, last ? rho : nullptr, perm);
-    k64_displacement<<<nb, kBlock, 0, st>>>(nv, P[k], last ? nullptr : P[k + 1], pc, last ? 1 : 0, P[0], vs, perm,
-                                            x_out, v_out);
+    if (lanes == 2) {
+      k64_density_lambda<true><<<nbp, kBlock, 0, st>>>(nv, P[k], pc, aux ? aux[k] : nullptr, last ? rho : nullptr,
+                                                       perm);
+      k64_displacement<true><<<nbp, kBlock, 0, st>>>(nv, P[k], last ? nullptr : P[k + 1], pc, last ? 1 : 0, P[0], vs,
+                                                     perm, x_out, v_out);
+    } else {
+      k64_density_lambda<false><<<nb, kBlock, 0, st>>>(nv, P[k], pc, aux ? aux[k] : nullptr, last ? rho : nullptr,
+                                                       perm);
+      k64_displacement<false><<<nb, kBlock, 0, st>>>(nv, P[k], last ? nullptr : P[k + 1], pc, last ? 1 : 0, P[0], vs,
+                                                     perm, x_out, v_out);
+    }
   }
 }
 
 void launch64_adjoint_step(const NbrView& nv, double4* const* P, double4* const* aux, int K, const PhysConst64& pc,
-                           const int* perm, double* xbar, double* vbar, const AdjBufs64& ab, hipStream_t st) {
-  const int n = nv.n, nb = blocks_of(n);
+                           const int* perm, double* xbar, double* vbar, const AdjBufs64& ab, hipStream_t st,
+                           int lanes) {
+  const int n = nv.n, nb = blocks_of(n), nbp = (int)((2ll * n + kBlock - 1) / kBlock);
   k64_adj_begin<<<nb, kBlock, 0, st>>>(n, perm, xbar, vbar, pc.dt, ab.e, ab.c, ab.f, ab.vsb);
   double4* e = ab.e;
   double4* e2 = ab.e2;
   for (int k = K - 1; k >= 0; --k) {
-    k64_adj_lambda<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.f, ab.q1, ab.q2, pc);
-    k64_adj_pos<<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.q1, ab.q2, e, ab.c, e2, ab.f, pc);
+    if (lanes == 2) {
+      k64_adj_lambda<true><<<nbp, kBlock, 0, st>>>(nv, P[k], aux[k], ab.f, ab.q1, ab.q2, pc);
+      k64_adj_pos<true><<<nbp, kBlock, 0, st>>>(nv, P[k], aux[k], ab.q1, ab.q2, e, ab.c, e2, ab.f, pc);
+    } else {
+      k64_adj_lambda<false><<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.f, ab.q1, ab.q2, pc);
+      k64_adj_pos<false><<<nb, kBlock, 0, st>>>(nv, P[k], aux[k], ab.q1, ab.q2, e, ab.c, e2, ab.f, pc);
+    }
     double4* t = e;
     e = e2;
     e2 = t;

@@ -122,6 +122,29 @@ def test_f64_backward_repeatable_and_separate_from_fp32(fluid, scene):
     np.testing.assert_array_equal(outs[0][1], outs[1][1])
 
 
+def test_f64_lane_pair_identical(fluid, scene):
+    """The double forward and adjoint passes with a lane pair per particle give
+    the positions and gradients of one lane per particle, bit for bit, on
+    unstaged and staged lists."""
+    say("double rollout: lane pair vs one lane")
+    x0 = compressed_block(scene, (9, 8, 7), 0.0036, seed=23)
+    n = len(x0)
+    c = d64(scene.uniform01(24, 3 * n).reshape(n, 3) - 0.5)
+    for smin in (-1, 1):
+        outs = []
+        for lanes in (1, 2):
+            ctx = fluid.Context("cuda:0")
+            ctx.set_tuning(256, smin, 0, lanes)
+            x, v = d64(x0), d64(np.zeros((n, 3)))
+            tok = ctx.rollout_forward(x, v, None, 4, 2)
+            assert ctx.stats()["pass_lanes"] == lanes
+            xb, vb = c.clone(), torch.zeros_like(c)
+            ctx.rollout_backward(xb, vb, tok)
+            outs.append((x.cpu().numpy(), xb.cpu().numpy(), vb.cpu().numpy()))
+        for u, w in zip(*outs):
+            np.testing.assert_array_equal(u, w, err_msg=f"stage_min_particles {smin}")
+
+
 def test_f64_batched_equals_per_scene(fluid, scene):
     say("double rollout: batched gradients == per-scene gradients, 5 scenes")
     scenes = [compressed_block(scene, d, 0.0036, seed=50 + k) for k, d in
