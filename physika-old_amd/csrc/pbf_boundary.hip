@@ -235,39 +235,39 @@ struct VisPair {
   float4 x, v;
 };
 
-// One Jacobi iteration (K_ApplyViscosity, ImplicitViscosity.cu:20-62) in
-// sorted order over the step's lists: w_ij = 45/(13 pi h^3)(1 - r/h) for
-// r > eps; v_new = v_old/(1+b) + (sum w v_j / sum w) b/(1+b). The last
-// iteration writes positions and velocities back to original order.
-__global__ __launch_bounds__(kBlock) void k_viscosity(NbrView nv, const float4* __restrict__ xs,
-                                                      const float4* __restrict__ v_old,
-                                                      const float4* __restrict__ v_in, float4* __restrict__ v_next,
-                                                      VisConst vc, const int* __restrict__ perm,
-                                                      float* __restrict__ x_out, float* __restrict__ v_out) {
-  const int s = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (s >= nv.n) return;
-  const float4 xi = xs[s];
+// Sums of one viscosity iteration for a particle (or one entry parity of it).
+struct VisSums {
   float tw = 0.0f, dx = 0.0f, dy = 0.0f, dz = 0.0f;
-  for_each_neighbor(
-      nv, s, [&](int j) { return VisPair{xs[j], v_in[j]}; },
-      [&](int, const VisPair& q) {
+};
+
+// w_ij = 45/(13 pi h^3)(1 - r/h) for r > eps: adds w and w v_j.
+__device__ __forceinline__ void vis_term(VisSums& a, float4 xi, const VisPair& q, const VisConst& vc) {
 #pragma clang fp contract(off)
-        const PairGeom g = pair_geom(xi, q.x, vc.inv_h);
-        if (g.r > vc.eps) {
-          const float w = vc.cw * g.t;
-          tw += w;
-          dx = fmaf(w, q.v.x, dx);
-          dy = fmaf(w, q.v.y, dy);
-          dz = fmaf(w, q.v.z, dz);
-        }
-      });
-  const float b = tw < vc.eps ? 0.0f : vc.b;
-  tw = tw < vc.eps ? 1.0f : tw;
+  const PairGeom g = pair_geom(xi, q.x, vc.inv_h);
+  if (g.r > vc.eps) {
+    const float w = vc.cw * g.t;
+    a.tw += w;
+    a.dx = fmaf(w, q.v.x, a.dx);
+    a.dy = fmaf(w, q.v.y, a.dy);
+    a.dz = fmaf(w, q.v.z, a.dz);
+  }
+}
+
+// v_new = v_old/(1+b) + (sum w v_j / sum w) b/(1+b) from the total sums t; the
+// last iteration writes positions and velocities back to original order.
+__device__ __forceinline__ void vis_finish(int s, float4 xi, VisSums t, const float4* __restrict__ v_old,
+                                           float4* __restrict__ v_next, const VisConst& vc,
+                                           const int* __restrict__ perm, float* __restrict__ x_out,
+                                           float* __restrict__ v_out) {
+  // nothing contracted, so both kernel forms round it alike
+#pragma clang fp contract(off)
+  const float b = t.tw < vc.eps ? 0.0f : vc.b;
+  const float tw = t.tw < vc.eps ? 1.0f : t.tw;
   const float ob = 1.0f + b;
   const float4 vo = v_old[s];
-  const float nx = vo.x / ob + (dx / tw) * b / ob;
-  const float ny = vo.y / ob + (dy / tw) * b / ob;
-  const float nz = vo.z / ob + (dz / tw) * b / ob;
+  const float nx = vo.x / ob + (t.dx / tw) * b / ob;
+  const float ny = vo.y / ob + (t.dy / tw) * b / ob;
+  const float nz = vo.z / ob + (t.dz / tw) * b / ob;
   if (!x_out) {
     v_next[s] = make_float4(nx, ny, nz, 0.0f);
     return;
@@ -281,9 +281,63 @@ __global__ __launch_bounds__(kBlock) void k_viscosity(NbrView nv, const float4* 
   v_out[3 * i + 2] = nz;
 }
 
+__device__ __forceinline__ VisSums add_parities(const VisSums& e, const VisSums& o) {
+  VisSums t;
+  t.tw = e.tw + o.tw;
+  t.dx = e.dx + o.dx;
+  t.dy = e.dy + o.dy;
+  t.dz = e.dz + o.dz;
+  return t;
+}
+
+// One Jacobi iteration (K_ApplyViscosity, ImplicitViscosity.cu:20-62) in
+// sorted order over the step's lists, the sums kept as one partial sum per
+// entry parity in the neighbor order (as the PBD passes). kPair: thread g
+// takes entry parity g & 1 of particle g >> 1 (a lane pair per particle, for
+// small searches; waves with 32-bit lists or none run the list on the even
+// lane); both forms give the same bits.
+template <bool kPair>
+__global__ __launch_bounds__(kBlock) void k_viscosity(NbrView nv, const float4* __restrict__ xs,
+                                                      const float4* __restrict__ v_old,
+                                                      const float4* __restrict__ v_in, float4* __restrict__ v_next,
+                                                      VisConst vc, const int* __restrict__ perm,
+                                                      float* __restrict__ x_out, float* __restrict__ v_out) {
+  const int g = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const int s = kPair ? g >> 1 : g, h = kPair ? g & 1 : 0;
+  if (s >= nv.n) return;
+  const float4 xi = xs[s];
+  auto gather = [&](int j) { return VisPair{xs[j], v_in[j]}; };
+  VisSums t;
+  if (kPair && for_each_neighbor_half_idx(nv, s, h, gather, [&](int, const VisPair& q) { vis_term(t, xi, q, vc); })) {
+    VisSums o;
+    o.tw = __shfl_xor(t.tw, 1);
+    o.dx = __shfl_xor(t.dx, 1);
+    o.dy = __shfl_xor(t.dy, 1);
+    o.dz = __shfl_xor(t.dz, 1);
+    if (h == 0) vis_finish(s, xi, add_parities(t, o), v_old, v_next, vc, perm, x_out, v_out);
+    return;
+  }
+  if (h != 0) return;
+  VisSums a[2];
+  for_each_neighbor_p(
+      nv, s, wave_list(nv, __builtin_amdgcn_readfirstlane(s >> 6)), gather,
+      [&](int, const VisPair& q, auto par) { vis_term(a[decltype(par)::value], xi, q, vc); },
+      [&](bool odd) {
+        const VisSums x = odd ? a[1] : a[0], y = odd ? a[0] : a[1];
+        a[0] = x;
+        a[1] = y;
+      });
+  vis_finish(s, xi, add_parities(a[0], a[1]), v_old, v_next, vc, perm, x_out, v_out);
+}
+
 void launch_viscosity(const NbrView& nv, const float4* xs, const float4* v_old, const float4* v_in, float4* v_next,
-                      const VisConst& vc, const int* perm, float* x_out, float* v_out, hipStream_t st) {
-  k_viscosity<<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, v_old, v_in, v_next, vc, perm, x_out, v_out);
+                      const VisConst& vc, const int* perm, float* x_out, float* v_out, hipStream_t st, int lanes) {
+  if (lanes == 2)
+    k_viscosity<true><<<(int)((2ll * nv.n + kBlock - 1) / kBlock), kBlock, 0, st>>>(nv, xs, v_old, v_in, v_next, vc,
+                                                                                    perm, x_out, v_out);
+  else
+    k_viscosity<false><<<(nv.n + kBlock - 1) / kBlock, kBlock, 0, st>>>(nv, xs, v_old, v_in, v_next, vc, perm, x_out,
+                                                                        v_out);
 }
 
 // Module path: one iteration over a caller CSR list in original order

@@ -977,7 +977,7 @@ int step_once(pbf_ctx* c, const float* x_in, const float* v_in, float* x_out, fl
       const bool last = t + 1 == T;
       const float4* vin = t == 0 ? v_old : V[(t - 1) & 1];
       launch_viscosity(nv, xk, v_old, vin, last ? nullptr : V[t & 1], c->vc, c->perm, last ? x_out : nullptr,
-                       last ? v_out : nullptr, st);
+                       last ? v_out : nullptr, st, lanes);
     }
   }
   PBF_LAUNCHED(c);

@@ -311,8 +311,9 @@ void launch64_adjoint_step(const NbrView& nv, double4* const* P, double4* const*
 void launch_box_sdf(const BoxField& f, const float lo[3], const float hi[3], hipStream_t st);
 void launch_box_constrain(const BoxField& f, const float* x_in, const float* v_in, float* x_out, float* v_out, int n,
                           const int* scene_off, int nscenes, float keep_normal, float damp, hipStream_t st);
+// lanes: 1, or 2 for a lane pair per particle (same results)
 void launch_viscosity(const NbrView& nv, const float4* xs, const float4* v_old, const float4* v_in, float4* v_next,
-                      const VisConst& vc, const int* perm, float* x_out, float* v_out, hipStream_t st);
+                      const VisConst& vc, const int* perm, float* x_out, float* v_out, hipStream_t st, int lanes = 1);
 void launch_csr_viscosity(const float4* xa, const Csr& nb, const float4* v_old, const float4* v_in, float4* v_next,
                           float* v_out, const VisConst& vc, hipStream_t st);
 
