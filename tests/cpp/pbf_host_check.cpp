@@ -1,6 +1,6 @@
 // Host-side checks of libpbf.so that need no GPU, built with AddressSanitizer
 // and UndefinedBehaviorSanitizer on the host code of the library and of this
-// driver (tests/test_sanitizers.py): parameter validation, the derivation of
+// check program (run by tests/test_sanitizers.py): parameter validation, the derivation of
 // the grid and the kernel constants inside pbf_create (it runs before the
 // first HIP call), the status strings, and every entry point's refusal of a
 // null context. Prints one PASS line per check; exits 1 on a failed check (a
