@@ -199,6 +199,27 @@ def test_empty_input(fluid):
     assert nl.index.numel() == 0 and nl.elements.numel() == 0 and nl.size() == 0
 
 
+def test_particle_count_limit(fluid, capi, oracle, scene):
+    """2^27 particles or more in one call (the passes' 32-bit byte offsets of
+    16-byte records) are refused with PBF_ERR_UNSUPPORTED before any access
+    to the caller's buffers, and the context keeps working afterwards."""
+    import ctypes as C
+    say("n = 2^27: refused, then a 512-particle step on the same context")
+    x_h = scene.jittered_lattice((8, 8, 8), (0.3, 0.3, 0.3), 0.005, 0.1, seed=4)
+    ctx = fluid.Context("cuda:0")
+    x, v = dev(x_h), torch.zeros((len(x_h), 3), dtype=torch.float32, device="cuda:0")
+    p = lambda t: C.c_void_p(t.data_ptr())  # noqa: E731
+    L = capi.lib()
+    assert L.pbf_step(ctx.handle, p(x), p(v), None, 1 << 27, 1, None) == capi.PBF_ERR_UNSUPPORTED
+    assert b"2^27" in L.pbf_last_error(ctx.handle)
+    off = torch.empty(16, dtype=torch.int32, device="cuda:0")
+    total = C.c_longlong()
+    st = L.pbf_neighbor_query(ctx.handle, p(x), 1 << 27, p(off), p(off), 16, C.byref(total), None)
+    assert st == capi.PBF_ERR_UNSUPPORTED
+    nl = ctx.neighbor_query(x)
+    assert_same_sets(nl, *oracle.neighbor_query(oracle.make_params(), x_h))
+
+
 @pytest.mark.parametrize("scene_name,reserve", [
     ("lattice", {"list_slots": 1}),                     # list full: words re-tested (kept in LDS only)
     ("blob", {"list_slots": 1, "mask_words": 512}),     # list full: dense waves read global bit masks
