@@ -98,28 +98,36 @@ __device__ __forceinline__ uint32_t test_word(const float4* p, float4 xi, int cn
   return (bits << (32 - k0)) & (~0u << (32 - cnt));
 }
 
-// test_word over the coordinate planes: two 16-byte loads per coordinate bring
-// eight consecutive candidates (the loads need only 4-byte alignment), six
-// loads where the float4 array needs eight. The planes are padded like xn.
+// test_word over the coordinate planes: kG / 4 16-byte loads per coordinate
+// bring kG consecutive candidates (the loads need only 4-byte alignment), six
+// loads per eight where the float4 array needs eight. kG = 16 halves the load
+// round trips of a word at the cost of testing up to 15 slots past the run
+// (the planes are padded like xn, by kXnPad >= 16 floats).
 typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+template <int kG = 8>
 __device__ __forceinline__ uint32_t test_word_planes(const float* cx, const float* cy, const float* cz, int j0,
                                                      float4 xi, int cnt, float thr2) {
+  static_assert(kG == 8 || kG == 16, "groups of 8 or 16 candidates");
+  static_assert(kXnPad >= kG, "the planes' padding covers a group past the last particle");
   uint32_t bits = 0;
   int k0 = 0;
   // one 32-bit byte offset for the three planes (scalar bases): the loads take
   // the scalar-base form and the loop advances a single register
   const char *bx = reinterpret_cast<const char*>(cx), *by = reinterpret_cast<const char*>(cy),
              *bz = reinterpret_cast<const char*>(cz);
-  for (; k0 < cnt; k0 += 8) {
+  for (; k0 < cnt; k0 += kG) {
     const uint32_t o = (uint32_t)(j0 + k0) << 2;
-    const f4u x0 = *reinterpret_cast<const f4u*>(bx + o), x1 = *reinterpret_cast<const f4u*>(bx + o + 16);
-    const f4u y0 = *reinterpret_cast<const f4u*>(by + o), y1 = *reinterpret_cast<const f4u*>(by + o + 16);
-    const f4u z0 = *reinterpret_cast<const f4u*>(bz + o), z1 = *reinterpret_cast<const f4u*>(bz + o + 16);
-    const float px[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
-    const float py[8] = {y0.x, y0.y, y0.z, y0.w, y1.x, y1.y, y1.z, y1.w};
-    const float pz[8] = {z0.x, z0.y, z0.z, z0.w, z1.x, z1.y, z1.z, z1.w};
+    f4u qx[kG / 4], qy[kG / 4], qz[kG / 4];
 #pragma unroll
-    for (int u = 0; u < 8; ++u) bits = push_bit(bits, sq_dist(xi, make_float4(px[u], py[u], pz[u], 0.0f)), thr2);
+    for (int g = 0; g < kG / 4; ++g) qx[g] = *reinterpret_cast<const f4u*>(bx + o + 16 * g);
+#pragma unroll
+    for (int g = 0; g < kG / 4; ++g) qy[g] = *reinterpret_cast<const f4u*>(by + o + 16 * g);
+#pragma unroll
+    for (int g = 0; g < kG / 4; ++g) qz[g] = *reinterpret_cast<const f4u*>(bz + o + 16 * g);
+#pragma unroll
+    for (int u = 0; u < kG; ++u)
+      bits = push_bit(bits, sq_dist(xi, make_float4(qx[u >> 2][u & 3], qy[u >> 2][u & 3], qz[u >> 2][u & 3], 0.0f)),
+                      thr2);
   }
   return (bits << (32 - k0)) & (~0u << (32 - cnt));
 }

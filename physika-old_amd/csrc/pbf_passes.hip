@@ -173,6 +173,7 @@ __device__ __forceinline__ void write_list_runs(uint16_t* list, long long lb, in
 // whose three intervals hold at most kStageCap particles together (and contain
 // every lane's runs, checked) writes those neighbors as stage slots, so that
 // the PBD passes read them from LDS.
+template <int kG>
 __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsigned long long* __restrict__ counters) {
   __shared__ uint2 lword[kNbLdsWords * 64];
   const int wave = xcd_block(blockIdx.x, gridDim.x);
@@ -259,7 +260,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
     const int nw = (rl[q] + 31) >> 5;
     for (int w = 0; w < nw; ++w) {
       const int jb = ra[q] + (w << 5);
-      const uint32_t bits = test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
+      const uint32_t bits = test_word_planes<kG>(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
       if (in_lds) {
         const uint32_t base = mode == kListIndex32
                                   ? (uint32_t)jb
@@ -289,7 +290,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
         const int jb = ra[q] + (w << 5);
         uint32_t bits = (!in_lds && mb >= 0)
                             ? gcol[(long long)(wo + w) << 6]
-                            : test_word_planes(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
+                            : test_word_planes<kG>(nv.cx, nv.cy, nv.cz, jb, xi, min(32, rl[q] - (w << 5)), nv.thr2);
         while (bits) {
           const int j = jb + pop_first(bits);
           const int slot = atomicAdd(&out.oog_rcnt[j], 1);
@@ -354,7 +355,7 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
         write_list_lds<0>(out.list, lb, lane, lcol, cnt, n_sl, pad_to);
     } else if (in_lds) {  // this lane's words did not fit: test its candidates again
       auto retest = [&](int q, int w, int) {
-        return test_word_planes(nv.cx, nv.cy, nv.cz, ra[q] + (w << 5), xi, min(32, rl[q] - (w << 5)), nv.thr2);
+        return test_word_planes<kG>(nv.cx, nv.cy, nv.cz, ra[q] + (w << 5), xi, min(32, rl[q] - (w << 5)), nv.thr2);
       };
       if (mode == kListIndex32)
         write_list_runs<1>(out.list, lb, lane, ra, rl, retest, jbase, sdel, 0, 0);
@@ -370,13 +371,29 @@ __global__ __launch_bounds__(64) void k_neighbors(NbrView nv, NbrOut out, unsign
   }
 }
 
-void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st,
-                      const KernelEvents* ev) {
+// A search of at most kWideGroupMaxParticles particles runs one wave or fewer
+// per SIMD, and each wave waits on its own candidate loads: it tests sixteen
+// candidates per load round trip. Larger searches keep groups of eight (issue
+// slots bound them, and a run's partly empty last group costs more in groups
+// of sixteen). Both give the same bits.
+constexpr int kWideGroupMaxParticles = 65536;
+
+template <int kG>
+void launch_neighbors_g(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st,
+                        const KernelEvents* ev) {
   if (ev)
-    hipExtLaunchKernelGGL(k_neighbors, dim3((nv.n + 63) / 64), dim3(64), 0, st, ev->start, ev->stop, 0, nv, out,
+    hipExtLaunchKernelGGL(k_neighbors<kG>, dim3((nv.n + 63) / 64), dim3(64), 0, st, ev->start, ev->stop, 0, nv, out,
                           counters);
   else
-    k_neighbors<<<(nv.n + 63) / 64, 64, 0, st>>>(nv, out, counters);
+    k_neighbors<kG><<<(nv.n + 63) / 64, 64, 0, st>>>(nv, out, counters);
+}
+
+void launch_neighbors(const NbrView& nv, const NbrOut& out, unsigned long long* counters, hipStream_t st,
+                      const KernelEvents* ev) {
+  if (nv.n <= kWideGroupMaxParticles)
+    launch_neighbors_g<16>(nv, out, counters, st, ev);
+  else
+    launch_neighbors_g<8>(nv, out, counters, st, ev);
 }
 
 // One block: sums and maxima over the per-wave statistics.
